@@ -1,0 +1,443 @@
+/*
+ * C entry points over the real TNL headers (Devices::Host), compiled from the sources
+ * where they lie by oracle/Makefile into oracle/_ref/libtnlref.so.
+ *
+ * TEST INFRASTRUCTURE ONLY: it pins the oracle (oracle/oracle.cpp) to TNL's own results
+ * and serves as the timed CPU baseline.  Nothing in the product links it.
+ * The API mirrors oracle.h so tests can compare the two value by value.
+ */
+#include <TNL/Algorithms/Segments/BiEllpack.h>
+#include <TNL/Algorithms/Segments/CSR.h>
+#include <TNL/Algorithms/Segments/ChunkedEllpack.h>
+#include <TNL/Algorithms/Segments/Ellpack.h>
+#include <TNL/Algorithms/Segments/SlicedEllpack.h>
+#include <TNL/Algorithms/Segments/Kernels/CSRAdaptiveKernel.h>
+#include <TNL/Containers/Vector.h>
+#include <TNL/Matrices/SparseMatrix.h>
+#include <TNL/Solvers/Linear/BICGStab.h>
+#include <TNL/Solvers/Linear/CG.h>
+#include <TNL/Solvers/Linear/GMRES.h>
+#include <TNL/Solvers/Linear/Preconditioners/Diagonal.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <unistd.h>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include <omp.h>
+
+using namespace TNL;
+namespace Seg = TNL::Algorithms::Segments;
+using Host = TNL::Devices::Host;
+using IVec = Containers::Vector< int, Host, int >;
+using DVec = Containers::Vector< double, Host, int >;
+using Alloc = typename Allocators::Default< Host >::template Allocator< int >;
+
+namespace {
+
+// Expose the protected metadata of each segments class.
+template< Seg::ElementsOrganization O >
+struct PeekEllpack : Seg::Ellpack< Host, int, Alloc, O, 32 >
+{
+   using Base = Seg::Ellpack< Host, int, Alloc, O, 32 >;
+   std::vector< int > meta() const { return { this->segmentSize, this->size, this->alignedSize }; }
+};
+template< Seg::ElementsOrganization O >
+struct PeekSliced : Seg::SlicedEllpack< Host, int, Alloc, O, 32 >
+{
+   using Base = Seg::SlicedEllpack< Host, int, Alloc, O, 32 >;
+   using Base::sliceOffsets;
+   using Base::sliceSegmentSizes;
+   std::vector< int > meta() const { return { this->size, this->alignedSize }; }
+};
+template< Seg::ElementsOrganization O >
+struct PeekChunked : Seg::ChunkedEllpack< Host, int, Alloc, O >
+{
+   using Base = Seg::ChunkedEllpack< Host, int, Alloc, O >;
+   using Base::chunksToSegmentsMapping;
+   using Base::numberOfSlices;
+   using Base::rowPointers;
+   using Base::rowToChunkMapping;
+   using Base::rowToSliceMapping;
+   using Base::slices;
+   using Base::storageSize;
+};
+template< Seg::ElementsOrganization O >
+struct PeekBi : Seg::BiEllpack< Host, int, Alloc, O, 32 >
+{
+   using Base = Seg::BiEllpack< Host, int, Alloc, O, 32 >;
+   using Base::groupPointers;
+   using Base::rowPermArray;
+   using Base::storageSize;
+   using Base::virtualRows;
+};
+
+template< typename V >
+std::vector< int > toStd( const V& v )
+{
+   std::vector< int > r( v.getSize() );
+   for( int i = 0; i < v.getSize(); i++ )
+      r[ i ] = v[ i ];
+   return r;
+}
+
+struct Handle
+{
+   std::map< std::string, std::vector< int > > arrays;
+   long long storage = 0;
+};
+
+template< typename Peek >
+void fillChunked( Handle& h, const Peek& p )
+{
+   h.arrays[ "rowToChunkMapping" ] = toStd( p.rowToChunkMapping );
+   h.arrays[ "rowToSliceMapping" ] = toStd( p.rowToSliceMapping );
+   h.arrays[ "chunksToSegmentsMapping" ] = toStd( p.chunksToSegmentsMapping );
+   h.arrays[ "rowPointers" ] = toStd( p.rowPointers );
+   std::vector< int > sl;
+   for( int i = 0; i < p.numberOfSlices; i++ ) {
+      const auto& s = p.slices[ i ];
+      sl.insert( sl.end(), { s.size, s.chunkSize, s.firstSegment, s.pointer } );
+   }
+   h.arrays[ "slices" ] = sl;
+   h.arrays[ "chunked" ] = { p.numberOfSlices, p.storageSize };
+   h.storage = p.storageSize;
+}
+
+template< typename Peek >
+void fillBi( Handle& h, const Peek& p )
+{
+   h.arrays[ "rowPermArray" ] = toStd( p.rowPermArray );
+   h.arrays[ "groupPointers" ] = toStd( p.groupPointers );
+   h.storage = p.storageSize;
+   h.arrays[ "biellpack" ] = { p.virtualRows, (int) ( h.storage & 0xffffffff ), (int) ( h.storage >> 32 ) };
+}
+
+template< Seg::ElementsOrganization O >
+Handle* build( int kind, const IVec& caps )
+{
+   auto* h = new Handle;
+   switch( kind ) {
+      case 0: {
+         Seg::CSR< Host, int > s;
+         s.setSegmentsSizes( caps );
+         h->arrays[ "offsets" ] = toStd( s.getOffsets() );
+         h->storage = s.getStorageSize();
+         break;
+      }
+      case 1: {
+         PeekEllpack< O > s;
+         s.setSegmentsSizes( caps );
+         h->arrays[ "ellpack" ] = s.meta();
+         h->storage = s.getStorageSize();
+         break;
+      }
+      case 2: {
+         PeekSliced< O > s;
+         s.setSegmentsSizes( caps );
+         h->arrays[ "sliceOffsets" ] = toStd( s.sliceOffsets );
+         h->arrays[ "sliceSegmentSizes" ] = toStd( s.sliceSegmentSizes );
+         h->arrays[ "sliced" ] = s.meta();
+         h->storage = s.getStorageSize();
+         break;
+      }
+      case 3: {
+         PeekChunked< O > s;
+         s.setSegmentsSizes( caps );
+         fillChunked( *h, s );
+         break;
+      }
+      case 4: {
+         PeekBi< O > s;
+         s.setSegmentsSizes( caps );
+         fillBi( *h, s );
+         break;
+      }
+      default: delete h; return nullptr;
+   }
+   return h;
+}
+
+template< typename Matrix >
+void fillCSR( Matrix& A, int n, int ncols, const int* rp, const int* cols, const double* vals )
+{
+   A.setDimensions( n, ncols );
+   IVec caps( n );
+   for( int i = 0; i < n; i++ )
+      caps[ i ] = rp[ i + 1 ] - rp[ i ];
+   A.setRowCapacities( caps );
+   auto& c = A.getColumnIndexes();
+   auto& v = A.getValues();
+   const int nnz = rp[ n ];
+   for( int k = 0; k < nnz; k++ ) {
+      c[ k ] = cols[ k ];
+      v[ k ] = vals[ k ];
+   }
+}
+
+using CsrMatrix = Matrices::SparseMatrix< double, Host, int, Matrices::GeneralMatrix, Seg::CSRDefault >;
+
+template< template< typename, typename, typename > class S >
+int convertSpmv( int n, int ncols, const int* rp, const int* cols, const double* vals, const double* x, double* y,
+                 double* out_vals, int* out_cols, long long storage_cap )
+{
+   CsrMatrix A;
+   fillCSR( A, n, ncols, rp, cols, vals );
+   Matrices::SparseMatrix< double, Host, int, Matrices::GeneralMatrix, S > B;
+   B = A;
+   const long long st = B.getValues().getSize();
+   if( st > storage_cap )
+      return -1;
+   for( long long g = 0; g < st; g++ ) {
+      out_vals[ g ] = B.getValues()[ g ];
+      out_cols[ g ] = B.getColumnIndexes()[ g ];
+   }
+   DVec xv( ncols ), yv( n );
+   for( int i = 0; i < ncols; i++ )
+      xv[ i ] = x[ i ];
+   for( int i = 0; i < n; i++ )
+      yv[ i ] = y[ i ];
+   B.vectorProduct( xv, yv );
+   for( int i = 0; i < n; i++ )
+      y[ i ] = yv[ i ];
+   return 0;
+}
+
+template< typename D, typename I, typename A >
+using EllC = Seg::Ellpack< D, I, A, Seg::ColumnMajorOrder, 32 >;
+template< typename D, typename I, typename A >
+using EllRow = Seg::Ellpack< D, I, A, Seg::RowMajorOrder, 32 >;
+template< typename D, typename I, typename A >
+using SlC = Seg::SlicedEllpack< D, I, A, Seg::ColumnMajorOrder, 32 >;
+template< typename D, typename I, typename A >
+using SlR = Seg::SlicedEllpack< D, I, A, Seg::RowMajorOrder, 32 >;
+template< typename D, typename I, typename A >
+using ChC = Seg::ChunkedEllpack< D, I, A, Seg::ColumnMajorOrder >;
+template< typename D, typename I, typename A >
+using ChR = Seg::ChunkedEllpack< D, I, A, Seg::RowMajorOrder >;
+template< typename D, typename I, typename A >
+using BiC = Seg::BiEllpack< D, I, A, Seg::ColumnMajorOrder, 32 >;
+template< typename D, typename I, typename A >
+using BiR = Seg::BiEllpack< D, I, A, Seg::RowMajorOrder, 32 >;
+
+// residual history through IterativeSolver::setResidue (IterativeSolver.hpp:189-200)
+template< typename Solver >
+struct WithHistory : Solver
+{
+   void openHistory( const std::string& path )
+   {
+      this->residualHistoryFile.open( path );
+      this->residualHistoryFile.precision( 17 );
+   }
+   void closeHistory() { this->residualHistoryFile.close(); }
+};
+
+}  // namespace
+
+extern "C" {
+
+void* ref_segments_create( int kind, int n, const int* caps, int colmajor )
+{
+   IVec c( n );
+   for( int i = 0; i < n; i++ )
+      c[ i ] = caps[ i ];
+   return colmajor ? (void*) build< Seg::ColumnMajorOrder >( kind, c ) : (void*) build< Seg::RowMajorOrder >( kind, c );
+}
+
+void ref_segments_destroy( void* h )
+{
+   delete (Handle*) h;
+}
+
+long long ref_segments_storage_size( void* h )
+{
+   return ( (Handle*) h )->storage;
+}
+
+long long ref_segments_get( void* hv, const char* name, int* out )
+{
+   auto* h = (Handle*) hv;
+   auto it = h->arrays.find( name );
+   if( it == h->arrays.end() )
+      return -1;
+   if( out )
+      std::memcpy( out, it->second.data(), it->second.size() * sizeof( int ) );
+   return (long long) it->second.size();
+}
+
+// SparseMatrix<Host, kind>::operator=( CSR ) then vectorProduct( x, y ); exports the storage.
+int ref_convert_spmv( int kind, int colmajor, int n, int ncols, const int* rp, const int* cols, const double* vals,
+                      const double* x, double* y, double* out_vals, int* out_cols, long long storage_cap )
+{
+   switch( kind * 2 + ( colmajor ? 1 : 0 ) ) {
+      case 0:
+      case 1: return convertSpmv< Seg::CSRScalar >( n, ncols, rp, cols, vals, x, y, out_vals, out_cols, storage_cap );
+      case 2: return convertSpmv< EllRow >( n, ncols, rp, cols, vals, x, y, out_vals, out_cols, storage_cap );
+      case 3: return convertSpmv< EllC >( n, ncols, rp, cols, vals, x, y, out_vals, out_cols, storage_cap );
+      case 4: return convertSpmv< SlR >( n, ncols, rp, cols, vals, x, y, out_vals, out_cols, storage_cap );
+      case 5: return convertSpmv< SlC >( n, ncols, rp, cols, vals, x, y, out_vals, out_cols, storage_cap );
+      case 6: return convertSpmv< ChR >( n, ncols, rp, cols, vals, x, y, out_vals, out_cols, storage_cap );
+      case 7: return convertSpmv< ChC >( n, ncols, rp, cols, vals, x, y, out_vals, out_cols, storage_cap );
+      case 8: return convertSpmv< BiR >( n, ncols, rp, cols, vals, x, y, out_vals, out_cols, storage_cap );
+      case 9: return convertSpmv< BiC >( n, ncols, rp, cols, vals, x, y, out_vals, out_cols, storage_cap );
+   }
+   return -2;
+}
+
+// CSRAdaptiveKernel::init, blocks for sizeof(Value) == 2^value_size_log; 6 ints per block.
+long long ref_csr_adaptive_blocks( int n, const int* offsets, int value_size_log, int* out )
+{
+   struct PeekKernel : Seg::CSRAdaptiveKernel< int, Host >
+   {
+      using Seg::CSRAdaptiveKernel< int, Host >::blocksArray;
+   };
+   IVec off( n + 1 );
+   for( int i = 0; i <= n; i++ )
+      off[ i ] = offsets[ i ];
+   PeekKernel k;
+   k.init( off );
+   const auto& b = k.blocksArray[ value_size_log ];
+   if( out )
+      for( int i = 0; i < b.getSize(); i++ ) {
+         const auto d = b[ i ];
+         const int type = (int) d.getType();
+         const bool sized = type != 0 && i + 1 < b.getSize();
+         out[ 6 * i + 0 ] = type;
+         out[ 6 * i + 1 ] = d.getFirstSegment();
+         out[ 6 * i + 2 ] = sized ? d.getSize() : 0;
+         out[ 6 * i + 3 ] = sized ? d.getSegmentsInBlock() : 0;
+         out[ 6 * i + 4 ] = d.getWarpIdx();
+         out[ 6 * i + 5 ] = type == 0 ? d.getWarpsCount() : 0;  // set only for LONG blocks
+      }
+   return b.getSize();
+}
+
+// Timed Host CSR SpMV (OpenMP CSRScalarKernel): returns the median seconds per call.
+double ref_time_spmv_csr( int n, int ncols, const int* rp, const int* cols, const double* vals, const double* x,
+                          double* y, int loops, int threads )
+{
+   if( threads > 0 )
+      omp_set_num_threads( threads );
+   CsrMatrix A;
+   fillCSR( A, n, ncols, rp, cols, vals );
+   DVec xv( ncols ), yv( n );
+   for( int i = 0; i < ncols; i++ )
+      xv[ i ] = x[ i ];
+   A.vectorProduct( xv, yv );
+   std::vector< double > t;
+   for( int l = 0; l < loops; l++ ) {
+      const auto a = std::chrono::steady_clock::now();
+      A.vectorProduct( xv, yv );
+      const auto b = std::chrono::steady_clock::now();
+      t.push_back( std::chrono::duration< double >( b - a ).count() );
+   }
+   for( int i = 0; i < n; i++ )
+      y[ i ] = yv[ i ];
+   std::sort( t.begin(), t.end() );
+   return t.empty() ? 0.0 : t[ t.size() / 2 ];
+}
+
+typedef struct
+{
+   int max_iterations, min_iterations;
+   double convergence_residue, divergence_residue;
+   int gmres_variant, restarting_min, restarting_max, restarting_step_min, restarting_step_max;
+   int bicgstab_exact_residue, preconditioner;
+} ref_solver_params;
+
+// Solve on the Host with TNL's solvers; history via residual-history-file.  Returns
+// 1 converged, 0 not, -1 GMRES breakdown; *seconds = wall time of solve().
+int ref_solve( int method, int n, const int* rp, const int* cols, const double* vals, const double* b, double* x,
+               const ref_solver_params* p, int* iterations, double* residue, double* history, int history_cap,
+               int* history_len, int threads, double* seconds )
+{
+   if( threads > 0 )
+      omp_set_num_threads( threads );
+   auto A = std::make_shared< CsrMatrix >();
+   fillCSR( *A, n, n, rp, cols, vals );
+   DVec bv( n ), xv( n );
+   for( int i = 0; i < n; i++ ) {
+      bv[ i ] = b[ i ];
+      xv[ i ] = x[ i ];
+   }
+   char path[] = "/tmp/tnlref_histXXXXXX";
+   const int fd = mkstemp( path );
+   if( fd >= 0 )
+      close( fd );
+   using Pre = Solvers::Linear::Preconditioners::Diagonal< CsrMatrix >;
+   auto pre = std::make_shared< Pre >();
+   int rc = 0;
+   auto run = [ & ]( auto& solver )
+   {
+      solver.setMatrix( A );
+      if( p->preconditioner == 1 ) {
+         pre->update( A );
+         solver.setPreconditioner( pre );
+      }
+      solver.setMaxIterations( p->max_iterations );
+      solver.setMinIterations( p->min_iterations );
+      solver.setConvergenceResidue( p->convergence_residue );
+      solver.setDivergenceResidue( p->divergence_residue );
+      solver.openHistory( path );
+      const auto t0 = std::chrono::steady_clock::now();
+      try {
+         rc = solver.solve( bv, xv ) ? 1 : 0;
+      }
+      catch( int ) {
+         rc = -1;
+      }
+      const auto t1 = std::chrono::steady_clock::now();
+      if( seconds )
+         *seconds = std::chrono::duration< double >( t1 - t0 ).count();
+      solver.closeHistory();
+      *iterations = solver.getIterations();
+      *residue = solver.getResidue();
+   };
+   if( method == 0 ) {
+      WithHistory< Solvers::Linear::CG< CsrMatrix > > s;
+      run( s );
+   }
+   else if( method == 1 ) {
+      WithHistory< Solvers::Linear::BICGStab< CsrMatrix > > s;
+      Config::ParameterContainer prm;
+      prm.addParameter< bool >( "bicgstab-exact-residue", p->bicgstab_exact_residue != 0 );
+      s.setup( prm );
+      run( s );
+   }
+   else {
+      WithHistory< Solvers::Linear::GMRES< CsrMatrix > > s;
+      Config::ParameterContainer prm;
+      const char* names[] = { "CGS", "CGSR", "MGS", "MGSR", "CWY" };
+      prm.addParameter< String >( "gmres-variant", names[ p->gmres_variant ] );
+      prm.addParameter< int >( "gmres-restarting-min", p->restarting_min );
+      prm.addParameter< int >( "gmres-restarting-max", p->restarting_max );
+      prm.addParameter< int >( "gmres-restarting-step-min", p->restarting_step_min );
+      prm.addParameter< int >( "gmres-restarting-step-max", p->restarting_step_max );
+      s.setup( prm );
+      run( s );
+   }
+   for( int i = 0; i < n; i++ )
+      x[ i ] = xv[ i ];
+   // parse "iter \t residue" lines
+   std::ifstream in( path );
+   int count = 0;
+   long long itn;
+   double val;
+   while( in >> itn >> val ) {
+      if( history && count < history_cap )
+         history[ count ] = val;
+      count++;
+   }
+   if( history_len )
+      *history_len = count;
+   std::remove( path );
+   return rc;
+}
+
+}  // extern "C"

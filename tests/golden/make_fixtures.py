@@ -1,0 +1,135 @@
+"""Writes the golden fixtures under tests/golden/ from TNL's own Host path.
+
+The values come from oracle/_ref/libtnlref.so, which oracle/Makefile compiles from the
+reference headers where they lie (no reference source is copied).  Run in the container
+that holds the reference tree:
+
+    make -C oracle && python tests/golden/make_fixtures.py
+
+Outputs (numpy .npz, data only):
+  segments.npz  per case: CSR input, x, and for each segments kind the metadata arrays,
+                the converted storage (values, columns) and y = A x (column-major
+                organization, TNL's GPU default; metadata also for row-major)
+  adaptive.npz  CSR-Adaptive block descriptors per case and value size
+  solvers.npz   residual histories, iteration counts, convergence flags and solutions of
+                CG / BiCGStab / GMRES (all variants, with and without Jacobi) at one thread
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tnl-dev_amd"))
+
+import pyoracle as O  # noqa: E402
+import workloads as W  # noqa: E402
+
+
+def cases():
+    """Small matrices covering the edge cases of every format."""
+    out = {}
+    out["lap2d_16"] = W.laplacian_2d(16)
+    out["lap3d_10"] = W.laplacian_3d(10)  # n = 1000: partial last strip / slice
+    out["irr_3000"] = W.irregular(3000, mean=16, sigma=1.2, max_len=1024, seed=7)
+    out["convdiff_8"] = W.convection_diffusion_3d(8, 10.0)
+    # explicit zeros, empty rows and a ragged tail of 33 rows (one-row last strip)
+    rng = np.random.default_rng(11)
+    rows = []
+    for r in range(33):
+        L = int(rng.integers(0, 40)) if r % 5 else 0
+        cs = np.sort(rng.choice(64, size=L, replace=False))
+        vs = rng.standard_normal(L)
+        vs[rng.random(L) < 0.2] = 0.0  # explicit zeros are dropped by the conversion
+        rows.append((cs, vs))
+    rp = np.zeros(34, np.int32)
+    rp[1:] = np.cumsum([len(c) for c, _ in rows])
+    out["ragged_zeros_33x64"] = (rp, np.concatenate([c for c, _ in rows]).astype(np.int32),
+                                 np.concatenate([v for _, v in rows]).astype(np.float64))
+    return out
+
+
+def main():
+    if not O.ref_available():
+        raise SystemExit("oracle/_ref/libtnlref.so is missing: run make -C oracle in the reference container")
+    seg = {}
+    for name, (rp, c, v) in cases().items():
+        n = len(rp) - 1
+        ncols = 64 if name.startswith("ragged") else n
+        x = W.uniform_vector(ncols, 1234)
+        seg[f"{name}/row_ptr"] = rp
+        seg[f"{name}/cols"] = c
+        seg[f"{name}/vals"] = v
+        seg[f"{name}/x"] = x
+        seg[f"{name}/ncols"] = np.array([ncols], np.int32)
+        caps = np.diff(rp).astype(np.int32)
+        for kind, k in O.KINDS.items():
+            for cm in (True, False):
+                s = O.Segments(k, caps, cm, backend="ref")
+                org = "col" if cm else "row"
+                for nm, arr in s.metadata().items():
+                    seg[f"{name}/{kind}/{org}/{nm}"] = arr
+                seg[f"{name}/{kind}/{org}/storage"] = np.array([s.storage_size()], np.int64)
+            vals, cols, y = O.ref_convert_spmv(k, rp, c, v, ncols, x, column_major=True)
+            if kind == "BiEllpack":
+                # TNL allocates 32 x groupPointers[last]; everything past groupPointers[last]
+                # is padding, so only the used part is kept
+                used = int(seg[f"{name}/{kind}/col/groupPointers"][-1])
+                assert np.all(cols[used:] == -1) and np.all(vals[used:] == 0.0)
+                vals, cols = vals[:used], cols[:used]
+            if len(vals) <= 300000:  # padding-heavy Ellpack storage of the irregular case is left out
+                seg[f"{name}/{kind}/col/values"] = vals
+                seg[f"{name}/{kind}/col/columns"] = cols
+            seg[f"{name}/{kind}/col/y"] = y
+    np.savez_compressed(os.path.join(HERE, "segments.npz"), **seg)
+
+    ad = {}
+    for name, (rp, c, v) in cases().items():
+        for vs in (4, 8):
+            ad[f"{name}/{vs}"] = O.ref_csr_adaptive_blocks(rp, vs)
+    np.savez_compressed(os.path.join(HERE, "adaptive.npz"), **ad)
+
+    sol = {}
+    systems = {
+        "lap2d_24": W.laplacian_2d(24),
+        "irr_2000": W.irregular(2000, mean=16, sigma=1.2, max_len=512, seed=3),
+        "convdiff_8_pe10": W.convection_diffusion_3d(8, 10.0),
+    }
+    runs = [("lap2d_24", "cg", {}), ("lap2d_24", "cg", {"preconditioner": 1}),
+            ("lap2d_24", "bicgstab", {}), ("lap2d_24", "bicgstab", {"preconditioner": 1}),
+            ("lap2d_24", "bicgstab", {"bicgstab_exact_residue": 1})]
+    runs += [("lap2d_24", "gmres", {"gmres_variant": var}) for var in range(5)]
+    runs += [("lap2d_24", "gmres", {"gmres_variant": 4, "preconditioner": 1, "restarting_min": 5,
+                                    "restarting_max": 15})]
+    runs += [("irr_2000", "bicgstab", {"preconditioner": 1, "convergence_residue": 1e-10})]
+    runs += [("convdiff_8_pe10", "gmres", {"gmres_variant": var, "preconditioner": 1, "restarting_min": 30,
+                                           "restarting_max": 30, "convergence_residue": 1e-10})
+             for var in (3, 4)]
+    for name, (rp, c, v) in systems.items():
+        sol[f"{name}/row_ptr"] = rp
+        sol[f"{name}/cols"] = c
+        sol[f"{name}/vals"] = v
+        sol[f"{name}/b"] = W.uniform_vector(len(rp) - 1, 42)
+    for i, (name, method, kw) in enumerate(runs):
+        rp, c, v = systems[name]
+        kw = dict(kw)
+        kw.setdefault("convergence_residue", 1e-8)
+        kw.setdefault("max_iterations", 5000)
+        r = O.solve(method, rp, c, v, sol[f"{name}/b"], backend="ref", threads=1, **kw)
+        key = f"run{i:02d}"
+        sol[f"{key}/system"] = np.array(name)
+        sol[f"{key}/method"] = np.array(method)
+        sol[f"{key}/params"] = np.array(repr(sorted(kw.items())))
+        sol[f"{key}/history"] = r["history"]
+        sol[f"{key}/iterations"] = np.array([r["iterations"]])
+        sol[f"{key}/converged"] = np.array([r["converged"]])
+        sol[f"{key}/x"] = r["x"]
+    np.savez_compressed(os.path.join(HERE, "solvers.npz"), **sol)
+    for f in ("segments.npz", "adaptive.npz", "solvers.npz"):
+        print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()
