@@ -1,0 +1,237 @@
+/*
+ * tnlhip — C ABI of the MI355X (gfx950) implementation of TNL's sparse matrix-vector
+ * product, its segment formats, the reduction / BLAS-1 kernels and the Krylov solvers
+ * CG, BiCGStab and GMRES.
+ *
+ * TNL selects the device by a template tag (Devices::Host / Devices::Cuda); its GPU
+ * boundary is SparseMatrix<Real, Device, Index, GeneralMatrix, Segments> with the Device
+ * tag routed to the CUDA allocator and kernel launches.  This header is the boundary a
+ * new Devices::Hip tag binds to: plain pointers, sizes and status codes, no C++ or torch
+ * types.  Each entry point names the reference interface it replaces (paths relative to
+ * TNL's src/).  The C++ front in tnl-dev_amd/TNL/ wraps it with TNL's own class names.
+ *
+ * Conventions
+ *  - Index type is int32 (TNL's default `int`).
+ *  - Device pointers are HIP device pointers; `stream` is a hipStream_t (NULL = default).
+ *  - Vectors are float64 for F64 matrices and float32 for F32 and BF16 matrices.
+ *  - Every call returns a tnlhip_status; tnlhip_last_error() describes the last failure
+ *    on the calling thread.
+ *  - Calls that take a stream are asynchronous on it; calls that return host values
+ *    (dot, norm, solve) synchronise that stream before returning, as TNL's do.
+ */
+#ifndef TNLHIP_H
+#define TNLHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+
+typedef enum
+{
+   TNLHIP_SUCCESS = 0,
+   TNLHIP_ERROR_INVALID_ARGUMENT = 1, /* TNL_ASSERT_* / std::invalid_argument */
+   TNLHIP_ERROR_RUNTIME = 2,          /* Exceptions::CudaRuntimeError (TNL_CHECK_CUDA_DEVICE) */
+   TNLHIP_ERROR_OUT_OF_MEMORY = 3,    /* std::bad_alloc from Allocators::Cuda */
+   TNLHIP_ERROR_CAPACITY = 4,         /* std::logic_error on row capacity overflow, SparseMatrixView.hpp:304-312 */
+   TNLHIP_ERROR_UNSUPPORTED = 5,
+   TNLHIP_ERROR_BREAKDOWN = 6         /* GMRES zero Hessenberg diagonal: `throw 1`, Solvers/Linear/GMRES.hpp:648-660 */
+} tnlhip_status;
+
+const char* tnlhip_status_string( int status );
+const char* tnlhip_last_error( void );
+
+/* Library version, device name and compute-unit count of the current HIP device. */
+int tnlhip_device_info( char* name, int name_len, int* compute_units, int64_t* global_memory );
+
+/* ------------------------------------------------------------------ sparse matrix */
+
+/* Segments of SparseMatrix<..., Segments> and, for CSR, its kernel
+ * (Algorithms/Segments/CSR.h:577-605, Ellpack.h, SlicedEllpack.h, ChunkedEllpack.h, BiEllpack.h).
+ * All padded formats use TNL's GPU default ElementsOrganization, column-major
+ * (Algorithms/Segments/ElementsOrganization.h:21-32). */
+typedef enum
+{
+   TNLHIP_CSR_SCALAR = 0,   /* CSRScalar: one lane per row, sequential left-to-right sum */
+   TNLHIP_CSR_VECTOR = 1,   /* CSRVector: one wavefront (64 lanes) per row */
+   TNLHIP_CSR_LIGHT = 2,    /* CSRLight: 2..64 lanes per row chosen from nnz / rows */
+   TNLHIP_CSR_HYBRID = 3,   /* CSRHybrid: 2^ceil(log2(nnz/rows)) lanes per row, at most 64 */
+   TNLHIP_CSR_ADAPTIVE = 4, /* CSRAdaptive: STREAM / VECTOR / LONG blocks */
+   TNLHIP_ELLPACK = 5,      /* Ellpack, alignment 32 */
+   TNLHIP_SLICED_ELLPACK = 6, /* SlicedEllpack, slice size 32 */
+   TNLHIP_CHUNKED_ELLPACK = 7, /* ChunkedEllpack, 256 chunks per slice, desired chunk size 16 */
+   TNLHIP_BIELLPACK = 8     /* BiEllpack, strips of 32 rows */
+} tnlhip_format;
+
+typedef enum
+{
+   TNLHIP_F64 = 0,
+   TNLHIP_F32 = 1,
+   TNLHIP_BF16 = 2 /* values stored as bfloat16, ComputeReal = float (SparseMatrix.h:40-49) */
+} tnlhip_dtype;
+
+typedef struct tnlhip_matrix_s* tnlhip_matrix;
+
+typedef struct
+{
+   int32_t rows;
+   int32_t columns;
+   int32_t format;
+   int32_t value_dtype;
+   int64_t storage_size;  /* Segments::getStorageSize() as TNL reports it */
+   int64_t allocated;     /* elements held in HBM (BiEllpack: groupPointers[last], not 32x) */
+   int64_t nonzeros;      /* getNonzeroElementsCount() */
+   int32_t lanes_per_row; /* CSR light / hybrid / vector kernel parameter (0 otherwise) */
+   int32_t reserved;
+} tnlhip_matrix_info;
+
+/* SparseMatrix<Real, Devices::Hip, int, GeneralMatrix, Segments> A;  A = csr;
+ *   Matrices/SparseMatrix.hpp:1070-1180 (operator= from a CSR matrix): row capacities are
+ *   the source row lengths, entries are copied in source order, explicit zeros are
+ *   dropped and the remaining slots hold the padding index -1
+ *   (SparseMatrix.hpp:1315-1318).  setRowCapacities, SparseMatrix.hpp:284-305.
+ * row_pointers[rows+1], column_indexes[nnz], values[nnz] (float64) are host pointers when
+ * host_pointers != 0, HIP device pointers otherwise. */
+int tnlhip_matrix_create_csr( int format, int value_dtype, int32_t rows, int32_t columns, const int32_t* row_pointers,
+                              const int32_t* column_indexes, const double* values, int host_pointers,
+                              tnlhip_matrix* out );
+
+/* B = A across formats / value types on the device (SparseMatrix.hpp:1093-1107). */
+int tnlhip_matrix_convert( tnlhip_matrix source, int format, int value_dtype, tnlhip_matrix* out );
+
+int tnlhip_matrix_destroy( tnlhip_matrix A );
+
+int tnlhip_matrix_get_info( tnlhip_matrix A, tnlhip_matrix_info* info );
+
+/* Copies one segments metadata array to the host; returns its length (or -1 for an
+ * unknown name), out may be NULL to query the length.  Names follow TNL's members:
+ *   CSR:            offsets                                   (detail/CSR.h:22-34)
+ *   Ellpack:        ellpack = {segmentSize, size, alignedSize} (Ellpack.hpp:72-83)
+ *   SlicedEllpack:  sliceOffsets, sliceSegmentSizes, sliced = {size, alignedSize}
+ *   ChunkedEllpack: rowToChunkMapping, rowToSliceMapping, chunksToSegmentsMapping,
+ *                   rowPointers, slices (4 ints per slice: size, chunkSize,
+ *                   firstSegment, pointer), chunked = {numberOfSlices, storageSize}
+ *   BiEllpack:      rowPermArray, groupPointers, biellpack = {virtualRows, storage lo, hi}
+ *   CSRAdaptive:    adaptiveBlocks (6 ints per block: type, firstSegment, blockSize,
+ *                   segmentsInBlock, warpIdx, warpsCount) */
+int64_t tnlhip_matrix_get_metadata( tnlhip_matrix A, const char* name, int32_t* out );
+
+/* Host-only: the segments metadata `format` builds for n segments of the given capacities,
+ * without touching the GPU (Segments::setSegmentsSizes on Devices::Host; e.g.
+ * ChunkedEllpack.hpp:194-235, BiEllpack.hpp:246-279).  Same names and return convention as
+ * tnlhip_matrix_get_metadata, plus "storage" = {lo, hi} of getStorageSize(). */
+int64_t tnlhip_segments_metadata( int format, int32_t n, const int32_t* capacities, const char* name, int32_t* out );
+
+/* Copies the allocated storage (values as float64, column indexes) to the host. */
+int tnlhip_matrix_get_storage( tnlhip_matrix A, double* values, int32_t* columns );
+
+/* Diagonal a_ii of every row (SparseMatrixView::getElement(i, i)), device pointer
+ * of the vector type. */
+int tnlhip_matrix_get_diagonal( tnlhip_matrix A, void* diagonal, void* stream );
+
+/* Overrides a kernel parameter chosen at construction ("lanes_per_row" for CSR
+ * light / hybrid / vector; "xcd_remap" 0/1 for every kernel). */
+int tnlhip_matrix_set_param( tnlhip_matrix A, const char* name, int32_t value );
+
+/* SparseMatrix::vectorProduct( x, y, alpha, beta, firstRow, lastRow ):
+ *   y[first, last) = alpha * A x + beta * y; last == 0 means all rows; beta == 0 does not
+ *   read y.  Matrices/SparseMatrix.h:1078-1085, SparseMatrixView.hpp:390-491. */
+int tnlhip_spmv( tnlhip_matrix A, const void* x, void* y, double alpha, double beta, int32_t first, int32_t last,
+                 void* stream );
+
+/* ------------------------------------------------------------------ reductions / BLAS-1 */
+
+/* (a, b) = Algorithms::reduce( a * b, Plus ), Containers/Expressions/ExpressionTemplates.h:393-403.
+ * Deterministic: fixed partition and fixed combining order. */
+int tnlhip_dot( int dtype, int64_t n, const void* a, const void* b, double* result, void* stream );
+
+/* lpNorm( a, 2.0 ) = sqrt( (a, a) ), ExpressionTemplates.h:458-478. */
+int tnlhip_norm2( int dtype, int64_t n, const void* a, double* result, void* stream );
+
+/* y = alpha * x + beta * y (VectorAssignment over ParallelFor, Containers/detail/VectorAssignment.h). */
+int tnlhip_axpby( int dtype, int64_t n, double alpha, const void* x, double beta, void* y, void* stream );
+
+/* result[k] = sum_i V[i + k*ld] * w[i], k < count: Algorithms::Multireduction::reduce with the
+ * GMRES fetch (Algorithms/Multireduction.hpp:30-233, Solvers/Linear/GMRES.hpp:218-230). */
+int tnlhip_multidot( int dtype, int64_t n, int32_t count, const void* V, int64_t ld, const void* w, double* result,
+                     void* stream );
+
+/* y = alpha * V x + beta * y with V column-major n x count, x on the host:
+ * Matrices::MatrixOperations::gemv, Matrices/MatrixOperations.h:46-150, 232-345. */
+int tnlhip_gemv( int dtype, int64_t n, int32_t count, double alpha, const void* V, int64_t ld, const double* x,
+                 double beta, void* y, void* stream );
+
+/* In-place exclusive prefix sum of int32 (Algorithms::inplaceExclusiveScan, Algorithms/scan.h);
+ * out[n] receives the total when out has n+1 elements (total_out != 0). */
+int tnlhip_exclusive_scan_i32( int64_t n, const int32_t* in, int32_t* out, int total_out, void* stream );
+
+/* ------------------------------------------------------------------ Krylov solvers */
+
+typedef enum
+{
+   TNLHIP_CG = 0,       /* Solvers/Linear/CG.hpp */
+   TNLHIP_BICGSTAB = 1, /* Solvers/Linear/BICGStab.hpp */
+   TNLHIP_GMRES = 2     /* Solvers/Linear/GMRES.hpp */
+} tnlhip_method;
+
+typedef enum
+{
+   TNLHIP_GMRES_CGS = 0,
+   TNLHIP_GMRES_CGSR = 1,
+   TNLHIP_GMRES_MGS = 2,
+   TNLHIP_GMRES_MGSR = 3,
+   TNLHIP_GMRES_CWY = 4
+} tnlhip_gmres_variant;
+
+/* IterativeSolver / solver setup parameters; defaults as IterativeSolver.h:226-245 and
+ * GMRES.h:220-237 (CWY, restarting 10/10, steps 3/3). */
+typedef struct
+{
+   int32_t max_iterations;
+   int32_t min_iterations;
+   double convergence_residue;
+   double divergence_residue;
+   int32_t gmres_variant;
+   int32_t restarting_min;
+   int32_t restarting_max;
+   int32_t restarting_step_min;
+   int32_t restarting_step_max;
+   int32_t bicgstab_exact_residue;
+   int32_t preconditioner; /* 0 none, 1 Preconditioners::Diagonal (Jacobi) */
+} tnlhip_solver_params;
+
+void tnlhip_solver_params_default( tnlhip_solver_params* params );
+
+typedef struct tnlhip_solver_s* tnlhip_solver;
+
+int tnlhip_solver_create( int method, tnlhip_solver* out );
+int tnlhip_solver_destroy( tnlhip_solver solver );
+
+/* LinearSolver::setMatrix( shared_ptr<const Matrix> ), Solvers/Linear/LinearSolver.h:115-131.
+ * The matrix must outlive the solver's use of it. */
+int tnlhip_solver_set_matrix( tnlhip_solver solver, tnlhip_matrix A );
+
+/* setup( parameters ) / setMaxIterations, setConvergenceResidue, ...; when
+ * params->preconditioner == 1 this is setPreconditioner( Diagonal ) followed by its
+ * update( matrix ) (Preconditioners/Diagonal.hpp:22-41). */
+int tnlhip_solver_set_params( tnlhip_solver solver, const tnlhip_solver_params* params );
+
+/* bool LinearSolver::solve( b, x ), x holds the initial guess on entry; *converged is
+ * checkConvergence() (IterativeSolver.hpp:127-150).  Synchronous. */
+int tnlhip_solver_solve( tnlhip_solver solver, const void* b, void* x, int32_t* converged, void* stream );
+
+/* getIterations(), getResidue() (IterativeSolver.h:70-200). */
+int tnlhip_solver_get_status( tnlhip_solver solver, int32_t* iterations, double* residue );
+
+/* Every value passed to setResidue during the last solve, in order (the residual history
+ * file of IterativeSolver.hpp:189-200); returns the count, out may be NULL. */
+int64_t tnlhip_solver_get_history( tnlhip_solver solver, double* out, int64_t capacity );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TNLHIP_H */

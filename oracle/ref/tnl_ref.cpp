@@ -277,7 +277,9 @@ int ref_convert_spmv( int kind, int colmajor, int n, int ncols, const int* rp, c
 {
    switch( kind * 2 + ( colmajor ? 1 : 0 ) ) {
       case 0:
-      case 1: return convertSpmv< Seg::CSRScalar >( n, ncols, rp, cols, vals, x, y, out_vals, out_cols, storage_cap );
+      // CSRVector: a CSR type different from the source's, so the converting operator= runs
+      // (same-type assignment is a plain copy that keeps explicit zeros)
+      case 1: return convertSpmv< Seg::CSRVector >( n, ncols, rp, cols, vals, x, y, out_vals, out_cols, storage_cap );
       case 2: return convertSpmv< EllRow >( n, ncols, rp, cols, vals, x, y, out_vals, out_cols, storage_cap );
       case 3: return convertSpmv< EllC >( n, ncols, rp, cols, vals, x, y, out_vals, out_cols, storage_cap );
       case 4: return convertSpmv< SlR >( n, ncols, rp, cols, vals, x, y, out_vals, out_cols, storage_cap );

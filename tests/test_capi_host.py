@@ -1,0 +1,79 @@
+"""The C-ABI library on the host: it loads, exports every function include/tnlhip.h declares,
+and its host-side segments metadata is TNL's, bit for bit (no GPU needed)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FORMAT_OF_KIND = {"CSR": "CSRScalar", "Ellpack": "Ellpack", "SlicedEllpack": "SlicedEllpack",
+                  "ChunkedEllpack": "ChunkedEllpack", "BiEllpack": "BiEllpack"}
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "tnlhip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(tnlhip_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function(tnl):
+    names = declared_functions()
+    assert len(names) >= 25
+    for name in names:
+        assert hasattr(tnl.lib, name), name
+    assert sorted(tnl.EXPORTS) == names
+
+
+def test_status_strings(tnl):
+    assert tnl.lib.tnlhip_status_string(0) == b"success"
+    assert tnl.lib.tnlhip_status_string(6) == b"solver breakdown"
+
+
+@pytest.mark.parametrize("kind", list(FORMAT_OF_KIND))
+def test_host_metadata_matches_tnl_fixtures(tnl, golden, kind):
+    d = golden["segments"]
+    cases = sorted({k.split("/")[0] for k in d.files})
+    for case in cases:
+        caps = np.diff(d[f"{case}/row_ptr"]).astype(np.int32)
+        for name in O.METADATA_NAMES[O.KINDS[kind]]:
+            np.testing.assert_array_equal(tnl.segments_metadata(FORMAT_OF_KIND[kind], caps, name),
+                                          d[f"{case}/{kind}/col/{name}"], err_msg=f"{case} {name}")
+        assert tnl.segments_metadata(FORMAT_OF_KIND[kind], caps, "storage") == int(d[f"{case}/{kind}/col/storage"][0])
+
+
+@pytest.mark.parametrize("kind", list(FORMAT_OF_KIND))
+def test_host_metadata_at_full_size(tnl, kind):
+    """160^3 Laplacian (the SpMV benchmark matrix) and a 200k irregular matrix: bit-exact
+    against the oracle."""
+    for rp in (W.laplacian_3d(160)[0], W.irregular(200000, seed=3)[0]):
+        caps = np.diff(rp).astype(np.int32)
+        s = O.Segments(kind, caps)
+        for name in O.METADATA_NAMES[O.KINDS[kind]]:
+            np.testing.assert_array_equal(tnl.segments_metadata(FORMAT_OF_KIND[kind], caps, name), s.get(name))
+        assert tnl.segments_metadata(FORMAT_OF_KIND[kind], caps, "storage") == s.storage_size()
+
+
+def test_edge_capacities(tnl):
+    """Empty matrix, all-empty rows, a single row, one-row last strip, rows longer than a slice."""
+    for caps in ([], [0], [0] * 40, [5], [3] * 33, [0, 7, 0, 4096, 1, 0], [300] * 3 + [1] * 70):
+        caps = np.asarray(caps, np.int32)
+        for kind, fmt in FORMAT_OF_KIND.items():
+            s = O.Segments(kind, caps)
+            for name in O.METADATA_NAMES[O.KINDS[kind]]:
+                np.testing.assert_array_equal(tnl.segments_metadata(fmt, caps, name), s.get(name),
+                                              err_msg=f"{kind} {name} {caps[:5]}")
+
+
+def test_adaptive_blocks_follow_the_tnl_split(tnl):
+    """With wave64 parameters the product's CSR-Adaptive split is the same greedy algorithm
+    (CSRAdaptiveKernel::initValueSize) as the oracle's with the same parameters."""
+    rp, _, _ = W.irregular(20000, seed=4)
+    caps = np.diff(rp).astype(np.int32)
+    got = tnl.segments_metadata("CSRAdaptive", caps, "adaptiveBlocks").reshape(-1, 6)
+    want = O.csr_adaptive_blocks(rp, 8, warp_size=64, block_size=256, streamed_shared_bytes=4 * 8192,
+                                 max_adaptive_elements=1024)
+    np.testing.assert_array_equal(got, want)

@@ -1,0 +1,213 @@
+"""The oracle pinned: against TNL's own results (golden fixtures written from the reference
+build, and the reference build itself when present), TNL's unit-test golden values, closed
+forms and brute force."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+import tnl_cases
+import workloads as W
+
+KINDS = list(O.KINDS)
+
+
+def _cases(golden):
+    d = golden["segments"]
+    return sorted({k.split("/")[0] for k in d.files})
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("org", ["col", "row"])
+def test_metadata_matches_tnl_fixtures(golden, kind, org):
+    d = golden["segments"]
+    for case in _cases(golden):
+        caps = np.diff(d[f"{case}/row_ptr"]).astype(np.int32)
+        s = O.Segments(kind, caps, column_major=(org == "col"))
+        for name in O.METADATA_NAMES[O.KINDS[kind]]:
+            np.testing.assert_array_equal(s.get(name), d[f"{case}/{kind}/{org}/{name}"], err_msg=f"{case} {name}")
+        assert s.storage_size() == int(d[f"{case}/{kind}/{org}/storage"][0])
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_conversion_and_spmv_match_tnl_fixtures(golden, kind):
+    d = golden["segments"]
+    for case in _cases(golden):
+        rp, c, v, x = (d[f"{case}/{k}"] for k in ("row_ptr", "cols", "vals", "x"))
+        s = O.Segments(kind, np.diff(rp))
+        vals, cols = s.convert(rp, c, v)
+        key = f"{case}/{kind}/col/values"
+        if key in d.files:
+            used = len(d[key])
+            np.testing.assert_array_equal(vals[:used], d[key])
+            np.testing.assert_array_equal(cols[:used], d[f"{case}/{kind}/col/columns"])
+            assert np.all(cols[used:] == -1)
+        y = s.spmv(vals, cols, x)
+        np.testing.assert_array_equal(y, d[f"{case}/{kind}/col/y"], err_msg=case)
+
+
+def test_csr_adaptive_blocks_match_tnl_fixtures(golden):
+    d = golden["adaptive"]
+    s = golden["segments"]
+    for key in d.files:
+        case, vs = key.split("/")
+        rp = s[f"{case}/row_ptr"]
+        np.testing.assert_array_equal(O.csr_adaptive_blocks(rp, int(vs)), d[key], err_msg=key)
+
+
+def _solver_runs(golden):
+    d = golden["solvers"]
+    runs = sorted({k.split("/")[0] for k in d.files if k.startswith("run")})
+    for r in runs:
+        sysname = str(d[f"{r}/system"])
+        params = dict(eval(str(d[f"{r}/params"])))  # list of (key, value) pairs written by make_fixtures.py
+        yield r, sysname, str(d[f"{r}/method"]), params
+
+
+def test_solvers_match_tnl_histories(golden):
+    """Single-thread TNL Host solves: same iterations, same residual history, same x (bit for
+    bit under the no-contraction build of both)."""
+    d = golden["solvers"]
+    for run, sysname, method, params in _solver_runs(golden):
+        rp, c, v, b = (d[f"{sysname}/{k}"] for k in ("row_ptr", "cols", "vals", "b"))
+        r = O.solve(method, rp, c, v, b, **params)
+        assert r["iterations"] == int(d[f"{run}/iterations"][0]), run
+        assert r["converged"] == int(d[f"{run}/converged"][0]), run
+        np.testing.assert_array_equal(r["history"], d[f"{run}/history"], err_msg=run)
+        np.testing.assert_array_equal(r["x"], d[f"{run}/x"], err_msg=run)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_tnl_unit_test_golden_values(kind):
+    for name, case in tnl_cases.all_cases().items():
+        rp, cols, vals = tnl_cases.to_csr(case)
+        s = O.Segments(kind, np.diff(rp))
+        sv, sc = s.convert(rp, cols, vals)
+        x = np.full(case["m"], case["x"])
+        y = s.spmv(sv, sc, x)
+        np.testing.assert_array_equal(y, np.asarray(case["y"], np.float64), err_msg=f"{kind} {name}")
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_spmv_brute_force_and_keepers(kind):
+    rng = np.random.default_rng(5)
+    for n, m, dens in [(1, 1, 1.0), (7, 5, 0.5), (33, 40, 0.2), (100, 90, 0.05), (65, 65, 0.0)]:
+        A = rng.standard_normal((n, m)) * (rng.random((n, m)) < dens)
+        rp, c, v = W.dense_to_csr(A)
+        s = O.Segments(kind, np.diff(rp))
+        sv, sc = s.convert(rp, c, v)
+        x = rng.standard_normal(m)
+        y0 = rng.standard_normal(n)
+        np.testing.assert_allclose(s.spmv(sv, sc, x), A @ x, rtol=1e-12, atol=1e-12)
+        for alpha, beta in [(1.0, 0.0), (2.5, 0.0), (1.0, -0.5), (-1.0, 1.0), (0.75, 3.0)]:
+            first, last = (1, n - 1) if n > 2 else (0, n)
+            got = s.spmv(sv, sc, x, y0, alpha, beta, first, last)
+            want = y0.copy()
+            if not (kind == "BiEllpack" and s.storage_size() == 0):
+                # TNL's Host BiEllpack returns before touching y when nothing is stored
+                # (BiEllpackView.hpp:248-249); the oracle keeps that behaviour
+                want[first:last] = alpha * (A @ x)[first:last] + beta * y0[first:last]
+            np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_laplacian_row_sums_closed_form():
+    """A * 1 of the Dirichlet Laplacians = number of missing neighbours per row."""
+    for N, gen, dims in [(13, W.laplacian_2d, 2), (9, W.laplacian_3d, 3)]:
+        rp, c, v = gen(N)
+        n = len(rp) - 1
+        s = O.Segments("CSR", np.diff(rp))
+        y = s.spmv(v, c, np.ones(n))
+        idx = np.arange(n)
+        want = np.zeros(n)
+        for d in range(dims):
+            coord = (idx // N ** d) % N
+            want += (coord == 0).astype(float) + (coord == N - 1).astype(float)
+        np.testing.assert_array_equal(y, want)
+
+
+def test_reductions_closed_forms():
+    # reduceTest.h:91-128: sum of ones = size; sum of iota = n(n-1)/2
+    for size in [1, 2, 127, 128, 255, 256, 257, 1000, 9377, 100000]:
+        ones = np.ones(size)
+        assert O.dot(ones, ones) == size
+        iota = np.arange(size, dtype=np.float64)
+        assert O.dot(iota, ones) == size * (size - 1) / 2
+
+
+def test_cg_converges_in_distinct_eigenvalue_count():
+    """CG on an SPD diagonal matrix with k distinct eigenvalues reaches the solution in k steps."""
+    n, k = 300, 4
+    diag = np.repeat(np.array([1.0, 2.0, 5.0, 9.0]), n // k)
+    rp = np.arange(n + 1, dtype=np.int32)
+    c = np.arange(n, dtype=np.int32)
+    b = W.uniform_vector(n, 1)
+    r = O.solve("cg", rp, c, diag, b, convergence_residue=1e-12, max_iterations=50)
+    assert r["converged"] == 1
+    assert r["iterations"] <= k + 1
+    np.testing.assert_allclose(r["x"], b / diag, rtol=1e-10)
+
+
+def test_jacobi_closed_form():
+    """With the Diagonal preconditioner on a diagonal matrix, CG and GMRES solve in one step.
+    BiCGStab breaks down there: s = 0 after the half step gives omega = 0/0 (TNL has no
+    half-step convergence test, BICGStab.hpp:72-76), so its residue is NaN and solve() fails."""
+    n = 200
+    d = np.linspace(1.0, 3.0, n)
+    rp = np.arange(n + 1, dtype=np.int32)
+    c = np.arange(n, dtype=np.int32)
+    b = W.uniform_vector(n, 2)
+    for method in ("cg", "gmres"):
+        r = O.solve(method, rp, c, d, b, convergence_residue=1e-12, max_iterations=20, preconditioner=1)
+        assert r["converged"] == 1
+        np.testing.assert_allclose(r["x"], b / d, rtol=1e-12, atol=1e-14)
+    r = O.solve("bicgstab", rp, c, d, b, convergence_residue=1e-12, max_iterations=20, preconditioner=1)
+    assert r["converged"] == 0 and np.isnan(r["residue"])
+
+
+def test_gmres_residual_non_increasing_within_cycles():
+    rp, c, v = W.convection_diffusion_3d(6, 10.0)
+    b = W.uniform_vector(len(rp) - 1, 3)
+    for variant in ("CGS", "MGS", "CWY"):
+        r = O.solve("gmres", rp, c, v, b, convergence_residue=1e-10, max_iterations=500, gmres_variant=variant,
+                    restarting_min=30, restarting_max=30)
+        h = r["history"]
+        assert r["converged"] == 1
+        assert np.all(np.diff(h[1:31]) <= 1e-12 * h[0])
+
+
+def test_true_residual_of_solutions():
+    rp, c, v = W.laplacian_2d(20)
+    n = len(rp) - 1
+    A = W.csr_to_dense(rp, c, v, n)
+    b = W.uniform_vector(n, 4)
+    for method in ("cg", "bicgstab", "gmres"):
+        r = O.solve(method, rp, c, v, b, convergence_residue=1e-10, max_iterations=5000)
+        assert np.linalg.norm(b - A @ r["x"]) / np.linalg.norm(b) < 1e-9
+
+
+def test_planted_mistake_is_caught():
+    """A transposed operand in the product must fail the brute-force check used above."""
+    rng = np.random.default_rng(9)
+    A = rng.standard_normal((20, 20)) * (rng.random((20, 20)) < 0.3)
+    rp, c, v = W.dense_to_csr(A.T)
+    s = O.Segments("CSR", np.diff(rp))
+    x = rng.standard_normal(20)
+    assert not np.allclose(s.spmv(v, c, x), A @ x)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="TNL reference build not present")
+@pytest.mark.parametrize("kind", KINDS)
+def test_oracle_matches_live_reference_build(kind):
+    rp, c, v = W.irregular(5000, mean=12, sigma=1.4, max_len=2000, seed=21)
+    caps = np.diff(rp)
+    for cm in (True, False):
+        a = O.Segments(kind, caps, cm)
+        b = O.Segments(kind, caps, cm, backend="ref")
+        for name in O.METADATA_NAMES[O.KINDS[kind]]:
+            np.testing.assert_array_equal(a.get(name), b.get(name))
+    x = W.uniform_vector(len(caps), 8)
+    if kind != "Ellpack":
+        vals, cols, y = O.ref_convert_spmv(kind, rp, c, v, len(caps), x)
+        a = O.Segments(kind, caps)
+        ov, oc = a.convert(rp, c, v)
+        np.testing.assert_array_equal(ov, vals[:len(ov)])
+        np.testing.assert_array_equal(a.spmv(ov, oc, x), y)

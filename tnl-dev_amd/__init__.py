@@ -1,0 +1,430 @@
+"""Python mirror of TNL's SpMV + Krylov interface over the tnlhip C ABI (libtnlhip.so).
+
+Names follow TNL (src/TNL/Matrices/SparseMatrix.h, src/TNL/Solvers/Linear/*.h):
+SparseMatrix.vectorProduct, LinearSolver.setMatrix / setPreconditioner / solve, and the
+IterativeSolver setters.  Vectors are torch tensors on the GPU (float64 for FP64 matrices,
+float32 for FP32 / BF16 matrices); torch provides device memory and streams only.
+
+The HIP library is required: importing this package raises if libtnlhip.so is missing,
+and there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtnlhip.so")
+
+FORMATS = {
+    "CSRScalar": 0, "CSRVector": 1, "CSRLight": 2, "CSRHybrid": 3, "CSRAdaptive": 4,
+    "Ellpack": 5, "SlicedEllpack": 6, "ChunkedEllpack": 7, "BiEllpack": 8,
+}
+FORMAT_NAMES = {v: k for k, v in FORMATS.items()}
+DTYPES = {"f64": 0, "f32": 1, "bf16": 2}
+METHODS = {"cg": 0, "bicgstab": 1, "gmres": 2}
+GMRES_VARIANTS = {"CGS": 0, "CGSR": 1, "MGS": 2, "MGSR": 3, "CWY": 4}
+
+# every function declared in include/tnlhip.h
+EXPORTS = [
+    "tnlhip_status_string", "tnlhip_last_error", "tnlhip_device_info",
+    "tnlhip_matrix_create_csr", "tnlhip_matrix_convert", "tnlhip_matrix_destroy", "tnlhip_matrix_get_info",
+    "tnlhip_matrix_get_metadata", "tnlhip_segments_metadata", "tnlhip_matrix_get_storage", "tnlhip_matrix_get_diagonal",
+    "tnlhip_matrix_set_param", "tnlhip_spmv", "tnlhip_dot", "tnlhip_norm2", "tnlhip_axpby", "tnlhip_multidot",
+    "tnlhip_gemv", "tnlhip_exclusive_scan_i32", "tnlhip_solver_params_default", "tnlhip_solver_create",
+    "tnlhip_solver_destroy", "tnlhip_solver_set_matrix", "tnlhip_solver_set_params", "tnlhip_solver_solve",
+    "tnlhip_solver_get_status", "tnlhip_solver_get_history",
+]
+
+
+class TnlHipError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"tnlhip status {status}: {msg}")
+        self.status = status
+
+
+class CapacityError(TnlHipError):
+    """std::logic_error in TNL: a row capacity was exceeded."""
+
+
+class BreakdownError(TnlHipError):
+    """GMRES hit a zero Hessenberg diagonal (TNL throws 1)."""
+
+
+class MatrixInfo(C.Structure):
+    _fields_ = [("rows", C.c_int32), ("columns", C.c_int32), ("format", C.c_int32), ("value_dtype", C.c_int32),
+                ("storage_size", C.c_int64), ("allocated", C.c_int64), ("nonzeros", C.c_int64),
+                ("lanes_per_row", C.c_int32), ("reserved", C.c_int32)]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("max_iterations", C.c_int32), ("min_iterations", C.c_int32), ("convergence_residue", C.c_double),
+                ("divergence_residue", C.c_double), ("gmres_variant", C.c_int32), ("restarting_min", C.c_int32),
+                ("restarting_max", C.c_int32), ("restarting_step_min", C.c_int32),
+                ("restarting_step_max", C.c_int32), ("bicgstab_exact_residue", C.c_int32),
+                ("preconditioner", C.c_int32)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libtnlhip.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    sig = {
+        "tnlhip_status_string": (C.c_char_p, [C.c_int]),
+        "tnlhip_last_error": (C.c_char_p, []),
+        "tnlhip_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(i64)]),
+        "tnlhip_matrix_create_csr": (C.c_int, [C.c_int, C.c_int, i32, i32, vp, vp, vp, C.c_int, C.POINTER(vp)]),
+        "tnlhip_matrix_convert": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(vp)]),
+        "tnlhip_matrix_destroy": (C.c_int, [vp]),
+        "tnlhip_matrix_get_info": (C.c_int, [vp, C.POINTER(MatrixInfo)]),
+        "tnlhip_matrix_get_metadata": (i64, [vp, C.c_char_p, vp]),
+        "tnlhip_segments_metadata": (i64, [C.c_int, i32, vp, C.c_char_p, vp]),
+        "tnlhip_matrix_get_storage": (C.c_int, [vp, vp, vp]),
+        "tnlhip_matrix_get_diagonal": (C.c_int, [vp, vp, vp]),
+        "tnlhip_matrix_set_param": (C.c_int, [vp, C.c_char_p, i32]),
+        "tnlhip_spmv": (C.c_int, [vp, vp, vp, f64, f64, i32, i32, vp]),
+        "tnlhip_dot": (C.c_int, [C.c_int, i64, vp, vp, C.POINTER(f64), vp]),
+        "tnlhip_norm2": (C.c_int, [C.c_int, i64, vp, C.POINTER(f64), vp]),
+        "tnlhip_axpby": (C.c_int, [C.c_int, i64, f64, vp, f64, vp, vp]),
+        "tnlhip_multidot": (C.c_int, [C.c_int, i64, i32, vp, i64, vp, vp, vp]),
+        "tnlhip_gemv": (C.c_int, [C.c_int, i64, i32, f64, vp, i64, vp, f64, vp, vp]),
+        "tnlhip_exclusive_scan_i32": (C.c_int, [i64, vp, vp, C.c_int, vp]),
+        "tnlhip_solver_params_default": (None, [C.POINTER(SolverParams)]),
+        "tnlhip_solver_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "tnlhip_solver_destroy": (C.c_int, [vp]),
+        "tnlhip_solver_set_matrix": (C.c_int, [vp, vp]),
+        "tnlhip_solver_set_params": (C.c_int, [vp, C.POINTER(SolverParams)]),
+        "tnlhip_solver_solve": (C.c_int, [vp, vp, vp, C.POINTER(i32), vp]),
+        "tnlhip_solver_get_status": (C.c_int, [vp, C.POINTER(i32), C.POINTER(f64)]),
+        "tnlhip_solver_get_history": (i64, [vp, vp, i64]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+lib = _load()
+
+
+def _check(status):
+    if status != 0:
+        msg = lib.tnlhip_last_error().decode(errors="replace")
+        if status == 4:
+            raise CapacityError(status, msg)
+        if status == 6:
+            raise BreakdownError(status, msg)
+        raise TnlHipError(status, msg)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return C.c_void_p(stream if isinstance(stream, int) else stream.cuda_stream)
+
+
+def device_info():
+    name = C.create_string_buffer(128)
+    cus = C.c_int(0)
+    mem = C.c_int64(0)
+    _check(lib.tnlhip_device_info(name, 128, C.byref(cus), C.byref(mem)))
+    return {"name": name.value.decode(), "compute_units": cus.value, "memory": mem.value}
+
+
+def _dtype_id(dtype):
+    if isinstance(dtype, int):
+        return dtype
+    return DTYPES[str(dtype).replace("torch.", "").replace("float64", "f64").replace("float32", "f32")
+                  .replace("bfloat16", "bf16")]
+
+
+class SparseMatrix:
+    """SparseMatrix< Real, Devices::Hip, int, GeneralMatrix, Segments > (Matrices/SparseMatrix.h:40-49)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if isinstance(handle, int) else handle
+        info = MatrixInfo()
+        _check(lib.tnlhip_matrix_get_info(self._h, C.byref(info)))
+        self.info = info
+
+    @classmethod
+    def from_csr(cls, row_pointers, column_indexes, values, columns=None, format="CSRScalar", dtype="f64"):
+        """A = csr (SparseMatrix::operator=, Matrices/SparseMatrix.hpp:1070-1180) from host numpy
+        arrays or device torch tensors (int32 row pointers / columns, float64 values)."""
+        import numpy as np
+        fmt = FORMATS[format] if isinstance(format, str) else format
+        dt = _dtype_id(dtype)
+        h = C.c_void_p(0)
+        if isinstance(row_pointers, np.ndarray):
+            rp = np.ascontiguousarray(row_pointers, np.int32)
+            ci = np.ascontiguousarray(column_indexes, np.int32)
+            va = np.ascontiguousarray(values, np.float64)
+            rows = len(rp) - 1
+            cols = rows if columns is None else columns
+            _check(lib.tnlhip_matrix_create_csr(fmt, dt, rows, cols, rp.ctypes.data_as(C.c_void_p),
+                                                ci.ctypes.data_as(C.c_void_p), va.ctypes.data_as(C.c_void_p),
+                                                1, C.byref(h)))
+        else:
+            rows = row_pointers.numel() - 1
+            cols = rows if columns is None else columns
+            _check(lib.tnlhip_matrix_create_csr(fmt, dt, rows, cols, _ptr(row_pointers), _ptr(column_indexes),
+                                                _ptr(values), 0, C.byref(h)))
+        return cls(h)
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib.tnlhip_matrix_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    # --- TNL-style accessors
+    def getRows(self):
+        return self.info.rows
+
+    def getColumns(self):
+        return self.info.columns
+
+    def getNonzeroElementsCount(self):
+        return self.info.nonzeros
+
+    def getStorageSize(self):
+        return self.info.storage_size
+
+    @property
+    def format(self):
+        return FORMAT_NAMES[self.info.format]
+
+    @property
+    def value_dtype(self):
+        return {0: "f64", 1: "f32", 2: "bf16"}[self.info.value_dtype]
+
+    @property
+    def vector_dtype(self):
+        import torch
+        return torch.float64 if self.info.value_dtype == 0 else torch.float32
+
+    def convert(self, format, dtype=None):
+        """B = A into another format / value type, on the device."""
+        fmt = FORMATS[format] if isinstance(format, str) else format
+        dt = self.info.value_dtype if dtype is None else _dtype_id(dtype)
+        h = C.c_void_p(0)
+        _check(lib.tnlhip_matrix_convert(self._h, fmt, dt, C.byref(h)))
+        return SparseMatrix(h)
+
+    def metadata(self, name):
+        import numpy as np
+        n = lib.tnlhip_matrix_get_metadata(self._h, name.encode(), None)
+        if n < 0:
+            raise KeyError(name)
+        out = np.zeros(max(n, 1), np.int32)
+        lib.tnlhip_matrix_get_metadata(self._h, name.encode(), out.ctypes.data_as(C.c_void_p))
+        return out[:n]
+
+    def storage(self):
+        """(values as float64, column indexes) of the allocated storage, host copies."""
+        import numpy as np
+        n = self.info.allocated
+        v = np.zeros(max(n, 1), np.float64)
+        c = np.zeros(max(n, 1), np.int32)
+        _check(lib.tnlhip_matrix_get_storage(self._h, v.ctypes.data_as(C.c_void_p), c.ctypes.data_as(C.c_void_p)))
+        return v[:n], c[:n]
+
+    def diagonal(self, out, stream=None):
+        _check(lib.tnlhip_matrix_get_diagonal(self._h, _ptr(out), _stream(stream)))
+        return out
+
+    def set_param(self, name, value):
+        _check(lib.tnlhip_matrix_set_param(self._h, name.encode(), int(value)))
+        info = MatrixInfo()
+        _check(lib.tnlhip_matrix_get_info(self._h, C.byref(info)))
+        self.info = info
+
+    def vectorProduct(self, inVector, outVector, matrixMultiplicator=1.0, outVectorMultiplicator=0.0,
+                      firstRow=0, lastRow=0, stream=None):
+        """outVector = matrixMultiplicator * A inVector + outVectorMultiplicator * outVector over rows
+        [firstRow, lastRow) (lastRow 0 = all), SparseMatrix.h:1078-1085.  Asynchronous on the stream."""
+        dt = self.vector_dtype
+        if inVector.dtype != dt or outVector.dtype != dt:
+            raise TypeError(f"vectors must be {dt}")
+        if inVector.numel() != self.info.columns or outVector.numel() != self.info.rows:
+            raise ValueError("vector sizes do not match the matrix")
+        _check(lib.tnlhip_spmv(self._h, _ptr(inVector), _ptr(outVector), float(matrixMultiplicator),
+                               float(outVectorMultiplicator), int(firstRow), int(lastRow), _stream(stream)))
+        return outVector
+
+
+def segments_metadata(format, capacities, name):
+    """Host-only segments metadata (no GPU needed): setSegmentsSizes on the host."""
+    import numpy as np
+    fmt = FORMATS[format] if isinstance(format, str) else format
+    caps = np.ascontiguousarray(capacities, np.int32)
+    n = lib.tnlhip_segments_metadata(fmt, len(caps), caps.ctypes.data_as(C.c_void_p), name.encode(), None)
+    if n < 0:
+        raise KeyError(name)
+    out = np.zeros(max(n, 1), np.int32)
+    lib.tnlhip_segments_metadata(fmt, len(caps), caps.ctypes.data_as(C.c_void_p), name.encode(),
+                                 out.ctypes.data_as(C.c_void_p))
+    out = out[:n]
+    if name == "storage":
+        return int(out[0].astype(np.uint32)) + (int(out[1]) << 32)
+    return out
+
+
+# ------------------------------------------------------------------ BLAS-1
+
+
+def _vdt(t):
+    import torch
+    if t.dtype == torch.float64:
+        return 0
+    if t.dtype == torch.float32:
+        return 1
+    raise TypeError("vectors are float64 or float32")
+
+
+def dot(a, b, stream=None):
+    """(a, b) — deterministic device reduction."""
+    r = C.c_double(0)
+    _check(lib.tnlhip_dot(_vdt(a), a.numel(), _ptr(a), _ptr(b), C.byref(r), _stream(stream)))
+    return r.value
+
+
+def lpNorm2(a, stream=None):
+    r = C.c_double(0)
+    _check(lib.tnlhip_norm2(_vdt(a), a.numel(), _ptr(a), C.byref(r), _stream(stream)))
+    return r.value
+
+
+def axpby(alpha, x, beta, y, stream=None):
+    _check(lib.tnlhip_axpby(_vdt(x), x.numel(), float(alpha), _ptr(x), float(beta), _ptr(y), _stream(stream)))
+    return y
+
+
+def multidot(V, ld, count, w, stream=None):
+    import numpy as np
+    out = np.zeros(count, np.float64)
+    _check(lib.tnlhip_multidot(_vdt(w), w.numel(), count, _ptr(V), ld, _ptr(w), out.ctypes.data_as(C.c_void_p),
+                               _stream(stream)))
+    return out
+
+
+def gemv(alpha, V, ld, count, x, beta, y, stream=None):
+    import numpy as np
+    xx = np.ascontiguousarray(x, np.float64)
+    _check(lib.tnlhip_gemv(_vdt(y), y.numel(), count, float(alpha), _ptr(V), ld, xx.ctypes.data_as(C.c_void_p),
+                           float(beta), _ptr(y), _stream(stream)))
+    return y
+
+
+def exclusive_scan(inp, out, total=True, stream=None):
+    n = inp.numel()
+    _check(lib.tnlhip_exclusive_scan_i32(n, _ptr(inp), _ptr(out), 1 if total else 0, _stream(stream)))
+    return out
+
+
+# ------------------------------------------------------------------ solvers
+
+
+class Diagonal:
+    """Preconditioners::Diagonal (Jacobi), Solvers/Linear/Preconditioners/Diagonal.h."""
+
+
+class LinearSolver:
+    """LinearSolver< Matrix > (Solvers/Linear/LinearSolver.h:115-175) with IterativeSolver's setters."""
+    method = None
+
+    def __init__(self):
+        h = C.c_void_p(0)
+        _check(lib.tnlhip_solver_create(METHODS[self.method], C.byref(h)))
+        self._h = h
+        self.params = SolverParams()
+        lib.tnlhip_solver_params_default(C.byref(self.params))
+        self._matrix = None
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib.tnlhip_solver_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def setMatrix(self, matrix):
+        self._matrix = matrix
+        _check(lib.tnlhip_solver_set_matrix(self._h, matrix._h))
+
+    def setPreconditioner(self, preconditioner):
+        self.params.preconditioner = 1 if preconditioner is not None else 0
+
+    def setMaxIterations(self, n):
+        self.params.max_iterations = int(n)
+
+    def setMinIterations(self, n):
+        self.params.min_iterations = int(n)
+
+    def setConvergenceResidue(self, r):
+        self.params.convergence_residue = float(r)
+
+    def setDivergenceResidue(self, r):
+        self.params.divergence_residue = float(r)
+
+    def setup(self, **parameters):
+        """Config-style parameters: max_iterations, convergence_residue, gmres_variant, ..."""
+        for k, v in parameters.items():
+            k = k.replace("-", "_")
+            if k == "gmres_variant" and isinstance(v, str):
+                v = GMRES_VARIANTS[v]
+            if k == "preconditioner" and isinstance(v, str):
+                v = {"none": 0, "diagonal": 1}[v]
+            setattr(self.params, k, v)
+        return True
+
+    def solve(self, b, x, stream=None):
+        """bool solve( b, x ); x holds the initial guess.  Synchronous."""
+        _check(lib.tnlhip_solver_set_params(self._h, C.byref(self.params)))
+        conv = C.c_int32(0)
+        _check(lib.tnlhip_solver_solve(self._h, _ptr(b), _ptr(x), C.byref(conv), _stream(stream)))
+        return bool(conv.value)
+
+    def getIterations(self):
+        it = C.c_int32(0)
+        r = C.c_double(0)
+        _check(lib.tnlhip_solver_get_status(self._h, C.byref(it), C.byref(r)))
+        return it.value
+
+    def getResidue(self):
+        it = C.c_int32(0)
+        r = C.c_double(0)
+        _check(lib.tnlhip_solver_get_status(self._h, C.byref(it), C.byref(r)))
+        return r.value
+
+    def getResidualHistory(self):
+        import numpy as np
+        n = lib.tnlhip_solver_get_history(self._h, None, 0)
+        out = np.zeros(max(n, 1), np.float64)
+        lib.tnlhip_solver_get_history(self._h, out.ctypes.data_as(C.c_void_p), n)
+        return out[:n]
+
+
+class CG(LinearSolver):
+    method = "cg"
+
+
+class BICGStab(LinearSolver):
+    method = "bicgstab"
+
+
+class GMRES(LinearSolver):
+    method = "gmres"
+
+
+def getLinearSolver(name):
+    """LinearSolverTypeResolver::getLinearSolver by name (Solvers/LinearSolverTypeResolver.h:36-150)."""
+    return {"cg": CG, "bicgstab": BICGStab, "gmres": GMRES}[name]()

@@ -1,0 +1,555 @@
+// extern "C" entry points of include/tnlhip.h: argument checks, matrix construction and
+// conversion, and the stand-alone SpMV / BLAS-1 calls.  Solver entry points live in
+// solvers.hip.
+#include "blas.hpp"
+#include "matrix.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+// Opaque handle type of the C ABI.
+struct tnlhip_matrix_s : tnlhip::Matrix
+{};
+
+namespace tnlhip {
+
+namespace {
+thread_local std::string g_lastError;
+}
+
+void setLastError( const std::string& msg )
+{
+   g_lastError = msg;
+}
+
+const DeviceProps& deviceProps()
+{
+   static DeviceProps props;
+   static std::once_flag once;
+   std::call_once( once, [] {
+      int dev = 0;
+      if( hipGetDevice( &dev ) == hipSuccess ) {
+         hipDeviceProp_t p;
+         if( hipGetDeviceProperties( &p, dev ) == hipSuccess ) {
+            props.cus = p.multiProcessorCount > 0 ? p.multiProcessorCount : 256;
+            props.memory = (int64_t) p.totalGlobalMem;
+            props.name = p.gcnArchName;
+         }
+      }
+   } );
+   return props;
+}
+
+Workspace& threadWorkspace()
+{
+   thread_local Workspace ws;
+   ws.ensure( 4, kMaxMulti, 64 );
+   return ws;
+}
+
+void Matrix::uploadMetadata()
+{
+   switch( meta.kind ) {
+      case KIND_CSR: offsets.upload( meta.offsets ); break;
+      case KIND_SLICED:
+         sliceOffsets.upload( meta.sliceOffsets );
+         sliceSizes.upload( meta.sliceSegmentSizes );
+         break;
+      case KIND_CHUNKED:
+         rowToChunk.upload( meta.rowToChunk );
+         rowToSlice.upload( meta.rowToSlice );
+         slices.upload( meta.slices );
+         break;
+      case KIND_BIELLPACK:
+         rowPerm.upload( meta.rowPerm );
+         groupPointers.upload( meta.groupPointers );
+         break;
+      default: break;
+   }
+   if( format == TNLHIP_CSR_ADAPTIVE ) {
+      // device descriptor: {type | warpIdx << 8 | warpsCount << 16, first, size, segments}
+      const int32_t nb = (int32_t) ( adaptiveHost.size() / 6 ) - 1;  // drop the closing block
+      std::vector< int32_t > d( (size_t) 4 * std::max( nb, 0 ) );
+      for( int32_t i = 0; i < nb; i++ ) {
+         const int32_t* b = &adaptiveHost[ 6 * i ];
+         d[ 4 * i + 0 ] = b[ 0 ] | ( b[ 4 ] << 8 ) | ( b[ 5 ] << 16 );
+         d[ 4 * i + 1 ] = b[ 1 ];
+         d[ 4 * i + 2 ] = b[ 2 ];
+         d[ 4 * i + 3 ] = b[ 3 ];
+      }
+      adaptive.upload( d );
+      adaptiveBlocks = std::max( nb, 0 );
+   }
+   TNLHIP_CHECK( hipDeviceSynchronize() );
+}
+
+// Wave64 parameters of the adaptive split: 256-lane workgroups, 8 KiB of LDS per wave for
+// STREAM blocks (1024 FP64 / 2048 FP32 products), VECTOR rows up to 2048 elements.
+constexpr int kAdaptiveStreamBytes = 4 * 8192;
+constexpr int kAdaptiveMaxElements = 1024;
+
+namespace {
+
+int lightLanes( int64_t nnz, int32_t rows )
+{
+   // CSRLightKernel::init (Kernels/CSRLightKernel.hpp:475-500), extended to wave64
+   if( rows <= 0 )
+      return 2;
+   const int64_t per = ( nnz + rows - 1 ) / rows;
+   if( per <= 2 )
+      return 2;
+   if( per <= 4 )
+      return 4;
+   if( per <= 8 )
+      return 8;
+   if( per <= 16 )
+      return 16;
+   if( per <= 32 )
+      return 32;
+   return 64;
+}
+
+int hybridLanes( int64_t nnz, int32_t rows )
+{
+   // CSRHybridKernel::init (Kernels/CSRHybridKernel.hpp:159-175): 2^ceil(log2(avg)), capped
+   // at the wavefront width
+   if( rows <= 0 )
+      return 1;
+   const int64_t per = (int64_t) std::ceil( (double) nnz / (double) rows );
+   int g = 1;
+   while( g < per && g < kWave )
+      g *= 2;
+   return g;
+}
+
+tnlhip_matrix_s* buildEmpty( int format, int dtype, int32_t rows, int32_t cols, const std::vector< int32_t >& caps )
+{
+   TNLHIP_REQUIRE( format >= TNLHIP_CSR_SCALAR && format <= TNLHIP_BIELLPACK, "unknown format" );
+   TNLHIP_REQUIRE( dtype >= TNLHIP_F64 && dtype <= TNLHIP_BF16, "unknown value type" );
+   TNLHIP_REQUIRE( rows >= 0 && cols >= 0, "negative dimensions" );
+   auto M = std::make_unique< tnlhip_matrix_s >();
+   M->format = format;
+   M->dtype = dtype;
+   M->rows = rows;
+   M->cols = cols;
+   buildSegments( M->meta, kindOf( format ), rows, caps.data() );
+   if( format == TNLHIP_CSR_ADAPTIVE ) {
+      const int vsize = dtype == TNLHIP_F64 ? 8 : 4;  // LDS holds products in the compute type
+      M->adaptiveHost = buildAdaptiveBlocks( rows, M->meta.offsets.data(), vsize, kWave, 256, kAdaptiveStreamBytes,
+                                             kAdaptiveMaxElements );
+   }
+   int64_t capTotal = 0;
+   for( int32_t c : caps )
+      capTotal += c;
+   if( format == TNLHIP_CSR_LIGHT )
+      M->lanes = lightLanes( capTotal, rows );
+   else if( format == TNLHIP_CSR_HYBRID || format == TNLHIP_CSR_ADAPTIVE )
+      M->lanes = hybridLanes( capTotal, rows );
+   else if( format == TNLHIP_CSR_VECTOR )
+      M->lanes = kWave;
+   M->values.resize( M->meta.allocated * (int64_t) dtypeBytes( dtype ) );
+   M->columns.resize( M->meta.allocated );
+   M->uploadMetadata();
+   return M.release();
+}
+
+__global__ void countKernel( int64_t n, const int32_t* __restrict__ cols, ReduceSlot slot, int64_t* result )
+{
+   double acc[ 1 ] = { 0.0 };
+   for( int64_t i = (int64_t) blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += (int64_t) gridDim.x * kReduceBlock )
+      acc[ 0 ] += cols[ i ] != -1 ? 1.0 : 0.0;
+   double out[ 1 ];
+   if( finishReduction< 1 >( acc, slot, out ) && threadIdx.x == 0 )
+      *result = (int64_t) out[ 0 ];
+}
+
+int64_t countStored( const Matrix& M, hipStream_t st )
+{
+   if( M.meta.allocated == 0 )
+      return 0;
+   Workspace& ws = threadWorkspace();
+   int64_t* r = reinterpret_cast< int64_t* >( ws.scalars.ptr );
+   countKernel<<< kReduceGrid, kReduceBlock, 0, st >>>( M.meta.allocated, M.columns.ptr, ws.slot( 0 ), r );
+   TNLHIP_CHECK( hipGetLastError() );
+   int64_t h = 0;
+   TNLHIP_CHECK( hipMemcpyAsync( &h, r, sizeof( h ), hipMemcpyDeviceToHost, st ) );
+   TNLHIP_CHECK( hipStreamSynchronize( st ) );
+   return h;
+}
+
+inline float bf16ToFloat( uint16_t b )
+{
+   uint32_t u = (uint32_t) b << 16;
+   float f;
+   std::memcpy( &f, &u, 4 );
+   return f;
+}
+
+}  // namespace
+
+}  // namespace tnlhip
+
+using namespace tnlhip;
+
+extern "C" {
+
+const char* tnlhip_status_string( int s )
+{
+   switch( s ) {
+      case TNLHIP_SUCCESS: return "success";
+      case TNLHIP_ERROR_INVALID_ARGUMENT: return "invalid argument";
+      case TNLHIP_ERROR_RUNTIME: return "HIP runtime error";
+      case TNLHIP_ERROR_OUT_OF_MEMORY: return "out of memory";
+      case TNLHIP_ERROR_CAPACITY: return "row capacity exceeded";
+      case TNLHIP_ERROR_UNSUPPORTED: return "unsupported";
+      case TNLHIP_ERROR_BREAKDOWN: return "solver breakdown";
+   }
+   return "unknown status";
+}
+
+const char* tnlhip_last_error( void )
+{
+   return g_lastError.c_str();
+}
+
+int tnlhip_device_info( char* name, int name_len, int* compute_units, int64_t* global_memory )
+{
+   return guarded( [ & ] {
+      int count = 0;
+      TNLHIP_CHECK( hipGetDeviceCount( &count ) );
+      TNLHIP_REQUIRE( count > 0, "no HIP device" );
+      const DeviceProps& p = deviceProps();
+      if( name && name_len > 0 ) {
+         std::strncpy( name, p.name.c_str(), (size_t) name_len - 1 );
+         name[ name_len - 1 ] = 0;
+      }
+      if( compute_units )
+         *compute_units = p.cus;
+      if( global_memory )
+         *global_memory = p.memory;
+   } );
+}
+
+int tnlhip_matrix_create_csr( int format, int value_dtype, int32_t rows, int32_t columns, const int32_t* row_pointers,
+                              const int32_t* column_indexes, const double* values, int host_pointers,
+                              tnlhip_matrix* out )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( out != nullptr, "out is NULL" );
+      TNLHIP_REQUIRE( rows >= 0 && columns >= 0, "negative dimensions" );
+      TNLHIP_REQUIRE( row_pointers != nullptr || rows == 0, "row_pointers is NULL" );
+      std::vector< int32_t > rp( (size_t) rows + 1, 0 );
+      if( rows > 0 ) {
+         if( host_pointers )
+            std::memcpy( rp.data(), row_pointers, sizeof( int32_t ) * rp.size() );
+         else
+            TNLHIP_CHECK( hipMemcpy( rp.data(), row_pointers, sizeof( int32_t ) * rp.size(), hipMemcpyDeviceToHost ) );
+      }
+      TNLHIP_REQUIRE( rp[ 0 ] == 0, "row_pointers[0] must be 0" );
+      std::vector< int32_t > caps( rows );
+      for( int32_t i = 0; i < rows; i++ ) {
+         caps[ i ] = rp[ i + 1 ] - rp[ i ];
+         TNLHIP_REQUIRE( caps[ i ] >= 0, "row_pointers must be non-decreasing" );
+      }
+      const int64_t nnzIn = rp[ rows ];
+      if( host_pointers && nnzIn > 0 ) {
+         for( int64_t k = 0; k < nnzIn; k++ )
+            TNLHIP_REQUIRE( column_indexes[ k ] >= -1 && column_indexes[ k ] < columns, "column index out of range" );
+      }
+      std::unique_ptr< tnlhip_matrix_s > M( buildEmpty( format, value_dtype, rows, columns, caps ) );
+      DeviceArray< int32_t > drp, dcol;
+      DeviceArray< double > dval;
+      const int32_t* prp = row_pointers;
+      const int32_t* pcol = column_indexes;
+      const double* pval = values;
+      if( host_pointers ) {
+         drp.upload( rp );
+         dcol.upload( column_indexes, nnzIn );
+         dval.upload( values, nnzIn );
+         prp = drp.ptr;
+         pcol = dcol.ptr;
+         pval = dval.ptr;
+      }
+      convertFromCsrDevice( *M, prp, pcol, pval, nullptr );
+      M->nnz = countStored( *M, nullptr );
+      *out = M.release();
+   } );
+}
+
+int tnlhip_matrix_convert( tnlhip_matrix source, int format, int value_dtype, tnlhip_matrix* out )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( source && out, "NULL argument" );
+      const Matrix& S = *source;
+      DeviceArray< int32_t > dcaps( S.rows );
+      capacitiesOf( S, dcaps.ptr, nullptr );
+      std::vector< int32_t > caps = dcaps.download();
+      std::unique_ptr< tnlhip_matrix_s > M( buildEmpty( format, value_dtype, S.rows, S.cols, caps ) );
+      std::vector< int32_t > rp( (size_t) S.rows + 1, 0 );
+      for( int32_t i = 0; i < S.rows; i++ )
+         rp[ i + 1 ] = rp[ i ] + caps[ i ];
+      DeviceArray< int32_t > drp, dcol( rp[ S.rows ] );
+      DeviceArray< double > dval( rp[ S.rows ] );
+      drp.upload( rp );
+      gatherToCsr( S, drp.ptr, dcol.ptr, dval.ptr, nullptr );
+      convertFromCsrDevice( *M, drp.ptr, dcol.ptr, dval.ptr, nullptr );
+      M->nnz = countStored( *M, nullptr );
+      *out = M.release();
+   } );
+}
+
+int tnlhip_matrix_destroy( tnlhip_matrix A )
+{
+   return guarded( [ & ] { delete A; } );
+}
+
+int tnlhip_matrix_get_info( tnlhip_matrix A, tnlhip_matrix_info* info )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( A && info, "NULL argument" );
+      info->rows = A->rows;
+      info->columns = A->cols;
+      info->format = A->format;
+      info->value_dtype = A->dtype;
+      info->storage_size = A->meta.storage;
+      info->allocated = A->meta.allocated;
+      info->nonzeros = A->nnz;
+      info->lanes_per_row = A->lanes;
+      info->reserved = 0;
+   } );
+}
+
+int64_t tnlhip_matrix_get_metadata( tnlhip_matrix A, const char* name_c, int32_t* out )
+{
+   if( ! A || ! name_c )
+      return -1;
+   const HostSegments& s = A->meta;
+   const std::string name( name_c );
+   auto put = [ & ]( const std::vector< int32_t >& v ) -> int64_t
+   {
+      if( out && ! v.empty() )
+         std::memcpy( out, v.data(), v.size() * sizeof( int32_t ) );
+      return (int64_t) v.size();
+   };
+   switch( s.kind ) {
+      case KIND_CSR:
+         if( name == "offsets" )
+            return put( s.offsets );
+         if( name == "adaptiveBlocks" && A->format == TNLHIP_CSR_ADAPTIVE )
+            return put( A->adaptiveHost );
+         break;
+      case KIND_ELLPACK:
+         if( name == "ellpack" )
+            return put( { s.segmentSize, s.n, s.alignedSize } );
+         break;
+      case KIND_SLICED:
+         if( name == "sliceOffsets" )
+            return put( s.sliceOffsets );
+         if( name == "sliceSegmentSizes" )
+            return put( s.sliceSegmentSizes );
+         if( name == "sliced" )
+            return put( { s.slicedSize, (int32_t) s.storage } );
+         break;
+      case KIND_CHUNKED:
+         if( name == "rowToChunkMapping" )
+            return put( s.rowToChunk );
+         if( name == "rowToSliceMapping" )
+            return put( s.rowToSlice );
+         if( name == "chunksToSegmentsMapping" )
+            return put( s.chunksToSegments );
+         if( name == "rowPointers" )
+            return put( s.rowPointers );
+         if( name == "slices" )
+            return put( s.slices );
+         if( name == "chunked" )
+            return put( { s.numberOfSlices, (int32_t) s.storage } );
+         break;
+      case KIND_BIELLPACK:
+         if( name == "rowPermArray" )
+            return put( s.rowPerm );
+         if( name == "groupPointers" )
+            return put( s.groupPointers );
+         if( name == "biellpack" )
+            return put( { s.virtualRows, (int32_t) ( s.storage & 0xffffffff ), (int32_t) ( s.storage >> 32 ) } );
+         break;
+   }
+   return -1;
+}
+
+int64_t tnlhip_segments_metadata( int format, int32_t n, const int32_t* capacities, const char* name, int32_t* out )
+{
+   if( format < TNLHIP_CSR_SCALAR || format > TNLHIP_BIELLPACK || n < 0 || ( n > 0 && ! capacities ) || ! name )
+      return -1;
+   try {
+      tnlhip_matrix_s M;
+      M.format = format;
+      M.rows = n;
+      buildSegments( M.meta, kindOf( format ), n, capacities );
+      if( format == TNLHIP_CSR_ADAPTIVE )
+         M.adaptiveHost = buildAdaptiveBlocks( n, M.meta.offsets.data(), 8, kWave, 256, kAdaptiveStreamBytes,
+                                               kAdaptiveMaxElements );
+      if( std::string( name ) == "storage" ) {
+         if( out ) {
+            out[ 0 ] = (int32_t) ( M.meta.storage & 0xffffffff );
+            out[ 1 ] = (int32_t) ( M.meta.storage >> 32 );
+         }
+         return 2;
+      }
+      return tnlhip_matrix_get_metadata( &M, name, out );
+   }
+   catch( const std::exception& e ) {
+      setLastError( e.what() );
+      return -1;
+   }
+}
+
+int tnlhip_matrix_get_storage( tnlhip_matrix A, double* values, int32_t* columns )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( A, "NULL matrix" );
+      const int64_t n = A->meta.allocated;
+      if( n == 0 )
+         return;
+      if( columns )
+         TNLHIP_CHECK( hipMemcpy( columns, A->columns.ptr, sizeof( int32_t ) * n, hipMemcpyDeviceToHost ) );
+      if( values ) {
+         if( A->dtype == TNLHIP_F64 )
+            TNLHIP_CHECK( hipMemcpy( values, A->values.ptr, sizeof( double ) * n, hipMemcpyDeviceToHost ) );
+         else if( A->dtype == TNLHIP_F32 ) {
+            std::vector< float > t( n );
+            TNLHIP_CHECK( hipMemcpy( t.data(), A->values.ptr, sizeof( float ) * n, hipMemcpyDeviceToHost ) );
+            for( int64_t i = 0; i < n; i++ )
+               values[ i ] = t[ i ];
+         }
+         else {
+            std::vector< uint16_t > t( n );
+            TNLHIP_CHECK( hipMemcpy( t.data(), A->values.ptr, sizeof( uint16_t ) * n, hipMemcpyDeviceToHost ) );
+            for( int64_t i = 0; i < n; i++ )
+               values[ i ] = bf16ToFloat( t[ i ] );
+         }
+      }
+   } );
+}
+
+int tnlhip_matrix_get_diagonal( tnlhip_matrix A, void* diagonal, void* stream )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( A && diagonal, "NULL argument" );
+      extractDiagonal( *A, diagonal, (hipStream_t) stream );
+   } );
+}
+
+int tnlhip_matrix_set_param( tnlhip_matrix A, const char* name_c, int32_t value )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( A && name_c, "NULL argument" );
+      const std::string name( name_c );
+      if( name == "lanes_per_row" ) {
+         TNLHIP_REQUIRE( value >= 1 && value <= kWave && ( value & ( value - 1 ) ) == 0,
+                         "lanes_per_row must be a power of two in [1, 64]" );
+         TNLHIP_REQUIRE( kindOf( A->format ) == KIND_CSR && A->format != TNLHIP_CSR_SCALAR,
+                         "lanes_per_row applies to CSR light / hybrid / vector / adaptive fallback" );
+         A->lanes = value;
+      }
+      else if( name == "xcd_remap" )
+         A->xcdRemap = value != 0;
+      else
+         throw Error( TNLHIP_ERROR_INVALID_ARGUMENT, "unknown parameter " + name );
+   } );
+}
+
+int tnlhip_spmv( tnlhip_matrix A, const void* x, void* y, double alpha, double beta, int32_t first, int32_t last,
+                 void* stream )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( A, "NULL matrix" );
+      TNLHIP_REQUIRE( ( x && y ) || A->rows == 0, "NULL vector" );
+      if( last == 0 )
+         last = A->rows;
+      TNLHIP_REQUIRE( first >= 0 && first <= last && last <= A->rows, "row range out of bounds" );
+      spmvLaunch( *A, x, y, alpha, beta, first, last, nullptr, (hipStream_t) stream );
+   } );
+}
+
+int tnlhip_dot( int dtype, int64_t n, const void* a, const void* b, double* result, void* stream )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( result, "NULL result" );
+      TNLHIP_REQUIRE( dtype == TNLHIP_F64 || dtype == TNLHIP_F32, "vectors are F64 or F32" );
+      if( n <= 0 ) {
+         *result = 0.0;
+         return;
+      }
+      Workspace& ws = threadWorkspace();
+      deviceDot( dtype, n, a, b, ws.scalars.ptr, false, ws, (hipStream_t) stream );
+      TNLHIP_CHECK( hipMemcpyAsync( result, ws.scalars.ptr, sizeof( double ), hipMemcpyDeviceToHost, (hipStream_t) stream ) );
+      TNLHIP_CHECK( hipStreamSynchronize( (hipStream_t) stream ) );
+   } );
+}
+
+int tnlhip_norm2( int dtype, int64_t n, const void* a, double* result, void* stream )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( result, "NULL result" );
+      TNLHIP_REQUIRE( dtype == TNLHIP_F64 || dtype == TNLHIP_F32, "vectors are F64 or F32" );
+      if( n <= 0 ) {
+         *result = 0.0;
+         return;
+      }
+      Workspace& ws = threadWorkspace();
+      deviceDot( dtype, n, a, a, ws.scalars.ptr, true, ws, (hipStream_t) stream );
+      TNLHIP_CHECK( hipMemcpyAsync( result, ws.scalars.ptr, sizeof( double ), hipMemcpyDeviceToHost, (hipStream_t) stream ) );
+      TNLHIP_CHECK( hipStreamSynchronize( (hipStream_t) stream ) );
+   } );
+}
+
+int tnlhip_axpby( int dtype, int64_t n, double alpha, const void* x, double beta, void* y, void* stream )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( dtype == TNLHIP_F64 || dtype == TNLHIP_F32, "vectors are F64 or F32" );
+      deviceAxpby( dtype, n, alpha, x, beta, y, (hipStream_t) stream );
+   } );
+}
+
+int tnlhip_multidot( int dtype, int64_t n, int32_t count, const void* V, int64_t ld, const void* w, double* result,
+                     void* stream )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( dtype == TNLHIP_F64 || dtype == TNLHIP_F32, "vectors are F64 or F32" );
+      TNLHIP_REQUIRE( n > 0 && count > 0 && ld >= n, "bad multidot shape" );
+      Workspace& ws = threadWorkspace();
+      ws.ensure( 4, kMaxMulti, count );
+      deviceMultidot( dtype, n, count, V, ld, w, ws.scalars.ptr, ws, (hipStream_t) stream );
+      TNLHIP_CHECK( hipMemcpyAsync( result, ws.scalars.ptr, sizeof( double ) * count, hipMemcpyDeviceToHost,
+                                    (hipStream_t) stream ) );
+      TNLHIP_CHECK( hipStreamSynchronize( (hipStream_t) stream ) );
+   } );
+}
+
+int tnlhip_gemv( int dtype, int64_t n, int32_t count, double alpha, const void* V, int64_t ld, const double* x,
+                 double beta, void* y, void* stream )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( dtype == TNLHIP_F64 || dtype == TNLHIP_F32, "vectors are F64 or F32" );
+      TNLHIP_REQUIRE( n > 0 && count > 0 && count <= 64 && ld >= n, "bad gemv shape" );
+      Workspace& ws = threadWorkspace();
+      ws.ensure( 4, kMaxMulti, 64 );
+      TNLHIP_CHECK( hipMemcpyAsync( ws.scalars.ptr, x, sizeof( double ) * count, hipMemcpyHostToDevice,
+                                    (hipStream_t) stream ) );
+      deviceGemv( dtype, n, count, alpha, V, ld, ws.scalars.ptr, beta, y, (hipStream_t) stream );
+      TNLHIP_CHECK( hipStreamSynchronize( (hipStream_t) stream ) );
+   } );
+}
+
+int tnlhip_exclusive_scan_i32( int64_t n, const int32_t* in, int32_t* out, int total_out, void* stream )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( n >= 0, "negative size" );
+      Workspace& ws = threadWorkspace();
+      deviceExclusiveScan( n, in, out, total_out ? out + n : nullptr, ws, (hipStream_t) stream );
+   } );
+}
+
+}  // extern "C"

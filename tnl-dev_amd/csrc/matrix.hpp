@@ -1,0 +1,109 @@
+// Sparse matrix handle: segments metadata (host copy for export, device copy for the
+// kernels), value / column storage in HBM and the per-matrix kernel parameters.
+#pragma once
+
+#include "common.hpp"
+
+#include <map>
+
+namespace tnlhip {
+
+enum SegKind
+{
+   KIND_CSR = 0,
+   KIND_ELLPACK = 1,
+   KIND_SLICED = 2,
+   KIND_CHUNKED = 3,
+   KIND_BIELLPACK = 4
+};
+
+inline int kindOf( int format )
+{
+   switch( format ) {
+      case TNLHIP_ELLPACK: return KIND_ELLPACK;
+      case TNLHIP_SLICED_ELLPACK: return KIND_SLICED;
+      case TNLHIP_CHUNKED_ELLPACK: return KIND_CHUNKED;
+      case TNLHIP_BIELLPACK: return KIND_BIELLPACK;
+      default: return KIND_CSR;
+   }
+}
+
+// TNL constants of the segment formats
+constexpr int32_t kEllpackAlignment = 32;   // Ellpack.h:21
+constexpr int32_t kSliceSize = 32;          // SlicedEllpack.h:22
+constexpr int32_t kChunksInSlice = 256;     // ChunkedEllpack.h:182
+constexpr int32_t kDesiredChunkSize = 16;   // ChunkedEllpack.h:182
+constexpr int32_t kStrip = 32;              // BiEllpack.h:22 (WarpSize template parameter)
+constexpr int32_t kLogStrip = 5;
+constexpr int32_t kGroupsPerStrip = kLogStrip + 1;
+
+// Segment metadata built on the host from the row capacities (the way TNL builds it for
+// every device, e.g. ChunkedEllpack.hpp:194-235 builds a Host copy and assigns it).
+struct HostSegments
+{
+   int kind = KIND_CSR;
+   int32_t n = 0;
+   int64_t storage = 0;    // getStorageSize()
+   int64_t allocated = 0;  // elements that can hold data
+   // CSR
+   std::vector< int32_t > offsets;
+   // Ellpack
+   int32_t segmentSize = 0, alignedSize = 0;
+   // SlicedEllpack
+   std::vector< int32_t > sliceOffsets, sliceSegmentSizes;
+   int32_t slicedSize = 0;
+   // ChunkedEllpack
+   int32_t numberOfSlices = 0;
+   std::vector< int32_t > rowToChunk, rowToSlice, chunksToSegments, rowPointers, slices;  // slices: 4 per slice
+   // BiEllpack
+   int32_t virtualRows = 0;
+   std::vector< int32_t > rowPerm, groupPointers;
+};
+
+void buildSegments( HostSegments& s, int kind, int32_t n, const int32_t* capacities );
+
+// CSR-Adaptive blocks (same greedy split as CSRAdaptiveKernel::initValueSize) for the
+// given wave width / block size / streamed LDS budget.  6 int32 per block.
+std::vector< int32_t > buildAdaptiveBlocks( int32_t n, const int32_t* offsets, int value_size, int wave_size,
+                                            int block_size, int streamed_bytes, int max_adaptive_elements );
+
+// Adaptive block types (detail/CSRAdaptiveKernelBlockDescriptor.h)
+enum { BLOCK_LONG = 0, BLOCK_STREAM = 1, BLOCK_VECTOR = 2 };
+
+struct Matrix
+{
+   int format = TNLHIP_CSR_SCALAR;
+   int dtype = TNLHIP_F64;
+   int32_t rows = 0, cols = 0;
+   int64_t nnz = 0;
+   HostSegments meta;
+   DeviceArray< uint8_t > values;   // dtypeBytes(dtype) * meta.allocated
+   DeviceArray< int32_t > columns;  // meta.allocated
+   // device metadata
+   DeviceArray< int32_t > offsets, sliceOffsets, sliceSizes, rowToChunk, rowToSlice, slices, rowPerm, groupPointers;
+   DeviceArray< int32_t > adaptive;  // 4 int32 per block: type|warpIdx<<8|warpsCount<<16, first, size, segs
+   std::vector< int32_t > adaptiveHost;
+   int32_t adaptiveBlocks = 0;
+   int32_t lanes = 0;      // lanes per row for CSR light / hybrid / vector
+   int32_t xcdRemap = 0;
+   DeviceArray< uint8_t > diagonal;  // cached Jacobi diagonal (vector type)
+   bool diagonalValid = false;
+
+   void uploadMetadata();
+};
+
+// Device launches implemented in spmv.hip / convert.hip
+void spmvLaunch( const Matrix& A, const void* x, void* y, double alpha, double beta, int32_t first, int32_t last,
+                 const void* diag, hipStream_t stream );
+// Gated form used inside solvers: skipped on the device when gate[0] != 0 (gateMode 1) or,
+// with gateMode 2, unless gate[1] != 0.  last < 0 means all rows.
+void spmvLaunchGated( const Matrix& A, const void* x, void* y, double alpha, double beta, const void* diag,
+                      const int32_t* gate, int gateMode, hipStream_t stream, int32_t first = 0, int32_t last = -1 );
+void convertFromCsrDevice( Matrix& A, const int32_t* rowPtr, const int32_t* cols, const double* vals,
+                           hipStream_t stream );
+void gatherToCsr( const Matrix& A, const int32_t* rpDevice, int32_t* colsDevice, double* valsDevice,
+                  hipStream_t stream );
+void extractDiagonal( const Matrix& A, void* diag, hipStream_t stream );
+void capacitiesOf( const Matrix& A, int32_t* capsDevice, hipStream_t stream );
+
+}  // namespace tnlhip

@@ -1,0 +1,1554 @@
+// Krylov solvers CG, BiCGStab and GMRES (CGS / CGSR / MGS / MGSR / CWY) with the Diagonal
+// (Jacobi) preconditioner, following TNL's algorithms step for step:
+//   Solvers/Linear/CG.hpp:17-107, BICGStab.hpp:39-147, GMRES.hpp:79-801,
+//   Preconditioners/Diagonal.hpp:22-47, stopping rules Solvers/IterativeSolver.hpp:103-150.
+//
+// MI355X design: every scalar of the iteration (dots, alpha, beta, Hessenberg, Givens, the
+// iteration counter, the residue and the stopping decision) lives in device memory and is
+// produced by the last workgroup of a fused reduction kernel, so an iteration is a fixed
+// sequence of launches with no host round trip.  Every kernel is gated by device flags: once
+// the stopping rule fires, the remaining queued kernels return immediately, so the host can
+// queue iterations in chunks and look at the state only between chunks.  The host decides
+// only what TNL's host decides across GMRES restart cycles (the adaptive restart length).
+#include "blas.hpp"
+#include "matrix.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace tnlhip {
+
+// ------------------------------------------------------------------ device state
+
+struct State
+{
+   double normb, residue, s1, s2, alpha, beta, omega, rho, rho_old, r_ast_sqnorm, normw, tmp, beta_gm;
+   double pad_;
+   int32_t it;
+   int32_t flags[ 2 ];  // [0] done (stop this solve / this GMRES cycle), [1] auxiliary gate
+   int32_t histLen;
+   int32_t o_steps;
+   int32_t breakdown;
+   int32_t stop;  // solve finished (GMRES: the outer loop must not continue)
+   int32_t pad2_;
+};
+
+struct Stop
+{
+   int32_t maxIt, minIt;
+   double conv, div;
+};
+
+// IterativeSolver::checkNextIteration (IterativeSolver.hpp:116-124)
+__device__ __forceinline__ bool checkNext( const State& s, const Stop& p )
+{
+   const double r = s.residue;
+   return ! ( isnan( r ) || s.it > p.maxIt || ( r > p.div && s.it >= p.minIt ) || ( r < p.conv && s.it >= p.minIt ) );
+}
+
+// IterativeSolver::setResidue with the residual history (IterativeSolver.hpp:189-200)
+__device__ __forceinline__ void setResidue( State& s, double r, double* hist, int64_t cap )
+{
+   s.residue = r;
+   if( s.histLen < cap )
+      hist[ s.histLen ] = r;
+   s.histLen++;
+}
+
+// Gate: 0 always run; 1 run while !done; 2 run while !done && aux; 3 run while !done && !aux
+struct Gate
+{
+   const int32_t* flags;
+   int mode;
+   __device__ __forceinline__ bool open() const
+   {
+      if( mode == 0 )
+         return true;
+      const int32_t done = __builtin_nontemporal_load( flags );
+      if( done )
+         return false;
+      if( mode == 1 )
+         return true;
+      const int32_t aux = __builtin_nontemporal_load( flags + 1 );
+      return mode == 2 ? aux != 0 : aux == 0;
+   }
+};
+
+// ------------------------------------------------------------------ generic kernels
+
+// Elementwise pass over n entries; Op::load() reads device scalars once per thread.
+template< typename Op >
+__global__ __launch_bounds__( 256 ) void elementwiseKernel( int64_t n, Op op, Gate gate )
+{
+   if( ! gate.open() )
+      return;
+   op.load();
+   const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride )
+      op( i );
+}
+
+// Elementwise pass fused with W simultaneous sums; Fin runs once with the totals.
+template< int W, typename Op, typename Fin >
+__global__ __launch_bounds__( kReduceBlock ) void fusedReduceKernel( int64_t n, Op op, Fin fin, ReduceSlot slot,
+                                                                     Gate gate )
+{
+   if( ! gate.open() )
+      return;
+   op.load();
+   double acc[ W ];
+#pragma unroll
+   for( int k = 0; k < W; k++ )
+      acc[ k ] = 0.0;
+   const int64_t stride = (int64_t) gridDim.x * kReduceBlock;
+   for( int64_t i = (int64_t) blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += stride )
+      op( i, acc );
+   double out[ W ];
+   if( finishReduction< W >( acc, slot, out ) && threadIdx.x == 0 )
+      fin( out );
+}
+
+// Serial step on one lane (Hessenberg / Givens / small triangular algebra).
+template< typename Fin >
+__global__ void scalarKernel( Fin fin, Gate gate )
+{
+   if( ! gate.open() )
+      return;
+   if( threadIdx.x == 0 && blockIdx.x == 0 )
+      fin();
+}
+
+template< typename Op >
+void launchElementwise( int64_t n, const Op& op, Gate gate, hipStream_t st )
+{
+   if( n <= 0 )
+      return;
+   elementwiseKernel<<< gridFor( n, 256 ), 256, 0, st >>>( n, op, gate );
+   TNLHIP_CHECK( hipGetLastError() );
+}
+
+template< int W, typename Op, typename Fin >
+void launchReduce( int64_t n, const Op& op, const Fin& fin, ReduceSlot slot, Gate gate, hipStream_t st )
+{
+   fusedReduceKernel< W ><<< kReduceGrid, kReduceBlock, 0, st >>>( n, op, fin, slot, gate );
+   TNLHIP_CHECK( hipGetLastError() );
+}
+
+template< typename Fin >
+void launchScalar( const Fin& fin, Gate gate, hipStream_t st )
+{
+   scalarKernel<<< 1, 64, 0, st >>>( fin, gate );
+   TNLHIP_CHECK( hipGetLastError() );
+}
+
+// ------------------------------------------------------------------ elementwise operations
+
+struct NoScalars
+{
+   __device__ void load() {}
+};
+
+// y = x (copy)
+template< typename T >
+struct OpCopy : NoScalars
+{
+   T* y;
+   const T* x;
+   __device__ void operator()( int64_t i ) const { y[ i ] = x[ i ]; }
+};
+
+// y = x / d   (Diagonal::solve)
+template< typename T >
+struct OpDivide : NoScalars
+{
+   T* y;
+   const T* x;
+   const T* d;
+   __device__ void operator()( int64_t i ) const { y[ i ] = x[ i ] / d[ i ]; }
+};
+
+// y = a * x with a = (*scalar) or 1 / (*scalar)
+template< typename T >
+struct OpScale
+{
+   T* y;
+   const T* x;
+   const double* scalar;
+   bool reciprocal;
+   double a;
+   __device__ void load() { a = reciprocal ? 1.0 / *scalar : *scalar; }
+   __device__ void operator()( int64_t i ) const { y[ i ] = (T) ( a * (double) x[ i ] ); }
+};
+
+// sum of squares of x (norm); optional elementwise copy dst = src before
+template< typename T >
+struct OpSquares : NoScalars
+{
+   const T* x;
+   __device__ void operator()( int64_t i, double ( &acc )[ 1 ] ) const
+   {
+      const double v = (double) x[ i ];
+      acc[ 0 ] += v * v;
+   }
+};
+
+template< typename T >
+struct OpDot : NoScalars
+{
+   const T* a;
+   const T* b;
+   __device__ void operator()( int64_t i, double ( &acc )[ 1 ] ) const { acc[ 0 ] += (double) a[ i ] * (double) b[ i ]; }
+};
+
+// ------------------------------------------------------------------ solver object
+
+struct SolverImpl
+{
+   int method = TNLHIP_CG;
+   tnlhip_solver_params params;
+   Matrix* A = nullptr;
+   int vdtype = TNLHIP_F64;
+   int64_t n = 0;
+   size_t vbytes = 8;
+   Workspace ws;
+   DeviceArray< uint8_t > diag;
+   bool diagValid = false;
+   DeviceArray< State > state;
+   DeviceArray< double > hist;
+   int64_t histCap = 0;
+   DeviceArray< uint8_t > vecs;  // solver work vectors
+   int64_t ld = 0;               // column stride of V / Y (GMRES)
+   DeviceArray< double > small;  // GMRES Hessenberg / Givens / T / YL
+   hipStream_t stream = nullptr;
+   // results of the last solve
+   int32_t iterations = 0;
+   double residue = 0;
+   std::vector< double > history;
+
+   SolverImpl() { tnlhip_solver_params_default( &params ); }
+   ~SolverImpl()
+   {
+      if( stream )
+         (void) hipStreamDestroy( stream );
+   }
+   Stop stopRule() const
+   {
+      return Stop{ params.max_iterations, params.min_iterations, params.convergence_residue, params.divergence_residue };
+   }
+   void* vec( int k ) { return vecs.ptr + (size_t) k * ld * vbytes; }
+};
+
+namespace {
+
+void ensureDiagonal( SolverImpl& S )
+{
+   if( S.params.preconditioner != 1 || S.diagValid )
+      return;
+   S.diag.resize( S.n * (int64_t) S.vbytes );
+   extractDiagonal( *S.A, S.diag.ptr, S.stream );
+   S.diagValid = true;
+}
+
+// TNL's checkConvergence on the host from the final (iterations, residue)
+bool checkConvergenceHost( const SolverImpl& S )
+{
+   const double r = S.residue;
+   if( std::isnan( r ) )
+      return false;
+   if( r > S.params.divergence_residue && S.iterations > S.params.min_iterations )
+      return false;
+   if( S.iterations >= S.params.max_iterations )
+      return false;
+   if( r > S.params.convergence_residue )
+      return false;
+   return true;
+}
+
+State readState( SolverImpl& S )
+{
+   State h;
+   TNLHIP_CHECK( hipMemcpyAsync( &h, S.state.ptr, sizeof( State ), hipMemcpyDeviceToHost, S.stream ) );
+   TNLHIP_CHECK( hipStreamSynchronize( S.stream ) );
+   return h;
+}
+
+void finishSolve( SolverImpl& S )
+{
+   const State h = readState( S );
+   S.iterations = h.it;
+   S.residue = h.residue;
+   const int64_t len = std::min< int64_t >( h.histLen, S.histCap );
+   S.history.resize( len );
+   if( len > 0 )
+      TNLHIP_CHECK( hipMemcpy( S.history.data(), S.hist.ptr, sizeof( double ) * len, hipMemcpyDeviceToHost ) );
+}
+
+void resetState( SolverImpl& S )
+{
+   S.state.resize( 1 );
+   State z;
+   std::memset( &z, 0, sizeof( z ) );
+   TNLHIP_CHECK( hipMemcpyAsync( S.state.ptr, &z, sizeof( State ), hipMemcpyHostToDevice, S.stream ) );
+   const int64_t cap = std::min< int64_t >( (int64_t) S.params.max_iterations + 64, 1 << 20 );
+   if( S.histCap != cap ) {
+      S.hist.resize( cap );
+      S.histCap = cap;
+   }
+}
+
+// Precondition-aware SpMV: y = M^{-1} A x or A x, gated.
+void matvec( SolverImpl& S, const void* x, void* y, Gate gate )
+{
+   spmvLaunchGated( *S.A, x, y, 1.0, 0.0, S.params.preconditioner == 1 ? S.diag.ptr : nullptr, gate.flags, gate.mode,
+                    S.stream );
+}
+
+// r = M^{-1} ( b - A x ): r = b; r = b - A x (keeper beta = 1, alpha = -1); r /= diag.
+template< typename T >
+void residual( SolverImpl& S, const void* b, const void* x, T* r, Gate gate )
+{
+   launchElementwise( S.n, OpCopy< T >{ {}, r, (const T*) b }, gate, S.stream );
+   spmvLaunchGated( *S.A, x, r, -1.0, 1.0, nullptr, gate.flags, gate.mode, S.stream );
+   if( S.params.preconditioner == 1 )
+      launchElementwise( S.n, OpDivide< T >{ {}, r, r, (const T*) S.diag.ptr }, gate, S.stream );
+}
+
+// normb = || M^{-1} b || (1 when zero), written to state.normb
+template< typename T >
+struct FinNormb
+{
+   State* st;
+   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   {
+      double nb = sqrt( v[ 0 ] );
+      st->normb = nb == 0.0 ? 1.0 : nb;
+   }
+};
+
+template< typename T >
+void computeNormb( SolverImpl& S, const void* b, T* tmp )
+{
+   const T* src = (const T*) b;
+   if( S.params.preconditioner == 1 ) {
+      launchElementwise( S.n, OpDivide< T >{ {}, tmp, (const T*) b, (const T*) S.diag.ptr }, Gate{ nullptr, 0 },
+                         S.stream );
+      src = tmp;
+   }
+   launchReduce< 1 >( S.n, OpSquares< T >{ {}, src }, FinNormb< T >{ S.state.ptr }, S.ws.slot( 0 ), Gate{ nullptr, 0 },
+                      S.stream );
+}
+
+// Polls the device state every `chunk` queued iterations until `done`.
+template< typename Iter >
+void runChunks( SolverImpl& S, Iter&& iteration )
+{
+   int chunk = 4;
+   int64_t queued = 0;
+   const int64_t limit = (int64_t) S.params.max_iterations + 2;
+   while( true ) {
+      for( int k = 0; k < chunk && queued < limit; k++, queued++ )
+         iteration();
+      const State h = readState( S );
+      if( h.flags[ 0 ] || queued >= limit )
+         break;
+      chunk = std::min( chunk * 2, 64 );
+   }
+}
+
+// ================================================================== CG
+
+template< typename T >
+struct CgInitOp : NoScalars
+{
+   // z = r / diag (when preconditioned); p = z or r; acc += r * z
+   T* p;
+   T* z;
+   const T* r;
+   const T* diag;
+   __device__ void operator()( int64_t i, double ( &acc )[ 1 ] ) const
+   {
+      const T ri = r[ i ];
+      T zi = ri;
+      if( diag ) {
+         zi = ri / diag[ i ];
+         z[ i ] = zi;
+      }
+      p[ i ] = zi;
+      acc[ 0 ] += (double) ri * (double) zi;
+   }
+};
+
+struct FinCgInit
+{
+   State* st;
+   double* hist;
+   int64_t cap;
+   Stop stop;
+   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   {
+      State& s = *st;
+      s.s1 = v[ 0 ];
+      setResidue( s, sqrt( s.s1 ) / s.normb, hist, cap );
+      // while( nextIteration() ): check, then count
+      if( ! checkNext( s, stop ) )
+         s.flags[ 0 ] = 1;
+      s.it++;
+   }
+};
+
+struct FinCgAlpha
+{
+   State* st;
+   double* hist;
+   int64_t cap;
+   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   {
+      State& s = *st;
+      s.s2 = v[ 0 ];
+      if( s.s2 == 0.0 ) {
+         // p = 0 => r = 0: the solution has been reached (CG.hpp:64-67)
+         setResidue( s, 0.0, hist, cap );
+         s.flags[ 0 ] = 1;
+         return;
+      }
+      s.alpha = s.s1 / s.s2;
+   }
+};
+
+template< typename T >
+struct CgUpdateOp
+{
+   // x += alpha p; r -= alpha Ap; z = r / diag; acc += r * (z or r)
+   T* x;
+   T* r;
+   T* z;
+   const T* p;
+   const T* Ap;
+   const T* diag;
+   const State* st;
+   double alpha;
+   __device__ void load() { alpha = st->alpha; }
+   __device__ void operator()( int64_t i, double ( &acc )[ 1 ] ) const
+   {
+      x[ i ] = (T) ( (double) x[ i ] + alpha * (double) p[ i ] );
+      const T ri = (T) ( (double) r[ i ] - alpha * (double) Ap[ i ] );
+      r[ i ] = ri;
+      T zi = ri;
+      if( diag ) {
+         zi = ri / diag[ i ];
+         z[ i ] = zi;
+      }
+      acc[ 0 ] += (double) ri * (double) zi;
+   }
+};
+
+struct FinCgBeta
+{
+   State* st;
+   double* hist;
+   int64_t cap;
+   Stop stop;
+   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   {
+      State& s = *st;
+      s.s2 = s.s1;
+      s.s1 = v[ 0 ];
+      s.beta = s.s2 == 0.0 ? 0.0 : s.s1 / s.s2;
+      setResidue( s, sqrt( s.s1 ) / s.normb, hist, cap );
+      if( ! checkNext( s, stop ) )
+         s.flags[ 0 ] = 1;
+      s.it++;
+   }
+};
+
+template< typename T >
+struct CgDirectionOp
+{
+   // p = (z or r) + beta p
+   T* p;
+   const T* zr;
+   const State* st;
+   double beta;
+   __device__ void load() { beta = st->beta; }
+   __device__ void operator()( int64_t i ) const { p[ i ] = (T) ( (double) zr[ i ] + beta * (double) p[ i ] ); }
+};
+
+template< typename T >
+void solveCG( SolverImpl& S, const void* b, void* xv )
+{
+   // vectors: r, p, Ap, z
+   T* r = (T*) S.vec( 0 );
+   T* p = (T*) S.vec( 1 );
+   T* Ap = (T*) S.vec( 2 );
+   T* z = (T*) S.vec( 3 );
+   T* x = (T*) xv;
+   const T* diag = S.params.preconditioner == 1 ? (const T*) S.diag.ptr : nullptr;
+   const Gate always{ nullptr, 0 }, running{ S.state.ptr->flags, 1 };
+   const Stop stop = S.stopRule();
+   computeNormb< T >( S, b, z );
+   residual< T >( S, b, x, r, always );
+   launchReduce< 1 >( S.n, CgInitOp< T >{ {}, p, z, r, diag }, FinCgInit{ S.state.ptr, S.hist.ptr, S.histCap, stop },
+                      S.ws.slot( 0 ), always, S.stream );
+   runChunks( S, [ & ] {
+      spmvLaunchGated( *S.A, p, Ap, 1.0, 0.0, nullptr, running.flags, running.mode, S.stream );
+      launchReduce< 1 >( S.n, OpDot< T >{ {}, Ap, p }, FinCgAlpha{ S.state.ptr, S.hist.ptr, S.histCap },
+                         S.ws.slot( 1 ), running, S.stream );
+      launchReduce< 1 >( S.n, CgUpdateOp< T >{ x, r, z, p, Ap, diag, S.state.ptr, 0 },
+                         FinCgBeta{ S.state.ptr, S.hist.ptr, S.histCap, stop }, S.ws.slot( 2 ), running, S.stream );
+      launchElementwise( S.n, CgDirectionOp< T >{ p, diag ? z : r, S.state.ptr, 0 }, running, S.stream );
+   } );
+}
+
+// ================================================================== BiCGStab
+
+struct FinRhoInit
+{
+   // rho = r_ast_sqnorm = (r, r_ast) = (r, r); residue; first nextIteration
+   State* st;
+   double* hist;
+   int64_t cap;
+   Stop stop;
+   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   {
+      State& s = *st;
+      s.rho = s.r_ast_sqnorm = v[ 0 ];
+      setResidue( s, sqrt( s.rho ) / s.normb, hist, cap );
+      if( ! checkNext( s, stop ) )
+         s.flags[ 0 ] = 1;
+      s.it++;
+   }
+};
+
+template< typename T >
+struct OpCopy2 : NoScalars
+{
+   // p = r_ast = r (BICGStab.hpp:63); s = 0; acc += r * r
+   T* p;
+   T* rast;
+   T* s;
+   const T* r;
+   __device__ void operator()( int64_t i, double ( &acc )[ 1 ] ) const
+   {
+      const T ri = r[ i ];
+      p[ i ] = ri;
+      rast[ i ] = ri;
+      s[ i ] = 0;
+      acc[ 0 ] += (double) ri * (double) ri;
+   }
+};
+
+struct FinBiAlpha
+{
+   State* st;
+   __device__ void operator()( const double ( &v )[ 1 ] ) const { st->alpha = st->rho / v[ 0 ]; }
+};
+
+template< typename T >
+struct OpBiS
+{
+   // s = r - alpha Ap
+   T* s;
+   const T* r;
+   const T* Ap;
+   const State* st;
+   double alpha;
+   __device__ void load() { alpha = st->alpha; }
+   __device__ void operator()( int64_t i ) const { s[ i ] = (T) ( (double) r[ i ] - alpha * (double) Ap[ i ] ); }
+};
+
+template< typename T >
+struct OpBiOmega : NoScalars
+{
+   const T* As;
+   const T* s;
+   __device__ void operator()( int64_t i, double ( &acc )[ 2 ] ) const
+   {
+      const double a = (double) As[ i ];
+      acc[ 0 ] += a * (double) s[ i ];
+      acc[ 1 ] += a * a;
+   }
+};
+
+struct FinBiOmega
+{
+   State* st;
+   __device__ void operator()( const double ( &v )[ 2 ] ) const { st->omega = v[ 0 ] / v[ 1 ]; }
+};
+
+template< typename T >
+struct OpBiUpdate
+{
+   // x += alpha p + omega s; r = s - omega As; acc = { (r, r_ast), (r, r) }
+   T* x;
+   T* r;
+   const T* p;
+   const T* s;
+   const T* As;
+   const T* rast;
+   const State* st;
+   double alpha, omega;
+   __device__ void load()
+   {
+      alpha = st->alpha;
+      omega = st->omega;
+   }
+   __device__ void operator()( int64_t i, double ( &acc )[ 2 ] ) const
+   {
+      const double si = (double) s[ i ];
+      x[ i ] = (T) ( (double) x[ i ] + ( alpha * (double) p[ i ] + omega * si ) );
+      const T ri = (T) ( si - omega * (double) As[ i ] );
+      r[ i ] = ri;
+      acc[ 0 ] += (double) ri * (double) rast[ i ];
+      acc[ 1 ] += (double) ri * (double) ri;
+   }
+};
+
+struct FinBiRho
+{
+   // rho_old = rho; rho = (r, r_ast); restart test; beta; residue (non-exact) + next
+   State* st;
+   double* hist;
+   int64_t cap;
+   Stop stop;
+   double eps2;
+   bool exact;
+   __device__ void operator()( const double ( &v )[ 2 ] ) const
+   {
+      State& s = *st;
+      s.rho_old = s.rho;
+      s.rho = v[ 0 ];
+      if( fabs( s.rho ) < eps2 * s.r_ast_sqnorm ) {
+         s.flags[ 1 ] = 1;  // restart with a new r0 (BICGStab.hpp:88-95)
+         return;
+      }
+      s.beta = ( s.rho / s.rho_old ) * ( s.alpha / s.omega );
+      if( ! exact ) {
+         setResidue( s, sqrt( v[ 1 ] ) / s.normb, hist, cap );
+         if( ! checkNext( s, stop ) )
+            s.stop = 1;
+         s.it++;
+      }
+   }
+};
+
+template< typename T >
+struct OpBiRestartCopy : NoScalars
+{
+   // r_ast = r; acc += r * r
+   T* rast;
+   const T* r;
+   __device__ void operator()( int64_t i, double ( &acc )[ 1 ] ) const
+   {
+      const T ri = r[ i ];
+      rast[ i ] = ri;
+      acc[ 0 ] += (double) ri * (double) ri;
+   }
+};
+
+struct FinBiRestart
+{
+   State* st;
+   double* hist;
+   int64_t cap;
+   Stop stop;
+   bool exact;
+   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   {
+      State& s = *st;
+      s.rho = s.r_ast_sqnorm = v[ 0 ];
+      s.flags[ 1 ] = 0;
+      s.beta = ( s.rho / s.rho_old ) * ( s.alpha / s.omega );
+      if( ! exact ) {
+         setResidue( s, sqrt( v[ 0 ] ) / s.normb, hist, cap );
+         if( ! checkNext( s, stop ) )
+            s.stop = 1;
+         s.it++;
+      }
+   }
+};
+
+template< typename T >
+struct OpBiDirection
+{
+   // p = r + beta p - (beta omega) Ap
+   T* p;
+   const T* r;
+   const T* Ap;
+   const State* st;
+   double beta, bo;
+   __device__ void load()
+   {
+      beta = st->beta;
+      bo = st->beta * st->omega;
+   }
+   __device__ void operator()( int64_t i ) const
+   {
+      p[ i ] = (T) ( (double) r[ i ] + beta * (double) p[ i ] - bo * (double) Ap[ i ] );
+   }
+};
+
+struct FinExactResidue
+{
+   State* st;
+   double* hist;
+   int64_t cap;
+   Stop stop;
+   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   {
+      State& s = *st;
+      setResidue( s, sqrt( v[ 0 ] ) / s.normb, hist, cap );
+      if( ! checkNext( s, stop ) )
+         s.stop = 1;
+      s.it++;
+   }
+};
+
+struct FinPromoteStop
+{
+   State* st;
+   __device__ void operator()() const
+   {
+      if( st->stop )
+         st->flags[ 0 ] = 1;
+   }
+};
+
+template< typename T >
+void solveBiCGStab( SolverImpl& S, const void* b, void* xv )
+{
+   // vectors: r, r_ast, p, s, Ap, As, M_tmp
+   T* r = (T*) S.vec( 0 );
+   T* rast = (T*) S.vec( 1 );
+   T* p = (T*) S.vec( 2 );
+   T* s = (T*) S.vec( 3 );
+   T* Ap = (T*) S.vec( 4 );
+   T* As = (T*) S.vec( 5 );
+   T* Mt = (T*) S.vec( 6 );
+   T* x = (T*) xv;
+   const bool exact = S.params.bicgstab_exact_residue != 0;
+   const Gate always{ nullptr, 0 }, running{ S.state.ptr->flags, 1 }, restart{ S.state.ptr->flags, 2 };
+   const Stop stop = S.stopRule();
+   const double eps = std::numeric_limits< T >::epsilon();
+   computeNormb< T >( S, b, Mt );
+   residual< T >( S, b, x, r, always );
+   launchReduce< 1 >( S.n, OpCopy2< T >{ {}, p, rast, s, r }, FinRhoInit{ S.state.ptr, S.hist.ptr, S.histCap, stop },
+                      S.ws.slot( 0 ), always, S.stream );
+   runChunks( S, [ & ] {
+      matvec( S, p, Ap, running );
+      launchReduce< 1 >( S.n, OpDot< T >{ {}, Ap, rast }, FinBiAlpha{ S.state.ptr }, S.ws.slot( 0 ), running, S.stream );
+      launchElementwise( S.n, OpBiS< T >{ s, r, Ap, S.state.ptr, 0 }, running, S.stream );
+      matvec( S, s, As, running );
+      launchReduce< 2 >( S.n, OpBiOmega< T >{ {}, As, s }, FinBiOmega{ S.state.ptr }, S.ws.slot( 1 ), running,
+                         S.stream );
+      launchReduce< 2 >( S.n, OpBiUpdate< T >{ x, r, p, s, As, rast, S.state.ptr, 0, 0 },
+                         FinBiRho{ S.state.ptr, S.hist.ptr, S.histCap, stop, eps * eps, exact }, S.ws.slot( 2 ),
+                         running, S.stream );
+      // restart path (rare): r = M^{-1}(b - A x); r_ast = r; rho = r_ast_sqnorm = (r, r)
+      residual< T >( S, b, x, r, restart );
+      launchReduce< 1 >( S.n, OpBiRestartCopy< T >{ {}, rast, r },
+                         FinBiRestart{ S.state.ptr, S.hist.ptr, S.histCap, stop, exact }, S.ws.slot( 3 ), restart,
+                         S.stream );
+      launchElementwise( S.n, OpBiDirection< T >{ p, r, Ap, S.state.ptr, 0, 0 }, running, S.stream );
+      if( exact ) {
+         residual< T >( S, b, x, s, running );
+         launchReduce< 1 >( S.n, OpSquares< T >{ {}, s }, FinExactResidue{ S.state.ptr, S.hist.ptr, S.histCap, stop },
+                            S.ws.slot( 0 ), running, S.stream );
+      }
+      launchScalar( FinPromoteStop{ S.state.ptr }, running, S.stream );
+   } );
+}
+
+// ================================================================== GMRES
+
+// Small dense state of one restart cycle, in device memory (m = restarting_max):
+//   H (m+1) x m column-major, s, cs, sn (m+1), T (m+1)^2, YL (m+1)^2, y, aux (m+1)
+struct Small
+{
+   double* H;
+   double* s;
+   double* cs;
+   double* sn;
+   double* T;
+   double* YL;
+   double* y;
+   double* aux;
+   int m;    // current restart length
+   int mx;   // restarting_max
+};
+
+__device__ void generatePlaneRotation( double dx, double dy, double& cs, double& sn )
+{
+   // GMRES.hpp:722-744
+   if( dy == 0.0 ) {
+      cs = 1.0;
+      sn = 0.0;
+   }
+   else if( fabs( dy ) > fabs( dx ) ) {
+      const double t = dx / dy;
+      sn = 1.0 / sqrt( 1.0 + t * t );
+      cs = t * sn;
+   }
+   else {
+      const double t = dy / dx;
+      cs = 1.0 / sqrt( 1.0 + t * t );
+      sn = t * cs;
+   }
+}
+
+__device__ void applyPlaneRotation( double& dx, double& dy, double cs, double sn )
+{
+   const double t = cs * dx + sn * dy;
+   dy = cs * dy - sn * dx;
+   dx = t;
+}
+
+__device__ void applyGivens( const Small& q, int i )
+{
+   // GMRES.hpp:751-763
+   const int m = q.m;
+   double* Hc = q.H + i * ( m + 1 );
+   for( int k = 0; k < i; k++ )
+      applyPlaneRotation( Hc[ k ], Hc[ k + 1 ], q.cs[ k ], q.sn[ k ] );
+   if( Hc[ i + 1 ] != 0.0 ) {
+      generatePlaneRotation( Hc[ i ], Hc[ i + 1 ], q.cs[ i ], q.sn[ i ] );
+      applyPlaneRotation( Hc[ i ], Hc[ i + 1 ], q.cs[ i ], q.sn[ i ] );
+      applyPlaneRotation( q.s[ i ], q.s[ i + 1 ], q.cs[ i ], q.sn[ i ] );
+   }
+}
+
+struct FinGmresCycleStart
+{
+   // H = 0, s = 0, (T = 0), s[0] = beta (Gram-Schmidt variants), o_steps = m
+   State* st;
+   Small q;
+   bool cwy;
+   __device__ void operator()() const
+   {
+      for( int k = 0; k < ( q.m + 1 ) * q.m; k++ )
+         q.H[ k ] = 0.0;
+      for( int k = 0; k <= q.m; k++ )
+         q.s[ k ] = 0.0;
+      if( cwy )
+         for( int k = 0; k < ( q.mx + 1 ) * ( q.mx + 1 ); k++ )
+            q.T[ k ] = 0.0;
+      else
+         q.s[ 0 ] = st->beta_gm;
+      st->o_steps = q.m;
+      st->flags[ 0 ] = 0;
+   }
+};
+
+struct FinGmresStepStart
+{
+   // top of `for( i < m && nextIteration() )`: check and count; zero H(0..i, i)
+   State* st;
+   Small q;
+   Stop stop;
+   int i;
+   __device__ void operator()() const
+   {
+      State& s = *st;
+      const bool ok = checkNext( s, stop );
+      s.it++;
+      if( ! ok ) {
+         s.flags[ 0 ] = 1;  // loop ends, orthogonalize returns m
+         return;
+      }
+      for( int k = 0; k <= i; k++ )
+         q.H[ k + i * ( q.m + 1 ) ] = 0.0;
+   }
+};
+
+// MGS chain, fused: w -= h_prev v_prev; acc = (w, v_next)  (either part optional)
+template< typename T >
+struct OpMgs
+{
+   T* w;
+   const T* vprev;
+   const T* vnext;
+   const double* hprev;  // device scalar or null
+   double h;
+   __device__ void load() { h = hprev ? *hprev : 0.0; }
+   __device__ void operator()( int64_t i, double ( &acc )[ 1 ] ) const
+   {
+      double wi = (double) w[ i ];
+      if( vprev ) {
+         wi = wi - h * (double) vprev[ i ];
+         w[ i ] = (T) wi;
+         wi = (double) w[ i ];
+      }
+      acc[ 0 ] += vnext ? wi * (double) vnext[ i ] : wi * wi;
+   }
+};
+
+struct FinMgsCoef
+{
+   // H_{k,i} += (w, v_k); remember the coefficient for the next subtraction
+   State* st;
+   double* Hki;
+   double* coef;
+   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   {
+      *Hki += v[ 0 ];
+      *coef = v[ 0 ];
+   }
+};
+
+struct FinNormW
+{
+   State* st;
+   double* Hi1i;
+   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   {
+      const double nw = sqrt( v[ 0 ] );
+      st->normw = nw;
+      *Hi1i = nw;
+   }
+};
+
+struct FinGmresStepEnd
+{
+   // Givens, residue = |s[i+1]| / normb, `if( ! checkNextIteration() ) return i;`
+   State* st;
+   Small q;
+   Stop stop;
+   double* hist;
+   int64_t cap;
+   int i;
+   int resIndex;    // s index of the residue
+   int returnValue; // orthogonalize's return value when stopping here
+   bool givens;
+   int givensIndex;
+   bool checkStop;  // CWY tests the stopping rule only after step 0
+   __device__ void operator()() const
+   {
+      State& s = *st;
+      if( givens )
+         applyGivens( q, givensIndex );
+      setResidue( s, fabs( q.s[ resIndex ] ) / s.normb, hist, cap );
+      if( checkStop && ! checkNext( s, stop ) ) {
+         s.o_steps = returnValue;
+         s.flags[ 0 ] = 1;
+      }
+   }
+};
+
+// CGS: multi-reduction acc_k = (V_k, w) fused with the previous subtraction w -= V h_prev
+template< typename T >
+struct OpCgs
+{
+   T* w;
+   const T* V;
+   int64_t ld;
+   int count;          // columns of V involved
+   const double* hprev;  // subtraction coefficients (null: none)
+   bool norm;          // accumulate |w|^2 into acc[0] instead of the projections
+   __device__ void load() {}
+   __device__ void operator()( int64_t i, double ( &acc )[ kMaxMulti ] ) const
+   {
+      double wi = (double) w[ i ];
+      if( hprev ) {
+         double t = 0.0;
+         for( int k = 0; k < count; k++ )
+            t += (double) V[ i + k * ld ] * hprev[ k ];
+         wi = wi - t;
+         w[ i ] = (T) wi;
+         wi = (double) w[ i ];
+      }
+      if( norm )
+         acc[ 0 ] += wi * wi;
+      else
+         for( int k = 0; k < count; k++ )
+            acc[ k ] += (double) V[ i + k * ld ] * wi;
+   }
+};
+
+struct FinCgsCoef
+{
+   // H(0..i, i) += H_l; keep H_l for the subtraction
+   double* Hcol;
+   double* Hl;
+   int count;
+   __device__ void operator()( const double ( &v )[ kMaxMulti ] ) const
+   {
+      for( int k = 0; k < count; k++ ) {
+         Hcol[ k ] += v[ k ];
+         Hl[ k ] = v[ k ];
+      }
+   }
+};
+
+struct FinCgsNorm
+{
+   State* st;
+   double* Hi1i;
+   __device__ void operator()( const double ( &v )[ kMaxMulti ] ) const
+   {
+      const double nw = sqrt( v[ 0 ] );
+      st->normw = nw;
+      *Hi1i = nw;
+   }
+};
+
+// ---- update: back substitution and x += V y (GMRES.hpp:641-719)
+
+struct FinBacksolve
+{
+   State* st;
+   Small q;
+   bool cwy;
+   __device__ void operator()() const
+   {
+      State& s = *st;
+      const int m = q.m;
+      const int k = s.o_steps < m ? s.o_steps : m - 1;
+      double* y = q.y;
+      for( int i = 0; i <= m; i++ )
+         y[ i ] = q.s[ i ];
+      for( int i = k; i >= 0; i-- ) {
+         if( q.H[ i + i * ( m + 1 ) ] == 0 ) {
+            s.breakdown = 1;
+            s.flags[ 0 ] = 1;
+            return;
+         }
+         y[ i ] /= q.H[ i + i * ( m + 1 ) ];
+         for( int j = i - 1; j >= 0; j-- )
+            y[ j ] -= q.H[ j + i * ( m + 1 ) ] * y[ i ];
+      }
+      if( cwy ) {
+         const int mx = q.mx;
+         double* aux = q.aux;
+         for( int j = 0; j <= k; j++ )
+            aux[ j ] = 0;
+         for( int j = 0; j <= k; j++ )
+            for( int i = 0; i <= j; i++ )
+               aux[ i ] += y[ j ] * q.YL[ j + i * ( mx + 1 ) ];
+         for( int i = 0; i <= k; i++ ) {
+            double a2 = 0.0;
+            for( int j = i; j <= k; j++ )
+               a2 += q.T[ i + j * ( mx + 1 ) ] * aux[ j ];
+            aux[ i ] = a2;
+         }
+      }
+      s.tmp = (double) ( k + 1 );  // number of columns
+   }
+};
+
+template< typename T >
+struct OpUpdateX
+{
+   // x += sign * V c  (c = y for Gram-Schmidt, c = aux with sign -1 for CWY); CWY also x[j] += y[j]
+   T* x;
+   const T* V;
+   int64_t ld;
+   const double* c;
+   const double* y;
+   const State* st;
+   double sign;
+   bool cwy;
+   int cols;
+   __device__ void load() { cols = (int) st->tmp; }
+   __device__ void operator()( int64_t i ) const
+   {
+      double t = 0.0;
+      for( int k = 0; k < cols; k++ )
+         t += (double) V[ i + k * ld ] * c[ k ];
+      double xi = (double) x[ i ] + sign * t;
+      if( cwy && i < cols )
+         xi = (double) (T) xi + y[ i ];
+      x[ i ] = (T) xi;
+   }
+};
+
+// ---- CWY (Householder) pieces, GMRES.hpp:347-428, 464-638
+
+template< typename T >
+struct OpTruncate : NoScalars
+{
+   // y_i = z with the first i entries zeroed; acc = |y_i|^2
+   T* yi;
+   const T* z;
+   int i;
+   __device__ void operator()( int64_t j, double ( &acc )[ 1 ] ) const
+   {
+      const T v = j < i ? (T) 0 : z[ j ];
+      yi[ j ] = v;
+      acc[ 0 ] += (double) v * (double) v;
+   }
+};
+
+template< typename T >
+struct FinHouseholder
+{
+   // y_ii +- normz, t_i = 2 / |y_i|^2; copy the head of y_i to YL (apply_trunc)
+   State* st;
+   Small q;
+   T* yi;
+   int i;
+   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   {
+      const double normz = sqrt( v[ 0 ] );
+      const double y_ii = (double) yi[ i ];
+      yi[ i ] = (T) ( y_ii > 0.0 ? y_ii + normz : y_ii - normz );
+      const double norm_yi_squared = 2 * ( normz * normz + fabs( y_ii ) * normz );
+      const int mx = q.mx;
+      q.T[ i + i * ( mx + 1 ) ] = 2.0 / norm_yi_squared;
+   }
+};
+
+template< typename T >
+struct FinYL
+{
+   // YL_i = y_i[0..mx]; out = z[0..i] - y_i[0..i] (hauseholder_apply_trunc)
+   State* st;
+   Small q;
+   const T* yi;
+   const T* z;
+   int i;
+   int64_t n;
+   __device__ void operator()() const
+   {
+      const int mx = q.mx;
+      double* YLi = q.YL + i * ( mx + 1 );
+      for( int k = 0; k <= mx; k++ )
+         YLi[ k ] = k < n ? (double) yi[ k ] : 0.0;
+      double* out = i == 0 ? q.s : q.H + ( i - 1 ) * ( q.m + 1 );
+      for( int k = 0; k <= i; k++ )
+         out[ k ] = (double) z[ k ] - (double) yi[ k ];
+   }
+};
+
+struct FinTUpdate
+{
+   // [T_i]_{0..i-1} = - T_{i-1} * t_i * aux
+   Small q;
+   int i;
+   __device__ void operator()( const double ( &aux )[ kMaxMulti ] ) const
+   {
+      const int mx = q.mx;
+      const double ti = q.T[ i + i * ( mx + 1 ) ];
+      for( int k = 0; k < i; k++ ) {
+         double t = 0.0;
+         for( int j = k; j < i; j++ )
+            t -= q.T[ k + j * ( mx + 1 ) ] * ( ti * aux[ j ] );
+         q.T[ k + i * ( mx + 1 ) ] = t;
+      }
+   }
+};
+
+struct FinCwyAux
+{
+   // aux = T_i * Y_i^T e_i  (from YL)
+   Small q;
+   int i;
+   __device__ void operator()() const
+   {
+      const int mx = q.mx;
+      double* aux = q.aux;
+      for( int k = 0; k <= i; k++ )
+         aux[ k ] = q.YL[ i + k * ( mx + 1 ) ];
+      for( int k = 0; k <= i; k++ ) {
+         double a2 = 0.0;
+         for( int j = k; j <= i; j++ )
+            a2 += q.T[ k + j * ( mx + 1 ) ] * aux[ j ];
+         aux[ k ] = a2;
+      }
+   }
+};
+
+template< typename T >
+struct OpCwyV
+{
+   // v = e_i - Y_i aux
+   T* v;
+   const T* Y;
+   int64_t ld;
+   const double* auxp;
+   int count;
+   int i;
+   __device__ void load() {}
+   __device__ void operator()( int64_t j ) const
+   {
+      double t = 0.0;
+      for( int k = 0; k < count; k++ )
+         t += (double) Y[ j + k * ld ] * ( -auxp[ k ] );
+      T vj = (T) t;
+      if( j == i )
+         vj = (T) ( 1.0 + (double) vj );
+      v[ j ] = vj;
+   }
+};
+
+template< typename T >
+struct OpMulti
+{
+   // acc_k = (Y_k, w), k < count
+   const T* Y;
+   int64_t ld;
+   const T* w;
+   int count;
+   __device__ void load() {}
+   __device__ void operator()( int64_t i, double ( &acc )[ kMaxMulti ] ) const
+   {
+      const double wi = (double) w[ i ];
+      for( int k = 0; k < count; k++ )
+         acc[ k ] += (double) Y[ i + k * ld ] * wi;
+   }
+};
+
+struct FinCwyTransposed
+{
+   // aux = T_i^T * (Y_i^T w)
+   Small q;
+   int i;
+   __device__ void operator()( const double ( &v )[ kMaxMulti ] ) const
+   {
+      const int mx = q.mx;
+      double a[ kMaxMulti ];
+      for( int k = 0; k <= i; k++ )
+         a[ k ] = v[ k ];
+      for( int k = i; k >= 0; k-- ) {
+         double a2 = 0.0;
+         for( int j = 0; j <= k; j++ )
+            a2 += q.T[ j + k * ( mx + 1 ) ] * a[ j ];
+         a[ k ] = a2;
+      }
+      for( int k = 0; k <= i; k++ )
+         q.aux[ k ] = a[ k ];
+   }
+};
+
+template< typename T >
+struct OpCwyZ
+{
+   // z = w - Y_i aux
+   T* z;
+   const T* w;
+   const T* Y;
+   int64_t ld;
+   const double* auxp;
+   int count;
+   __device__ void load() {}
+   __device__ void operator()( int64_t j ) const
+   {
+      double t = 0.0;
+      for( int k = 0; k < count; k++ )
+         t += (double) Y[ j + k * ld ] * auxp[ k ];
+      z[ j ] = (T) ( (double) w[ j ] - t );
+   }
+};
+
+struct FinBeta
+{
+   // beta = |r|; residue = beta / normb
+   State* st;
+   double* hist;
+   int64_t cap;
+   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   {
+      State& s = *st;
+      s.beta_gm = sqrt( v[ 0 ] );
+      setResidue( s, s.beta_gm / s.normb, hist, cap );
+   }
+};
+
+template< typename T >
+void solveGMRES( SolverImpl& S, const void* b, void* xv )
+{
+   const tnlhip_solver_params& P = S.params;
+   if( P.restarting_min <= 0 || P.restarting_max <= 0 || P.restarting_min > P.restarting_max )
+      throw Error( TNLHIP_ERROR_INVALID_ARGUMENT, "wrong GMRES restarting parameters" );
+   if( P.restarting_step_min < 0 || P.restarting_step_max < 0 || P.restarting_step_min > P.restarting_step_max )
+      throw Error( TNLHIP_ERROR_INVALID_ARGUMENT, "wrong GMRES restarting adjustment parameters" );
+   const int variant = P.gmres_variant;
+   const bool cwy = variant == TNLHIP_GMRES_CWY;
+   const int mx = P.restarting_max;
+   TNLHIP_REQUIRE( mx + 1 <= kMaxMulti, "GMRES restarting_max must be at most 31 on this device path" );
+   // vectors: r, w, M_tmp, z; then V (or Y) with mx+1 columns
+   T* r = (T*) S.vec( 0 );
+   T* w = (T*) S.vec( 1 );
+   T* Mt = (T*) S.vec( 2 );
+   T* z = (T*) S.vec( 3 );
+   T* V = (T*) S.vec( 4 );  // Gram-Schmidt basis or Householder vectors Y
+   T* vtmp = (T*) S.vec( 4 + mx + 1 );  // CWY: the current basis vector v_i
+   T* x = (T*) xv;
+   const int64_t ld = S.ld;
+   const int64_t n = S.n;
+   const Stop stop = S.stopRule();
+   const Gate always{ nullptr, 0 }, cycle{ S.state.ptr->flags, 1 };
+   State* st = S.state.ptr;
+   // small arrays
+   const int m1 = mx + 1;
+   double* base = S.small.ptr;
+   Small q{ base, base + m1 * mx, base + m1 * mx + m1, base + m1 * mx + 2 * m1, base + m1 * mx + 3 * m1,
+            base + m1 * mx + 3 * m1 + m1 * m1, base + m1 * mx + 3 * m1 + 2 * m1 * m1,
+            base + m1 * mx + 4 * m1 + 2 * m1 * m1, mx, mx };
+   double* coef = q.aux + m1;  // MGS coefficient / CGS H_l (m1 entries)
+
+   computeNormb< T >( S, b, Mt );
+   residual< T >( S, b, x, r, always );
+   launchReduce< 1 >( n, OpSquares< T >{ {}, r }, FinBeta{ st, S.hist.ptr, S.histCap }, S.ws.slot( 0 ), always,
+                      S.stream );
+
+   double beta_ratio = 1;
+   const double max_beta_ratio = 0.99, min_beta_ratio = 0.175;
+   int restart_cycles = 0;
+   int m = mx;
+   State h = readState( S );
+   double beta_host = h.beta_gm;
+   while( true ) {
+      // while( checkNextIteration() )
+      {
+         const bool ok = ! ( std::isnan( h.residue ) || h.it > stop.maxIt || ( h.residue > stop.div && h.it >= stop.minIt )
+                             || ( h.residue < stop.conv && h.it >= stop.minIt ) );
+         if( ! ok )
+            break;
+      }
+      if( P.restarting_max > P.restarting_min && restart_cycles > 0 ) {
+         if( beta_ratio > max_beta_ratio )
+            m = P.restarting_max;
+         else if( beta_ratio >= min_beta_ratio ) {
+            const int step = P.restarting_step_min
+                           + (float) ( P.restarting_step_max - P.restarting_step_min )
+                                / ( P.restarting_max - P.restarting_min ) * ( m - P.restarting_min );
+            if( m - step >= P.restarting_min )
+               m -= step;
+            else
+               m = P.restarting_max;
+         }
+      }
+      q.m = m;
+      launchScalar( FinGmresCycleStart{ st, q, cwy }, always, S.stream );
+      if( ! cwy ) {
+         // v_0 = r / beta
+         launchElementwise( n, OpScale< T >{ V, r, &st->beta_gm, true, 0 }, always, S.stream );
+         for( int i = 0; i < m; i++ ) {
+            T* vi = V + i * ld;
+            launchScalar( FinGmresStepStart{ st, q, stop, i }, cycle, S.stream );
+            matvec( S, vi, w, cycle );
+            double* Hcol = q.H + i * ( m + 1 );
+            if( variant == TNLHIP_GMRES_MGS || variant == TNLHIP_GMRES_MGSR ) {
+               const int passes = variant == TNLHIP_GMRES_MGSR ? 2 : 1;
+               const T* vprev = nullptr;
+               for( int l = 0; l < passes; l++ )
+                  for( int k = 0; k <= i; k++ ) {
+                     launchReduce< 1 >( n, OpMgs< T >{ w, vprev, V + k * ld, vprev ? coef : nullptr, 0 },
+                                        FinMgsCoef{ st, Hcol + k, coef }, S.ws.slot( 0 ), cycle, S.stream );
+                     vprev = V + k * ld;
+                  }
+               launchReduce< 1 >( n, OpMgs< T >{ w, vprev, nullptr, coef, 0 }, FinNormW{ st, Hcol + i + 1 },
+                                  S.ws.slot( 0 ), cycle, S.stream );
+            }
+            else {
+               const int passes = variant == TNLHIP_GMRES_CGSR ? 2 : 1;
+               for( int l = 0; l < passes; l++ )
+                  launchReduce< kMaxMulti >( n, OpCgs< T >{ w, V, ld, i + 1, l == 0 ? nullptr : coef, false },
+                                             FinCgsCoef{ Hcol, coef, i + 1 }, S.ws.slot( 0 ), cycle, S.stream );
+               launchReduce< kMaxMulti >( n, OpCgs< T >{ w, V, ld, i + 1, coef, true }, FinCgsNorm{ st, Hcol + i + 1 },
+                                          S.ws.slot( 0 ), cycle, S.stream );
+            }
+            // v_{i+1} = w / |w|
+            launchElementwise( n, OpScale< T >{ V + ( i + 1 ) * ld, w, &st->normw, true, 0 }, cycle, S.stream );
+            launchScalar( FinGmresStepEnd{ st, q, stop, S.hist.ptr, S.histCap, i, i + 1, i, true, i, true }, cycle,
+                          S.stream );
+         }
+      }
+      else {
+         // z = r (GMRES.hpp:357)
+         launchElementwise( n, OpCopy< T >{ {}, z, r }, always, S.stream );
+         for( int i = 0; i <= m; i++ ) {
+            T* yi = V + i * ld;
+            launchScalar( FinGmresStepStart{ st, q, stop, -1 }, cycle, S.stream );
+            // hauseholder_generate
+            launchReduce< 1 >( n, OpTruncate< T >{ {}, yi, z, i }, FinHouseholder< T >{ st, q, yi, i }, S.ws.slot( 0 ),
+                               cycle, S.stream );
+            if( i > 0 )
+               launchReduce< kMaxMulti >( n, OpMulti< T >{ V, ld, yi, i }, FinTUpdate{ q, i }, S.ws.slot( 1 ), cycle,
+                                          S.stream );
+            // hauseholder_apply_trunc (s or H column i-1)
+            launchScalar( FinYL< T >{ st, q, yi, z, i, n }, cycle, S.stream );
+            // hauseholder_cwy: v = e_i - Y_i T_i Y_i^T e_i
+            launchScalar( FinCwyAux{ q, i }, cycle, S.stream );
+            launchElementwise( n, OpCwyV< T >{ vtmp, V, ld, q.aux, i + 1, i }, cycle, S.stream );
+            if( i < m ) {
+               matvec( S, vtmp, w, cycle );
+               // z = (I - Y_i T_i^T Y_i^T) w
+               launchReduce< kMaxMulti >( n, OpMulti< T >{ V, ld, w, i + 1 }, FinCwyTransposed{ q, i }, S.ws.slot( 2 ),
+                                          cycle, S.stream );
+               launchElementwise( n, OpCwyZ< T >{ z, w, V, ld, q.aux, i + 1 }, cycle, S.stream );
+            }
+            launchScalar( FinGmresStepEnd{ st, q, stop, S.hist.ptr, S.histCap, i, i, i - 1, i > 0, i - 1, i > 0 },
+                          cycle, S.stream );
+         }
+      }
+      h = readState( S );
+      const int o_steps = h.o_steps;
+      // update( k ) with k = o_steps (early) or m - 1
+      launchScalar( FinBacksolve{ st, q, cwy }, always, S.stream );
+      h = readState( S );
+      if( h.breakdown )
+         throw Error( TNLHIP_ERROR_BREAKDOWN, "GMRES: zero diagonal entry of the Hessenberg matrix" );
+      if( ! cwy )
+         launchElementwise( n, OpUpdateX< T >{ x, V, ld, q.y, q.y, st, 1.0, false, 0 }, always, S.stream );
+      else
+         launchElementwise( n, OpUpdateX< T >{ x, V, ld, q.aux, q.y, st, -1.0, true, 0 }, always, S.stream );
+      if( o_steps < m )
+         break;
+      // new residual, beta, restart adaptivity
+      residual< T >( S, b, x, r, always );
+      launchReduce< 1 >( n, OpSquares< T >{ {}, r }, FinBeta{ st, S.hist.ptr, S.histCap }, S.ws.slot( 0 ), always,
+                         S.stream );
+      h = readState( S );
+      const double beta_old = beta_host;
+      beta_host = h.beta_gm;
+      ++restart_cycles;
+      beta_ratio = beta_host / beta_old;
+   }
+}
+
+}  // namespace
+
+}  // namespace tnlhip
+
+using namespace tnlhip;
+
+struct tnlhip_solver_s : SolverImpl
+{};
+
+namespace {
+
+template< typename T >
+void solveTyped( SolverImpl& S, const void* b, void* x )
+{
+   switch( S.method ) {
+      case TNLHIP_CG: solveCG< T >( S, b, x ); break;
+      case TNLHIP_BICGSTAB: solveBiCGStab< T >( S, b, x ); break;
+      case TNLHIP_GMRES: solveGMRES< T >( S, b, x ); break;
+   }
+}
+
+void prepare( SolverImpl& S )
+{
+   TNLHIP_REQUIRE( S.A, "no matrix was set; call tnlhip_solver_set_matrix first" );
+   TNLHIP_REQUIRE( S.A->rows == S.A->cols, "the matrix must be square" );
+   S.vdtype = vectorDtype( S.A->dtype );
+   S.vbytes = dtypeBytes( S.vdtype );
+   S.n = S.A->rows;
+   if( ! S.stream )
+      TNLHIP_CHECK( hipStreamCreateWithFlags( &S.stream, hipStreamNonBlocking ) );
+   // column stride aligned to 256 bytes, as GMRES::setSize does for GPUs (GMRES.hpp:765-773)
+   const int64_t per = 256 / (int64_t) S.vbytes;
+   S.ld = ( S.n + per - 1 ) / per * per;
+   int nvec = 4;
+   if( S.method == TNLHIP_BICGSTAB )
+      nvec = 7;
+   if( S.method == TNLHIP_GMRES )
+      nvec = 4 + ( S.params.restarting_max + 1 ) + 1;
+   S.vecs.resize( (int64_t) nvec * S.ld * (int64_t) S.vbytes );
+   S.ws.ensure( 4, kMaxMulti, 64 );
+   if( S.method == TNLHIP_GMRES ) {
+      const int64_t m1 = S.params.restarting_max + 1;
+      S.small.resize( m1 * S.params.restarting_max + 6 * m1 + 2 * m1 * m1 + 64 );
+   }
+   ensureDiagonal( S );
+   resetState( S );
+}
+
+}  // namespace
+
+extern "C" {
+
+void tnlhip_solver_params_default( tnlhip_solver_params* p )
+{
+   if( ! p )
+      return;
+   p->max_iterations = 1000000000;
+   p->min_iterations = 0;
+   p->convergence_residue = 1e-6;
+   p->divergence_residue = std::numeric_limits< double >::max();
+   p->gmres_variant = TNLHIP_GMRES_CWY;
+   p->restarting_min = 10;
+   p->restarting_max = 10;
+   p->restarting_step_min = 3;
+   p->restarting_step_max = 3;
+   p->bicgstab_exact_residue = 0;
+   p->preconditioner = 0;
+}
+
+int tnlhip_solver_create( int method, tnlhip_solver* out )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( out, "NULL out" );
+      TNLHIP_REQUIRE( method >= TNLHIP_CG && method <= TNLHIP_GMRES, "unknown method" );
+      auto* s = new tnlhip_solver_s;
+      s->method = method;
+      *out = s;
+   } );
+}
+
+int tnlhip_solver_destroy( tnlhip_solver s )
+{
+   return guarded( [ & ] { delete s; } );
+}
+
+int tnlhip_solver_set_matrix( tnlhip_solver s, tnlhip_matrix A )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( s && A, "NULL argument" );
+      s->A = reinterpret_cast< Matrix* >( A );
+      s->diagValid = false;
+   } );
+}
+
+int tnlhip_solver_set_params( tnlhip_solver s, const tnlhip_solver_params* p )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( s && p, "NULL argument" );
+      TNLHIP_REQUIRE( p->preconditioner == 0 || p->preconditioner == 1, "unknown preconditioner" );
+      TNLHIP_REQUIRE( p->gmres_variant >= 0 && p->gmres_variant <= 4, "unknown GMRES variant" );
+      const bool pre = p->preconditioner != s->params.preconditioner;
+      s->params = *p;
+      if( pre )
+         s->diagValid = false;
+   } );
+}
+
+int tnlhip_solver_solve( tnlhip_solver s, const void* b, void* x, int32_t* converged, void* stream )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( s && b && x, "NULL argument" );
+      TNLHIP_CHECK( hipStreamSynchronize( (hipStream_t) stream ) );
+      prepare( *s );
+      if( s->vdtype == TNLHIP_F64 )
+         solveTyped< double >( *s, b, x );
+      else
+         solveTyped< float >( *s, b, x );
+      finishSolve( *s );
+      if( converged )
+         *converged = checkConvergenceHost( *s ) ? 1 : 0;
+   } );
+}
+
+int tnlhip_solver_get_status( tnlhip_solver s, int32_t* iterations, double* residue )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( s, "NULL solver" );
+      if( iterations )
+         *iterations = s->iterations;
+      if( residue )
+         *residue = s->residue;
+   } );
+}
+
+int64_t tnlhip_solver_get_history( tnlhip_solver s, double* out, int64_t capacity )
+{
+   if( ! s )
+      return -1;
+   const int64_t n = (int64_t) s->history.size();
+   if( out )
+      std::memcpy( out, s->history.data(), sizeof( double ) * std::min( n, capacity ) );
+   return n;
+}
+
+}  // extern "C"
