@@ -1,0 +1,165 @@
+"""Benchmark: FP64 SpMV on the 3D 7-point Laplacian, 160^3 (n = 4,096,000, nnz = 28,518,400),
+in every segment format on one MI355X; one JSON line on stdout.
+
+A step is one SpMV y = A x of the whole matrix with x ~ U[-1, 1], inputs resident in HBM.
+Each timed SpMV is bracketed by HIP events on the launch stream; between steps a 512 MiB
+buffer is rewritten so the 256 MiB MALL holds nothing of A, x or y (cold-cache HBM numbers).
+GB/s counts the CSR-compulsory bytes of one product:
+    nnz*(8+4) + (n+1)*4 + n*8 (x read once) + n*8 (y written) = 424,140,804 B,
+whatever the format (padding and format metadata are overhead).  TNL's own figure
+nnz*(2*8+4)/2^30 per second is reported beside it.  The CPU baseline is TNL's Host CSR
+vectorProduct (OpenMP) compiled from the reference headers (oracle/_ref), or the oracle's
+single-thread restatement when that build is absent.
+
+usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--formats all] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tnl-dev_amd"))
+
+HBM_PEAK_GBS = 8000.0
+FORMATS = ["CSRScalar", "CSRVector", "CSRLight", "CSRHybrid", "CSRAdaptive", "Ellpack", "SlicedEllpack",
+           "ChunkedEllpack", "BiEllpack"]
+HEADLINE = "CSRLight"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_spmv(tnl, torch, A, x, y, steps, warmup, flush):
+    st = torch.cuda.current_stream()
+    for _ in range(warmup):
+        A.vectorProduct(x, y)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(steps):
+        flush.add_(1.0)  # evict A, x, y from the MALL / L2
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        A.vectorProduct(x, y)
+        e1.record(st)
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1) * 1e-3)
+    return float(np.mean(times)), float(np.median(times)), float(np.min(times))
+
+
+def cpu_baseline(rp, cols, vals, x, ncols):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    n = len(rp) - 1
+    nnz = len(cols)
+    if pyoracle.ref_available():
+        loops = 40
+        t, _ = pyoracle.ref_time_spmv_csr(rp, cols, vals, ncols, x, loops=loops, threads=threads)
+        kind, cores, sample = "reference", threads, f"TNL Host CSR vectorProduct, median of {loops} calls on the full matrix"
+    else:
+        seg = pyoracle.Segments("CSR", np.diff(rp))
+        loops = 5
+        t0 = time.perf_counter()
+        for _ in range(loops):
+            seg.spmv(vals, cols, x)
+        t = (time.perf_counter() - t0) / loops
+        kind, cores, sample = "port", 1, f"oracle CSR SpMV, mean of {loops} calls on the full matrix"
+    from workloads import algorithmic_bytes
+    gbs = algorithmic_bytes(n, ncols, nnz, 8) / t / 1e9
+    return {"value": round(gbs, 2), "unit": "GB/s", "cores": cores, "kind": kind, "sample": sample,
+            "seconds_per_spmv": t}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--formats", default="all")
+    ap.add_argument("--grid", type=int, default=160)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
+                                   "(requests shard across GPUs by request, see DESIGN.md)"}))
+        sys.exit(2)
+
+    import torch
+    from __graft_entry__ import load_package
+    import workloads as W
+
+    tnl = load_package()
+    dev = torch.device("cuda:0")
+    N = args.grid
+    rp, cols, vals = W.laplacian_3d(N)
+    n = len(rp) - 1
+    nnz = len(cols)
+    x_host = W.uniform_vector(n, 2024)
+    B = W.algorithmic_bytes(n, n, nnz, 8)
+    tnl_bytes = nnz * (2 * 8 + 4)
+    log(f"matrix {N}^3: n={n} nnz={nnz} algorithmic bytes/SpMV={B}")
+    formats = FORMATS if args.formats == "all" else args.formats.split(",")
+    x = torch.tensor(x_host, device=dev)
+    y = torch.zeros(n, dtype=torch.float64, device=dev)
+    flush = torch.zeros(64 << 20, dtype=torch.float64, device=dev)  # 512 MiB
+    results = {}
+    for fmt in formats:
+        t0 = time.perf_counter()
+        A = tnl.SparseMatrix.from_csr(rp, cols, vals, format=fmt, dtype="f64")
+        setup = time.perf_counter() - t0
+        mean, med, best = time_spmv(tnl, torch, A, x, y, args.steps, args.warmup, flush)
+        gbs = B / mean / 1e9
+        results[fmt] = {"us": round(mean * 1e6, 2), "us_median": round(med * 1e6, 2), "GB/s": round(gbs, 1),
+                        "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "tnl_GiB/s": round(tnl_bytes / mean / 2 ** 30, 1),
+                        "setup_s": round(setup, 3), "storage": A.getStorageSize()}
+        log(f"{fmt:15s} {mean * 1e6:9.2f} us  {gbs:8.1f} GB/s  {100 * gbs / HBM_PEAK_GBS:5.1f}% of HBM peak")
+        del A
+        torch.cuda.empty_cache()
+    # FP32 values/vectors for the headline kernel
+    A32 = tnl.SparseMatrix.from_csr(rp, cols, vals, format=HEADLINE, dtype="f32")
+    x32, y32 = x.float(), y.float()
+    mean32, _, _ = time_spmv(tnl, torch, A32, x32, y32, args.steps, args.warmup, flush)
+    B32 = W.algorithmic_bytes(n, n, nnz, 4)
+    results[HEADLINE + "_f32"] = {"us": round(mean32 * 1e6, 2), "GB/s": round(B32 / mean32 / 1e9, 1),
+                                  "frac_hbm": round(B32 / mean32 / 1e9 / HBM_PEAK_GBS, 4)}
+    del A32
+
+    head = results.get(HEADLINE) or next(iter(results.values()))
+    head_fmt = HEADLINE if HEADLINE in results else next(iter(results))
+    mean = head["us"] * 1e-6
+    out = {
+        "metric": "SpMV GB/s & %HBM roofline; solves/s + p99 latency at 1e-10 rel. residual",
+        "value": head["GB/s"],
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mean * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3D 7-point Laplacian {N}^3 (n={n}, nnz={nnz}), FP64 values, int32 indices, "
+                               f"SpMV y = A x, x ~ U[-1,1], {head_fmt} kernel, cold MALL",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": head["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": head["frac_hbm"], "traffic": None,
+                     "algorithmic_bytes_per_launch": B},
+        "formats": results,
+    }
+    if not args.no_cpu:
+        log("timing the CPU baseline ...")
+        out["cpu_baseline"] = cpu_baseline(rp, cols, vals, x_host, n)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,234 @@
+// TNL-style C++ user code on Devices::Hip: the SpMV unit tests of
+// src/UnitTests/Matrices/SparseMatrixVectorProductTest.hpp (smallMatrix1, mediumSizeMatrix1,
+// largeMatrix, longRowsMatrix) written against the TNL class names, for every segments type,
+// and a CG / GMRES solve through LinearSolver::setMatrix / solve.  Prints one line per check;
+// exits non-zero on the first failure.
+#include <TNL/Matrices/SparseMatrix.h>
+#include <TNL/Solvers/Linear/CG.h>
+#include <TNL/Solvers/Linear/GMRES.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+using namespace TNL;
+namespace Seg = TNL::Algorithms::Segments;
+
+static int failures = 0;
+#define EXPECT_EQ( a, b )                                                                           \
+   do {                                                                                             \
+      if( !( ( a ) == ( b ) ) ) {                                                                   \
+         std::printf( "FAIL %s:%d: %s != %s (%g vs %g)\n", __FILE__, __LINE__, #a, #b, (double) ( a ), \
+                      (double) ( b ) );                                                             \
+         failures++;                                                                                \
+      }                                                                                             \
+   } while( 0 )
+
+template< typename Matrix >
+void test_smallMatrix1()
+{
+   using Vector = Containers::Vector< typename Matrix::VectorReal, Devices::Hip, int >;
+   Matrix m( 4, 4 );
+   m.setRowCapacities( std::vector< int >{ 1, 2, 1, 1 } );
+   double value = 1;
+   m.setElement( 0, 0, value++ );
+   m.setElement( 1, 1, value++ );
+   m.setElement( 1, 3, value++ );
+   m.setElement( 2, 1, value++ );
+   m.setElement( 3, 2, value++ );
+   Vector in( 4, 2.0 ), out( 4, 0.0 );
+   m.vectorProduct( in, out );
+   EXPECT_EQ( out.getElement( 0 ), 2 );
+   EXPECT_EQ( out.getElement( 1 ), 10 );
+   EXPECT_EQ( out.getElement( 2 ), 8 );
+   EXPECT_EQ( out.getElement( 3 ), 10 );
+}
+
+template< typename Matrix >
+void test_mediumSizeMatrix1()
+{
+   using Vector = Containers::Vector< typename Matrix::VectorReal, Devices::Hip, int >;
+   Matrix m( 8, 8 );
+   m.setRowCapacities( std::vector< int >{ 4, 4, 5, 4, 4, 4, 5, 5 } );
+   double v = 1;
+   for( int i = 0; i < 3; i++ ) m.setElement( 0, i, v++ );
+   m.setElement( 0, 5, v++ );
+   for( int i = 1; i < 5; i++ ) m.setElement( 1, i, v++ );
+   for( int i = 0; i < 5; i++ ) m.setElement( 2, i, v++ );
+   for( int i = 1; i < 5; i++ ) m.setElement( 3, i, v++ );
+   for( int i = 2; i < 6; i++ ) m.setElement( 4, i, v++ );
+   for( int i = 3; i < 7; i++ ) m.setElement( 5, i, v++ );
+   for( int i = 0; i < 5; i++ ) m.setElement( 6, i, v++ );
+   for( int i = 0; i < 5; i++ ) m.setElement( 7, i, v++ );
+   Vector in( 8, 2.0 ), out( 8, 0.0 );
+   m.vectorProduct( in, out );
+   const double want[] = { 20, 52, 110, 124, 156, 188, 280, 330 };
+   for( int i = 0; i < 8; i++ )
+      EXPECT_EQ( out.getElement( i ), want[ i ] );
+}
+
+template< typename Matrix >
+void test_largeMatrix()
+{
+   using Vector = Containers::Vector< typename Matrix::VectorReal, Devices::Hip, int >;
+   const int size = 1051;
+   Matrix m2( size, size );
+   std::vector< int > caps( size );
+   for( int i = 0; i < size; i++ ) caps[ i ] = i + 1;
+   m2.setRowCapacities( caps );
+   // lower triangle a_ij = j + 1, assembled as host CSR and assigned (operator= from CSR)
+   std::vector< int > rp( size + 1, 0 ), cols;
+   std::vector< double > vals;
+   for( int i = 0; i < size; i++ ) {
+      for( int j = 0; j <= i; j++ ) {
+         cols.push_back( j );
+         vals.push_back( j + 1 );
+      }
+      rp[ i + 1 ] = (int) cols.size();
+   }
+   m2.setFromCSR( size, size, rp.data(), cols.data(), vals.data() );
+   Vector in( size, 1.0 ), out( size, 0.0 );
+   m2.vectorProduct( in, out );
+   auto h = out.toHost();
+   for( int i = 0; i < size; i++ )
+      EXPECT_EQ( h[ i ], ( i + 1 ) * ( i + 2 ) / 2 );
+}
+
+template< typename Matrix >
+void test_longRowsMatrix()
+{
+   using Vector = Containers::Vector< typename Matrix::VectorReal, Devices::Hip, int >;
+   for( int columns : { 64, 65, 128, 129, 256, 257, 512, 513, 1024, 1025, 2048, 2049, 3000 } ) {
+      const int rows = 33;
+      std::vector< int > rp( rows + 1, 0 ), cols;
+      std::vector< double > vals;
+      for( int r = 0; r < rows; r++ ) {
+         for( int l = 0; l < columns; l++ ) {
+            cols.push_back( l );
+            vals.push_back( l + r );
+         }
+         rp[ r + 1 ] = (int) cols.size();
+      }
+      Matrix m;
+      m.setFromCSR( rows, columns, rp.data(), cols.data(), vals.data() );
+      Vector in( columns, 1.0 ), out( rows, 0.0 );
+      m.vectorProduct( in, out );
+      auto h = out.toHost();
+      for( int r = 0; r < rows; r++ )
+         EXPECT_EQ( h[ r ], (double) columns * ( columns - 1 ) / 2.0 + columns * r );
+   }
+}
+
+template< template< typename, typename, typename > class S >
+void runFormat( const char* name )
+{
+   using M64 = Matrices::SparseMatrix< double, Devices::Hip, int, Matrices::GeneralMatrix, S >;
+   using M32 = Matrices::SparseMatrix< float, Devices::Hip, int, Matrices::GeneralMatrix, S >;
+   const int before = failures;
+   test_smallMatrix1< M64 >();
+   test_mediumSizeMatrix1< M64 >();
+   test_largeMatrix< M64 >();
+   test_longRowsMatrix< M64 >();
+   test_smallMatrix1< M32 >();
+   test_mediumSizeMatrix1< M32 >();
+   // cross-format conversion: B = A on the device
+   M64 a( 4, 4 );
+   a.setRowCapacities( std::vector< int >{ 1, 2, 1, 1 } );
+   a.setElement( 0, 0, 1 );
+   a.setElement( 1, 1, 2 );
+   a.setElement( 1, 3, 3 );
+   a.setElement( 2, 1, 4 );
+   a.setElement( 3, 2, 5 );
+   Matrices::SparseMatrix< double, Devices::Hip, int, Matrices::GeneralMatrix, Seg::BiEllpack > b;
+   b = a;
+   Containers::Vector< double, Devices::Hip, int > in( 4, 2.0 ), out( 4, 0.0 );
+   b.vectorProduct( in, out );
+   EXPECT_EQ( out.getElement( 1 ), 10 );
+   std::printf( "%s %s\n", failures == before ? "PASS" : "FAIL", name );
+}
+
+int main()
+{
+   runFormat< Seg::CSRScalar >( "CSRScalar" );
+   runFormat< Seg::CSRVector >( "CSRVector" );
+   runFormat< Seg::CSRLight >( "CSRLight" );
+   runFormat< Seg::CSRHybrid >( "CSRHybrid" );
+   runFormat< Seg::CSRAdaptive >( "CSRAdaptive" );
+   runFormat< Seg::Ellpack >( "Ellpack" );
+   runFormat< Seg::SlicedEllpack >( "SlicedEllpack" );
+   runFormat< Seg::ChunkedEllpack >( "ChunkedEllpack" );
+   runFormat< Seg::BiEllpack >( "BiEllpack" );
+
+   // capacity overflow is std::logic_error, as in SparseMatrixView::setElement
+   {
+      Matrices::SparseMatrix< double > m( 2, 2 );
+      m.setRowCapacities( std::vector< int >{ 1, 1 } );
+      m.setElement( 0, 0, 1.0 );
+      bool thrown = false;
+      try {
+         m.setElement( 0, 1, 1.0 );
+      }
+      catch( const std::logic_error& ) {
+         thrown = true;
+      }
+      EXPECT_EQ( thrown, true );
+   }
+
+   // Solvers: 2D 5-point Laplacian 64^2, CG and GMRES (CWY) with Jacobi to 1e-8
+   {
+      using Matrix = Matrices::SparseMatrix< double, Devices::Hip, int, Matrices::GeneralMatrix, Seg::CSRLight >;
+      using Vector = Containers::Vector< double, Devices::Hip, int >;
+      const int N = 64, n = N * N;
+      auto A = std::make_shared< Matrix >( n, n );
+      std::vector< int > caps( n, 5 );
+      A->setRowCapacities( caps );
+      for( int j = 0; j < N; j++ )
+         for( int i = 0; i < N; i++ ) {
+            const int r = i + N * j;
+            if( j > 0 ) A->setElement( r, r - N, -1 );
+            if( i > 0 ) A->setElement( r, r - 1, -1 );
+            A->setElement( r, r, 4 );
+            if( i < N - 1 ) A->setElement( r, r + 1, -1 );
+            if( j < N - 1 ) A->setElement( r, r + N, -1 );
+         }
+      std::vector< double > bh( n );
+      for( int i = 0; i < n; i++ ) bh[ i ] = std::sin( 0.37 * i );
+      Vector b( bh ), x( n, 0.0 );
+      Solvers::Linear::CG< Matrix > cg;
+      cg.setMatrix( A );
+      cg.setConvergenceResidue( 1e-8 );
+      cg.setMaxIterations( 10000 );
+      const bool ok = cg.solve( b, x );
+      std::printf( "CG converged=%d iterations=%d residue=%.3e\n", (int) ok, cg.getIterations(), cg.getResidue() );
+      EXPECT_EQ( ok, true );
+      Vector r( n );
+      A->vectorProduct( x, r );
+      auto rh = r.toHost();
+      double num = 0, den = 0;
+      for( int i = 0; i < n; i++ ) {
+         num += ( bh[ i ] - rh[ i ] ) * ( bh[ i ] - rh[ i ] );
+         den += bh[ i ] * bh[ i ];
+      }
+      std::printf( "CG true residual %.3e\n", std::sqrt( num / den ) );
+      EXPECT_EQ( std::sqrt( num / den ) < 1e-7, true );
+
+      Solvers::Linear::GMRES< Matrix > gmres;
+      gmres.setMatrix( A );
+      gmres.setPreconditioner( std::make_shared< Solvers::Linear::Preconditioners::Diagonal< Matrix > >() );
+      Config::ParameterContainer prm;
+      prm.addParameter< std::string >( "gmres-variant", "CWY" );
+      prm.addParameter< int >( "gmres-restarting-min", 30 );
+      prm.addParameter< int >( "gmres-restarting-max", 30 );
+      prm.addParameter< double >( "convergence-residue", 1e-8 );
+      prm.addParameter< int >( "max-iterations", 20000 );
+      gmres.setup( prm );
+      Vector x2( n, 0.0 );
+      const bool ok2 = gmres.solve( b, x2 );
+      std::printf( "GMRES converged=%d iterations=%d residue=%.3e\n", (int) ok2, gmres.getIterations(),
+                   gmres.getResidue() );
+      EXPECT_EQ( ok2, true );
+   }
+   std::printf( failures ? "FAILURES %d\n" : "ALL PASSED\n", failures );
+   return failures ? 1 : 0;
+}

@@ -1,0 +1,158 @@
+"""Reductions, BLAS-1 and the Krylov solvers on the GPU through the C ABI, against the oracle
+(single-thread TNL semantics, pinned to TNL's own residual histories) and the mathematics."""
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as O
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def t(a, dev, dt=torch.float64):
+    return torch.tensor(np.asarray(a), dtype=dt, device=dev)
+
+
+def test_dot_norm_closed_forms_and_determinism(tnl, dev):
+    # reduceTest.h:91-128 closed forms; same bits on repeated calls
+    for size in [1, 2, 127, 128, 255, 256, 257, 1000, 9377, 100000, 4096000]:
+        ones = torch.ones(size, dtype=torch.float64, device=dev)
+        assert tnl.dot(ones, ones) == size
+        iota = torch.arange(size, dtype=torch.float64, device=dev)
+        assert tnl.dot(iota, ones) == size * (size - 1) / 2
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal(1 << 20)
+    b = rng.standard_normal(1 << 20)
+    ad, bd = t(a, dev), t(b, dev)
+    d1 = tnl.dot(ad, bd)
+    assert all(tnl.dot(ad, bd) == d1 for _ in range(5))
+    assert abs(d1 - float(np.dot(a, b))) <= 1e-12 * float(np.abs(a) @ np.abs(b))
+    assert abs(tnl.lpNorm2(ad) - np.linalg.norm(a)) <= 1e-13 * np.linalg.norm(a)
+    a32 = t(a, dev, torch.float32)
+    assert abs(tnl.dot(a32, a32) - float(np.dot(a.astype(np.float32).astype(np.float64),
+                                                 a.astype(np.float32).astype(np.float64)))) < 1e-9 * len(a)
+
+
+def test_axpby_multidot_gemv_scan(tnl, dev):
+    rng = np.random.default_rng(2)
+    n, k = 100003, 31
+    ld = ((n + 31) // 32) * 32
+    V = rng.standard_normal((k, ld))
+    w = rng.standard_normal(n)
+    Vd, wd = t(V.ravel(), dev), t(w, dev)
+    got = tnl.multidot(Vd, ld, k, wd)
+    want = V[:, :n] @ w
+    assert np.all(np.abs(got - want) <= 1e-12 * (np.abs(V[:, :n]) @ np.abs(w)))
+    c = rng.standard_normal(20)
+    y = rng.standard_normal(n)
+    yd = t(y, dev)
+    tnl.gemv(-1.0, Vd, ld, 20, c, 0.5, yd)
+    want = -(V[:20, :n].T @ c) + 0.5 * y
+    np.testing.assert_allclose(yd.cpu().numpy(), want, rtol=1e-12, atol=1e-12)
+    x = rng.standard_normal(n)
+    yd = t(y, dev)
+    tnl.axpby(2.0, t(x, dev), -3.0, yd)
+    np.testing.assert_allclose(yd.cpu().numpy(), 2 * x - 3 * y, rtol=1e-15, atol=1e-15)
+    caps = rng.integers(0, 50, 1234567).astype(np.int32)
+    cd = torch.tensor(caps, device=dev)
+    out = torch.zeros(len(caps) + 1, dtype=torch.int32, device=dev)
+    tnl.exclusive_scan(cd, out)
+    want = np.zeros(len(caps) + 1, np.int64)
+    np.cumsum(caps, out=want[1:])
+    np.testing.assert_array_equal(out.cpu().numpy(), want.astype(np.int32))
+
+
+def _run_device_solver(tnl, dev, method, rp, c, v, b, fmt="CSRLight", dtype="f64", **params):
+    A = tnl.SparseMatrix.from_csr(rp, c, v, format=fmt, dtype=dtype)
+    s = tnl.getLinearSolver(method)
+    s.setMatrix(A)
+    s.setup(**params)
+    bd = t(b, dev, A.vector_dtype)
+    xd = torch.zeros(len(b), dtype=A.vector_dtype, device=dev)
+    ok = s.solve(bd, xd)
+    return ok, s.getIterations(), s.getResidue(), s.getResidualHistory(), xd.cpu().numpy().astype(np.float64)
+
+
+def test_solvers_follow_tnl_histories(tnl, golden, dev):
+    """Every fixture run (CG, BiCGStab, all GMRES variants, with and without Jacobi): same
+    convergence flag, iteration count within 2, residual history within 1e-6 relative over the
+    first iterations (the device sums in a different order), true residual at the tolerance."""
+    d = golden["solvers"]
+    runs = sorted({k.split("/")[0] for k in d.files if k.startswith("run")})
+    errors = []
+    for run in runs:
+        try:
+            _check_run(tnl, d, dev, run)
+        except AssertionError as e:
+            errors.append(f"{run}: {e}")
+    assert not errors, "\n".join(errors)
+
+
+def _check_run(tnl, d, dev, run):
+    sysname = str(d[f"{run}/system"])
+    method = str(d[f"{run}/method"])
+    params = dict(eval(str(d[f"{run}/params"])))
+    print(f"run {run} {sysname} {method} {params}", flush=True)
+    rp, c, v, b = (d[f"{sysname}/{k}"] for k in ("row_ptr", "cols", "vals", "b"))
+    ok, its, res, hist, x = _run_device_solver(tnl, dev, method, rp, c, v, b, **params)
+    ref_its = int(d[f"{run}/iterations"][0])
+    assert ok == bool(d[f"{run}/converged"][0]), run
+    assert abs(its - ref_its) <= 2, (run, its, ref_its)
+    h_ref = d[f"{run}/history"]
+    m = min(len(hist), len(h_ref), 20)
+    np.testing.assert_allclose(hist[:m], h_ref[:m], rtol=1e-6, err_msg=run)
+    A = W.csr_to_dense(rp, c, v, len(b))
+    true_res = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    tol = params.get("convergence_residue", 1e-6)
+    assert true_res < 50 * tol, (run, true_res)
+
+
+def test_cg_laplacian_2d_256_config(tnl, dev):
+    """The CPU configuration: 256^2 5-point Laplacian, CG to 1e-8 from x0 = 0 (756 iterations on
+    TNL's Host path for its seed); device and oracle agree on the iteration count."""
+    rp, c, v = W.laplacian_2d(256)
+    b = W.uniform_vector(len(rp) - 1, 42)
+    ref = O.solve("cg", rp, c, v, b, convergence_residue=1e-8, max_iterations=5000)
+    ok, its, res, hist, x = _run_device_solver(tnl, dev, "cg", rp, c, v, b, convergence_residue=1e-8,
+                                               max_iterations=5000)
+    assert ok and abs(its - ref["iterations"]) <= 3
+    assert res < 1e-8
+    np.testing.assert_allclose(hist[:50], ref["history"][:50], rtol=1e-8)
+
+
+@pytest.mark.parametrize("fmt", ["CSRScalar", "CSRAdaptive", "Ellpack", "SlicedEllpack", "ChunkedEllpack", "BiEllpack"])
+def test_solvers_independent_of_format(tnl, dev, fmt):
+    rp, c, v = W.convection_diffusion_3d(10, 10.0)
+    b = W.uniform_vector(len(rp) - 1, 3)
+    ok, its, res, hist, x = _run_device_solver(tnl, dev, "bicgstab", rp, c, v, b, fmt=fmt, convergence_residue=1e-10,
+                                               max_iterations=2000, preconditioner=1)
+    ref = O.solve("bicgstab", rp, c, v, b, convergence_residue=1e-10, max_iterations=2000, preconditioner=1)
+    assert ok and abs(its - ref["iterations"]) <= 2
+
+
+def test_failure_paths(tnl, dev):
+    """max_iterations reached -> solve() is false; NaN in b -> false; b = 0 -> immediate
+    convergence with x = 0 (normb := 1, CG.hpp:32-33)."""
+    rp, c, v = W.laplacian_2d(32)
+    n = len(rp) - 1
+    b = W.uniform_vector(n, 1)
+    ok, its, *_ = _run_device_solver(tnl, dev, "cg", rp, c, v, b, convergence_residue=1e-12, max_iterations=5)
+    assert not ok
+    ref = O.solve("cg", rp, c, v, b, convergence_residue=1e-12, max_iterations=5)
+    assert its == ref["iterations"]
+    bn = b.copy()
+    bn[7] = np.nan
+    ok, *_ = _run_device_solver(tnl, dev, "gmres", rp, c, v, bn, convergence_residue=1e-8, max_iterations=100)
+    assert not ok
+    ok, its, res, hist, x = _run_device_solver(tnl, dev, "bicgstab", rp, c, v, np.zeros(n), convergence_residue=1e-8)
+    assert ok and np.all(x == 0)
+
+
+def test_float32_solve(tnl, dev):
+    rp, c, v = W.laplacian_3d(16)
+    b = W.uniform_vector(len(rp) - 1, 9)
+    ok, its, res, hist, x = _run_device_solver(tnl, dev, "cg", rp, c, v, b, dtype="f32", convergence_residue=1e-5,
+                                               max_iterations=1000)
+    ref = O.solve("cg", rp, c, v, b, convergence_residue=1e-5, max_iterations=1000)
+    assert ok and abs(its - ref["iterations"]) <= 3

@@ -1,0 +1,207 @@
+"""SpMV parity on the GPU through the C ABI: storage and metadata bit-exact against TNL's
+(golden fixtures written from the reference build) and the oracle; products bit-exact for the
+one-lane-per-row kernels (same summation order, no contraction) and within
+|dy_i| <= 1e-13 * sum_j |a_ij x_j| for the cooperative kernels (FP64)."""
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as O
+import tnl_cases
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+FORMATS = ["CSRScalar", "CSRVector", "CSRLight", "CSRHybrid", "CSRAdaptive", "Ellpack", "SlicedEllpack",
+           "ChunkedEllpack", "BiEllpack"]
+KIND = {"CSRScalar": "CSR", "CSRVector": "CSR", "CSRLight": "CSR", "CSRHybrid": "CSR", "CSRAdaptive": "CSR",
+        "Ellpack": "Ellpack", "SlicedEllpack": "SlicedEllpack", "ChunkedEllpack": "ChunkedEllpack",
+        "BiEllpack": "BiEllpack"}
+# kernels that sum a row left to right on one lane, products and sums rounded separately
+EXACT = {"CSRScalar", "Ellpack", "SlicedEllpack"}
+FP64_TOL = 1e-13
+
+
+def scale_of(rp, cols, vals, x):
+    """sum_j |a_ij x_j| per row (the scale of the per-entry tolerance)."""
+    n = len(rp) - 1
+    ax = np.abs(vals) * np.abs(np.where(cols >= 0, x[np.clip(cols, 0, None)], 0.0))
+    return np.add.reduceat(np.concatenate([ax, [0.0]]), np.minimum(rp[:-1], len(ax)))[:n] * (np.diff(rp) > 0)
+
+
+def check_product(fmt, y, y_ref, scale, tol=FP64_TOL):
+    if fmt in EXACT:
+        np.testing.assert_array_equal(y, y_ref)
+    else:
+        err = np.abs(y - y_ref)
+        bad = err > tol * scale + 1e-300
+        assert not bad.any(), f"{fmt}: {bad.sum()} entries off, worst {np.max(err / np.maximum(scale, 1e-300)):.3e}"
+
+
+def run_spmv(tnl, A, x, dev, y0=None, alpha=1.0, beta=0.0, first=0, last=0):
+    dt = A.vector_dtype
+    xd = torch.tensor(x, dtype=dt, device=dev)
+    yd = torch.tensor(y0, dtype=dt, device=dev) if y0 is not None else torch.zeros(A.getRows(), dtype=dt, device=dev)
+    A.vectorProduct(xd, yd, alpha, beta, first, last)
+    torch.cuda.synchronize()
+    return yd.cpu().numpy().astype(np.float64)
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_fixture_storage_metadata_and_product(tnl, golden, dev, fmt):
+    d = golden["segments"]
+    kind = KIND[fmt]
+    cases = sorted({k.split("/")[0] for k in d.files})
+    for case in cases:
+        print(f"case {case} {fmt}", flush=True)
+        rp, c, v, x = (d[f"{case}/{k}"] for k in ("row_ptr", "cols", "vals", "x"))
+        ncols = int(d[f"{case}/ncols"][0])
+        A = tnl.SparseMatrix.from_csr(rp, c, v, columns=ncols, format=fmt, dtype="f64")
+        for name in O.METADATA_NAMES[O.KINDS[kind]]:
+            np.testing.assert_array_equal(A.metadata(name), d[f"{case}/{kind}/col/{name}"], err_msg=name)
+        assert A.getStorageSize() == int(d[f"{case}/{kind}/col/storage"][0])
+        vals, cols = A.storage()
+        key = f"{case}/{kind}/col/values"
+        if key in d.files:
+            np.testing.assert_array_equal(vals, d[key])
+            np.testing.assert_array_equal(cols, d[f"{case}/{kind}/col/columns"])
+        y = run_spmv(tnl, A, x, dev)
+        check_product(fmt, y, d[f"{case}/{kind}/col/y"], scale_of(rp, c, v, x))
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_tnl_unit_test_golden_values(tnl, dev, fmt):
+    for name, case in tnl_cases.all_cases().items():
+        rp, cols, vals = tnl_cases.to_csr(case)
+        for dt in ("f64", "f32"):
+            A = tnl.SparseMatrix.from_csr(rp, cols, vals, columns=case["m"], format=fmt, dtype=dt)
+            y = run_spmv(tnl, A, np.full(case["m"], case["x"]), dev)
+            np.testing.assert_array_equal(y, np.asarray(case["y"], np.float64), err_msg=f"{fmt} {dt} {name}")
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_keepers_row_ranges_and_value_types(tnl, dev, fmt):
+    rng = np.random.default_rng(3)
+    rp, c, v = W.irregular(4000, mean=10, sigma=1.3, max_len=1500, seed=8)
+    n = len(rp) - 1
+    x = rng.uniform(-1, 1, n)
+    y0 = rng.uniform(-1, 1, n)
+    seg = O.Segments("CSR", np.diff(rp))
+    sv, sc = seg.convert(rp, c, v)
+    scale = scale_of(rp, c, v, x)
+    A = tnl.SparseMatrix.from_csr(rp, c, v, format=fmt, dtype="f64")
+    for alpha, beta, first, last in [(1.0, 0.0, 0, 0), (2.5, 0.0, 0, 0), (1.0, -0.5, 17, 3001), (-1.0, 1.0, 0, n),
+                                     (0.75, 3.0, 1000, 1001)]:
+        want = seg.spmv(sv, sc, x, y0, alpha, beta, first, last)
+        got = run_spmv(tnl, A, x, dev, y0, alpha, beta, first, last)
+        hi = n if last == 0 else last
+        untouched = np.r_[0:first, hi:n]
+        np.testing.assert_array_equal(got[untouched], y0[untouched])
+        err = np.abs(got - want)
+        tol = FP64_TOL * (abs(alpha) * scale + abs(beta) * np.abs(y0)) + 1e-300
+        assert np.all(err <= tol), (alpha, beta, first, last, np.max(err / tol))
+    # FP32 values: oracle in float32 on float32-rounded inputs, tolerance scaled by 2^-20
+    A32 = tnl.SparseMatrix.from_csr(rp, c, v, format=fmt, dtype="f32")
+    want = seg.spmv(sv.astype(np.float32), sc, x.astype(np.float32), dtype=np.float32).astype(np.float64)
+    got = run_spmv(tnl, A32, x, dev)
+    assert np.all(np.abs(got - want) <= 2.0 ** -20 * scale * 8 + 1e-30)
+    # BF16 values, FP32 compute: values rounded to bfloat16 on the device
+    Ab = tnl.SparseMatrix.from_csr(rp, c, v, format=fmt, dtype="bf16")
+    vb, cb = Ab.storage()
+    vb_full = np.zeros(len(c))
+    # the bf16 storage read back holds the rounded values; compare with a float64 product of them
+    segb = O.Segments(KIND[fmt], np.diff(rp))
+    ref_vals, ref_cols = segb.convert(rp, c, v)
+    assert np.array_equal(cb, ref_cols[:len(cb)])
+    assert np.all(np.abs(vb - ref_vals[:len(vb)]) <= 2.0 ** -8 * np.abs(ref_vals[:len(vb)]))
+    want_b = segb.spmv(vb, cb, x.astype(np.float32).astype(np.float64))
+    got = run_spmv(tnl, Ab, x, dev)
+    assert np.all(np.abs(got - want_b) <= 2.0 ** -20 * scale * 8 + 1e-30)
+    del vb_full
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_full_size_laplacian_160(tnl, dev, fmt):
+    """The benchmark matrix: 160^3 7-point Laplacian.  A*1 is exact (row sums are small
+    integers), metadata bit-exact against the oracle, and a random x within tolerance."""
+    rp, c, v = W.laplacian_3d(160)
+    n = len(rp) - 1
+    A = tnl.SparseMatrix.from_csr(rp, c, v, format=fmt, dtype="f64")
+    assert A.getNonzeroElementsCount() == 28518400
+    seg = O.Segments(KIND[fmt], np.diff(rp))
+    for name in O.METADATA_NAMES[O.KINDS[KIND[fmt]]]:
+        np.testing.assert_array_equal(A.metadata(name), seg.get(name), err_msg=name)
+    y = run_spmv(tnl, A, np.ones(n), dev)
+    idx = np.arange(n)
+    want = np.zeros(n)
+    for d in range(3):
+        coord = (idx // 160 ** d) % 160
+        want += (coord == 0).astype(float) + (coord == 159).astype(float)
+    np.testing.assert_array_equal(y, want)
+    x = W.uniform_vector(n, 11)
+    ov, oc = O.Segments("CSR", np.diff(rp)).convert(rp, c, v)
+    y_ref = O.Segments("CSR", np.diff(rp)).spmv(ov, oc, x)
+    check_product(fmt, run_spmv(tnl, A, x, dev), y_ref, scale_of(rp, c, v, x))
+
+
+def test_device_conversion_between_formats(tnl, dev):
+    """B = A across formats on the device gives TNL's storage for B (capacities of A's
+    segments), checked against the oracle's conversion chain."""
+    rp, c, v = W.irregular(3000, mean=8, sigma=1.0, max_len=300, seed=12)
+    n = len(rp) - 1
+    x = W.uniform_vector(n, 5)
+    A = tnl.SparseMatrix.from_csr(rp, c, v, format="ChunkedEllpack", dtype="f64")
+    # TNL: capacities of B are A's segment sizes (getRowCapacities); entries in A's local order
+    src = O.Segments("ChunkedEllpack", np.diff(rp))
+    caps = np.array([O.lib().orc_segments_segment_size(src._h, r) for r in range(n)], np.int32)
+    sv, sc = src.convert(rp, c, v)
+    rp2 = np.zeros(n + 1, np.int32)
+    rp2[1:] = np.cumsum(caps)
+    c2 = np.full(rp2[-1], -1, np.int32)
+    v2 = np.zeros(rp2[-1])
+    for r in range(n):
+        k = rp2[r]
+        for l in range(caps[r]):
+            g = src.global_index(r, l)
+            if sc[g] != -1:
+                c2[k] = sc[g]
+                v2[k] = sv[g]
+                k += 1
+    for fmt in FORMATS:
+        B = A.convert(fmt)
+        dst = O.Segments(KIND[fmt], caps)
+        dv, dc = dst.convert(rp2, c2, v2)
+        bv, bc = B.storage()
+        np.testing.assert_array_equal(bc, dc[:len(bc)], err_msg=fmt)
+        np.testing.assert_array_equal(bv, dv[:len(bv)], err_msg=fmt)
+        y = run_spmv(tnl, B, x, dev)
+        y_ref = dst.spmv(dv, dc, x)
+        check_product("x", y, y_ref, scale_of(rp, c, v, x))
+
+
+def test_empty_and_degenerate_matrices(tnl, dev):
+    for fmt in FORMATS:
+        for rows, caps in [(1, [0]), (40, [0] * 40), (33, [1] * 33), (5, [0, 3, 0, 0, 2])]:
+            rp = np.zeros(rows + 1, np.int32)
+            rp[1:] = np.cumsum(caps)
+            c = np.array([k % rows for k in range(rp[-1])], np.int32)
+            v = np.arange(1, rp[-1] + 1, dtype=np.float64)
+            A = tnl.SparseMatrix.from_csr(rp, c, v, format=fmt)
+            x = np.linspace(1, 2, rows)
+            y = run_spmv(tnl, A, x, dev, np.full(rows, 7.0), 1.0, 0.0)
+            want = W.csr_to_dense(rp, c, v, rows) @ x
+            np.testing.assert_allclose(y, want, rtol=1e-15, atol=0, err_msg=f"{fmt} {caps}")
+
+
+def test_invalid_arguments_raise(tnl, dev):
+    rp, c, v = W.laplacian_2d(4)
+    A = tnl.SparseMatrix.from_csr(rp, c, v)
+    x = torch.zeros(16, dtype=torch.float64, device=dev)
+    with pytest.raises(ValueError):
+        A.vectorProduct(x, torch.zeros(15, dtype=torch.float64, device=dev))
+    with pytest.raises(tnl.TnlHipError):
+        A.vectorProduct(x, torch.zeros(16, dtype=torch.float64, device=dev), 1.0, 0.0, 5, 3)
+    bad = c.copy()
+    bad[3] = 99
+    with pytest.raises(tnl.TnlHipError):
+        tnl.SparseMatrix.from_csr(rp, bad, v)

@@ -1,0 +1,232 @@
+// Matrices::SparseMatrix< Real, Devices::Hip, Index, GeneralMatrix, Segments, ComputeReal >:
+// TNL's sparse matrix interface (src/TNL/Matrices/SparseMatrix.h:40-1085) over the tnlhip C ABI.
+//
+// Elements set with setElement / addElement are staged on the host in TNL's row layout
+// (capacities from setRowCapacities, columns kept sorted, SparseMatrixView.hpp:270-345) and
+// moved to HBM on first use, through the same conversion TNL's cross-device operator= does
+// (explicit zeros dropped, padding index -1).  vectorProduct, format conversion and the
+// solvers then run on the GPU.
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+#include "../Algorithms/Segments/Segments.h"
+#include "../Containers/Vector.h"
+
+namespace TNL {
+
+// bfloat16 storage for SparseMatrix values (ComputeReal float)
+struct bfloat16
+{
+   uint16_t bits;
+};
+
+namespace Matrices {
+
+struct GeneralMatrix
+{
+   static constexpr bool isSymmetric() { return false; }
+};
+
+template< typename Real,
+          typename Device = Devices::Hip,
+          typename Index = int,
+          typename MatrixType = GeneralMatrix,
+          template< typename, typename, typename > class Segments = Algorithms::Segments::CSRDefault,
+          typename ComputeReal = typename std::conditional< std::is_same< Real, bfloat16 >::value, float, Real >::type >
+class SparseMatrix
+{
+   static_assert( std::is_same< Device, Devices::Hip >::value, "this SparseMatrix is the Devices::Hip one" );
+   static_assert( std::is_same< Index, int >::value, "Index is int (int32)" );
+   static_assert( ! MatrixType::isSymmetric(), "GeneralMatrix only" );
+
+public:
+   using RealType = Real;
+   using DeviceType = Device;
+   using IndexType = Index;
+   using ComputeRealType = ComputeReal;
+   using VectorReal = typename std::conditional< std::is_same< Real, double >::value, double, float >::type;
+   using SegmentsTag = Segments< Device, Index, void >;
+   static constexpr int format = SegmentsTag::format;
+   static constexpr int valueDtype =
+      std::is_same< Real, double >::value ? TNLHIP_F64 : std::is_same< Real, float >::value ? TNLHIP_F32 : TNLHIP_BF16;
+
+   SparseMatrix() = default;
+   SparseMatrix( Index rows, Index columns ) { setDimensions( rows, columns ); }
+   SparseMatrix( const SparseMatrix& o ) { *this = o; }
+   ~SparseMatrix() { release(); }
+
+   // ---- setup (SparseMatrix.hpp:284-305 and the setters)
+   void setDimensions( Index rows, Index columns )
+   {
+      release();
+      this->rows = rows;
+      this->columns = columns;
+      offsets.assign( (size_t) rows + 1, 0 );
+      cols.clear();
+      vals.clear();
+      used.assign( rows, 0 );
+   }
+
+   template< typename Capacities >
+   void setRowCapacities( const Capacities& capacities )
+   {
+      if( (Index) capacities.size() != rows )
+         throw std::invalid_argument( "Number of matrix rows does not fit with rowCapacities vector size." );
+      release();
+      for( Index r = 0; r < rows; r++ )
+         offsets[ r + 1 ] = offsets[ r ] + capacities[ r ];
+      cols.assign( offsets[ rows ], -1 );
+      vals.assign( offsets[ rows ], 0.0 );
+      used.assign( rows, 0 );
+   }
+
+   void setElement( Index row, Index column, double value ) { addElement( row, column, value, 0.0 ); }
+
+   void addElement( Index row, Index column, double value, double thisElementMultiplicator = 1.0 )
+   {
+      stage();
+      const Index b = offsets[ row ], e = b + used[ row ];
+      Index k = b;
+      while( k < e && cols[ k ] < column )
+         k++;
+      if( k < e && cols[ k ] == column ) {
+         vals[ k ] = thisElementMultiplicator * vals[ k ] + value;
+         return;
+      }
+      if( e == offsets[ row + 1 ] )
+         throw std::logic_error( "The capacity of the sparse matrix row was exceeded." );
+      for( Index j = e; j > k; j-- ) {
+         cols[ j ] = cols[ j - 1 ];
+         vals[ j ] = vals[ j - 1 ];
+      }
+      cols[ k ] = column;
+      vals[ k ] = value;
+      used[ row ]++;
+   }
+
+   double getElement( Index row, Index column ) const
+   {
+      const_cast< SparseMatrix* >( this )->stage();
+      for( Index k = offsets[ row ]; k < offsets[ row ] + used[ row ]; k++ )
+         if( cols[ k ] == column )
+            return vals[ k ];
+      return 0.0;
+   }
+
+   // A = csr given by host arrays (row pointers, columns, values): operator= from a host
+   // CSR matrix (SparseMatrix.hpp:1070-1180)
+   void setFromCSR( Index rows, Index columns, const Index* rowPointers, const Index* columnIndexes,
+                    const double* values )
+   {
+      release();
+      this->rows = rows;
+      this->columns = columns;
+      detail::check( tnlhip_matrix_create_csr( format, valueDtype, rows, columns, rowPointers, columnIndexes, values, 1,
+                                               &handle ) );
+      staged = false;
+   }
+
+   // B = A across formats and value types, on the device (SparseMatrix.hpp:1093-1107)
+   template< typename R2, template< typename, typename, typename > class S2, typename C2 >
+   SparseMatrix& operator=( const SparseMatrix< R2, Device, Index, MatrixType, S2, C2 >& other )
+   {
+      release();
+      rows = other.getRows();
+      columns = other.getColumns();
+      detail::check( tnlhip_matrix_convert( other.getHandle(), format, valueDtype, &handle ) );
+      staged = false;
+      return *this;
+   }
+   SparseMatrix& operator=( const SparseMatrix& other )
+   {
+      if( this != &other ) {
+         release();
+         rows = other.rows;
+         columns = other.columns;
+         detail::check( tnlhip_matrix_convert( other.getHandle(), format, valueDtype, &handle ) );
+         staged = false;
+      }
+      return *this;
+   }
+
+   // ---- queries
+   Index getRows() const { return rows; }
+   Index getColumns() const { return columns; }
+   tnlhip_matrix_info getInfo() const
+   {
+      tnlhip_matrix_info info;
+      detail::check( tnlhip_matrix_get_info( getHandle(), &info ) );
+      return info;
+   }
+   Index getNonzeroElementsCount() const { return (Index) getInfo().nonzeros; }
+   int64_t getStorageSize() const { return getInfo().storage_size; }
+   std::vector< int32_t > getMetadata( const char* name ) const
+   {
+      const int64_t n = tnlhip_matrix_get_metadata( getHandle(), name, nullptr );
+      if( n < 0 )
+         throw std::invalid_argument( name );
+      std::vector< int32_t > v( n );
+      tnlhip_matrix_get_metadata( getHandle(), name, v.data() );
+      return v;
+   }
+
+   // ---- SparseMatrix::vectorProduct (SparseMatrix.h:1078-1085); synchronous like TNL's
+   template< typename InVector, typename OutVector >
+   void vectorProduct( const InVector& inVector, OutVector& outVector, ComputeReal matrixMultiplicator = 1.0,
+                       ComputeReal outVectorMultiplicator = 0.0, Index firstRow = 0, Index lastRow = 0 ) const
+   {
+      if( inVector.getSize() != columns || outVector.getSize() != rows )
+         throw std::invalid_argument( "vector sizes do not fit the matrix" );
+      detail::check( tnlhip_spmv( getHandle(), inVector.getData(), outVector.getData(), matrixMultiplicator,
+                                  outVectorMultiplicator, firstRow, lastRow, nullptr ) );
+      detail::checkHip( hipStreamSynchronize( nullptr ) );
+   }
+
+   // the C-ABI handle (moves staged elements to the device first)
+   tnlhip_matrix getHandle() const
+   {
+      if( ! handle ) {
+         if( rows > 0 && offsets.size() != (size_t) rows + 1 )
+            throw std::logic_error( "matrix dimensions were not set" );
+         auto* self = const_cast< SparseMatrix* >( this );
+         detail::check( tnlhip_matrix_create_csr( format, valueDtype, rows, columns, offsets.data(),
+                                                  cols.empty() ? nullptr : cols.data(),
+                                                  vals.empty() ? nullptr : vals.data(), 1, &self->handle ) );
+      }
+      return handle;
+   }
+
+private:
+   // back to host staging before an element update (the device copy is rebuilt on next use)
+   void stage()
+   {
+      if( handle && ! staged )
+         throw std::logic_error( "setElement after the matrix moved to the device is not supported; "
+                                 "build the matrix, then use it" );
+      if( handle ) {
+         tnlhip_matrix_destroy( handle );
+         handle = nullptr;
+      }
+      staged = true;
+   }
+   void release()
+   {
+      if( handle )
+         tnlhip_matrix_destroy( handle );
+      handle = nullptr;
+   }
+
+   Index rows = 0, columns = 0;
+   std::vector< Index > offsets{ 0 }, cols, used;
+   std::vector< double > vals;
+   mutable tnlhip_matrix handle = nullptr;
+   bool staged = true;
+};
+
+}  // namespace Matrices
+}  // namespace TNL

@@ -1,0 +1,161 @@
+// LinearSolver< Matrix > with IterativeSolver's setters (src/TNL/Solvers/Linear/LinearSolver.h:115-175,
+// src/TNL/Solvers/IterativeSolver.h:70-245) and the CG / BICGStab / GMRES front classes, over
+// the tnlhip solver entry points.  solve() returns checkConvergence() like TNL's.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../Config/ParameterContainer.h"
+#include "../../Containers/Vector.h"
+#include "Preconditioners/Diagonal.h"
+
+namespace TNL {
+namespace Solvers {
+namespace Linear {
+
+template< typename Matrix, int Method >
+class LinearSolverBase
+{
+public:
+   using MatrixType = Matrix;
+   using MatrixPointer = std::shared_ptr< const Matrix >;
+   using PreconditionerPointer = std::shared_ptr< Preconditioners::Preconditioner< Matrix > >;
+   using VectorType = Containers::Vector< typename Matrix::VectorReal, Devices::Hip, int >;
+
+   LinearSolverBase()
+   {
+      detail::check( tnlhip_solver_create( Method, &handle ) );
+      tnlhip_solver_params_default( &params );
+   }
+   virtual ~LinearSolverBase() { tnlhip_solver_destroy( handle ); }
+   LinearSolverBase( const LinearSolverBase& ) = delete;
+   LinearSolverBase& operator=( const LinearSolverBase& ) = delete;
+
+   void setMatrix( const MatrixPointer& m )
+   {
+      matrix = m;
+      detail::check( tnlhip_solver_set_matrix( handle, m->getHandle() ) );
+   }
+   void setPreconditioner( const PreconditionerPointer& p )
+   {
+      preconditioner = p;
+      params.preconditioner = p ? p->kind() : 0;
+   }
+   void setMaxIterations( int n ) { params.max_iterations = n; }
+   void setMinIterations( int n ) { params.min_iterations = n; }
+   void setConvergenceResidue( double r ) { params.convergence_residue = r; }
+   void setDivergenceResidue( double r ) { params.divergence_residue = r; }
+   int getMaxIterations() const { return params.max_iterations; }
+   double getConvergenceResidue() const { return params.convergence_residue; }
+
+   // setup( parameters, prefix ): IterativeSolver.hpp:41-64 plus the solver-specific keys
+   virtual bool setup( const Config::ParameterContainer& p, const std::string& prefix = "" )
+   {
+      if( p.checkParameter( prefix + "max-iterations" ) )
+         params.max_iterations = p.getInt( prefix + "max-iterations" );
+      if( p.checkParameter( prefix + "min-iterations" ) )
+         params.min_iterations = p.getInt( prefix + "min-iterations" );
+      if( p.checkParameter( prefix + "convergence-residue" ) )
+         params.convergence_residue = p.getDouble( prefix + "convergence-residue" );
+      if( p.checkParameter( prefix + "divergence-residue" ) )
+         params.divergence_residue = p.getDouble( prefix + "divergence-residue" );
+      return true;
+   }
+
+   // bool solve( b, x ), x holds the initial guess
+   bool solve( const VectorType& b, VectorType& x )
+   {
+      if( ! matrix )
+         throw std::logic_error( "No matrix was set. Call setMatrix() before solve()." );
+      detail::check( tnlhip_solver_set_params( handle, &params ) );
+      int32_t converged = 0;
+      detail::check( tnlhip_solver_solve( handle, b.getData(), x.getData(), &converged, nullptr ) );
+      return converged != 0;
+   }
+
+   int getIterations() const
+   {
+      int32_t it = 0;
+      double r = 0;
+      detail::check( tnlhip_solver_get_status( handle, &it, &r ) );
+      return it;
+   }
+   double getResidue() const
+   {
+      int32_t it = 0;
+      double r = 0;
+      detail::check( tnlhip_solver_get_status( handle, &it, &r ) );
+      return r;
+   }
+   std::vector< double > getResidualHistory() const
+   {
+      const int64_t n = tnlhip_solver_get_history( handle, nullptr, 0 );
+      std::vector< double > h( n > 0 ? n : 0 );
+      if( n > 0 )
+         tnlhip_solver_get_history( handle, h.data(), n );
+      return h;
+   }
+
+protected:
+   tnlhip_solver handle = nullptr;
+   tnlhip_solver_params params;
+   MatrixPointer matrix;
+   PreconditionerPointer preconditioner;
+};
+
+template< typename Matrix >
+class CG : public LinearSolverBase< Matrix, TNLHIP_CG >
+{};
+
+template< typename Matrix >
+class BICGStab : public LinearSolverBase< Matrix, TNLHIP_BICGSTAB >
+{
+public:
+   bool setup( const Config::ParameterContainer& p, const std::string& prefix = "" ) override
+   {
+      if( p.checkParameter( prefix + "bicgstab-exact-residue" ) )
+         this->params.bicgstab_exact_residue = p.getInt( prefix + "bicgstab-exact-residue" );
+      return LinearSolverBase< Matrix, TNLHIP_BICGSTAB >::setup( p, prefix );
+   }
+};
+
+template< typename Matrix >
+class GMRES : public LinearSolverBase< Matrix, TNLHIP_GMRES >
+{
+public:
+   // GMRES.hpp:47-75: gmres-variant CGS | CGSR | MGS | MGSR | CWY, restarting parameters
+   bool setup( const Config::ParameterContainer& p, const std::string& prefix = "" ) override
+   {
+      if( p.checkParameter( prefix + "gmres-variant" ) ) {
+         const std::string v = p.getString( prefix + "gmres-variant" );
+         const char* names[] = { "CGS", "CGSR", "MGS", "MGSR", "CWY" };
+         int id = -1;
+         for( int i = 0; i < 5; i++ )
+            if( v == names[ i ] )
+               id = i;
+         if( id < 0 )
+            return false;
+         this->params.gmres_variant = id;
+      }
+      if( p.checkParameter( prefix + "gmres-restarting-min" ) )
+         this->params.restarting_min = p.getInt( prefix + "gmres-restarting-min" );
+      if( p.checkParameter( prefix + "gmres-restarting-max" ) )
+         this->params.restarting_max = p.getInt( prefix + "gmres-restarting-max" );
+      if( p.checkParameter( prefix + "gmres-restarting-step-min" ) )
+         this->params.restarting_step_min = p.getInt( prefix + "gmres-restarting-step-min" );
+      if( p.checkParameter( prefix + "gmres-restarting-step-max" ) )
+         this->params.restarting_step_max = p.getInt( prefix + "gmres-restarting-step-max" );
+      return LinearSolverBase< Matrix, TNLHIP_GMRES >::setup( p, prefix );
+   }
+   void setRestarting( int rmin, int rmax )
+   {
+      this->params.restarting_min = rmin;
+      this->params.restarting_max = rmax;
+   }
+};
+
+}  // namespace Linear
+}  // namespace Solvers
+}  // namespace TNL

@@ -1,0 +1,49 @@
+// Preconditioners::Diagonal< Matrix > (Jacobi), src/TNL/Solvers/Linear/Preconditioners/Diagonal.h.
+// On Devices::Hip the solver applies it inside its kernels (the division by the diagonal is
+// fused into the matrix-vector product); update() extracts the diagonal for inspection.
+#pragma once
+
+#include <memory>
+
+#include "../../../Containers/Vector.h"
+
+namespace TNL {
+namespace Solvers {
+namespace Linear {
+namespace Preconditioners {
+
+template< typename Matrix >
+class Preconditioner
+{
+public:
+   using MatrixPointer = std::shared_ptr< const Matrix >;
+   virtual ~Preconditioner() = default;
+   virtual void update( const MatrixPointer& ) {}
+   virtual int kind() const { return 0; }
+};
+
+template< typename Matrix >
+class Diagonal : public Preconditioner< Matrix >
+{
+public:
+   using MatrixPointer = std::shared_ptr< const Matrix >;
+   using VectorType = Containers::Vector< typename Matrix::VectorReal, Devices::Hip, int >;
+
+   // Diagonal.hpp:22-41: diagonal[i] = A(i, i)
+   void update( const MatrixPointer& matrix ) override
+   {
+      diagonal.setSize( matrix->getRows() );
+      detail::check( tnlhip_matrix_get_diagonal( matrix->getHandle(), diagonal.getData(), nullptr ) );
+      detail::checkHip( hipStreamSynchronize( nullptr ) );
+   }
+   int kind() const override { return 1; }
+   const VectorType& getDiagonal() const { return diagonal; }
+
+private:
+   VectorType diagonal;
+};
+
+}  // namespace Preconditioners
+}  // namespace Linear
+}  // namespace Solvers
+}  // namespace TNL

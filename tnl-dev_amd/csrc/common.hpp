@@ -157,8 +157,16 @@ inline int vectorDtype( int value_dtype )
 
 // Multiply and add with rounding after each operation, no contraction: this reproduces
 // TNL's Host sum `aux = aux + v * x` under a no-FMA policy bit for bit.
-__device__ __forceinline__ double mulAddExact( double acc, double a, double b ) { return __dadd_rn( acc, __dmul_rn( a, b ) ); }
-__device__ __forceinline__ float mulAddExact( float acc, float a, float b ) { return __fadd_rn( acc, __fmul_rn( a, b ) ); }
+__device__ __forceinline__ double mulAddExact( double acc, double a, double b )
+{
+#pragma clang fp contract( off )
+   return acc + a * b;
+}
+__device__ __forceinline__ float mulAddExact( float acc, float a, float b )
+{
+#pragma clang fp contract( off )
+   return acc + a * b;
+}
 
 // ------------------------------------------------------------------ wavefront reductions
 

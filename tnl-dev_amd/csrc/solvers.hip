@@ -290,6 +290,7 @@ void resetState( SolverImpl& S )
    State z;
    std::memset( &z, 0, sizeof( z ) );
    TNLHIP_CHECK( hipMemcpyAsync( S.state.ptr, &z, sizeof( State ), hipMemcpyHostToDevice, S.stream ) );
+   TNLHIP_CHECK( hipStreamSynchronize( S.stream ) );  // z lives on this stack frame
    const int64_t cap = std::min< int64_t >( (int64_t) S.params.max_iterations + 64, 1 << 20 );
    if( S.histCap != cap ) {
       S.hist.resize( cap );
@@ -304,13 +305,14 @@ void matvec( SolverImpl& S, const void* x, void* y, Gate gate )
                     S.stream );
 }
 
-// r = M^{-1} ( b - A x ): r = b; r = b - A x (keeper beta = 1, alpha = -1); r /= diag.
+// r = M^{-1} ( b - A x ) (compute_residue of BiCGStab / GMRES) or, with precondition =
+// false, r = b - A x (CG): r = b; r = b - A x (keeper beta = 1, alpha = -1); r /= diag.
 template< typename T >
-void residual( SolverImpl& S, const void* b, const void* x, T* r, Gate gate )
+void residual( SolverImpl& S, const void* b, const void* x, T* r, Gate gate, bool precondition = true )
 {
    launchElementwise( S.n, OpCopy< T >{ {}, r, (const T*) b }, gate, S.stream );
    spmvLaunchGated( *S.A, x, r, -1.0, 1.0, nullptr, gate.flags, gate.mode, S.stream );
-   if( S.params.preconditioner == 1 )
+   if( precondition && S.params.preconditioner == 1 )
       launchElementwise( S.n, OpDivide< T >{ {}, r, r, (const T*) S.diag.ptr }, gate, S.stream );
 }
 
@@ -487,7 +489,7 @@ void solveCG( SolverImpl& S, const void* b, void* xv )
    const Gate always{ nullptr, 0 }, running{ S.state.ptr->flags, 1 };
    const Stop stop = S.stopRule();
    computeNormb< T >( S, b, z );
-   residual< T >( S, b, x, r, always );
+   residual< T >( S, b, x, r, always, false );  // CG.hpp:40-41: r = b - A x
    launchReduce< 1 >( S.n, CgInitOp< T >{ {}, p, z, r, diag }, FinCgInit{ S.state.ptr, S.hist.ptr, S.histCap, stop },
                       S.ws.slot( 0 ), always, S.stream );
    runChunks( S, [ & ] {
