@@ -1,7 +1,9 @@
 // Config::ParameterContainer subset used by the solvers' setup() (src/TNL/Config/ParameterContainer.h).
 #pragma once
 
+#include <iomanip>
 #include <map>
+#include <sstream>
 #include <string>
 
 namespace TNL {
@@ -25,7 +27,12 @@ private:
    static std::string toString( const char* s ) { return s; }
    static std::string toString( bool b ) { return b ? "1" : "0"; }
    template< typename T >
-   static std::string toString( const T& v ) { return std::to_string( v ); }
+   static std::string toString( const T& v )
+   {
+      std::ostringstream o;
+      o << std::setprecision( 17 ) << v;
+      return o.str();
+   }
    std::map< std::string, std::string > values;
 };
 
