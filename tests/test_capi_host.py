@@ -74,6 +74,6 @@ def test_adaptive_blocks_follow_the_tnl_split(tnl):
     rp, _, _ = W.irregular(20000, seed=4)
     caps = np.diff(rp).astype(np.int32)
     got = tnl.segments_metadata("CSRAdaptive", caps, "adaptiveBlocks").reshape(-1, 6)
-    want = O.csr_adaptive_blocks(rp, 8, warp_size=64, block_size=256, streamed_shared_bytes=4 * 8192,
+    want = O.csr_adaptive_blocks(rp, 8, warp_size=64, block_size=256, streamed_shared_bytes=4 * 4096,
                                  max_adaptive_elements=1024)
     np.testing.assert_array_equal(got, want)

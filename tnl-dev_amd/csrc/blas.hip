@@ -12,9 +12,23 @@ __global__ __launch_bounds__( kReduceBlock ) void dotKernel( int64_t n, const T*
                                                              const T* __restrict__ b, ReduceSlot slot,
                                                              double* __restrict__ result, int sqrtResult )
 {
+   // each workgroup streams one contiguous range, 4 x 2 elements in flight per lane
    double acc[ 1 ] = { 0.0 };
-   const int64_t stride = (int64_t) gridDim.x * kReduceBlock;
-   for( int64_t i = (int64_t) blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += stride ) {
+   const int64_t per = ( ( n + gridDim.x - 1 ) / gridDim.x + 511 ) / 512 * 512;
+   const int64_t i0 = (int64_t) blockIdx.x * per, i1 = min( i0 + per, n );
+   int64_t i = i0 + threadIdx.x;
+   for( ; i + 3 * kReduceBlock < i1; i += 4 * kReduceBlock ) {
+      T av[ 4 ], bv[ 4 ];
+#pragma unroll
+      for( int j = 0; j < 4; j++ ) {
+         av[ j ] = __builtin_nontemporal_load( a + i + j * kReduceBlock );
+         bv[ j ] = SQUARE ? av[ j ] : __builtin_nontemporal_load( b + i + j * kReduceBlock );
+      }
+#pragma unroll
+      for( int j = 0; j < 4; j++ )
+         acc[ 0 ] += (double) av[ j ] * (double) bv[ j ];
+   }
+   for( ; i < i1; i += kReduceBlock ) {
       const double x = (double) a[ i ];
       acc[ 0 ] += SQUARE ? x * x : x * (double) b[ i ];
    }

@@ -85,9 +85,9 @@ void Matrix::uploadMetadata()
    TNLHIP_CHECK( hipDeviceSynchronize() );
 }
 
-// Wave64 parameters of the adaptive split: 256-lane workgroups, 8 KiB of LDS per wave for
-// STREAM blocks (1024 FP64 / 2048 FP32 products), VECTOR rows up to 2048 elements.
-constexpr int kAdaptiveStreamBytes = 4 * 8192;
+// Wave64 parameters of the adaptive split: 256-lane workgroups (4 waves), 4 KiB of LDS per
+// wave for STREAM blocks, VECTOR rows up to 2048 elements.
+constexpr int kAdaptiveStreamBytes = 4 * kAdaptiveStreamBytesPerWave;
 constexpr int kAdaptiveMaxElements = 1024;
 
 namespace {
@@ -453,8 +453,18 @@ int tnlhip_matrix_set_param( tnlhip_matrix A, const char* name_c, int32_t value 
                          "lanes_per_row applies to CSR light / hybrid / vector / adaptive fallback" );
          A->lanes = value;
       }
-      else if( name == "xcd_remap" )
-         A->xcdRemap = value != 0;
+      else if( name == "xcd_remap" ) {
+         // 2 (contiguous blocks without the XCD remap) is switched off: a run of it ended in a
+         // GPU memory fault whose cause is not established
+         TNLHIP_REQUIRE( value == 0 || value == 1, "xcd_remap is 0 (grid-stride) or 1 (XCD-remapped blocks)" );
+         A->xcdRemap = value;
+      }
+      else if( name == "pair_rows" )
+         A->pairRows = value != 0;
+      else if( name == "blocks_per_cu" ) {
+         TNLHIP_REQUIRE( value >= 1 && value <= 64, "blocks_per_cu must be in [1, 64]" );
+         A->blocksPerCu = value;
+      }
       else
          throw Error( TNLHIP_ERROR_INVALID_ARGUMENT, "unknown parameter " + name );
    } );

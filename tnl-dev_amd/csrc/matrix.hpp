@@ -67,6 +67,10 @@ void buildSegments( HostSegments& s, int kind, int32_t n, const int32_t* capacit
 std::vector< int32_t > buildAdaptiveBlocks( int32_t n, const int32_t* offsets, int value_size, int wave_size,
                                             int block_size, int streamed_bytes, int max_adaptive_elements );
 
+// LDS bytes per wavefront for CSR-Adaptive STREAM blocks (512 FP64 / 1024 FP32 products):
+// 4 KiB per wave keeps 32 waves per CU within the 160 KiB of LDS.
+constexpr int kAdaptiveStreamBytesPerWave = 4096;
+
 // Adaptive block types (detail/CSRAdaptiveKernelBlockDescriptor.h)
 enum { BLOCK_LONG = 0, BLOCK_STREAM = 1, BLOCK_VECTOR = 2 };
 
@@ -85,7 +89,9 @@ struct Matrix
    std::vector< int32_t > adaptiveHost;
    int32_t adaptiveBlocks = 0;
    int32_t lanes = 0;      // lanes per row for CSR light / hybrid / vector
-   int32_t xcdRemap = 0;
+   int32_t xcdRemap = 0;      // row distribution of the padded kernels: 0 grid-stride, 1 XCD-remapped blocks
+   int32_t pairRows = 0;      // padded kernels: two adjacent rows per lane with 8 / 16-byte loads
+   int32_t blocksPerCu = 16;  // workgroups per CU the grid of the padded kernels is sized for
    DeviceArray< uint8_t > diagonal;  // cached Jacobi diagonal (vector type)
    bool diagonalValid = false;
 

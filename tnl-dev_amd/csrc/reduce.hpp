@@ -10,7 +10,7 @@
 namespace tnlhip {
 
 constexpr int kReduceBlock = 256;
-constexpr int kReduceGrid = 1024;  // partials per reduction (4 workgroups per CU on 256 CUs)
+constexpr int kReduceGrid = 2048;  // partials per reduction (8 workgroups per CU on 256 CUs)
 constexpr int kMaxMulti = 32;      // columns per multi-reduction pass
 
 // Scratch of one reduction slot: partials, ticket counter.

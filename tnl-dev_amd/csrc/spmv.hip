@@ -57,6 +57,53 @@ __device__ __forceinline__ A madd( A acc, A a, A b )
       return acc + a * b;
 }
 
+// Streaming loads of the matrix arrays bypass the caches' normal retention (non-temporal),
+// so the 256 MiB MALL and the L2 keep x, which is gathered many times.
+__device__ __forceinline__ int32_t ldnt( const int32_t* p ) { return __builtin_nontemporal_load( p ); }
+__device__ __forceinline__ double ldnt( const double* p ) { return __builtin_nontemporal_load( p ); }
+__device__ __forceinline__ float ldnt( const float* p ) { return __builtin_nontemporal_load( p ); }
+__device__ __forceinline__ bf16 ldnt( const bf16* p )
+{
+   return __ushort_as_bfloat16( __builtin_nontemporal_load( reinterpret_cast< const unsigned short* >( p ) ) );
+}
+
+// Workgroup -> row block mapping.  Workgroups are dispatched round-robin over the 8 XCDs;
+// with `remap` the logical index puts consecutive logical blocks on the same XCD, so each
+// XCD streams one contiguous eighth of the rows and x's neighbourhoods stay in its own L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t logicalBlock( int remap )
+{
+   if( ! remap )
+      return blockIdx.x;
+   const unsigned per = gridDim.x / kXcds;
+   return (int64_t) ( blockIdx.x % kXcds ) * per + blockIdx.x / kXcds;
+}
+
+// Row sequence of a lane taking P consecutive rows at a time.  dist 0: grid-stride over the
+// whole range; 1: each workgroup owns `per` consecutive rows, logical blocks remapped so an
+// XCD owns a contiguous eighth.
+struct RowSeq
+{
+   int64_t start, end, step;
+};
+template< int P >
+__device__ __forceinline__ RowSeq rowSequence( int64_t base, int64_t last, int64_t per, int dist )
+{
+   if( dist == 0 )
+      return RowSeq{ base + ( (int64_t) blockIdx.x * blockDim.x + threadIdx.x ) * P, last,
+                     (int64_t) gridDim.x * blockDim.x * P };
+   const int64_t r0 = base + logicalBlock( 1 ) * per;
+   return RowSeq{ r0 + (int64_t) threadIdx.x * P, min( r0 + per, last ), (int64_t) blockDim.x * P };
+}
+
+// Wave-local LDS handoff between lanes of one wavefront (no workgroup barrier).
+__device__ __forceinline__ void waveSync()
+{
+   __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+   __builtin_amdgcn_wave_barrier();
+   __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+}
+
 // ------------------------------------------------------------------ CSR: G lanes per row
 
 // G = 1: CSRScalar (one lane per row, exact Host order); G = 2..32: CSRLight / CSRHybrid;
@@ -90,11 +137,13 @@ __global__ __launch_bounds__( 256 ) void csrGroupedKernel( const int32_t* __rest
 
 // ------------------------------------------------------------------ CSR-Adaptive
 
-// One wavefront per block descriptor (Kernels/CSRAdaptiveKernelView.hpp:33-170 for the
-// scheme).  STREAM: the wave streams the elements of several short rows into its LDS
-// slice with fully coalesced loads, then every lane sums whole rows out of LDS.
-// VECTOR: one row per wave.  LONG: one row split over the waves left in the workgroup,
-// partial sums combined in LDS in a fixed order.
+// One wavefront per block descriptor (the scheme of Kernels/CSRAdaptiveKernelView.hpp:33-170,
+// with wave64 budgets).  STREAM: the wave loads the elements of several short rows with fully
+// coalesced, unrolled loads (up to STREAM/64 loads in flight per lane), keeps the products in
+// its own LDS slice, then every lane sums whole rows out of LDS.  VECTOR: one row per wave.
+// LONG: one row split over the waves left in the workgroup, partial sums combined in LDS in
+// a fixed order.  One descriptor per wave and no grid-stride loop, so the single workgroup
+// barrier (needed only by LONG rows) is uniform.
 template< int STREAM, typename V, typename A, typename Y >
 __global__ __launch_bounds__( 256 ) void csrAdaptiveKernel( int32_t nblocks, const int4* __restrict__ desc,
                                                             const int32_t* __restrict__ off,
@@ -104,116 +153,240 @@ __global__ __launch_bounds__( 256 ) void csrAdaptiveKernel( int32_t nblocks, con
 {
    if( keep.closed() )
       return;
+   constexpr int K = STREAM / kWave;
    __shared__ A lds[ 4 ][ STREAM ];
    __shared__ A longPart[ 4 ];
    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-   const int32_t groups = ( nblocks + 3 ) / 4;
-   for( int32_t wg = blockIdx.x; wg < groups; wg += gridDim.x ) {
-      const int32_t di = wg * 4 + wave;
-      const bool active = di < nblocks;
-      const int4 d = active ? desc[ di ] : make_int4( 0, 0, 0, 0 );
-      const int type = d.x & 0xff;
-      const int32_t firstRow = d.y;
-      A vsum = 0;
-      if( active ) {
-         const int32_t b = off[ firstRow ];
-         if( type == BLOCK_STREAM ) {
-            for( int32_t e = lane; e < d.z; e += kWave ) {
-               const int32_t c = col[ b + e ];
-               lds[ wave ][ e ] = c >= 0 ? toCompute( val[ b + e ], A() ) * (A) x[ c ] : (A) 0;
-            }
-         }
-         else {
-            const int32_t e = off[ firstRow + 1 ];
-            const int32_t widx = type == BLOCK_LONG ? ( d.x >> 8 ) & 0xff : 0;
-            const int32_t wcnt = type == BLOCK_LONG ? ( d.x >> 16 ) & 0xff : 1;
-            A p = 0;
-            for( int32_t k = b + widx * kWave + lane; k < e; k += kWave * wcnt ) {
-               const int32_t c = col[ k ];
-               if( c >= 0 )
-                  p += toCompute( val[ k ], A() ) * (A) x[ c ];
-            }
-            vsum = groupSum< kWave >( p );
-            if( type == BLOCK_LONG && lane == 0 )
-               longPart[ wave ] = vsum;
-         }
+   const int32_t di = blockIdx.x * 4 + wave;
+   const bool active = di < nblocks;
+   const int4 d = active ? desc[ di ] : make_int4( BLOCK_VECTOR, 0, 0, 0 );
+   const int type = d.x & 0xff;
+   const int32_t firstRow = d.y;
+   const int32_t b = active ? off[ firstRow ] : 0;
+   A vsum = 0;
+   if( active && type == BLOCK_STREAM && d.z == 0 ) {
+      for( int32_t r = lane; r < d.w; r += kWave )  // a block of empty rows
+         keep( firstRow + r, (A) 0 );
+   }
+   else if( active && type == BLOCK_STREAM ) {
+      const int32_t size = d.z;
+      // offsets of the first 64 rows of the block, fetched alongside the element stream
+      const int32_t rbPre = lane <= d.w ? off[ firstRow + lane ] : 0;
+      int32_t c[ K ];
+      V v[ K ];
+#pragma unroll
+      for( int k = 0; k < K; k++ ) {
+         const int32_t e = min( lane + k * kWave, size - 1 );
+         c[ k ] = ldnt( col + b + e );
+         v[ k ] = ldnt( val + b + e );
       }
-      __syncthreads();
-      if( active ) {
-         if( type == BLOCK_STREAM ) {
-            const int32_t b = off[ firstRow ];
-            for( int32_t r = lane; r < d.w; r += kWave ) {
-               const int32_t rb = off[ firstRow + r ] - b, re = off[ firstRow + r + 1 ] - b;
-               A s = 0;
-               for( int32_t k = rb; k < re; k++ )
-                  s += lds[ wave ][ k ];
-               keep( firstRow + r, s );
-            }
-         }
-         else if( type == BLOCK_VECTOR ) {
-            if( lane == 0 )
-               keep( firstRow, vsum );
-         }
-         else if( ( ( d.x >> 8 ) & 0xff ) == 0 && lane == 0 ) {
-            const int32_t wcnt = ( d.x >> 16 ) & 0xff;
-            A s = 0;
-            for( int32_t w = 0; w < wcnt; w++ )
-               s += longPart[ wave + w ];
-            keep( firstRow, s );
-         }
+      Y xv[ K ];
+#pragma unroll
+      for( int k = 0; k < K; k++ )
+         xv[ k ] = x[ c[ k ] < 0 ? 0 : c[ k ] ];
+#pragma unroll
+      for( int k = 0; k < K; k++ ) {
+         const int32_t e = lane + k * kWave;
+         if( e < size )
+            lds[ wave ][ e ] = c[ k ] >= 0 ? toCompute( v[ k ], A() ) * (A) xv[ k ] : (A) 0;
       }
-      __syncthreads();
+      waveSync();
+      const int32_t reNext = __shfl( rbPre, min( lane + 1, kWave - 1 ), kWave );
+      for( int32_t r = lane; r < d.w; r += kWave ) {
+         const int32_t rb = ( r < kWave ? rbPre : off[ firstRow + r ] ) - b;
+         const int32_t re = ( r + 1 < kWave ? reNext : off[ firstRow + r + 1 ] ) - b;
+         A s = 0;
+         for( int32_t k = rb; k < re; k++ )
+            s += lds[ wave ][ k ];
+         keep( firstRow + r, s );
+      }
+   }
+   else if( active ) {
+      const int32_t e = off[ firstRow + 1 ];
+      const int32_t widx = type == BLOCK_LONG ? ( d.x >> 8 ) & 0xff : 0;
+      const int32_t wcnt = type == BLOCK_LONG ? ( d.x >> 16 ) & 0xff : 1;
+      A p = 0;
+      for( int32_t k = b + widx * kWave + lane; k < e; k += kWave * wcnt ) {
+         const int32_t c = ldnt( col + k );
+         if( c >= 0 )
+            p += toCompute( ldnt( val + k ), A() ) * (A) x[ c ];
+      }
+      vsum = groupSum< kWave >( p );
+      if( type == BLOCK_VECTOR && lane == 0 )
+         keep( firstRow, vsum );
+      if( type == BLOCK_LONG && lane == 0 )
+         longPart[ wave ] = vsum;
+   }
+   __syncthreads();
+   if( active && type == BLOCK_LONG && ( ( d.x >> 8 ) & 0xff ) == 0 && lane == 0 ) {
+      const int32_t wcnt = ( d.x >> 16 ) & 0xff;
+      A s = 0;
+      for( int32_t w = 0; w < wcnt; w++ )
+         s += longPart[ wave + w ];
+      keep( firstRow, s );
    }
 }
 
 // ------------------------------------------------------------------ Ellpack / SlicedEllpack
 
 // One lane per row, column-major storage: consecutive lanes read consecutive addresses
-// for every local index (EllpackView.hpp:361-378, SlicedEllpackView.hpp:248-290).
-// Padding is trailing in each row, so predication on column >= 0 gives the Host sum.
-template< typename V, typename A, typename Y >
-__global__ __launch_bounds__( 256 ) void ellpackKernel( int32_t segmentSize, int32_t aligned,
-                                                        const int32_t* __restrict__ col,
-                                                        const V* __restrict__ val, const Y* __restrict__ x,
-                                                        Keeper< A, Y > keep, int32_t first, int32_t last )
+// for every local index (EllpackView.hpp:361-378, SlicedEllpackView.hpp:248-290).  The
+// slots of a row are read eight at a time with unconditional loads (indexes past the row
+// end are clamped to its last slot, padding gathers x[0]), so 24 loads per lane are in
+// flight; the sum still runs left to right over the non-padding slots only, which is the
+// Host result bit for bit.  Each workgroup owns `rowsPerBlock` consecutive rows.
+constexpr int kChunk = 8;
+
+// Two vertically adjacent slots (rows r, r+1 at one local index) are one aligned 8-byte
+// index load and one 2-element value load.
+typedef double double2v __attribute__( ( ext_vector_type( 2 ) ) );
+typedef float float2v __attribute__( ( ext_vector_type( 2 ) ) );
+template< typename V >
+struct Pair;
+template<>
+struct Pair< double >
 {
-   if( keep.closed() )
-      return;
-   const int64_t stride = (int64_t) gridDim.x * blockDim.x;
-   for( int64_t r = first + (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < last; r += stride ) {
-      A s = 0;
-#pragma unroll 4
-      for( int32_t l = 0; l < segmentSize; l++ ) {
-         const int64_t g = r + (int64_t) aligned * l;
-         const int32_t c = col[ g ];
-         if( c >= 0 )
-            s = mulAddExact( s, toCompute( val[ g ], A() ), (A) x[ c ] );
+   using T = double2v;
+};
+template<>
+struct Pair< float >
+{
+   using T = float2v;
+};
+template<>
+struct Pair< bf16 >
+{
+   using T = unsigned int;
+};
+__device__ __forceinline__ void split( double2v p, double& a, double& b )
+{
+   a = p.x;
+   b = p.y;
+}
+__device__ __forceinline__ void split( float2v p, float& a, float& b )
+{
+   a = p.x;
+   b = p.y;
+}
+__device__ __forceinline__ void split( unsigned int p, bf16& a, bf16& b )
+{
+   a = __ushort_as_bfloat16( (unsigned short) ( p & 0xffff ) );
+   b = __ushort_as_bfloat16( (unsigned short) ( p >> 16 ) );
+}
+template< typename V >
+__device__ __forceinline__ typename Pair< V >::T ldnt2( const V* p )
+{
+   using P = typename Pair< V >::T;
+   return __builtin_nontemporal_load( reinterpret_cast< const P* >( p ) );
+}
+__device__ __forceinline__ int2 ldnt2( const int32_t* p )
+{
+   const long long v = __builtin_nontemporal_load( reinterpret_cast< const long long* >( p ) );
+   return make_int2( (int32_t) ( v & 0xffffffff ), (int32_t) ( v >> 32 ) );
+}
+
+// One or two (PAIR) rows per lane.  Slot index of row r at local index l: base( r ) + step * l.
+template< bool PAIR, typename V, typename A, typename Y, typename Slots >
+__device__ __forceinline__ void paddedRows( const Slots& slots, const int32_t* __restrict__ col,
+                                            const V* __restrict__ val, const Y* __restrict__ x, Keeper< A, Y > keep,
+                                            int32_t first, RowSeq seq )
+{
+   for( int64_t r = seq.start; r < seq.end; r += seq.step ) {
+      const int64_t b = slots.base( r );
+      const int32_t width = slots.width( r );
+      const int64_t step = slots.step();
+      A s0 = 0, s1 = 0;
+      for( int32_t l0 = 0; l0 < width; l0 += kChunk ) {
+         if constexpr( PAIR ) {
+            int2 c[ kChunk ];
+            typename Pair< V >::T v[ kChunk ];
+#pragma unroll
+            for( int j = 0; j < kChunk; j++ ) {
+               const int64_t g = b + step * min( l0 + j, width - 1 );
+               c[ j ] = ldnt2( col + g );
+               v[ j ] = ldnt2( val + g );
+            }
+            Y x0[ kChunk ], x1[ kChunk ];
+#pragma unroll
+            for( int j = 0; j < kChunk; j++ ) {
+               x0[ j ] = x[ c[ j ].x < 0 ? 0 : c[ j ].x ];
+               x1[ j ] = x[ c[ j ].y < 0 ? 0 : c[ j ].y ];
+            }
+#pragma unroll
+            for( int j = 0; j < kChunk; j++ ) {
+               V a, bb;
+               split( v[ j ], a, bb );
+               if( l0 + j < width && c[ j ].x >= 0 )
+                  s0 = mulAddExact( s0, toCompute( a, A() ), (A) x0[ j ] );
+               if( l0 + j < width && c[ j ].y >= 0 )
+                  s1 = mulAddExact( s1, toCompute( bb, A() ), (A) x1[ j ] );
+            }
+         }
+         else {
+            int32_t c[ kChunk ];
+            V v[ kChunk ];
+#pragma unroll
+            for( int j = 0; j < kChunk; j++ ) {
+               const int64_t g = b + step * min( l0 + j, width - 1 );
+               c[ j ] = ldnt( col + g );
+               v[ j ] = ldnt( val + g );
+            }
+            Y xv[ kChunk ];
+#pragma unroll
+            for( int j = 0; j < kChunk; j++ )
+               xv[ j ] = x[ c[ j ] < 0 ? 0 : c[ j ] ];
+#pragma unroll
+            for( int j = 0; j < kChunk; j++ )
+               if( l0 + j < width && c[ j ] >= 0 )
+                  s0 = mulAddExact( s0, toCompute( v[ j ], A() ), (A) xv[ j ] );
+         }
       }
-      keep( r, s );
+      if( r >= first )
+         keep( r, s0 );
+      if( PAIR && r + 1 < seq.end )
+         keep( r + 1, s1 );
    }
 }
 
-template< typename V, typename A, typename Y >
-__global__ __launch_bounds__( 256 ) void slicedKernel( const int32_t* __restrict__ sliceOffsets,
-                                                       const int32_t* __restrict__ col,
-                                                       const V* __restrict__ val, const Y* __restrict__ x,
-                                                       Keeper< A, Y > keep, int32_t first, int32_t last )
+struct EllpackSlots
+{
+   int32_t segmentSize, aligned;
+   __device__ int64_t base( int64_t r ) const { return r; }
+   __device__ int32_t width( int64_t ) const { return segmentSize; }
+   __device__ int64_t step() const { return aligned; }
+};
+
+struct SlicedSlots
+{
+   const int32_t* sliceOffsets;
+   const int32_t* sliceSizes;
+   __device__ int64_t base( int64_t r ) const { return sliceOffsets[ r / kSliceSize ] + r % kSliceSize; }
+   __device__ int32_t width( int64_t r ) const { return sliceSizes[ r / kSliceSize ]; }
+   __device__ int64_t step() const { return kSliceSize; }
+};
+
+template< bool PAIR, typename V, typename A, typename Y >
+__global__ __launch_bounds__( 256 ) void ellpackKernel( EllpackSlots slots, const int32_t* __restrict__ col,
+                                                        const V* __restrict__ val, const Y* __restrict__ x,
+                                                        Keeper< A, Y > keep, int32_t first, int32_t last,
+                                                        int32_t rowsPerBlock, int dist )
 {
    if( keep.closed() )
       return;
-   const int64_t stride = (int64_t) gridDim.x * blockDim.x;
-   for( int64_t r = first + (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < last; r += stride ) {
-      const int32_t sl = (int32_t) ( r / kSliceSize );
-      const int32_t b = sliceOffsets[ sl ] + (int32_t) ( r % kSliceSize ), e = sliceOffsets[ sl + 1 ];
-      A s = 0;
-#pragma unroll 4
-      for( int32_t g = b; g < e; g += kSliceSize ) {
-         const int32_t c = col[ g ];
-         if( c >= 0 )
-            s = mulAddExact( s, toCompute( val[ g ], A() ), (A) x[ c ] );
-      }
-      keep( r, s );
-   }
+   const int64_t base = PAIR ? ( first & ~1 ) : first;
+   paddedRows< PAIR >( slots, col, val, x, keep, first, rowSequence< PAIR ? 2 : 1 >( base, last, rowsPerBlock, dist ) );
+}
+
+template< bool PAIR, typename V, typename A, typename Y >
+__global__ __launch_bounds__( 256 ) void slicedKernel( SlicedSlots slots, const int32_t* __restrict__ col,
+                                                       const V* __restrict__ val, const Y* __restrict__ x,
+                                                       Keeper< A, Y > keep, int32_t first, int32_t last,
+                                                       int32_t rowsPerBlock, int dist )
+{
+   if( keep.closed() )
+      return;
+   const int64_t base = PAIR ? ( first & ~1 ) : first;
+   paddedRows< PAIR >( slots, col, val, x, keep, first, rowSequence< PAIR ? 2 : 1 >( base, last, rowsPerBlock, dist ) );
 }
 
 // ------------------------------------------------------------------ ChunkedEllpack
@@ -238,11 +411,23 @@ __global__ __launch_bounds__( 256 ) void chunkedKernel( int32_t nslices, const i
       if( s.z >= last || s.z + s.x <= first )
          continue;  // uniform over the workgroup
       A p = 0;
-      for( int32_t k = 0; k < s.y; k++ ) {
-         const int64_t g = (int64_t) s.w + t + (int64_t) kChunksInSlice * k;
-         const int32_t c = col[ g ];
-         if( c >= 0 )
-            p += toCompute( val[ g ], A() ) * (A) x[ c ];
+      for( int32_t k0 = 0; k0 < s.y; k0 += kChunk ) {
+         int32_t c[ kChunk ];
+         V v[ kChunk ];
+#pragma unroll
+         for( int j = 0; j < kChunk; j++ ) {
+            const int64_t g = (int64_t) s.w + t + (int64_t) kChunksInSlice * min( k0 + j, s.y - 1 );
+            c[ j ] = ldnt( col + g );
+            v[ j ] = ldnt( val + g );
+         }
+         Y xv[ kChunk ];
+#pragma unroll
+         for( int j = 0; j < kChunk; j++ )
+            xv[ j ] = x[ c[ j ] < 0 ? 0 : c[ j ] ];
+#pragma unroll
+         for( int j = 0; j < kChunk; j++ )
+            if( k0 + j < s.y && c[ j ] >= 0 )
+               p += toCompute( v[ j ], A() ) * (A) xv[ j ];
       }
       part[ t ] = p;
       __syncthreads();
@@ -292,10 +477,23 @@ __global__ __launch_bounds__( 256 ) void biellpackKernel( int32_t n, int32_t str
             const int32_t base = gp[ grp ], size = gp[ grp + 1 ] - base;
             const int h = kStrip >> grp;
             A p = 0;
-            for( int32_t e = l; e < size; e += kStrip ) {
-               const int32_t c = col[ base + e ];
-               if( c >= 0 )
-                  p += toCompute( val[ base + e ], A() ) * (A) x[ c ];
+            for( int32_t e0 = l; e0 < size; e0 += 4 * kStrip ) {
+               int32_t c[ 4 ];
+               V v[ 4 ];
+#pragma unroll
+               for( int j = 0; j < 4; j++ ) {
+                  const int32_t e = min( e0 + j * kStrip, size - 1 );
+                  c[ j ] = ldnt( col + base + e );
+                  v[ j ] = ldnt( val + base + e );
+               }
+               Y xv[ 4 ];
+#pragma unroll
+               for( int j = 0; j < 4; j++ )
+                  xv[ j ] = x[ c[ j ] < 0 ? 0 : c[ j ] ];
+#pragma unroll
+               for( int j = 0; j < 4; j++ )
+                  if( e0 + j * kStrip < size && c[ j ] >= 0 )
+                     p += toCompute( v[ j ], A() ) * (A) xv[ j ];
             }
             for( int o = h; o < kStrip; o <<= 1 )
                p += __shfl_xor( p, o, kWave );
@@ -353,9 +551,9 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
       case TNLHIP_CSR_HYBRID:
       case TNLHIP_CSR_ADAPTIVE: {
          if( M.format == TNLHIP_CSR_ADAPTIVE && first == 0 && last == M.rows && M.adaptiveBlocks > 0 ) {
-            constexpr int STREAM = 8192 / sizeof( A );
-            const int32_t groups = ( M.adaptiveBlocks + 3 ) / 4;
-            csrAdaptiveKernel< STREAM, V, A, Y ><<< gridFor( groups, 1, 8 ), 256, 0, st >>>(
+            constexpr int STREAM = kAdaptiveStreamBytesPerWave / sizeof( A );
+            const unsigned groups = ( M.adaptiveBlocks + 3 ) / 4;
+            csrAdaptiveKernel< STREAM, V, A, Y ><<< groups, 256, 0, st >>>(
                M.adaptiveBlocks, reinterpret_cast< const int4* >( M.adaptive.ptr ), M.offsets.ptr, col, val, x, keep );
             break;
          }
@@ -388,13 +586,35 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
          break;
       }
       case TNLHIP_ELLPACK:
-         ellpackKernel< V, A, Y ><<< gridFor( rows, 256 ), 256, 0, st >>>( M.meta.segmentSize, M.meta.alignedSize, col,
-                                                                            val, x, keep, first, last );
+      case TNLHIP_SLICED_ELLPACK: {
+         // grid a multiple of the XCD count; contiguous row blocks when dist != 0
+         const bool pair = M.pairRows != 0;
+         const int rowsPerThread = pair ? 2 : 1;
+         unsigned grid = gridFor( rows, 256 * rowsPerThread, M.blocksPerCu );
+         grid = ( grid + kXcds - 1 ) / kXcds * kXcds;
+         const int64_t span = last - ( pair ? ( first & ~1 ) : first );
+         const int64_t unit = 256 * rowsPerThread;
+         const int64_t per = ( ( span + grid - 1 ) / grid + unit - 1 ) / unit * unit;
+         const EllpackSlots es{ M.meta.segmentSize, M.meta.alignedSize };
+         const SlicedSlots ss{ M.sliceOffsets.ptr, M.sliceSizes.ptr };
+         if( M.format == TNLHIP_ELLPACK ) {
+            if( pair )
+               ellpackKernel< true, V, A, Y ><<< grid, 256, 0, st >>>( es, col, val, x, keep, first, last, (int32_t) per,
+                                                                        M.xcdRemap );
+            else
+               ellpackKernel< false, V, A, Y ><<< grid, 256, 0, st >>>( es, col, val, x, keep, first, last,
+                                                                         (int32_t) per, M.xcdRemap );
+         }
+         else {
+            if( pair )
+               slicedKernel< true, V, A, Y ><<< grid, 256, 0, st >>>( ss, col, val, x, keep, first, last, (int32_t) per,
+                                                                       M.xcdRemap );
+            else
+               slicedKernel< false, V, A, Y ><<< grid, 256, 0, st >>>( ss, col, val, x, keep, first, last,
+                                                                        (int32_t) per, M.xcdRemap );
+         }
          break;
-      case TNLHIP_SLICED_ELLPACK:
-         slicedKernel< V, A, Y >
-            <<< gridFor( rows, 256 ), 256, 0, st >>>( M.sliceOffsets.ptr, col, val, x, keep, first, last );
-         break;
+      }
       case TNLHIP_CHUNKED_ELLPACK:
          if( M.meta.numberOfSlices > 0 )
             chunkedKernel< V, A, Y ><<< gridFor( M.meta.numberOfSlices, 1 ), 256, 0, st >>>(
