@@ -28,6 +28,9 @@ sys.path.insert(0, os.path.join(ROOT, "tnl-dev_amd"))
 
 HBM_PEAK_GBS = 8000.0
 MALL_BYTES = 256 << 20
+# HBM bytes per launch at 160^3 FP64 from rocprofv3 --pmc FETCH_SIZE (x2, the factor a plain
+# 512 MiB stream reads) and WRITE_SIZE runs: profiles/spmv_laplacian160_traffic_pmc.md
+MEASURED_TRAFFIC = {"CSRAdaptive": 488.8e6, "Ellpack": 484.8e6, "ChunkedEllpack": 507.0e6}
 FORMATS = ["CSRScalar", "CSRVector", "CSRLight", "CSRHybrid", "CSRAdaptive", "Ellpack", "SlicedEllpack",
            "ChunkedEllpack", "BiEllpack"]
 HEADLINE = "CSRAdaptive"
@@ -149,18 +152,18 @@ def main():
                                   "frac_hbm": round(B32 / mean32 / 1e9 / HBM_PEAK_GBS, 4)}
     del A32
 
-    # calibration: what plain streaming reaches on this device (512 MiB read by the library's
-    # deterministic dot kernel; 512 MiB copy by torch)
+    # calibration: what plain streaming reaches on this device (512 MiB read by torch.sum,
+    # 512 MiB copy by torch)
     big = torch.rand(64 << 20, dtype=torch.float64, device=dev)
     dst = torch.empty_like(big)
     for _ in range(2):
-        tnl.lpNorm2(big)
+        big.sum()
         dst.copy_(big)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(5):
-        tnl.lpNorm2(big)
+        big.sum()
     e1.record()
     e1.synchronize()
     read_gbs = 5 * big.numel() * 8 / (e0.elapsed_time(e1) * 1e-3) / 1e9
@@ -190,10 +193,10 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"3D 7-point Laplacian {N}^3 (n={n}, nnz={nnz}), FP64 values, int32 indices, "
-                               f"SpMV y = A x, x ~ U[-1,1], {head_fmt} kernel, cold MALL",
+                               f"SpMV y = A x, x ~ U[-1,1], {head_fmt} kernel, back-to-back launches",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": head["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": head["frac_hbm"], "traffic": None,
+                     "frac": head["frac_hbm"], "traffic": MEASURED_TRAFFIC.get(head_fmt) if N == 160 else None,
                      "algorithmic_bytes_per_launch": B},
         "formats": results,
         "calibration": {"stream_read_GB/s": round(read_gbs, 1), "copy_GB/s": round(copy_gbs, 1)},
