@@ -92,6 +92,57 @@ def cpu_baseline(rp, cols, vals, x, ncols):
             "seconds_per_spmv": t}
 
 
+def timed(torch, fn):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = fn()
+    torch.cuda.synchronize()
+    return r, time.perf_counter() - t0
+
+
+def solver_suite(tnl, torch, W, dev, args):
+    """Convection-diffusion 7-point upwind (v = (1, 0.5, 0.25)) on a cd_grid^3 box, Jacobi
+    GMRES(30) (CWY) to 1e-10: plain FP64 against FP64 refinement over an FP32 and a BF16 inner
+    solve, for each Peclet number."""
+    N = args.cd_grid
+    res = {"workload": f"3D convection-diffusion upwind {N}^3, Jacobi GMRES(30) CWY, tol 1e-10, x0 = 0"}
+    for pe in [float(p) for p in args.peclet.split(",")]:
+        rp, c, v = W.convection_diffusion_3d(N, pe)
+        n = len(rp) - 1
+        b = torch.tensor(W.uniform_vector(n, 77), device=dev)
+        A = tnl.SparseMatrix.from_csr(rp, c, v, format="CSRAdaptive", dtype="f64")
+        entry = {}
+        s = tnl.GMRES()
+        s.setMatrix(A)
+        s.setup(convergence_residue=1e-10, max_iterations=200000, restarting_min=30, restarting_max=30,
+                gmres_variant="CWY", preconditioner=1)
+        x = torch.zeros(n, dtype=torch.float64, device=dev)
+        ok, t64 = timed(torch, lambda: s.solve(b, x))
+        entry["fp64"] = {"converged": ok, "iterations": s.getIterations(), "seconds": round(t64, 4),
+                         "residue": s.getResidue()}
+        log(f"Pe={pe:g} FP64 GMRES: {s.getIterations()} iterations, {t64:.3f} s, converged={ok}")
+        x64 = x.clone()
+        for dt in ("f32", "bf16"):
+            A_low = A.convert("CSRAdaptive", dt)
+            x = torch.zeros(n, dtype=torch.float64, device=dev)
+            r, t = timed(torch, lambda: tnl.refine_solve(A, b, x, A_low=A_low, tolerance=1e-10, inner_method="gmres",
+                                                         inner_dtype=dt, convergence_residue=1e-4,
+                                                         max_iterations=200000, restarting_min=30,
+                                                         restarting_max=30, gmres_variant="CWY", preconditioner=1))
+            err = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))
+            entry[dt + "_refined"] = {"converged": r["converged"], "outer": r["outer_iterations"],
+                                      "inner_iterations": r["inner_iterations"], "seconds": round(t, 4),
+                                      "residue": r["residue"], "speedup_vs_fp64": round(t64 / t, 3),
+                                      "rel_error_vs_fp64_solve": err}
+            log(f"Pe={pe:g} {dt} refined: outer {r['outer_iterations']} inner {r['inner_iterations']} "
+                f"{t:.3f} s speedup {t64 / t:.2f}x residue {r['residue']:.2e} err vs FP64 {err:.2e}")
+            del A_low
+        res[f"Pe={pe:g}"] = entry
+        del A
+        torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +151,9 @@ def main():
     ap.add_argument("--formats", default="all")
     ap.add_argument("--grid", type=int, default=160)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solvers", action="store_true")
+    ap.add_argument("--cd-grid", type=int, default=128, help="convection-diffusion grid side")
+    ap.add_argument("--peclet", default="1,10,100")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -201,6 +255,10 @@ def main():
         "formats": results,
         "calibration": {"stream_read_GB/s": round(read_gbs, 1), "copy_GB/s": round(copy_gbs, 1)},
     }
+    if not args.no_solvers:
+        del x, y, flush
+        torch.cuda.empty_cache()
+        out["solvers"] = solver_suite(tnl, torch, W, dev, args)
     if not args.no_cpu:
         log("timing the CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(rp, cols, vals, x_host, n)

@@ -233,6 +233,41 @@ int tnlhip_solver_get_status( tnlhip_solver solver, int32_t* iterations, double*
  * file of IterativeSolver.hpp:189-200); returns the count, out may be NULL. */
 int64_t tnlhip_solver_get_history( tnlhip_solver solver, double* out, int64_t capacity );
 
+/* ------------------------------------------------------------------ mixed-precision refinement */
+
+/* FP64 iterative refinement around a low-precision inner Krylov solve.  No counterpart in the
+ * reference (TNL's solvers run in one precision); the inner solver is the CG / BiCGStab /
+ * GMRES of this library.  Outer loop: r = b - A x in FP64, inner solve of A_low d = r/|r| in
+ * FP32 vectors (A_low values FP32 or BF16) to inner.convergence_residue, x += |r| d in FP64,
+ * until |r| / |b| < tolerance. */
+#define TNLHIP_REFINEMENT_HISTORY 64
+
+typedef struct
+{
+   double tolerance;             /* outer FP64 relative residual target */
+   int32_t max_outer_iterations;
+   int32_t inner_method;         /* tnlhip_method */
+   int32_t inner_value_dtype;    /* TNLHIP_F32 or TNLHIP_BF16 */
+   int32_t reserved;
+   tnlhip_solver_params inner;   /* inner stopping rule, GMRES variant / restart, preconditioner */
+} tnlhip_refinement_params;
+
+typedef struct
+{
+   int32_t converged;
+   int32_t outer_iterations;
+   int64_t inner_iterations;     /* summed over the outer iterations */
+   double residue;               /* final FP64 |b - A x| / |b| */
+   double history[ TNLHIP_REFINEMENT_HISTORY ]; /* FP64 residue before each outer step */
+} tnlhip_refinement_result;
+
+void tnlhip_refinement_params_default( tnlhip_refinement_params* params );
+
+/* A: FP64 matrix; A_low: the same matrix in FP32 / BF16 (NULL: converted internally from A,
+ * same format); b, x: FP64 device vectors, x holds the initial guess.  Synchronous. */
+int tnlhip_refine_solve( tnlhip_matrix A, tnlhip_matrix A_low, const tnlhip_refinement_params* params,
+                         const double* b, double* x, tnlhip_refinement_result* result, void* stream );
+
 #ifdef __cplusplus
 }
 #endif

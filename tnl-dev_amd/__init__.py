@@ -31,7 +31,8 @@ EXPORTS = [
     "tnlhip_matrix_set_param", "tnlhip_spmv", "tnlhip_dot", "tnlhip_norm2", "tnlhip_axpby", "tnlhip_multidot",
     "tnlhip_gemv", "tnlhip_exclusive_scan_i32", "tnlhip_solver_params_default", "tnlhip_solver_create",
     "tnlhip_solver_destroy", "tnlhip_solver_set_matrix", "tnlhip_solver_set_params", "tnlhip_solver_solve",
-    "tnlhip_solver_get_status", "tnlhip_solver_get_history",
+    "tnlhip_solver_get_status", "tnlhip_solver_get_history", "tnlhip_refinement_params_default",
+    "tnlhip_refine_solve",
 ]
 
 
@@ -61,6 +62,19 @@ class SolverParams(C.Structure):
                 ("restarting_max", C.c_int32), ("restarting_step_min", C.c_int32),
                 ("restarting_step_max", C.c_int32), ("bicgstab_exact_residue", C.c_int32),
                 ("preconditioner", C.c_int32)]
+
+
+REFINEMENT_HISTORY = 64
+
+
+class RefinementParams(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("max_outer_iterations", C.c_int32), ("inner_method", C.c_int32),
+                ("inner_value_dtype", C.c_int32), ("reserved", C.c_int32), ("inner", SolverParams)]
+
+
+class RefinementResult(C.Structure):
+    _fields_ = [("converged", C.c_int32), ("outer_iterations", C.c_int32), ("inner_iterations", C.c_int64),
+                ("residue", C.c_double), ("history", C.c_double * REFINEMENT_HISTORY)]
 
 
 def _load():
@@ -96,6 +110,9 @@ def _load():
         "tnlhip_solver_solve": (C.c_int, [vp, vp, vp, C.POINTER(i32), vp]),
         "tnlhip_solver_get_status": (C.c_int, [vp, C.POINTER(i32), C.POINTER(f64)]),
         "tnlhip_solver_get_history": (i64, [vp, vp, i64]),
+        "tnlhip_refinement_params_default": (None, [C.POINTER(RefinementParams)]),
+        "tnlhip_refine_solve": (C.c_int, [vp, vp, C.POINTER(RefinementParams), vp, vp,
+                                          C.POINTER(RefinementResult), vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -423,6 +440,28 @@ class BICGStab(LinearSolver):
 
 class GMRES(LinearSolver):
     method = "gmres"
+
+
+def refine_solve(A, b, x, A_low=None, tolerance=1e-10, inner_method="gmres", inner_dtype="f32",
+                 max_outer_iterations=50, stream=None, **inner):
+    """FP64 iterative refinement around a low-precision inner solve (no TNL counterpart).
+    Returns a dict with converged, outer_iterations, inner_iterations, residue, history."""
+    p = RefinementParams()
+    lib.tnlhip_refinement_params_default(C.byref(p))
+    p.tolerance = float(tolerance)
+    p.max_outer_iterations = int(max_outer_iterations)
+    p.inner_method = METHODS[inner_method] if isinstance(inner_method, str) else inner_method
+    p.inner_value_dtype = _dtype_id(inner_dtype)
+    for k, v in inner.items():
+        if k == "gmres_variant" and isinstance(v, str):
+            v = GMRES_VARIANTS[v]
+        setattr(p.inner, k, v)
+    r = RefinementResult()
+    _check(lib.tnlhip_refine_solve(A._h, A_low._h if A_low is not None else None, C.byref(p), _ptr(b), _ptr(x),
+                                   C.byref(r), _stream(stream)))
+    return {"converged": bool(r.converged), "outer_iterations": r.outer_iterations,
+            "inner_iterations": r.inner_iterations, "residue": r.residue,
+            "history": list(r.history[:min(r.outer_iterations + 1, REFINEMENT_HISTORY)])}
 
 
 def getLinearSolver(name):

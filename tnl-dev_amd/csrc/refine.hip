@@ -1,0 +1,172 @@
+// Mixed-precision iterative refinement: an FP64 outer loop around a low-precision Krylov solve.
+//
+//   r = b - A64 x                       (FP64 product and update on the device)
+//   while ||r|| / ||b|| > tolerance:
+//      rhat = (float)( r / ||r|| )       (scaled so FP32 / BF16 keep its significant digits)
+//      solve A_low d = rhat              (inner CG / BiCGStab / GMRES, FP32 vectors, values FP32
+//                                         or BF16, to inner_tolerance, from d = 0)
+//      x += ||r|| * (double) d           (FP64)
+//      r = b - A64 x                     (FP64)
+//
+// The inner solver is the library's own (solvers.hip) on a converted copy of A; the outer
+// residual and correction never leave FP64.  There is no reference counterpart: TNL has no
+// mixed-precision refinement; the inner solvers follow TNL's CG.hpp / BICGStab.hpp / GMRES.hpp.
+#include "blas.hpp"
+#include "matrix.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+namespace tnlhip {
+
+namespace {
+
+// rhat = (float)( r * scale[0] ), scale = 1 / ||r|| on the device
+__global__ __launch_bounds__( 256 ) void scaleToFloatKernel( int64_t n, const double* __restrict__ r,
+                                                             const double* __restrict__ norm, float* __restrict__ out )
+{
+   const double s = 1.0 / *norm;
+   const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride )
+      out[ i ] = (float) ( r[ i ] * s );
+}
+
+// x += ||r|| * d; then r = b (the next residual starts from b)
+__global__ __launch_bounds__( 256 ) void correctKernel( int64_t n, double* __restrict__ x, const float* __restrict__ d,
+                                                        const double* __restrict__ norm, double* __restrict__ r,
+                                                        const double* __restrict__ b )
+{
+   const double s = *norm;
+   const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride ) {
+      x[ i ] += s * (double) d[ i ];
+      r[ i ] = b[ i ];
+   }
+}
+
+__global__ __launch_bounds__( 256 ) void copyKernel( int64_t n, double* __restrict__ y, const double* __restrict__ x )
+{
+   const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride )
+      y[ i ] = x[ i ];
+}
+
+}  // namespace
+
+}  // namespace tnlhip
+
+using namespace tnlhip;
+
+extern "C" {
+
+void tnlhip_refinement_params_default( tnlhip_refinement_params* p )
+{
+   if( ! p )
+      return;
+   p->tolerance = 1e-10;
+   p->max_outer_iterations = 50;
+   p->inner_method = TNLHIP_GMRES;
+   p->inner_value_dtype = TNLHIP_F32;
+   tnlhip_solver_params_default( &p->inner );
+   p->inner.convergence_residue = 1e-4;
+   p->inner.max_iterations = 100000;
+   p->inner.gmres_variant = TNLHIP_GMRES_CWY;
+   p->inner.restarting_min = 30;
+   p->inner.restarting_max = 30;
+   p->inner.preconditioner = 1;
+}
+
+int tnlhip_refine_solve( tnlhip_matrix A, tnlhip_matrix A_low, const tnlhip_refinement_params* p, const double* b,
+                         double* x, tnlhip_refinement_result* result, void* stream_ )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( A && p && b && x && result, "NULL argument" );
+      const Matrix& M = *reinterpret_cast< Matrix* >( A );
+      TNLHIP_REQUIRE( M.dtype == TNLHIP_F64, "the outer matrix must hold FP64 values" );
+      TNLHIP_REQUIRE( M.rows == M.cols, "the matrix must be square" );
+      TNLHIP_REQUIRE( p->inner_value_dtype == TNLHIP_F32 || p->inner_value_dtype == TNLHIP_BF16,
+                      "inner values are F32 or BF16" );
+      hipStream_t st = (hipStream_t) stream_;
+      const int64_t n = M.rows;
+      std::memset( result, 0, sizeof( *result ) );
+      // the low-precision copy of A (converted on the device unless given)
+      tnlhip_matrix low = A_low;
+      tnlhip_matrix owned = nullptr;
+      if( ! low ) {
+         const int fmt = M.format;
+         int s = tnlhip_matrix_convert( A, fmt, p->inner_value_dtype, &owned );
+         if( s != TNLHIP_SUCCESS )
+            throw Error( s, std::string( "conversion of the inner matrix failed: " ) + tnlhip_last_error() );
+         low = owned;
+      }
+      struct Guard
+      {
+         tnlhip_matrix m;
+         tnlhip_solver s;
+         ~Guard()
+         {
+            if( m )
+               tnlhip_matrix_destroy( m );
+            if( s )
+               tnlhip_solver_destroy( s );
+         }
+      } guard{ owned, nullptr };
+      TNLHIP_REQUIRE( reinterpret_cast< Matrix* >( low )->dtype != TNLHIP_F64, "the inner matrix must be FP32 or BF16" );
+
+      DeviceArray< double > r( n ), scal( 4 );
+      DeviceArray< float > rhat( n ), d( n );
+      Workspace ws;
+      ws.ensure( 2, kMaxMulti, 8 );
+      int s = tnlhip_solver_create( p->inner_method, &guard.s );
+      if( s != TNLHIP_SUCCESS )
+         throw Error( s, tnlhip_last_error() );
+      TNLHIP_CHECK( hipStreamSynchronize( st ) );
+      tnlhip_solver_set_matrix( guard.s, low );
+      tnlhip_solver_params inner = p->inner;
+      tnlhip_solver_set_params( guard.s, &inner );
+
+      // ||b||
+      deviceDot( TNLHIP_F64, n, b, b, scal.ptr + 1, true, ws, st );
+      // r = b - A x
+      copyKernel<<< gridFor( n, 256 ), 256, 0, st >>>( n, r.ptr, b );
+      spmvLaunch( M, x, r.ptr, -1.0, 1.0, 0, M.rows, nullptr, st );
+      double normb = 0;
+      TNLHIP_CHECK( hipMemcpyAsync( &normb, scal.ptr + 1, sizeof( double ), hipMemcpyDeviceToHost, st ) );
+      TNLHIP_CHECK( hipStreamSynchronize( st ) );
+      if( normb == 0.0 )
+         normb = 1.0;
+      for( int k = 0;; k++ ) {
+         deviceDot( TNLHIP_F64, n, r.ptr, r.ptr, scal.ptr, true, ws, st );
+         double nr = 0;
+         TNLHIP_CHECK( hipMemcpyAsync( &nr, scal.ptr, sizeof( double ), hipMemcpyDeviceToHost, st ) );
+         TNLHIP_CHECK( hipStreamSynchronize( st ) );
+         const double res = nr / normb;
+         if( k < TNLHIP_REFINEMENT_HISTORY )
+            result->history[ k ] = res;
+         result->residue = res;
+         result->outer_iterations = k;
+         if( std::isnan( res ) || res < p->tolerance || nr == 0.0 || k >= p->max_outer_iterations )
+            break;
+         scaleToFloatKernel<<< gridFor( n, 256 ), 256, 0, st >>>( n, r.ptr, scal.ptr, rhat.ptr );
+         TNLHIP_CHECK( hipMemsetAsync( d.ptr, 0, sizeof( float ) * n, st ) );
+         int32_t conv = 0;
+         s = tnlhip_solver_solve( guard.s, rhat.ptr, d.ptr, &conv, st );
+         if( s != TNLHIP_SUCCESS )
+            throw Error( s, std::string( "inner solve failed: " ) + tnlhip_last_error() );
+         int32_t its = 0;
+         double ires = 0;
+         tnlhip_solver_get_status( guard.s, &its, &ires );
+         result->inner_iterations += its;
+         // x += ||r|| d; r = b - A x
+         correctKernel<<< gridFor( n, 256 ), 256, 0, st >>>( n, x, d.ptr, scal.ptr, r.ptr, b );
+         spmvLaunch( M, x, r.ptr, -1.0, 1.0, 0, M.rows, nullptr, st );
+      }
+      TNLHIP_CHECK( hipGetLastError() );
+      TNLHIP_CHECK( hipStreamSynchronize( st ) );
+      result->converged = result->residue < p->tolerance ? 1 : 0;
+   } );
+}
+
+}  // extern "C"
