@@ -248,7 +248,9 @@ typedef struct
    int32_t max_outer_iterations;
    int32_t inner_method;         /* tnlhip_method */
    int32_t inner_value_dtype;    /* TNLHIP_F32 or TNLHIP_BF16 */
-   int32_t reserved;
+   int32_t stagnation_limit;     /* stop (not converged) after this many consecutive outer steps that
+                                    do not halve the residual; 0 = never.  Refinement converges only
+                                    when cond(A) times the unit roundoff of A_low is below 1. */
    tnlhip_solver_params inner;   /* inner stopping rule, GMRES variant / restart, preconditioner */
 } tnlhip_refinement_params;
 
@@ -267,6 +269,79 @@ void tnlhip_refinement_params_default( tnlhip_refinement_params* params );
  * same format); b, x: FP64 device vectors, x holds the initial guess.  Synchronous. */
 int tnlhip_refine_solve( tnlhip_matrix A, tnlhip_matrix A_low, const tnlhip_refinement_params* params,
                          const double* b, double* x, tnlhip_refinement_result* result, void* stream );
+
+/* ------------------------------------------------------------------ solve server (batched) */
+
+/* Many uneven systems with 1-16 right-hand sides each, resident together and advanced by one
+ * fixed sequence of persistent kernels (replayed as a hipGraph): continuous batching with
+ * admission by HBM footprint.  Each request solves (A + shift I) X = B for a resident
+ * matrix A to an FP64 relative residual per right-hand side, by FP64 iterative refinement
+ * over an FP32 CG inner solve (Solvers/Linear/CG.hpp:17-107 recurrences).  No counterpart in
+ * the reference: TNL solves one system per LinearSolver::solve call (LinearSolver.h:115-162);
+ * a TNL caller that loops over solve() for many small systems is what this replaces. */
+typedef struct tnlhip_server_s* tnlhip_server;
+
+typedef struct
+{
+   int64_t matrix_capacity;        /* SlicedEllpack slots of all resident matrices (16 B each) */
+   int64_t slice_capacity;         /* slice offsets of all resident matrices (4 B each) */
+   int64_t vector_capacity;        /* (row, right-hand side) entries of all resident requests (40 B each) */
+   int64_t staging_capacity;       /* largest nnz + rows + 1 of a matrix upload */
+   int32_t max_requests;           /* resident requests at once */
+   int32_t iterations_per_launch;  /* iterations per replay (one host synchronization each) */
+   int32_t use_graph;              /* 1: capture the replay in a hipGraph */
+   int32_t reserved;
+} tnlhip_server_config;
+
+typedef struct
+{
+   double tolerance;               /* FP64 |b - (A + shift I) x| / |b| per right-hand side */
+   double inner_tolerance;         /* FP32 inner CG relative residual */
+   int32_t max_outer_iterations;
+   int32_t max_inner_iterations;   /* per outer iteration */
+   int32_t method;                 /* TNLHIP_CG */
+   int32_t stagnation_limit;       /* as tnlhip_refinement_params */
+} tnlhip_request_params;
+
+typedef struct
+{
+   int32_t converged;              /* every right-hand side reached the tolerance */
+   int32_t outer_iterations;       /* max over the right-hand sides */
+   int64_t inner_iterations;       /* summed over the right-hand sides */
+   double residue;                 /* max over the right-hand sides */
+   int64_t launches;               /* server replays completed when the request finished */
+} tnlhip_request_result;
+
+typedef struct
+{
+   int32_t resident_requests, queued_requests;
+   int64_t resident_entries, vector_free, matrix_free, launches, tiles;
+} tnlhip_server_stats_t;
+
+void tnlhip_server_config_default( tnlhip_server_config* config );
+void tnlhip_request_params_default( tnlhip_request_params* params );
+/* HBM bytes a matrix (rows, nnz, longest row) and a request (rows, nrhs) occupy. */
+int tnlhip_server_footprint( int32_t rows, int64_t nnz, int32_t max_row, int32_t nrhs, int64_t* matrix_bytes,
+                             int64_t* request_bytes );
+int tnlhip_server_create( const tnlhip_server_config* config, tnlhip_server* out );
+int tnlhip_server_destroy( tnlhip_server server );
+/* Upload a square CSR matrix (host arrays) and keep it resident; TNLHIP_ERROR_CAPACITY when the
+ * matrix arena has no room.  Synchronous. */
+int tnlhip_server_add_matrix( tnlhip_server server, int32_t rows, const int32_t* row_pointers, const int32_t* columns,
+                              const double* values, int32_t* matrix_id );
+/* The matrix is freed once no resident request uses it. */
+int tnlhip_server_remove_matrix( tnlhip_server server, int32_t matrix_id );
+/* Queue a request: b and x are host arrays of rows x nrhs (entry row * nrhs + j) that stay valid
+ * until the ticket finishes; x receives the solution (the initial guess is 0). */
+int tnlhip_server_submit( tnlhip_server server, int32_t matrix_id, double shift, int32_t nrhs, const double* b,
+                          double* x, const tnlhip_request_params* params, int64_t* ticket );
+/* Admit queued requests that fit (FIFO), run one replay, retire finished requests (their x is
+ * written).  Up to `capacity` finished tickets are stored in `finished`; *count gets how many
+ * finished. */
+int tnlhip_server_progress( tnlhip_server server, int64_t* finished, int32_t capacity, int32_t* count );
+/* Result of a finished ticket (once). */
+int tnlhip_server_result( tnlhip_server server, int64_t ticket, tnlhip_request_result* result );
+int tnlhip_server_stats( tnlhip_server server, tnlhip_server_stats_t* stats );
 
 #ifdef __cplusplus
 }

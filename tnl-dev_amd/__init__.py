@@ -10,6 +10,7 @@ and there is no CPU fallback.
 """
 import ctypes as C
 import os
+import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtnlhip.so")
@@ -32,7 +33,10 @@ EXPORTS = [
     "tnlhip_gemv", "tnlhip_exclusive_scan_i32", "tnlhip_solver_params_default", "tnlhip_solver_create",
     "tnlhip_solver_destroy", "tnlhip_solver_set_matrix", "tnlhip_solver_set_params", "tnlhip_solver_solve",
     "tnlhip_solver_get_status", "tnlhip_solver_get_history", "tnlhip_refinement_params_default",
-    "tnlhip_refine_solve",
+    "tnlhip_refine_solve", "tnlhip_server_config_default", "tnlhip_request_params_default",
+    "tnlhip_server_footprint", "tnlhip_server_create", "tnlhip_server_destroy", "tnlhip_server_add_matrix",
+    "tnlhip_server_remove_matrix", "tnlhip_server_submit", "tnlhip_server_progress", "tnlhip_server_result",
+    "tnlhip_server_stats",
 ]
 
 
@@ -69,12 +73,34 @@ REFINEMENT_HISTORY = 64
 
 class RefinementParams(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("max_outer_iterations", C.c_int32), ("inner_method", C.c_int32),
-                ("inner_value_dtype", C.c_int32), ("reserved", C.c_int32), ("inner", SolverParams)]
+                ("inner_value_dtype", C.c_int32), ("stagnation_limit", C.c_int32), ("inner", SolverParams)]
 
 
 class RefinementResult(C.Structure):
     _fields_ = [("converged", C.c_int32), ("outer_iterations", C.c_int32), ("inner_iterations", C.c_int64),
                 ("residue", C.c_double), ("history", C.c_double * REFINEMENT_HISTORY)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("matrix_capacity", C.c_int64), ("slice_capacity", C.c_int64), ("vector_capacity", C.c_int64),
+                ("staging_capacity", C.c_int64), ("max_requests", C.c_int32), ("iterations_per_launch", C.c_int32),
+                ("use_graph", C.c_int32), ("reserved", C.c_int32)]
+
+
+class RequestParams(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("inner_tolerance", C.c_double), ("max_outer_iterations", C.c_int32),
+                ("max_inner_iterations", C.c_int32), ("method", C.c_int32), ("stagnation_limit", C.c_int32)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("converged", C.c_int32), ("outer_iterations", C.c_int32), ("inner_iterations", C.c_int64),
+                ("residue", C.c_double), ("launches", C.c_int64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("resident_requests", C.c_int32), ("queued_requests", C.c_int32), ("resident_entries", C.c_int64),
+                ("vector_free", C.c_int64), ("matrix_free", C.c_int64), ("launches", C.c_int64),
+                ("tiles", C.c_int64)]
 
 
 def _load():
@@ -113,6 +139,17 @@ def _load():
         "tnlhip_refinement_params_default": (None, [C.POINTER(RefinementParams)]),
         "tnlhip_refine_solve": (C.c_int, [vp, vp, C.POINTER(RefinementParams), vp, vp,
                                           C.POINTER(RefinementResult), vp]),
+        "tnlhip_server_config_default": (None, [C.POINTER(ServerConfig)]),
+        "tnlhip_request_params_default": (None, [C.POINTER(RequestParams)]),
+        "tnlhip_server_footprint": (C.c_int, [i32, i64, i32, i32, C.POINTER(i64), C.POINTER(i64)]),
+        "tnlhip_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(vp)]),
+        "tnlhip_server_destroy": (C.c_int, [vp]),
+        "tnlhip_server_add_matrix": (C.c_int, [vp, i32, vp, vp, vp, C.POINTER(i32)]),
+        "tnlhip_server_remove_matrix": (C.c_int, [vp, i32]),
+        "tnlhip_server_submit": (C.c_int, [vp, i32, f64, i32, vp, vp, C.POINTER(RequestParams), C.POINTER(i64)]),
+        "tnlhip_server_progress": (C.c_int, [vp, vp, i32, C.POINTER(i32)]),
+        "tnlhip_server_result": (C.c_int, [vp, i64, C.POINTER(RequestResult)]),
+        "tnlhip_server_stats": (C.c_int, [vp, C.POINTER(ServerStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -443,13 +480,14 @@ class GMRES(LinearSolver):
 
 
 def refine_solve(A, b, x, A_low=None, tolerance=1e-10, inner_method="gmres", inner_dtype="f32",
-                 max_outer_iterations=50, stream=None, **inner):
+                 max_outer_iterations=50, stagnation_limit=2, stream=None, **inner):
     """FP64 iterative refinement around a low-precision inner solve (no TNL counterpart).
     Returns a dict with converged, outer_iterations, inner_iterations, residue, history."""
     p = RefinementParams()
     lib.tnlhip_refinement_params_default(C.byref(p))
     p.tolerance = float(tolerance)
     p.max_outer_iterations = int(max_outer_iterations)
+    p.stagnation_limit = int(stagnation_limit)
     p.inner_method = METHODS[inner_method] if isinstance(inner_method, str) else inner_method
     p.inner_value_dtype = _dtype_id(inner_dtype)
     for k, v in inner.items():
@@ -462,6 +500,105 @@ def refine_solve(A, b, x, A_low=None, tolerance=1e-10, inner_method="gmres", inn
     return {"converged": bool(r.converged), "outer_iterations": r.outer_iterations,
             "inner_iterations": r.inner_iterations, "residue": r.residue,
             "history": list(r.history[:min(r.outer_iterations + 1, REFINEMENT_HISTORY)])}
+
+
+class Server:
+    """Solve server: resident matrices, requests (A + shift I) X = B with 1-16 right-hand sides,
+    FP64 refinement over FP32 CG, advanced together by replayed persistent kernels (no TNL
+    counterpart).  B and X are host float64 arrays of shape (rows, k) in row-major order."""
+
+    def __init__(self, vector_capacity=None, matrix_capacity=None, slice_capacity=None, max_requests=None,
+                 iterations_per_launch=None, use_graph=True, staging_capacity=None):
+        cfg = ServerConfig()
+        lib.tnlhip_server_config_default(C.byref(cfg))
+        for k, v in dict(vector_capacity=vector_capacity, matrix_capacity=matrix_capacity,
+                         slice_capacity=slice_capacity, max_requests=max_requests,
+                         iterations_per_launch=iterations_per_launch, staging_capacity=staging_capacity).items():
+            if v is not None:
+                setattr(cfg, k, int(v))
+        cfg.use_graph = 1 if use_graph else 0
+        self.config = cfg
+        h = C.c_void_p()
+        _check(lib.tnlhip_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._live = {}  # ticket -> (b, x) kept alive until finished
+
+    def close(self):
+        if self._h:
+            lib.tnlhip_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_matrix(self, row_ptr, cols, vals):
+        rp = np.ascontiguousarray(row_ptr, np.int32)
+        c = np.ascontiguousarray(cols, np.int32)
+        v = np.ascontiguousarray(vals, np.float64)
+        mid = C.c_int32()
+        _check(lib.tnlhip_server_add_matrix(self._h, len(rp) - 1, rp.ctypes.data, c.ctypes.data, v.ctypes.data,
+                                            C.byref(mid)))
+        return mid.value
+
+    def remove_matrix(self, mid):
+        _check(lib.tnlhip_server_remove_matrix(self._h, int(mid)))
+
+    def submit(self, mid, B, shift=0.0, tolerance=1e-10, inner_tolerance=1e-4, max_outer_iterations=20,
+               max_inner_iterations=5000, stagnation_limit=2):
+        B = np.ascontiguousarray(B, np.float64)
+        if B.ndim == 1:
+            B = B[:, None]
+        X = np.zeros_like(B)
+        p = RequestParams()
+        lib.tnlhip_request_params_default(C.byref(p))
+        p.tolerance, p.inner_tolerance = float(tolerance), float(inner_tolerance)
+        p.max_outer_iterations, p.max_inner_iterations = int(max_outer_iterations), int(max_inner_iterations)
+        p.stagnation_limit = int(stagnation_limit)
+        t = C.c_int64()
+        _check(lib.tnlhip_server_submit(self._h, int(mid), float(shift), B.shape[1], B.ctypes.data, X.ctypes.data,
+                                        C.byref(p), C.byref(t)))
+        self._live[t.value] = (B, X)
+        return t.value
+
+    def progress(self):
+        """Admit, run one replay, retire; returns the finished tickets."""
+        cap = max(1, self.config.max_requests)
+        out = (C.c_int64 * cap)()
+        n = C.c_int32()
+        _check(lib.tnlhip_server_progress(self._h, out, cap, C.byref(n)))
+        return list(out[:min(n.value, cap)])
+
+    def result(self, ticket):
+        """(X, info) of a finished ticket."""
+        r = RequestResult()
+        _check(lib.tnlhip_server_result(self._h, int(ticket), C.byref(r)))
+        B, X = self._live.pop(ticket)
+        return X, {"converged": bool(r.converged), "outer_iterations": r.outer_iterations,
+                   "inner_iterations": r.inner_iterations, "residue": r.residue, "launches": r.launches}
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib.tnlhip_server_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def run_all(self):
+        """Progress until nothing is resident or queued; returns {ticket: (X, info)}."""
+        done = {}
+        while True:
+            for t in self.progress():
+                done[t] = self.result(t)
+            s = self.stats()
+            if s["resident_requests"] == 0 and s["queued_requests"] == 0:
+                return done
+
+
+def server_footprint(rows, nnz, max_row, nrhs):
+    mb, rb = C.c_int64(), C.c_int64()
+    _check(lib.tnlhip_server_footprint(int(rows), int(nnz), int(max_row), int(nrhs), C.byref(mb), C.byref(rb)))
+    return mb.value, rb.value
 
 
 def getLinearSolver(name):

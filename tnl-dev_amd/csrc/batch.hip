@@ -1,0 +1,1053 @@
+// Solve server: many uneven systems, each with 1-16 right-hand sides, solved together by one
+// fixed sequence of persistent kernels that a hipGraph replays.
+//
+// Each request is (A + shift I) X = B for a resident matrix A, k right-hand sides, FP64
+// iterative refinement over an FP32 conjugate-gradient inner solve (the recurrences of
+// Solvers/Linear/CG.hpp:17-107, unpreconditioned).  There is no batched counterpart in the
+// reference: TNL solves one system per solve() call.
+//
+// Layout in HBM:
+//   matrices     SlicedEllpack (slice 32, column-major: SlicedEllpackView.hpp:108-132), one
+//                column array and FP64 + FP32 value arrays shared by all resident matrices
+//   vectors      per request, n x k entries interleaved (entry = row * k + j):
+//                FP64 x, b, r and FP32 d, rr, p, q
+//   tiles        a tile is one workgroup's share of one request: 256 / kp rows times kp lanes
+//                (kp = k rounded up to a power of two); tiles of a request are numbered in row
+//                order and every reduction of a request sums its tiles' partials in that order,
+//                so a request's result does not depend on what else is resident.
+//
+// One iteration of the replayed sequence runs six kernels over all tiles; each kernel acts on
+// the right-hand sides in one phase (per-system phase machine below) and requests with no
+// system in that phase skip their tiles:
+//   innerMatvec  INNER    q = (A32 + s I) p, (p, q)          -> alpha
+//   innerUpdate  INNER    d += alpha p, rr -= alpha q, (rr, rr) -> beta, inner stop test
+//   innerDir     INNER    p = rr + beta p
+//   correct      CORRECT  x += |r| d                           (FP64)
+//   outerResid   CORRECT  r = b - (A64 + s I) x, (r, r), (b, b) -> outer stop test (FP64)
+//   restart      RESTART  rr = r / |r|, d = 0, p = rr, (rr, rr)
+#include "common.hpp"
+#include "matrix.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
+
+namespace tnlhip {
+
+namespace {
+
+constexpr int kTile = 256;
+constexpr int kMaxRhs = 16;
+constexpr int kServerGrid = 1024;
+
+enum Phase : int32_t { P_IDLE = 0, P_INNER = 1, P_CORRECT = 2, P_RESTART = 3, P_DONE = 4, P_FAILED = 5 };
+constexpr int32_t kLive = ( 1 << P_INNER ) | ( 1 << P_CORRECT ) | ( 1 << P_RESTART );
+
+// host-written at admission
+struct ReqStatic
+{
+   int32_t n, k, kp, rpt;        // rows, right-hand sides, padded k, rows per tile
+   int32_t tiles, sliceBase, maxOuter, maxInner;
+   int64_t vec;                  // entry offset in the vector arena
+   int64_t mat;                  // slot offset in the matrix arena
+   double shift, tol, innerTol2;
+   int32_t stagnation, pad;
+};
+
+// device-written
+struct ReqDyn
+{
+   int32_t mask;     // bit p set when some system of the request is in phase p
+   uint32_t counter; // tiles arrived in the current kernel
+};
+
+struct Sys
+{
+   int32_t phase, innerIt, outerIt, slow;
+   int64_t innerTotal;
+   double normb, scale, rho, rho0, alpha, beta, res, prevRes;
+};
+
+struct Ctl
+{
+   int32_t numTiles;
+};
+
+struct Arena
+{
+   const Ctl* ctl;
+   const int32_t* tileReq;
+   const int32_t* tileFirst;
+   const ReqStatic* rs;
+   ReqDyn* rd;
+   Sys* sys;
+   double* part;  // [tile][2][16]
+   const int32_t* cols;
+   const double* v64;
+   const float* v32;
+   const int32_t* slices;
+   double* a64;   // x, b, r per request: 3 E
+   float* a32;    // d, rr, p, q per request: 4 E
+};
+
+struct TileCtx
+{
+   int req, ti;
+   int32_t row, j;
+   bool active;   // this lane holds an (row, rhs) entry
+   int64_t e;     // entry index within the request
+};
+
+__device__ __forceinline__ TileCtx tileCtx( const Arena& A, int t, const ReqStatic& R )
+{
+   TileCtx c;
+   c.req = A.tileReq[ t ];
+   c.ti = t - A.tileFirst[ c.req ];
+   const int lr = threadIdx.x / R.kp;
+   c.j = threadIdx.x % R.kp;
+   c.row = c.ti * R.rpt + lr;
+   c.active = lr < R.rpt && c.row < R.n && c.j < R.k;
+   c.e = (int64_t) c.row * R.k + c.j;
+   return c;
+}
+
+// y = sum_k A[row, k] * v[col * K + j] over the SlicedEllpack row (padding column -1)
+template< typename V, typename X, typename Acc >
+__device__ __forceinline__ Acc rowProduct( const Arena& A, const ReqStatic& R, int32_t row, int32_t j, const V* vals,
+                                           const X* v )
+{
+   const int32_t s = row >> 5;
+   const int32_t b0 = A.slices[ R.sliceBase + s ], b1 = A.slices[ R.sliceBase + s + 1 ];
+   const int32_t width = ( b1 - b0 ) >> 5;
+   int64_t idx = R.mat + b0 + ( row & 31 );
+   Acc sum = 0;
+   for( int32_t kk = 0; kk < width; kk++, idx += 32 ) {
+      const int32_t col = A.cols[ idx ];
+      if( col >= 0 )
+         sum += (Acc) vals[ idx ] * (Acc) v[ (int64_t) col * R.k + j ];
+   }
+   return sum;
+}
+
+// Per-tile sums of W values per right-hand side into part[t][w][j], in a fixed order.
+template< int W >
+__device__ __forceinline__ void tilePartials( double* part, int t, int kp, const double ( &v )[ W ] )
+{
+   __shared__ double lds[ kTile / 64 ][ W ][ kMaxRhs ];
+   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+#pragma unroll
+   for( int w = 0; w < W; w++ ) {
+      double x = v[ w ];
+      for( int o = 32; o >= kp; o >>= 1 )
+         x += __shfl_xor( x, o );
+      if( lane < kp )
+         lds[ wave ][ w ][ lane ] = x;
+   }
+   __syncthreads();
+   if( threadIdx.x < W * kp ) {
+      const int w = threadIdx.x / kp, j = threadIdx.x % kp;
+      double s = lds[ 0 ][ w ][ j ];
+      for( int q = 1; q < kTile / 64; q++ )
+         s += lds[ q ][ w ][ j ];
+      part[ (int64_t) t * 2 * kMaxRhs + w * kMaxRhs + j ] = s;
+   }
+}
+
+// Ticket: returns true in every thread of the block that completes the request's last tile.
+__device__ __forceinline__ bool lastTile( ReqDyn& d, int tiles )
+{
+   __shared__ int isLast;
+   __threadfence();
+   __syncthreads();
+   if( threadIdx.x == 0 )
+      isLast = atomicAdd( &d.counter, 1u ) == (uint32_t) ( tiles - 1 );
+   __syncthreads();
+   if( isLast )
+      __threadfence();
+   return isLast;
+}
+
+// Totals over the request's tiles, tile order fixed; returns LDS holding tot[w * kMaxRhs + j],
+// valid for every thread on return.
+template< int W >
+__device__ __forceinline__ const double* requestTotals( const double* part, int first, int tiles, int kp )
+{
+   __shared__ double out[ 2 * kMaxRhs ];
+   const int pairs = W * kp;
+   const int G = min( 64, kTile / pairs );
+   const int tid = threadIdx.x;
+   const int pair = tid / G, sub = tid % G;
+   double s = 0;
+   if( pair < pairs ) {
+      const int w = pair / kp, j = pair % kp;
+      for( int ti = sub; ti < tiles; ti += G )
+         s += part[ (int64_t) ( first + ti ) * 2 * kMaxRhs + w * kMaxRhs + j ];
+   }
+   for( int o = G / 2; o > 0; o >>= 1 )
+      s += __shfl_xor( s, o );
+   if( pair < pairs && sub == 0 )
+      out[ ( pair / kp ) * kMaxRhs + pair % kp ] = s;
+   __syncthreads();
+   return out;
+}
+
+// Recompute the request's phase mask after threads j < k updated their systems.
+__device__ __forceinline__ void publishMask( const Arena& A, int req, const ReqStatic& R )
+{
+   __shared__ int m;
+   if( threadIdx.x == 0 )
+      m = 0;
+   __syncthreads();
+   if( threadIdx.x < R.k )
+      atomicOr( &m, 1 << A.sys[ req * kMaxRhs + threadIdx.x ].phase );
+   __syncthreads();
+   if( threadIdx.x == 0 ) {
+      A.rd[ req ].mask = m;
+      A.rd[ req ].counter = 0;
+   }
+}
+
+__device__ __forceinline__ float* a32( const Arena& A, const ReqStatic& R, int which )
+{
+   return A.a32 + 4 * R.vec + (int64_t) which * R.n * R.k;
+}
+__device__ __forceinline__ double* a64( const Arena& A, const ReqStatic& R, int which )
+{
+   return A.a64 + 3 * R.vec + (int64_t) which * R.n * R.k;
+}
+enum { X64 = 0, B64 = 1, R64 = 2 };
+enum { D32 = 0, RR32 = 1, P32 = 2, Q32 = 3 };
+
+// ---------------------------------------------------------------- kernels
+
+__global__ __launch_bounds__( kTile ) void innerMatvecKernel( Arena A )
+{
+   const int nt = A.ctl->numTiles;
+   for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
+      const int req = A.tileReq[ t ];
+      const ReqStatic R = A.rs[ req ];
+      if( ! ( A.rd[ req ].mask & ( 1 << P_INNER ) ) )
+         continue;
+      const TileCtx c = tileCtx( A, t, R );
+      double v[ 1 ] = { 0 };
+      if( c.active && A.sys[ req * kMaxRhs + c.j ].phase == P_INNER ) {
+         const float* p = a32( A, R, P32 );
+         float q = rowProduct< float, float, float >( A, R, c.row, c.j, A.v32, p );
+         q += (float) R.shift * p[ c.e ];
+         a32( A, R, Q32 )[ c.e ] = q;
+         v[ 0 ] = (double) p[ c.e ] * (double) q;
+      }
+      tilePartials< 1 >( A.part, t, R.kp, v );
+      if( lastTile( A.rd[ req ], R.tiles ) ) {
+         const double* tot = requestTotals< 1 >( A.part, A.tileFirst[ req ], R.tiles, R.kp );
+         if( threadIdx.x < R.k ) {
+            Sys& s = A.sys[ req * kMaxRhs + threadIdx.x ];
+            if( s.phase == P_INNER ) {
+               const double pq = tot[ threadIdx.x ];
+               if( ! ( pq > 0 ) || ! std::isfinite( pq ) )
+                  s.phase = P_CORRECT;  // breakdown: keep the current correction
+               else
+                  s.alpha = s.rho / pq;
+            }
+         }
+         __syncthreads();
+         publishMask( A, req, R );
+      }
+      __syncthreads();
+   }
+}
+
+__global__ __launch_bounds__( kTile ) void innerUpdateKernel( Arena A )
+{
+   const int nt = A.ctl->numTiles;
+   for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
+      const int req = A.tileReq[ t ];
+      const ReqStatic R = A.rs[ req ];
+      if( ! ( A.rd[ req ].mask & ( 1 << P_INNER ) ) )
+         continue;
+      const TileCtx c = tileCtx( A, t, R );
+      double v[ 1 ] = { 0 };
+      if( c.active ) {
+         const Sys& s = A.sys[ req * kMaxRhs + c.j ];
+         if( s.phase == P_INNER ) {
+            const float alpha = (float) s.alpha;
+            const float p = a32( A, R, P32 )[ c.e ];
+            a32( A, R, D32 )[ c.e ] += alpha * p;
+            float& rr = a32( A, R, RR32 )[ c.e ];
+            rr -= alpha * a32( A, R, Q32 )[ c.e ];
+            v[ 0 ] = (double) rr * (double) rr;
+         }
+      }
+      tilePartials< 1 >( A.part, t, R.kp, v );
+      if( lastTile( A.rd[ req ], R.tiles ) ) {
+         const double* tot = requestTotals< 1 >( A.part, A.tileFirst[ req ], R.tiles, R.kp );
+         if( threadIdx.x < R.k ) {
+            Sys& s = A.sys[ req * kMaxRhs + threadIdx.x ];
+            if( s.phase == P_INNER ) {
+               const double rho = tot[ threadIdx.x ];
+               s.innerIt++;
+               s.innerTotal++;
+               if( ! std::isfinite( rho ) || rho <= R.innerTol2 * s.rho0 || s.innerIt >= R.maxInner )
+                  s.phase = P_CORRECT;
+               else {
+                  s.beta = rho / s.rho;
+                  s.rho = rho;
+               }
+            }
+         }
+         __syncthreads();
+         publishMask( A, req, R );
+      }
+      __syncthreads();
+   }
+}
+
+__global__ __launch_bounds__( kTile ) void innerDirKernel( Arena A )
+{
+   const int nt = A.ctl->numTiles;
+   for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
+      const int req = A.tileReq[ t ];
+      const ReqStatic R = A.rs[ req ];
+      if( ! ( A.rd[ req ].mask & ( 1 << P_INNER ) ) )
+         continue;
+      const TileCtx c = tileCtx( A, t, R );
+      if( c.active ) {
+         const Sys& s = A.sys[ req * kMaxRhs + c.j ];
+         if( s.phase == P_INNER ) {
+            float& p = a32( A, R, P32 )[ c.e ];
+            p = a32( A, R, RR32 )[ c.e ] + (float) s.beta * p;
+         }
+      }
+   }
+}
+
+__global__ __launch_bounds__( kTile ) void correctKernel( Arena A )
+{
+   const int nt = A.ctl->numTiles;
+   for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
+      const int req = A.tileReq[ t ];
+      const ReqStatic R = A.rs[ req ];
+      if( ! ( A.rd[ req ].mask & ( 1 << P_CORRECT ) ) )
+         continue;
+      const TileCtx c = tileCtx( A, t, R );
+      if( c.active ) {
+         const Sys& s = A.sys[ req * kMaxRhs + c.j ];
+         if( s.phase == P_CORRECT )
+            a64( A, R, X64 )[ c.e ] += s.scale * (double) a32( A, R, D32 )[ c.e ];
+      }
+   }
+}
+
+__global__ __launch_bounds__( kTile ) void outerResidKernel( Arena A )
+{
+   const int nt = A.ctl->numTiles;
+   for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
+      const int req = A.tileReq[ t ];
+      const ReqStatic R = A.rs[ req ];
+      if( ! ( A.rd[ req ].mask & ( 1 << P_CORRECT ) ) )
+         continue;
+      const TileCtx c = tileCtx( A, t, R );
+      double v[ 2 ] = { 0, 0 };
+      if( c.active && A.sys[ req * kMaxRhs + c.j ].phase == P_CORRECT ) {
+         const double* x = a64( A, R, X64 );
+         const double bb = a64( A, R, B64 )[ c.e ];
+         const double ax = rowProduct< double, double, double >( A, R, c.row, c.j, A.v64, x ) + R.shift * x[ c.e ];
+         const double r = bb - ax;
+         a64( A, R, R64 )[ c.e ] = r;
+         v[ 0 ] = r * r;
+         v[ 1 ] = bb * bb;
+      }
+      tilePartials< 2 >( A.part, t, R.kp, v );
+      if( lastTile( A.rd[ req ], R.tiles ) ) {
+         const double* tot = requestTotals< 2 >( A.part, A.tileFirst[ req ], R.tiles, R.kp );
+         if( threadIdx.x < R.k ) {
+            Sys& s = A.sys[ req * kMaxRhs + threadIdx.x ];
+            if( s.phase == P_CORRECT ) {
+               if( s.normb < 0 ) {
+                  s.normb = sqrt( tot[ kMaxRhs + threadIdx.x ] );
+                  if( s.normb == 0 )
+                     s.normb = 1;
+               }
+               const double nr = sqrt( tot[ threadIdx.x ] );
+               const double res = nr / s.normb;
+               s.slow = ( s.outerIt > 0 && res > 0.5 * s.prevRes ) ? s.slow + 1 : 0;
+               s.res = res;
+               if( res < R.tol || nr == 0 )
+                  s.phase = P_DONE;
+               else if( ! std::isfinite( res ) || s.outerIt >= R.maxOuter
+                        || ( R.stagnation > 0 && s.slow >= R.stagnation ) )
+                  s.phase = P_FAILED;
+               else {
+                  s.phase = P_RESTART;
+                  s.scale = nr;
+                  s.prevRes = res;
+               }
+            }
+         }
+         __syncthreads();
+         publishMask( A, req, R );
+      }
+      __syncthreads();
+   }
+}
+
+__global__ __launch_bounds__( kTile ) void restartKernel( Arena A )
+{
+   const int nt = A.ctl->numTiles;
+   for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
+      const int req = A.tileReq[ t ];
+      const ReqStatic R = A.rs[ req ];
+      if( ! ( A.rd[ req ].mask & ( 1 << P_RESTART ) ) )
+         continue;
+      const TileCtx c = tileCtx( A, t, R );
+      double v[ 1 ] = { 0 };
+      if( c.active ) {
+         const Sys& s = A.sys[ req * kMaxRhs + c.j ];
+         if( s.phase == P_RESTART ) {
+            const float rr = (float) ( a64( A, R, R64 )[ c.e ] / s.scale );
+            a32( A, R, RR32 )[ c.e ] = rr;
+            a32( A, R, P32 )[ c.e ] = rr;
+            a32( A, R, D32 )[ c.e ] = 0.0f;
+            v[ 0 ] = (double) rr * (double) rr;
+         }
+      }
+      tilePartials< 1 >( A.part, t, R.kp, v );
+      if( lastTile( A.rd[ req ], R.tiles ) ) {
+         const double* tot = requestTotals< 1 >( A.part, A.tileFirst[ req ], R.tiles, R.kp );
+         if( threadIdx.x < R.k ) {
+            Sys& s = A.sys[ req * kMaxRhs + threadIdx.x ];
+            if( s.phase == P_RESTART ) {
+               s.rho = s.rho0 = tot[ threadIdx.x ];
+               s.innerIt = 0;
+               s.outerIt++;
+               s.phase = P_INNER;
+            }
+         }
+         __syncthreads();
+         publishMask( A, req, R );
+      }
+      __syncthreads();
+   }
+}
+
+// admission: x = 0, d = 0 over the request's entries
+__global__ __launch_bounds__( 256 ) void admitKernel( int64_t E, double* x, float* d )
+{
+   const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+   for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < E; i += stride ) {
+      x[ i ] = 0.0;
+      d[ i ] = 0.0f;
+   }
+}
+
+// CSR -> SlicedEllpack scatter of one matrix into the arena, one lane per row
+__global__ __launch_bounds__( 256 ) void scatterKernel( int32_t n, const int32_t* rp, const int32_t* ci, const double* vv,
+                                                        const int32_t* slices, int64_t mat, int32_t* cols, double* v64,
+                                                        float* v32 )
+{
+   const int32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+   if( row >= n )
+      return;
+   const int32_t s = row >> 5;
+   const int32_t width = ( slices[ s + 1 ] - slices[ s ] ) >> 5;
+   const int64_t base = mat + slices[ s ] + ( row & 31 );
+   const int32_t b = rp[ row ], len = rp[ row + 1 ] - b;
+   for( int32_t kk = 0; kk < width; kk++ ) {
+      const int64_t idx = base + 32 * (int64_t) kk;
+      if( kk < len ) {
+         cols[ idx ] = ci[ b + kk ];
+         v64[ idx ] = vv[ b + kk ];
+         v32[ idx ] = (float) vv[ b + kk ];
+      }
+      else {
+         cols[ idx ] = -1;
+         v64[ idx ] = 0.0;
+         v32[ idx ] = 0.0f;
+      }
+   }
+}
+
+// ---------------------------------------------------------------- host side
+
+// first-fit allocator over [0, capacity)
+struct RangeAlloc
+{
+   int64_t capacity = 0;
+   std::map< int64_t, int64_t > freeList;  // offset -> length
+   void init( int64_t cap )
+   {
+      capacity = cap;
+      freeList.clear();
+      if( cap > 0 )
+         freeList[ 0 ] = cap;
+   }
+   int64_t alloc( int64_t len )
+   {
+      if( len <= 0 )
+         len = 1;
+      for( auto it = freeList.begin(); it != freeList.end(); ++it )
+         if( it->second >= len ) {
+            const int64_t off = it->first, rest = it->second - len;
+            freeList.erase( it );
+            if( rest > 0 )
+               freeList[ off + len ] = rest;
+            return off;
+         }
+      return -1;
+   }
+   void release( int64_t off, int64_t len )
+   {
+      if( len <= 0 )
+         len = 1;
+      auto it = freeList.emplace( off, len ).first;
+      auto next = std::next( it );
+      if( next != freeList.end() && it->first + it->second == next->first ) {
+         it->second += next->second;
+         freeList.erase( next );
+      }
+      if( it != freeList.begin() ) {
+         auto prev = std::prev( it );
+         if( prev->first + prev->second == it->first ) {
+            prev->second += it->second;
+            freeList.erase( it );
+         }
+      }
+   }
+   int64_t largest() const
+   {
+      int64_t m = 0;
+      for( auto& f : freeList )
+         m = std::max( m, f.second );
+      return m;
+   }
+   int64_t freeTotal() const
+   {
+      int64_t m = 0;
+      for( auto& f : freeList )
+         m += f.second;
+      return m;
+   }
+};
+
+struct ResidentMatrix
+{
+   int32_t n = 0;
+   int64_t mat = -1, slots = 0;
+   int32_t sliceBase = -1, sliceCount = 0;
+   int users = 0;
+   bool removing = false;
+};
+
+struct Request
+{
+   int64_t ticket;
+   int32_t matrix, k;
+   double shift;
+   const double* b;
+   double* x;
+   tnlhip_request_params params;
+   int slot = -1;
+   double submitted = 0;
+};
+
+}  // namespace
+
+}  // namespace tnlhip
+
+using namespace tnlhip;
+
+struct tnlhip_server_s
+{
+   tnlhip_server_config cfg{};
+   hipStream_t stream = nullptr;
+   DeviceArray< int32_t > cols, slices, tileReq, tileFirst;
+   DeviceArray< double > v64, a64, part;
+   DeviceArray< float > v32, a32;
+   DeviceArray< ReqStatic > rs;
+   DeviceArray< ReqDyn > rd;
+   DeviceArray< Sys > sys;
+   DeviceArray< Ctl > ctl;
+   DeviceArray< int32_t > staging;  // CSR upload staging (row pointers + columns), int32
+   DeviceArray< double > stagingV;
+   RangeAlloc matAlloc, sliceAlloc, vecAlloc;
+   std::vector< ResidentMatrix > matrices;
+   std::deque< Request > queue;
+   std::vector< Request > slots;  // per resident slot (slot < 0: free)
+   std::vector< bool > slotUsed;
+   std::map< int64_t, tnlhip_request_result > finished;
+   int64_t nextTicket = 1;
+   int64_t maxTiles = 0;
+   bool tilesDirty = false;
+   // pinned mirrors
+   int32_t* hTileReq = nullptr;
+   int32_t* hTileFirst = nullptr;
+   ReqDyn* hDyn = nullptr;
+   ReqStatic* hStatic = nullptr;
+   Sys* hSys = nullptr;
+   Ctl* hCtl = nullptr;
+   hipGraph_t graph = nullptr;
+   hipGraphExec_t exec = nullptr;
+   int64_t launches = 0;
+
+   Arena arena() const
+   {
+      return Arena{ ctl.ptr, tileReq.ptr, tileFirst.ptr, rs.ptr, rd.ptr, sys.ptr, part.ptr,
+                    cols.ptr, v64.ptr, v32.ptr, slices.ptr, a64.ptr, a32.ptr };
+   }
+
+   void enqueueIterations( int iterations )
+   {
+      const Arena A = arena();
+      for( int i = 0; i < iterations; i++ ) {
+         innerMatvecKernel<<< kServerGrid, kTile, 0, stream >>>( A );
+         innerUpdateKernel<<< kServerGrid, kTile, 0, stream >>>( A );
+         innerDirKernel<<< kServerGrid, kTile, 0, stream >>>( A );
+         correctKernel<<< kServerGrid, kTile, 0, stream >>>( A );
+         outerResidKernel<<< kServerGrid, kTile, 0, stream >>>( A );
+         restartKernel<<< kServerGrid, kTile, 0, stream >>>( A );
+      }
+   }
+
+   ~tnlhip_server_s()
+   {
+      if( exec )
+         (void) hipGraphExecDestroy( exec );
+      if( graph )
+         (void) hipGraphDestroy( graph );
+      if( stream )
+         (void) hipStreamSynchronize( stream );
+      for( void* p : { (void*) hTileReq, (void*) hTileFirst, (void*) hDyn, (void*) hStatic, (void*) hSys, (void*) hCtl } )
+         if( p )
+            (void) hipHostFree( p );
+      if( stream )
+         (void) hipStreamDestroy( stream );
+   }
+};
+
+namespace {
+
+int64_t requestEntries( const tnlhip_server_s& S, const Request& q )
+{
+   return (int64_t) S.matrices[ q.matrix ].n * q.k;
+}
+
+int kpOf( int k )
+{
+   int kp = 1;
+   while( kp < k )
+      kp <<= 1;
+   return kp;
+}
+
+// admit the head of the queue while it fits; returns the number admitted
+int admit( tnlhip_server_s& S )
+{
+   int count = 0;
+   while( ! S.queue.empty() ) {
+      Request& q = S.queue.front();
+      const int64_t E = requestEntries( S, q );
+      int slot = -1;
+      for( int i = 0; i < S.cfg.max_requests; i++ )
+         if( ! S.slotUsed[ i ] ) {
+            slot = i;
+            break;
+         }
+      if( slot < 0 )
+         break;
+      const int64_t off = S.vecAlloc.alloc( E );
+      if( off < 0 )
+         break;
+      const ResidentMatrix& M = S.matrices[ q.matrix ];
+      ReqStatic R{};
+      R.n = M.n;
+      R.k = q.k;
+      R.kp = kpOf( q.k );
+      R.rpt = kTile / R.kp;
+      R.tiles = ( M.n + R.rpt - 1 ) / R.rpt;
+      R.sliceBase = M.sliceBase;
+      R.maxOuter = q.params.max_outer_iterations;
+      R.maxInner = q.params.max_inner_iterations;
+      R.vec = off;
+      R.mat = M.mat;
+      R.shift = q.shift;
+      R.tol = q.params.tolerance;
+      R.innerTol2 = q.params.inner_tolerance * q.params.inner_tolerance;
+      R.stagnation = q.params.stagnation_limit;
+      S.hStatic[ slot ] = R;
+      S.hDyn[ slot ] = ReqDyn{ 1 << P_CORRECT, 0u };
+      for( int j = 0; j < kMaxRhs; j++ ) {
+         Sys s{};
+         s.phase = j < q.k ? P_CORRECT : P_IDLE;
+         s.normb = -1;
+         s.scale = 0;
+         S.hSys[ slot * kMaxRhs + j ] = s;
+      }
+      hipStream_t st = S.stream;
+      TNLHIP_CHECK( hipMemcpyAsync( S.rs.ptr + slot, S.hStatic + slot, sizeof( ReqStatic ), hipMemcpyHostToDevice, st ) );
+      TNLHIP_CHECK( hipMemcpyAsync( S.rd.ptr + slot, S.hDyn + slot, sizeof( ReqDyn ), hipMemcpyHostToDevice, st ) );
+      TNLHIP_CHECK( hipMemcpyAsync( S.sys.ptr + slot * kMaxRhs, S.hSys + slot * kMaxRhs, sizeof( Sys ) * kMaxRhs,
+                                    hipMemcpyHostToDevice, st ) );
+      double* b = S.a64.ptr + 3 * off + E;
+      TNLHIP_CHECK( hipMemcpyAsync( b, q.b, sizeof( double ) * E, hipMemcpyHostToDevice, st ) );
+      admitKernel<<< gridFor( E, 256 ), 256, 0, st >>>( E, S.a64.ptr + 3 * off, S.a32.ptr + 4 * off );
+      TNLHIP_CHECK( hipGetLastError() );
+      q.slot = slot;
+      S.slotUsed[ slot ] = true;
+      S.slots[ slot ] = q;
+      S.matrices[ q.matrix ].users++;
+      S.queue.pop_front();
+      S.tilesDirty = true;
+      count++;
+   }
+   return count;
+}
+
+void rebuildTiles( tnlhip_server_s& S )
+{
+   int32_t t = 0;
+   for( int i = 0; i < S.cfg.max_requests; i++ ) {
+      S.hTileFirst[ i ] = t;
+      if( ! S.slotUsed[ i ] )
+         continue;
+      const ReqStatic& R = S.hStatic[ i ];
+      for( int q = 0; q < R.tiles; q++ )
+         S.hTileReq[ t + q ] = i;
+      t += R.tiles;
+   }
+   TNLHIP_REQUIRE( t <= S.maxTiles, "tile table overflow" );
+   S.hCtl->numTiles = t;
+   hipStream_t st = S.stream;
+   if( t > 0 )
+      TNLHIP_CHECK( hipMemcpyAsync( S.tileReq.ptr, S.hTileReq, sizeof( int32_t ) * t, hipMemcpyHostToDevice, st ) );
+   TNLHIP_CHECK( hipMemcpyAsync( S.tileFirst.ptr, S.hTileFirst, sizeof( int32_t ) * S.cfg.max_requests,
+                                 hipMemcpyHostToDevice, st ) );
+   TNLHIP_CHECK( hipMemcpyAsync( S.ctl.ptr, S.hCtl, sizeof( Ctl ), hipMemcpyHostToDevice, st ) );
+   S.tilesDirty = false;
+}
+
+void releaseMatrixIfUnused( tnlhip_server_s& S, int id )
+{
+   ResidentMatrix& M = S.matrices[ id ];
+   if( M.removing && M.users == 0 && M.mat >= 0 ) {
+      S.matAlloc.release( M.mat, M.slots );
+      S.sliceAlloc.release( M.sliceBase, M.sliceCount );
+      M.mat = -1;
+      M.sliceBase = -1;
+   }
+}
+
+}  // namespace
+
+extern "C" {
+
+void tnlhip_server_config_default( tnlhip_server_config* c )
+{
+   if( ! c )
+      return;
+   std::memset( c, 0, sizeof( *c ) );
+   c->matrix_capacity = 64ll << 20;   // 64 Mi slots: 1 GiB (int32 column + FP64 + FP32 value)
+   c->slice_capacity = 4ll << 20;
+   c->vector_capacity = 64ll << 20;   // 64 Mi entries: 2.5 GiB (3 FP64 + 4 FP32 vectors)
+   c->max_requests = 256;
+   c->iterations_per_launch = 8;
+   c->use_graph = 1;
+   c->staging_capacity = 32ll << 20;
+}
+
+void tnlhip_request_params_default( tnlhip_request_params* p )
+{
+   if( ! p )
+      return;
+   std::memset( p, 0, sizeof( *p ) );
+   p->tolerance = 1e-10;
+   p->inner_tolerance = 1e-4;
+   p->max_outer_iterations = 20;
+   p->max_inner_iterations = 5000;
+   p->method = TNLHIP_CG;
+   p->stagnation_limit = 2;
+}
+
+int tnlhip_server_footprint( int32_t rows, int64_t nnz, int32_t max_row, int32_t nrhs, int64_t* matrix_bytes,
+                             int64_t* request_bytes )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( rows >= 0 && nnz >= 0 && max_row >= 0 && nrhs >= 1 && nrhs <= kMaxRhs, "bad sizes" );
+      const int64_t slots = std::max< int64_t >( nnz, (int64_t) ( ( rows + 31 ) / 32 ) * 32 * max_row );
+      if( matrix_bytes )
+         *matrix_bytes = slots * ( 4 + 8 + 4 ) + 4 * ( rows / 32 + 2 );
+      if( request_bytes )
+         *request_bytes = (int64_t) rows * nrhs * ( 3 * 8 + 4 * 4 );
+   } );
+}
+
+int tnlhip_server_create( const tnlhip_server_config* config, tnlhip_server* out )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( out, "NULL argument" );
+      *out = nullptr;
+      auto S = std::make_unique< tnlhip_server_s >();
+      tnlhip_server_config_default( &S->cfg );
+      if( config )
+         S->cfg = *config;
+      tnlhip_server_config& c = S->cfg;
+      TNLHIP_REQUIRE( c.matrix_capacity > 0 && c.vector_capacity > 0 && c.slice_capacity > 1 && c.max_requests > 0
+                         && c.iterations_per_launch > 0 && c.staging_capacity > 0,
+                      "server capacities must be positive" );
+      TNLHIP_REQUIRE( c.vector_capacity < ( 1ll << 40 ) && c.matrix_capacity < ( 1ll << 40 ), "capacity too large" );
+      TNLHIP_CHECK( hipStreamCreate( &S->stream ) );
+      S->cols.resize( c.matrix_capacity );
+      S->v64.resize( c.matrix_capacity );
+      S->v32.resize( c.matrix_capacity );
+      S->slices.resize( c.slice_capacity );
+      S->a64.resize( 3 * c.vector_capacity );
+      S->a32.resize( 4 * c.vector_capacity );
+      S->maxTiles = c.vector_capacity / 128 + 2 * (int64_t) c.max_requests;
+      TNLHIP_REQUIRE( S->maxTiles < ( 1ll << 31 ), "too many tiles" );
+      S->tileReq.resize( S->maxTiles );
+      S->tileFirst.resize( c.max_requests );
+      S->part.resize( S->maxTiles * 2 * kMaxRhs );
+      S->rs.resize( c.max_requests );
+      S->rd.resize( c.max_requests );
+      S->sys.resize( (int64_t) c.max_requests * kMaxRhs );
+      S->ctl.resize( 1 );
+      S->staging.resize( c.staging_capacity );
+      S->stagingV.resize( c.staging_capacity );
+      S->matAlloc.init( c.matrix_capacity );
+      S->sliceAlloc.init( c.slice_capacity );
+      S->vecAlloc.init( c.vector_capacity );
+      S->slots.resize( c.max_requests );
+      S->slotUsed.assign( c.max_requests, false );
+      TNLHIP_CHECK( hipHostMalloc( (void**) &S->hTileReq, sizeof( int32_t ) * S->maxTiles ) );
+      TNLHIP_CHECK( hipHostMalloc( (void**) &S->hTileFirst, sizeof( int32_t ) * c.max_requests ) );
+      TNLHIP_CHECK( hipHostMalloc( (void**) &S->hDyn, sizeof( ReqDyn ) * c.max_requests ) );
+      TNLHIP_CHECK( hipHostMalloc( (void**) &S->hStatic, sizeof( ReqStatic ) * c.max_requests ) );
+      TNLHIP_CHECK( hipHostMalloc( (void**) &S->hSys, sizeof( Sys ) * c.max_requests * kMaxRhs ) );
+      TNLHIP_CHECK( hipHostMalloc( (void**) &S->hCtl, sizeof( Ctl ) ) );
+      std::memset( S->hStatic, 0, sizeof( ReqStatic ) * c.max_requests );
+      std::memset( S->hDyn, 0, sizeof( ReqDyn ) * c.max_requests );
+      TNLHIP_CHECK( hipMemcpyAsync( S->rs.ptr, S->hStatic, sizeof( ReqStatic ) * c.max_requests, hipMemcpyHostToDevice,
+                                    S->stream ) );
+      TNLHIP_CHECK(
+         hipMemcpyAsync( S->rd.ptr, S->hDyn, sizeof( ReqDyn ) * c.max_requests, hipMemcpyHostToDevice, S->stream ) );
+      rebuildTiles( *S );
+      TNLHIP_CHECK( hipStreamSynchronize( S->stream ) );
+      if( c.use_graph ) {
+         TNLHIP_CHECK( hipStreamBeginCapture( S->stream, hipStreamCaptureModeThreadLocal ) );
+         S->enqueueIterations( c.iterations_per_launch );
+         TNLHIP_CHECK( hipStreamEndCapture( S->stream, &S->graph ) );
+         TNLHIP_CHECK( hipGraphInstantiate( &S->exec, S->graph, nullptr, nullptr, 0 ) );
+      }
+      TNLHIP_CHECK( hipStreamSynchronize( S->stream ) );
+      *out = S.release();
+   } );
+}
+
+int tnlhip_server_destroy( tnlhip_server S )
+{
+   return guarded( [ & ] { delete S; } );
+}
+
+int tnlhip_server_add_matrix( tnlhip_server S, int32_t rows, const int32_t* row_pointers, const int32_t* columns,
+                              const double* values, int32_t* matrix_id )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( S && row_pointers && matrix_id && rows > 0, "bad argument" );
+      const int64_t nnz = row_pointers[ rows ];
+      TNLHIP_REQUIRE( nnz >= 0 && ( nnz == 0 || ( columns && values ) ), "bad CSR arrays" );
+      std::vector< int32_t > caps( rows );
+      for( int32_t i = 0; i < rows; i++ ) {
+         caps[ i ] = row_pointers[ i + 1 ] - row_pointers[ i ];
+         TNLHIP_REQUIRE( caps[ i ] >= 0, "row pointers must not decrease" );
+      }
+      for( int64_t i = 0; i < nnz; i++ )
+         TNLHIP_REQUIRE( columns[ i ] >= 0 && columns[ i ] < rows, "column index out of range (square matrices only)" );
+      HostSegments seg;
+      buildSegments( seg, KIND_SLICED, rows, caps.data() );
+      const int64_t slots = seg.sliceOffsets.back();
+      const int32_t nslices = (int32_t) seg.sliceOffsets.size();
+      TNLHIP_REQUIRE( rows + 1 + nnz <= S->cfg.staging_capacity && nnz <= S->cfg.staging_capacity,
+                      "matrix larger than the server's staging buffer" );
+      const int64_t mat = S->matAlloc.alloc( slots );
+      if( mat < 0 )
+         throw Error( TNLHIP_ERROR_CAPACITY, "no room for the matrix in the server's matrix arena" );
+      const int64_t sb = S->sliceAlloc.alloc( nslices );
+      if( sb < 0 ) {
+         S->matAlloc.release( mat, slots );
+         throw Error( TNLHIP_ERROR_CAPACITY, "no room for the matrix in the server's slice arena" );
+      }
+      hipStream_t st = S->stream;
+      TNLHIP_CHECK( hipStreamSynchronize( st ) );
+      TNLHIP_CHECK( hipMemcpyAsync( S->slices.ptr + sb, seg.sliceOffsets.data(), sizeof( int32_t ) * nslices,
+                                    hipMemcpyHostToDevice, st ) );
+      int32_t* drp = S->staging.ptr;
+      int32_t* dci = S->staging.ptr + rows + 1;
+      TNLHIP_CHECK( hipMemcpyAsync( drp, row_pointers, sizeof( int32_t ) * ( rows + 1 ), hipMemcpyHostToDevice, st ) );
+      if( nnz > 0 ) {
+         TNLHIP_CHECK( hipMemcpyAsync( dci, columns, sizeof( int32_t ) * nnz, hipMemcpyHostToDevice, st ) );
+         TNLHIP_CHECK( hipMemcpyAsync( S->stagingV.ptr, values, sizeof( double ) * nnz, hipMemcpyHostToDevice, st ) );
+      }
+      scatterKernel<<< ( rows + 255 ) / 256, 256, 0, st >>>( rows, drp, dci, S->stagingV.ptr, S->slices.ptr + sb, mat,
+                                                             S->cols.ptr, S->v64.ptr, S->v32.ptr );
+      TNLHIP_CHECK( hipGetLastError() );
+      TNLHIP_CHECK( hipStreamSynchronize( st ) );
+      ResidentMatrix M;
+      M.n = rows;
+      M.mat = mat;
+      M.slots = slots;
+      M.sliceBase = (int32_t) sb;
+      M.sliceCount = nslices;
+      int id = -1;
+      for( size_t i = 0; i < S->matrices.size(); i++ )
+         if( S->matrices[ i ].mat < 0 && S->matrices[ i ].removing ) {
+            id = (int) i;
+            break;
+         }
+      if( id < 0 ) {
+         id = (int) S->matrices.size();
+         S->matrices.push_back( M );
+      }
+      else
+         S->matrices[ id ] = M;
+      *matrix_id = id;
+   } );
+}
+
+int tnlhip_server_remove_matrix( tnlhip_server S, int32_t id )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( S && id >= 0 && id < (int) S->matrices.size() && ! S->matrices[ id ].removing, "unknown matrix" );
+      for( const Request& q : S->queue )
+         TNLHIP_REQUIRE( q.matrix != id, "the matrix has queued requests" );
+      S->matrices[ id ].removing = true;
+      releaseMatrixIfUnused( *S, id );
+   } );
+}
+
+int tnlhip_server_submit( tnlhip_server S, int32_t matrix_id, double shift, int32_t nrhs, const double* b, double* x,
+                          const tnlhip_request_params* params, int64_t* ticket )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( S && b && x && ticket, "NULL argument" );
+      TNLHIP_REQUIRE( matrix_id >= 0 && matrix_id < (int) S->matrices.size() && ! S->matrices[ matrix_id ].removing,
+                      "unknown matrix" );
+      TNLHIP_REQUIRE( nrhs >= 1 && nrhs <= kMaxRhs, "1 to 16 right-hand sides per request" );
+      Request q;
+      tnlhip_request_params_default( &q.params );
+      if( params )
+         q.params = *params;
+      if( q.params.method != TNLHIP_CG )
+         throw Error( TNLHIP_ERROR_UNSUPPORTED, "the solve server runs CG requests (symmetric positive definite A + shift I)" );
+      TNLHIP_REQUIRE( q.params.tolerance > 0 && q.params.inner_tolerance > 0 && q.params.max_outer_iterations >= 0
+                         && q.params.max_inner_iterations > 0,
+                      "bad request parameters" );
+      q.matrix = matrix_id;
+      q.k = nrhs;
+      q.shift = shift;
+      q.b = b;
+      q.x = x;
+      const int64_t E = requestEntries( *S, q );
+      if( E > S->cfg.vector_capacity )
+         throw Error( TNLHIP_ERROR_CAPACITY, "request larger than the server's vector arena" );
+      q.ticket = S->nextTicket++;
+      S->queue.push_back( q );
+      *ticket = q.ticket;
+   } );
+}
+
+int tnlhip_server_progress( tnlhip_server S, int64_t* finished, int32_t capacity, int32_t* count )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( S && count && ( capacity == 0 || finished ), "NULL argument" );
+      *count = 0;
+      hipStream_t st = S->stream;
+      admit( *S );
+      bool any = false;
+      for( int i = 0; i < S->cfg.max_requests; i++ )
+         any = any || S->slotUsed[ i ];
+      if( ! any )
+         return;
+      if( S->tilesDirty )
+         rebuildTiles( *S );
+      if( S->exec )
+         TNLHIP_CHECK( hipGraphLaunch( S->exec, st ) );
+      else
+         S->enqueueIterations( S->cfg.iterations_per_launch );
+      TNLHIP_CHECK( hipGetLastError() );
+      S->launches++;
+      TNLHIP_CHECK( hipMemcpyAsync( S->hDyn, S->rd.ptr, sizeof( ReqDyn ) * S->cfg.max_requests, hipMemcpyDeviceToHost,
+                                    st ) );
+      TNLHIP_CHECK( hipStreamSynchronize( st ) );
+      std::vector< int > done;
+      for( int i = 0; i < S->cfg.max_requests; i++ )
+         if( S->slotUsed[ i ] && ! ( S->hDyn[ i ].mask & kLive ) )
+            done.push_back( i );
+      if( done.empty() )
+         return;
+      for( int i : done ) {
+         const Request& q = S->slots[ i ];
+         const ReqStatic& R = S->hStatic[ i ];
+         const int64_t E = (int64_t) R.n * R.k;
+         TNLHIP_CHECK( hipMemcpyAsync( q.x, S->a64.ptr + 3 * R.vec, sizeof( double ) * E, hipMemcpyDeviceToHost, st ) );
+         TNLHIP_CHECK( hipMemcpyAsync( S->hSys + i * kMaxRhs, S->sys.ptr + i * kMaxRhs, sizeof( Sys ) * kMaxRhs,
+                                       hipMemcpyDeviceToHost, st ) );
+      }
+      TNLHIP_CHECK( hipStreamSynchronize( st ) );
+      for( int i : done ) {
+         const Request& q = S->slots[ i ];
+         tnlhip_request_result r{};
+         r.converged = 1;
+         for( int j = 0; j < q.k; j++ ) {
+            const Sys& s = S->hSys[ i * kMaxRhs + j ];
+            r.converged = r.converged && s.phase == P_DONE;
+            r.outer_iterations = std::max( r.outer_iterations, s.outerIt );
+            r.inner_iterations += s.innerTotal;
+            r.residue = std::max( r.residue, s.res );
+         }
+         r.launches = S->launches;
+         S->finished[ q.ticket ] = r;
+         S->vecAlloc.release( S->hStatic[ i ].vec, (int64_t) S->hStatic[ i ].n * S->hStatic[ i ].k );
+         S->matrices[ q.matrix ].users--;
+         releaseMatrixIfUnused( *S, q.matrix );
+         S->slotUsed[ i ] = false;
+         S->tilesDirty = true;
+         if( *count < capacity )
+            finished[ *count ] = q.ticket;
+         ( *count )++;
+      }
+   } );
+}
+
+int tnlhip_server_result( tnlhip_server S, int64_t ticket, tnlhip_request_result* result )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( S && result, "NULL argument" );
+      auto it = S->finished.find( ticket );
+      TNLHIP_REQUIRE( it != S->finished.end(), "ticket not finished (or already taken)" );
+      *result = it->second;
+      S->finished.erase( it );
+   } );
+}
+
+int tnlhip_server_stats( tnlhip_server S, tnlhip_server_stats_t* stats )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( S && stats, "NULL argument" );
+      std::memset( stats, 0, sizeof( *stats ) );
+      for( int i = 0; i < S->cfg.max_requests; i++ )
+         if( S->slotUsed[ i ] ) {
+            stats->resident_requests++;
+            stats->resident_entries += (int64_t) S->hStatic[ i ].n * S->hStatic[ i ].k;
+         }
+      stats->queued_requests = (int32_t) S->queue.size();
+      stats->vector_free = S->vecAlloc.freeTotal();
+      stats->matrix_free = S->matAlloc.freeTotal();
+      stats->launches = S->launches;
+      stats->tiles = S->hCtl->numTiles;
+   } );
+}
+
+}  // extern "C"

@@ -69,6 +69,7 @@ void tnlhip_refinement_params_default( tnlhip_refinement_params* p )
    p->max_outer_iterations = 50;
    p->inner_method = TNLHIP_GMRES;
    p->inner_value_dtype = TNLHIP_F32;
+   p->stagnation_limit = 2;
    tnlhip_solver_params_default( &p->inner );
    p->inner.convergence_residue = 1e-4;
    p->inner.max_iterations = 100000;
@@ -137,6 +138,8 @@ int tnlhip_refine_solve( tnlhip_matrix A, tnlhip_matrix A_low, const tnlhip_refi
       TNLHIP_CHECK( hipStreamSynchronize( st ) );
       if( normb == 0.0 )
          normb = 1.0;
+      int slow = 0;
+      double prev = 0;
       for( int k = 0;; k++ ) {
          deviceDot( TNLHIP_F64, n, r.ptr, r.ptr, scal.ptr, true, ws, st );
          double nr = 0;
@@ -149,6 +152,10 @@ int tnlhip_refine_solve( tnlhip_matrix A, tnlhip_matrix A_low, const tnlhip_refi
          result->outer_iterations = k;
          if( std::isnan( res ) || res < p->tolerance || nr == 0.0 || k >= p->max_outer_iterations )
             break;
+         slow = ( k > 0 && res > 0.5 * prev ) ? slow + 1 : 0;
+         if( p->stagnation_limit > 0 && slow >= p->stagnation_limit )
+            break;
+         prev = res;
          scaleToFloatKernel<<< gridFor( n, 256 ), 256, 0, st >>>( n, r.ptr, scal.ptr, rhat.ptr );
          TNLHIP_CHECK( hipMemsetAsync( d.ptr, 0, sizeof( float ) * n, st ) );
          int32_t conv = 0;

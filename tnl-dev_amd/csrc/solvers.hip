@@ -105,8 +105,13 @@ __global__ __launch_bounds__( kReduceBlock ) void fusedReduceKernel( int64_t n, 
    for( int64_t i = (int64_t) blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += stride )
       op( i, acc );
    double out[ W ];
-   if( finishReduction< W >( acc, slot, out ) && threadIdx.x == 0 )
-      fin( out );
+   __shared__ double tot[ W ];  // the totals, indexable by the finishing lane
+   if( finishReduction< W >( acc, slot, out ) && threadIdx.x == 0 ) {
+#pragma unroll
+      for( int k = 0; k < W; k++ )
+         tot[ k ] = out[ k ];
+      fin( tot );
+   }
 }
 
 // Serial step on one lane (Hessenberg / Givens / small triangular algebra).
@@ -321,7 +326,7 @@ template< typename T >
 struct FinNormb
 {
    State* st;
-   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   __device__ void operator()( const double* v ) const
    {
       double nb = sqrt( v[ 0 ] );
       st->normb = nb == 0.0 ? 1.0 : nb;
@@ -387,7 +392,7 @@ struct FinCgInit
    double* hist;
    int64_t cap;
    Stop stop;
-   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   __device__ void operator()( const double* v ) const
    {
       State& s = *st;
       s.s1 = v[ 0 ];
@@ -404,7 +409,7 @@ struct FinCgAlpha
    State* st;
    double* hist;
    int64_t cap;
-   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   __device__ void operator()( const double* v ) const
    {
       State& s = *st;
       s.s2 = v[ 0 ];
@@ -451,7 +456,7 @@ struct FinCgBeta
    double* hist;
    int64_t cap;
    Stop stop;
-   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   __device__ void operator()( const double* v ) const
    {
       State& s = *st;
       s.s2 = s.s1;
@@ -511,7 +516,7 @@ struct FinRhoInit
    double* hist;
    int64_t cap;
    Stop stop;
-   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   __device__ void operator()( const double* v ) const
    {
       State& s = *st;
       s.rho = s.r_ast_sqnorm = v[ 0 ];
@@ -543,7 +548,7 @@ struct OpCopy2 : NoScalars
 struct FinBiAlpha
 {
    State* st;
-   __device__ void operator()( const double ( &v )[ 1 ] ) const { st->alpha = st->rho / v[ 0 ]; }
+   __device__ void operator()( const double* v ) const { st->alpha = st->rho / v[ 0 ]; }
 };
 
 template< typename T >
@@ -575,7 +580,7 @@ struct OpBiOmega : NoScalars
 struct FinBiOmega
 {
    State* st;
-   __device__ void operator()( const double ( &v )[ 2 ] ) const { st->omega = v[ 0 ] / v[ 1 ]; }
+   __device__ void operator()( const double* v ) const { st->omega = v[ 0 ] / v[ 1 ]; }
 };
 
 template< typename T >
@@ -615,7 +620,7 @@ struct FinBiRho
    Stop stop;
    double eps2;
    bool exact;
-   __device__ void operator()( const double ( &v )[ 2 ] ) const
+   __device__ void operator()( const double* v ) const
    {
       State& s = *st;
       s.rho_old = s.rho;
@@ -655,7 +660,7 @@ struct FinBiRestart
    int64_t cap;
    Stop stop;
    bool exact;
-   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   __device__ void operator()( const double* v ) const
    {
       State& s = *st;
       s.rho = s.r_ast_sqnorm = v[ 0 ];
@@ -696,7 +701,7 @@ struct FinExactResidue
    double* hist;
    int64_t cap;
    Stop stop;
-   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   __device__ void operator()( const double* v ) const
    {
       State& s = *st;
       setResidue( s, sqrt( v[ 0 ] ) / s.normb, hist, cap );
@@ -890,7 +895,7 @@ struct FinMgsCoef
    State* st;
    double* Hki;
    double* coef;
-   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   __device__ void operator()( const double* v ) const
    {
       *Hki += v[ 0 ];
       *coef = v[ 0 ];
@@ -901,7 +906,7 @@ struct FinNormW
 {
    State* st;
    double* Hi1i;
-   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   __device__ void operator()( const double* v ) const
    {
       const double nw = sqrt( v[ 0 ] );
       st->normw = nw;
@@ -960,9 +965,12 @@ struct OpCgs
       }
       if( norm )
          acc[ 0 ] += wi * wi;
-      else
-         for( int k = 0; k < count; k++ )
-            acc[ k ] += (double) V[ i + k * ld ] * wi;
+      else {
+#pragma unroll
+         for( int k = 0; k < kMaxMulti; k++ )
+            if( k < count )
+               acc[ k ] += (double) V[ i + k * ld ] * wi;
+      }
    }
 };
 
@@ -972,7 +980,7 @@ struct FinCgsCoef
    double* Hcol;
    double* Hl;
    int count;
-   __device__ void operator()( const double ( &v )[ kMaxMulti ] ) const
+   __device__ void operator()( const double* v ) const
    {
       for( int k = 0; k < count; k++ ) {
          Hcol[ k ] += v[ k ];
@@ -985,7 +993,7 @@ struct FinCgsNorm
 {
    State* st;
    double* Hi1i;
-   __device__ void operator()( const double ( &v )[ kMaxMulti ] ) const
+   __device__ void operator()( const double* v ) const
    {
       const double nw = sqrt( v[ 0 ] );
       st->normw = nw;
@@ -1088,7 +1096,7 @@ struct FinHouseholder
    Small q;
    T* yi;
    int i;
-   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   __device__ void operator()( const double* v ) const
    {
       const double normz = sqrt( v[ 0 ] );
       const double y_ii = (double) yi[ i ];
@@ -1126,7 +1134,7 @@ struct FinTUpdate
    // [T_i]_{0..i-1} = - T_{i-1} * t_i * aux
    Small q;
    int i;
-   __device__ void operator()( const double ( &aux )[ kMaxMulti ] ) const
+   __device__ void operator()( const double* aux ) const
    {
       const int mx = q.mx;
       const double ti = q.T[ i + i * ( mx + 1 ) ];
@@ -1194,8 +1202,10 @@ struct OpMulti
    __device__ void operator()( int64_t i, double ( &acc )[ kMaxMulti ] ) const
    {
       const double wi = (double) w[ i ];
-      for( int k = 0; k < count; k++ )
-         acc[ k ] += (double) Y[ i + k * ld ] * wi;
+#pragma unroll
+      for( int k = 0; k < kMaxMulti; k++ )
+         if( k < count )
+            acc[ k ] += (double) Y[ i + k * ld ] * wi;
    }
 };
 
@@ -1204,10 +1214,10 @@ struct FinCwyTransposed
    // aux = T_i^T * (Y_i^T w)
    Small q;
    int i;
-   __device__ void operator()( const double ( &v )[ kMaxMulti ] ) const
+   __device__ void operator()( const double* v ) const
    {
       const int mx = q.mx;
-      double a[ kMaxMulti ];
+      double* a = q.aux;  // in place: step k reads a[0..k], which later steps no longer need
       for( int k = 0; k <= i; k++ )
          a[ k ] = v[ k ];
       for( int k = i; k >= 0; k-- ) {
@@ -1216,8 +1226,6 @@ struct FinCwyTransposed
             a2 += q.T[ j + k * ( mx + 1 ) ] * a[ j ];
          a[ k ] = a2;
       }
-      for( int k = 0; k <= i; k++ )
-         q.aux[ k ] = a[ k ];
    }
 };
 
@@ -1247,7 +1255,7 @@ struct FinBeta
    State* st;
    double* hist;
    int64_t cap;
-   __device__ void operator()( const double ( &v )[ 1 ] ) const
+   __device__ void operator()( const double* v ) const
    {
       State& s = *st;
       s.beta_gm = sqrt( v[ 0 ] );
