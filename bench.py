@@ -102,45 +102,219 @@ def timed(torch, fn):
 
 def solver_suite(tnl, torch, W, dev, args):
     """Convection-diffusion 7-point upwind (v = (1, 0.5, 0.25)) on a cd_grid^3 box, Jacobi
-    GMRES(30) (CWY) to 1e-10: plain FP64 against FP64 refinement over an FP32 and a BF16 inner
-    solve, for each Peclet number."""
+    GMRES(30) to 1e-10 from x0 = 0: plain FP64 against FP64 refinement over an FP32 and a BF16
+    inner GMRES(30) (inner tolerance 1e-4), for each Peclet number and orthogonalization
+    variant.  The matrix format of each precision is the autotuner's choice (tuning time is
+    reported, not timed)."""
     N = args.cd_grid
-    res = {"workload": f"3D convection-diffusion upwind {N}^3, Jacobi GMRES(30) CWY, tol 1e-10, x0 = 0"}
+    variants = args.gmres_variants.split(",")
+    res = {"workload": f"3D convection-diffusion upwind {N}^3, Jacobi GMRES(30), tol 1e-10, x0 = 0",
+           "variants": variants}
     for pe in [float(p) for p in args.peclet.split(",")]:
         rp, c, v = W.convection_diffusion_3d(N, pe)
         n = len(rp) - 1
         b = torch.tensor(W.uniform_vector(n, 77), device=dev)
-        A = tnl.SparseMatrix.from_csr(rp, c, v, format="CSRAdaptive", dtype="f64")
-        entry = {}
-        s = tnl.GMRES()
-        s.setMatrix(A)
-        s.setup(convergence_residue=1e-10, max_iterations=200000, restarting_min=30, restarting_max=30,
-                gmres_variant="CWY", preconditioner=1)
-        x = torch.zeros(n, dtype=torch.float64, device=dev)
-        ok, t64 = timed(torch, lambda: s.solve(b, x))
-        entry["fp64"] = {"converged": ok, "iterations": s.getIterations(), "seconds": round(t64, 4),
-                         "residue": s.getResidue()}
-        log(f"Pe={pe:g} FP64 GMRES: {s.getIterations()} iterations, {t64:.3f} s, converged={ok}")
-        x64 = x.clone()
+        t0 = time.perf_counter()
+        A, best64, _ = tnl.autotune(rp, c, v, dtype="f64", repetitions=5)
+        lows = {}
         for dt in ("f32", "bf16"):
-            A_low = A.convert("CSRAdaptive", dt)
+            lows[dt] = tnl.autotune(rp, c, v, dtype=dt, repetitions=5)
+        entry = {"formats": {"f64": best64["format"], "f32": lows["f32"][1]["format"],
+                             "bf16": lows["bf16"][1]["format"]},
+                 "tuning_seconds": round(time.perf_counter() - t0, 2)}
+        for variant in variants:
+            ve = {}
+            s = tnl.GMRES()
+            s.setMatrix(A)
+            s.setup(convergence_residue=1e-10, max_iterations=100000, restarting_min=30, restarting_max=30,
+                    gmres_variant=variant, preconditioner=1)
             x = torch.zeros(n, dtype=torch.float64, device=dev)
-            r, t = timed(torch, lambda: tnl.refine_solve(A, b, x, A_low=A_low, tolerance=1e-10, inner_method="gmres",
-                                                         inner_dtype=dt, convergence_residue=1e-4,
-                                                         max_iterations=200000, restarting_min=30,
-                                                         restarting_max=30, gmres_variant="CWY", preconditioner=1))
-            err = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))
-            entry[dt + "_refined"] = {"converged": r["converged"], "outer": r["outer_iterations"],
-                                      "inner_iterations": r["inner_iterations"], "seconds": round(t, 4),
-                                      "residue": r["residue"], "speedup_vs_fp64": round(t64 / t, 3),
-                                      "rel_error_vs_fp64_solve": err}
-            log(f"Pe={pe:g} {dt} refined: outer {r['outer_iterations']} inner {r['inner_iterations']} "
-                f"{t:.3f} s speedup {t64 / t:.2f}x residue {r['residue']:.2e} err vs FP64 {err:.2e}")
-            del A_low
+            ok, t64 = timed(torch, lambda: s.solve(b, x))
+            ve["fp64"] = {"converged": ok, "iterations": s.getIterations(), "seconds": round(t64, 4),
+                          "residue": s.getResidue()}
+            log(f"Pe={pe:g} {variant} FP64 GMRES ({best64['format']}): {s.getIterations()} iterations, "
+                f"{t64:.3f} s, converged={ok}")
+            x64 = x.clone()
+            for dt in ("f32", "bf16"):
+                A_low = lows[dt][0]
+                x = torch.zeros(n, dtype=torch.float64, device=dev)
+                r, t = timed(torch, lambda: tnl.refine_solve(A, b, x, A_low=A_low, tolerance=1e-10,
+                                                             inner_method="gmres", inner_dtype=dt,
+                                                             convergence_residue=1e-4, max_iterations=3000,
+                                                             restarting_min=30, restarting_max=30,
+                                                             gmres_variant=variant, preconditioner=1))
+                err = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))
+                ve[dt + "_refined"] = {"converged": r["converged"], "outer": r["outer_iterations"],
+                                       "inner_iterations": r["inner_iterations"], "seconds": round(t, 4),
+                                       "residue": r["residue"], "speedup_vs_fp64": round(t64 / t, 3),
+                                       "rel_error_vs_fp64_solve": err}
+                log(f"Pe={pe:g} {variant} {dt} refined ({lows[dt][1]['format']}): outer {r['outer_iterations']} "
+                    f"inner {r['inner_iterations']} {t:.3f} s speedup {t64 / t:.2f}x residue {r['residue']:.2e} "
+                    f"err vs FP64 {err:.2e} converged={r['converged']}")
+            entry[variant] = ve
         res[f"Pe={pe:g}"] = entry
-        del A
+        del A, lows
         torch.cuda.empty_cache()
     return res
+
+
+def irregular_suite(tnl, torch, W, dev, args):
+    """n = 2M rows, off-diagonal lengths lognormal (mean 16, sigma 1.2) clipped to [1, 4096],
+    columns uniform without repeats, values N(0, 1), diagonal 1 + sum |offdiag|: SpMV in
+    CSR-Adaptive, ChunkedEllpack and BiEllpack (FP64 and FP32), and Jacobi BiCGStab to 1e-10."""
+    n = args.irr_rows
+    t0 = time.perf_counter()
+    rp, c, v = W.irregular(n, 16.0, 1.2, 4096, seed=2024)
+    nnz = len(c)
+    log(f"irregular: n={n} nnz={nnz} max row {int(np.diff(rp).max())} ({time.perf_counter() - t0:.1f} s to generate)")
+    x_host = W.uniform_vector(n, 5)
+    flush = torch.zeros(64 << 20, dtype=torch.float64, device=dev)
+    out = {"workload": f"irregular lognormal rows n={n} nnz={nnz} (mean 16, sigma 1.2, max 4096), diagonal 1+sum|a|",
+           "spmv": {}}
+    for dt, vt, sv in (("f64", torch.float64, 8), ("f32", torch.float32, 4)):
+        B = W.algorithmic_bytes(n, n, nnz, sv)
+        x = torch.tensor(x_host, device=dev).to(vt)
+        y = torch.zeros(n, dtype=vt, device=dev)
+        for fmt in ("CSRAdaptive", "ChunkedEllpack", "BiEllpack"):
+            A = tnl.SparseMatrix.from_csr(rp, c, v, format=fmt, dtype=dt)
+            steady, cold = time_spmv(tnl, torch, A, x, y, args.steps, args.warmup, flush)
+            mean = steady if B > MALL_BYTES else cold
+            gbs = B / mean / 1e9
+            out["spmv"][f"{fmt}_{dt}"] = {"us": round(mean * 1e6, 2), "GB/s": round(gbs, 1),
+                                          "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "us_cold": round(cold * 1e6, 2)}
+            log(f"  {fmt:15s} {dt}: {mean * 1e6:8.2f} us {gbs:7.1f} GB/s {100 * gbs / HBM_PEAK_GBS:5.1f}%")
+            del A
+        del x, y
+    del flush
+    torch.cuda.empty_cache()
+    A = tnl.SparseMatrix.from_csr(rp, c, v, format="CSRAdaptive", dtype="f64")
+    b = torch.tensor(W.uniform_vector(n, 6), device=dev)
+    s = tnl.BICGStab()
+    s.setMatrix(A)
+    s.setup(convergence_residue=1e-10, max_iterations=10000, preconditioner=1)
+    for rep in range(2):
+        xs = torch.zeros(n, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ok = s.solve(b, xs)
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - t0
+    out["bicgstab_jacobi"] = {"converged": ok, "iterations": s.getIterations(), "seconds": round(secs, 4),
+                              "ms_per_iteration": round(1e3 * secs / max(1, s.getIterations()), 3),
+                              "residue": s.getResidue()}
+    log(f"  Jacobi BiCGStab: {s.getIterations()} iterations, {secs * 1e3:.1f} ms, converged={ok}")
+    del A
+    torch.cuda.empty_cache()
+    return out
+
+
+def serving_ladder(W):
+    """Resident matrices of the serving workload: 2D 5-point and 3D 7-point Laplacians whose
+    sizes step geometrically from about 1e3 to 1e6 rows (8 sizes per dimension)."""
+    out = []
+    for side in np.unique(np.round(np.geomspace(32, 1000, 8)).astype(int)):
+        out.append(("2d", int(side), W.laplacian_2d(int(side))))
+    for side in np.unique(np.round(np.geomspace(10, 100, 8)).astype(int)):
+        out.append(("3d", int(side), W.laplacian_3d(int(side))))
+    return out
+
+
+def serving_requests(ladder, count, seed):
+    """count requests: dimension 2 or 3, rows log-uniform in [1e3, 1e6] (the nearest ladder
+    matrix of that dimension), shift ~ U[0, 1], k ~ U{1..16} right-hand sides ~ U[-1, 1]."""
+    rng = np.random.default_rng(seed)
+    sizes = {d: [(i, len(m[0]) - 1) for i, (dd, _, m) in enumerate(ladder) if dd == d] for d in ("2d", "3d")}
+    reqs = []
+    for _ in range(count):
+        d = "2d" if rng.random() < 0.5 else "3d"
+        target = np.exp(rng.uniform(np.log(1e3), np.log(1e6)))
+        idx = min(sizes[d], key=lambda t: abs(np.log(t[1]) - np.log(target)))[0]
+        reqs.append((idx, float(rng.uniform(0, 1)), int(rng.integers(1, 17))))
+    return reqs
+
+
+def serve(tnl, server, mids, ladder, reqs, pool, arrivals):
+    """Submit each request at its arrival time (None: all at once), progress the server until
+    every request finished; returns (wall seconds, latencies, results)."""
+    offs = np.random.default_rng(99).integers(0, len(pool) - 16 * 10 ** 6, len(reqs))
+    n_of = [len(m[0]) - 1 for _, _, m in ladder]
+    lat = np.zeros(len(reqs))
+    info = [None] * len(reqs)
+    ticket_of = {}
+    t0 = time.perf_counter()
+    arrive = np.zeros(len(reqs)) if arrivals is None else arrivals
+    nxt = 0
+    done = 0
+    last_log = t0
+    while done < len(reqs):
+        now = time.perf_counter() - t0
+        while nxt < len(reqs) and arrive[nxt] <= now:
+            idx, shift, k = reqs[nxt]
+            n = n_of[idx]
+            B = pool[offs[nxt]:offs[nxt] + n * k].reshape(n, k)
+            ticket_of[server.submit(mids[idx], B, shift=shift)] = nxt
+            nxt += 1
+        st = server.stats()
+        if st["resident_requests"] == 0 and st["queued_requests"] == 0:
+            if nxt < len(reqs):
+                time.sleep(max(0.0, min(arrive[nxt] - now, 0.001)))
+            continue
+        fin = server.progress()
+        tf = time.perf_counter() - t0
+        for t in fin:
+            i = ticket_of.pop(t)
+            X, inf = server.result(t)
+            lat[i] = tf - arrive[i]
+            info[i] = inf
+            done += 1
+        if time.perf_counter() - last_log > 20:
+            last_log = time.perf_counter()
+            log(f"  serving: {done}/{len(reqs)} finished, {nxt} arrived")
+    return time.perf_counter() - t0, lat, info
+
+
+def serving_suite(tnl, W, args):
+    """Continuous-batching solve server: Laplacian + shift I requests with 1-16 right-hand sides,
+    FP64 refinement over FP32 CG to 1e-10 per right-hand side.  First all requests at once
+    (throughput), then Poisson arrivals at 70 % of that throughput (latency)."""
+    ladder = serving_ladder(W)
+    server = tnl.Server(vector_capacity=96 << 20, matrix_capacity=48 << 20, slice_capacity=4 << 20,
+                        max_requests=256, iterations_per_launch=args.serve_iterations)
+    t0 = time.perf_counter()
+    mids = [server.add_matrix(*m) for _, _, m in ladder]
+    setup = time.perf_counter() - t0
+    count = args.serve_requests
+    reqs = serving_requests(ladder, count, 2024)
+    pool = np.random.default_rng(7).uniform(-1, 1, 32 * 10 ** 6)
+    n_of = [len(m[0]) - 1 for _, _, m in ladder]
+    rhs = sum(k for _, _, k in reqs)
+    log(f"serving: {len(ladder)} resident matrices ({setup:.2f} s to upload), {count} requests, {rhs} right-hand sides")
+    # warm-up: a few requests through the whole path
+    serve(tnl, server, mids, ladder, reqs[:8], pool, None)
+    wall, lat_sat, info = serve(tnl, server, mids, ladder, reqs, pool, None)
+    conv = sum(1 for i in info if i["converged"])
+    rate = count / wall
+    log(f"serving (all at t=0): {count} requests in {wall:.2f} s = {rate:.1f} solves/s "
+        f"({rhs / wall:.1f} RHS/s), converged {conv}/{count}")
+    lam = 0.7 * rate
+    arrivals = np.cumsum(np.random.default_rng(11).exponential(1.0 / lam, count))
+    wall2, lat, info2 = serve(tnl, server, mids, ladder, reqs, pool, arrivals)
+    conv2 = sum(1 for i in info2 if i["converged"])
+    p50, p99 = np.percentile(lat, [50, 99])
+    log(f"serving (Poisson, {lam:.1f} req/s offered): p50 {p50 * 1e3:.1f} ms, p99 {p99 * 1e3:.1f} ms, "
+        f"converged {conv2}/{count}")
+    server.close()
+    entries = sum(n_of[i] * k for i, _, k in reqs)
+    return {"workload": "2D/3D Laplacian + shift*I, shift ~ U[0,1], rows log-uniform 1e3..1e6 (16 resident "
+                        "matrices), 1-16 RHS ~ U[-1,1] per request, FP64 refinement over FP32 CG, tol 1e-10 per RHS",
+            "requests": count, "rhs": rhs, "entries": entries,
+            "throughput": {"solves_per_s": round(rate, 2), "rhs_per_s": round(rhs / wall, 2),
+                           "seconds": round(wall, 3), "converged": conv},
+            "latency": {"offered_req_per_s": round(lam, 2), "p50_ms": round(p50 * 1e3, 2),
+                        "p99_ms": round(p99 * 1e3, 2), "mean_ms": round(float(lat.mean()) * 1e3, 2),
+                        "seconds": round(wall2, 3), "converged": conv2},
+            "inner_iterations_mean": round(float(np.mean([i["inner_iterations"] for i in info])), 1),
+            "iterations_per_launch": args.serve_iterations}
 
 
 def main():
@@ -154,6 +328,13 @@ def main():
     ap.add_argument("--no-solvers", action="store_true")
     ap.add_argument("--cd-grid", type=int, default=128, help="convection-diffusion grid side")
     ap.add_argument("--peclet", default="1,10,100")
+    ap.add_argument("--gmres-variants", default="CWY,CGS")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-irregular", action="store_true")
+    ap.add_argument("--irr-rows", type=int, default=2000000)
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-iterations", type=int, default=8)
+    ap.add_argument("--only", default="", help="run one suite: spmv | solvers | serving")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -166,6 +347,15 @@ def main():
 
     tnl = load_package()
     dev = torch.device("cuda:0")
+    if args.only == "serving":
+        print(json.dumps(serving_suite(tnl, W, args)))
+        return
+    if args.only == "irregular":
+        print(json.dumps(irregular_suite(tnl, torch, W, dev, args)))
+        return
+    if args.only == "solvers":
+        print(json.dumps(solver_suite(tnl, torch, W, dev, args)))
+        return
     N = args.grid
     rp, cols, vals = W.laplacian_3d(N)
     n = len(rp) - 1
@@ -259,6 +449,12 @@ def main():
         del x, y, flush
         torch.cuda.empty_cache()
         out["solvers"] = solver_suite(tnl, torch, W, dev, args)
+    if not args.no_irregular:
+        torch.cuda.empty_cache()
+        out["irregular"] = irregular_suite(tnl, torch, W, dev, args)
+    if not args.no_serving:
+        torch.cuda.empty_cache()
+        out["serving"] = serving_suite(tnl, W, args)
     if not args.no_cpu:
         log("timing the CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(rp, cols, vals, x_host, n)

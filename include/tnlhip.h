@@ -270,6 +270,31 @@ void tnlhip_refinement_params_default( tnlhip_refinement_params* params );
 int tnlhip_refine_solve( tnlhip_matrix A, tnlhip_matrix A_low, const tnlhip_refinement_params* params,
                          const double* b, double* x, tnlhip_refinement_result* result, void* stream );
 
+/* ------------------------------------------------------------------ autotuner */
+
+/* One timed configuration: format, lanes per row (CSR light / hybrid; 0 = the kernel's
+ * choice), workgroups per CU and rows per lane (padded formats; 0 / -1 = defaults), seconds per
+ * y = A x. */
+typedef struct
+{
+   int32_t format;
+   int32_t lanes;
+   int32_t blocks_per_cu;
+   int32_t pair_rows;
+   double seconds;
+} tnlhip_tuning;
+
+/* Build the CSR matrix (arguments as tnlhip_matrix_create_csr) in every format whose bit is set
+ * in format_mask (bit = tnlhip_format value), time `repetitions` products y = A x for each
+ * candidate launch configuration, and report the fastest in *best (all timings in `all`, up to
+ * all_capacity; *all_count gets the number timed).  With best_matrix non-NULL the fastest
+ * matrix is returned, configured, and owned by the caller.  No counterpart in the reference
+ * (the format is TNL's Segments template parameter, SparseMatrix.h:40-49). */
+int tnlhip_autotune_csr( int value_dtype, int32_t rows, int32_t columns, const int32_t* row_pointers,
+                         const int32_t* column_indexes, const double* values, int32_t host_pointers,
+                         uint32_t format_mask, int32_t repetitions, tnlhip_tuning* best, tnlhip_tuning* all,
+                         int32_t all_capacity, int32_t* all_count, tnlhip_matrix* best_matrix );
+
 /* ------------------------------------------------------------------ solve server (batched) */
 
 /* Many uneven systems with 1-16 right-hand sides each, resident together and advanced by one

@@ -36,7 +36,7 @@ EXPORTS = [
     "tnlhip_refine_solve", "tnlhip_server_config_default", "tnlhip_request_params_default",
     "tnlhip_server_footprint", "tnlhip_server_create", "tnlhip_server_destroy", "tnlhip_server_add_matrix",
     "tnlhip_server_remove_matrix", "tnlhip_server_submit", "tnlhip_server_progress", "tnlhip_server_result",
-    "tnlhip_server_stats",
+    "tnlhip_server_stats", "tnlhip_autotune_csr",
 ]
 
 
@@ -79,6 +79,11 @@ class RefinementParams(C.Structure):
 class RefinementResult(C.Structure):
     _fields_ = [("converged", C.c_int32), ("outer_iterations", C.c_int32), ("inner_iterations", C.c_int64),
                 ("residue", C.c_double), ("history", C.c_double * REFINEMENT_HISTORY)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("format", C.c_int32), ("lanes", C.c_int32), ("blocks_per_cu", C.c_int32), ("pair_rows", C.c_int32),
+                ("seconds", C.c_double)]
 
 
 class ServerConfig(C.Structure):
@@ -150,6 +155,8 @@ def _load():
         "tnlhip_server_progress": (C.c_int, [vp, vp, i32, C.POINTER(i32)]),
         "tnlhip_server_result": (C.c_int, [vp, i64, C.POINTER(RequestResult)]),
         "tnlhip_server_stats": (C.c_int, [vp, C.POINTER(ServerStats)]),
+        "tnlhip_autotune_csr": (C.c_int, [C.c_int, i32, i32, vp, vp, vp, i32, C.c_uint32, i32, C.POINTER(Tuning),
+                                          C.POINTER(Tuning), i32, C.POINTER(i32), C.POINTER(vp)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -593,6 +600,33 @@ class Server:
             s = self.stats()
             if s["resident_requests"] == 0 and s["queued_requests"] == 0:
                 return done
+
+
+def autotune(row_pointers, column_indexes, values, dtype="f64", formats=None, repetitions=10, columns=None):
+    """Time y = A x for every format (or the given list) and launch configuration; returns
+    (the fastest SparseMatrix, its configuration, every timed configuration).  Host numpy CSR."""
+    rp = np.ascontiguousarray(row_pointers, np.int32)
+    ci = np.ascontiguousarray(column_indexes, np.int32)
+    va = np.ascontiguousarray(values, np.float64)
+    rows = len(rp) - 1
+    cols = rows if columns is None else columns
+    names = list(FORMATS) if formats is None else list(formats)
+    mask = 0
+    for f in names:
+        mask |= 1 << FORMATS[f]
+    best = Tuning()
+    cap = 64
+    allt = (Tuning * cap)()
+    cnt = C.c_int32()
+    h = C.c_void_p()
+    _check(lib.tnlhip_autotune_csr(_dtype_id(dtype), rows, cols, rp.ctypes.data, ci.ctypes.data, va.ctypes.data, 1,
+                                   mask, int(repetitions), C.byref(best), allt, cap, C.byref(cnt), C.byref(h)))
+
+    def as_dict(t):
+        return {"format": FORMAT_NAMES[t.format], "lanes": t.lanes, "blocks_per_cu": t.blocks_per_cu,
+                "pair_rows": t.pair_rows, "seconds": t.seconds}
+
+    return SparseMatrix(h), as_dict(best), [as_dict(allt[i]) for i in range(min(cnt.value, cap))]
 
 
 def server_footprint(rows, nnz, max_row, nrhs):

@@ -16,9 +16,10 @@
 //                order and every reduction of a request sums its tiles' partials in that order,
 //                so a request's result does not depend on what else is resident.
 //
-// One iteration of the replayed sequence runs six kernels over all tiles; each kernel acts on
-// the right-hand sides in one phase (per-system phase machine below) and requests with no
-// system in that phase skip their tiles:
+// One iteration of the replayed sequence runs six tile kernels over all tiles; each acts on
+// the right-hand sides in one phase (per-system phase machine below), requests with no system
+// in that phase skip their tiles, and each reduction is finished by a second kernel with one
+// workgroup per request (sums of the tile partials in tile order, then the scalar step):
 //   innerMatvec  INNER    q = (A32 + s I) p, (p, q)          -> alpha
 //   innerUpdate  INNER    d += alpha p, rr -= alpha q, (rr, rr) -> beta, inner stop test
 //   innerDir     INNER    p = rr + beta p
@@ -61,8 +62,8 @@ struct ReqStatic
 // device-written
 struct ReqDyn
 {
-   int32_t mask;     // bit p set when some system of the request is in phase p
-   uint32_t counter; // tiles arrived in the current kernel
+   int32_t mask;  // bit p set when some system of the request is in phase p
+   int32_t pad;
 };
 
 struct Sys
@@ -92,6 +93,7 @@ struct Arena
    const int32_t* slices;
    double* a64;   // x, b, r per request: 3 E
    float* a32;    // d, rr, p, q per request: 4 E
+   int maxRequests;
 };
 
 struct TileCtx
@@ -157,20 +159,6 @@ __device__ __forceinline__ void tilePartials( double* part, int t, int kp, const
    }
 }
 
-// Ticket: returns true in every thread of the block that completes the request's last tile.
-__device__ __forceinline__ bool lastTile( ReqDyn& d, int tiles )
-{
-   __shared__ int isLast;
-   __threadfence();
-   __syncthreads();
-   if( threadIdx.x == 0 )
-      isLast = atomicAdd( &d.counter, 1u ) == (uint32_t) ( tiles - 1 );
-   __syncthreads();
-   if( isLast )
-      __threadfence();
-   return isLast;
-}
-
 // Totals over the request's tiles, tile order fixed; returns LDS holding tot[w * kMaxRhs + j],
 // valid for every thread on return.
 template< int W >
@@ -195,19 +183,31 @@ __device__ __forceinline__ const double* requestTotals( const double* part, int 
    return out;
 }
 
-// Recompute the request's phase mask after threads j < k updated their systems.
-__device__ __forceinline__ void publishMask( const Arena& A, int req, const ReqStatic& R )
+// Per-request finish of a reduction kernel: one workgroup per resident request whose mask
+// holds `phase` sums its tiles' partials, lanes j < k update system j, and the request's phase
+// mask is recomputed.  Step( Sys&, const double* tot, int j, const ReqStatic& ).
+template< int W, typename Step >
+__global__ __launch_bounds__( kTile ) void requestFinalKernel( Arena A, int phase, Step step )
 {
    __shared__ int m;
-   if( threadIdx.x == 0 )
-      m = 0;
-   __syncthreads();
-   if( threadIdx.x < R.k )
-      atomicOr( &m, 1 << A.sys[ req * kMaxRhs + threadIdx.x ].phase );
-   __syncthreads();
-   if( threadIdx.x == 0 ) {
-      A.rd[ req ].mask = m;
-      A.rd[ req ].counter = 0;
+   for( int req = blockIdx.x; req < A.maxRequests; req += gridDim.x ) {
+      if( ! ( A.rd[ req ].mask & ( 1 << phase ) ) )
+         continue;
+      const ReqStatic R = A.rs[ req ];
+      const double* tot = requestTotals< W >( A.part, A.tileFirst[ req ], R.tiles, R.kp );
+      if( threadIdx.x == 0 )
+         m = 0;
+      if( threadIdx.x < R.k ) {
+         Sys& s = A.sys[ req * kMaxRhs + threadIdx.x ];
+         step( s, tot, threadIdx.x, R );
+      }
+      __syncthreads();
+      if( threadIdx.x < R.k )
+         atomicOr( &m, 1 << A.sys[ req * kMaxRhs + threadIdx.x ].phase );
+      __syncthreads();
+      if( threadIdx.x == 0 )
+         A.rd[ req ].mask = m;
+      __syncthreads();
    }
 }
 
@@ -222,16 +222,16 @@ __device__ __forceinline__ double* a64( const Arena& A, const ReqStatic& R, int 
 enum { X64 = 0, B64 = 1, R64 = 2 };
 enum { D32 = 0, RR32 = 1, P32 = 2, Q32 = 3 };
 
-// ---------------------------------------------------------------- kernels
+// ---------------------------------------------------------------- tile kernels
 
 __global__ __launch_bounds__( kTile ) void innerMatvecKernel( Arena A )
 {
    const int nt = A.ctl->numTiles;
    for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
       const int req = A.tileReq[ t ];
-      const ReqStatic R = A.rs[ req ];
       if( ! ( A.rd[ req ].mask & ( 1 << P_INNER ) ) )
          continue;
+      const ReqStatic R = A.rs[ req ];
       const TileCtx c = tileCtx( A, t, R );
       double v[ 1 ] = { 0 };
       if( c.active && A.sys[ req * kMaxRhs + c.j ].phase == P_INNER ) {
@@ -242,33 +242,32 @@ __global__ __launch_bounds__( kTile ) void innerMatvecKernel( Arena A )
          v[ 0 ] = (double) p[ c.e ] * (double) q;
       }
       tilePartials< 1 >( A.part, t, R.kp, v );
-      if( lastTile( A.rd[ req ], R.tiles ) ) {
-         const double* tot = requestTotals< 1 >( A.part, A.tileFirst[ req ], R.tiles, R.kp );
-         if( threadIdx.x < R.k ) {
-            Sys& s = A.sys[ req * kMaxRhs + threadIdx.x ];
-            if( s.phase == P_INNER ) {
-               const double pq = tot[ threadIdx.x ];
-               if( ! ( pq > 0 ) || ! std::isfinite( pq ) )
-                  s.phase = P_CORRECT;  // breakdown: keep the current correction
-               else
-                  s.alpha = s.rho / pq;
-            }
-         }
-         __syncthreads();
-         publishMask( A, req, R );
-      }
       __syncthreads();
    }
 }
+
+struct StepAlpha
+{
+   __device__ void operator()( Sys& s, const double* tot, int j, const ReqStatic& ) const
+   {
+      if( s.phase != P_INNER )
+         return;
+      const double pq = tot[ j ];
+      if( ! ( pq > 0 ) || ! isfinite( pq ) )
+         s.phase = P_CORRECT;  // breakdown: keep the current correction
+      else
+         s.alpha = s.rho / pq;
+   }
+};
 
 __global__ __launch_bounds__( kTile ) void innerUpdateKernel( Arena A )
 {
    const int nt = A.ctl->numTiles;
    for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
       const int req = A.tileReq[ t ];
-      const ReqStatic R = A.rs[ req ];
       if( ! ( A.rd[ req ].mask & ( 1 << P_INNER ) ) )
          continue;
+      const ReqStatic R = A.rs[ req ];
       const TileCtx c = tileCtx( A, t, R );
       double v[ 1 ] = { 0 };
       if( c.active ) {
@@ -283,37 +282,36 @@ __global__ __launch_bounds__( kTile ) void innerUpdateKernel( Arena A )
          }
       }
       tilePartials< 1 >( A.part, t, R.kp, v );
-      if( lastTile( A.rd[ req ], R.tiles ) ) {
-         const double* tot = requestTotals< 1 >( A.part, A.tileFirst[ req ], R.tiles, R.kp );
-         if( threadIdx.x < R.k ) {
-            Sys& s = A.sys[ req * kMaxRhs + threadIdx.x ];
-            if( s.phase == P_INNER ) {
-               const double rho = tot[ threadIdx.x ];
-               s.innerIt++;
-               s.innerTotal++;
-               if( ! std::isfinite( rho ) || rho <= R.innerTol2 * s.rho0 || s.innerIt >= R.maxInner )
-                  s.phase = P_CORRECT;
-               else {
-                  s.beta = rho / s.rho;
-                  s.rho = rho;
-               }
-            }
-         }
-         __syncthreads();
-         publishMask( A, req, R );
-      }
       __syncthreads();
    }
 }
+
+struct StepBeta
+{
+   __device__ void operator()( Sys& s, const double* tot, int j, const ReqStatic& R ) const
+   {
+      if( s.phase != P_INNER )
+         return;
+      const double rho = tot[ j ];
+      s.innerIt++;
+      s.innerTotal++;
+      if( ! isfinite( rho ) || rho <= R.innerTol2 * s.rho0 || s.innerIt >= R.maxInner )
+         s.phase = P_CORRECT;
+      else {
+         s.beta = rho / s.rho;
+         s.rho = rho;
+      }
+   }
+};
 
 __global__ __launch_bounds__( kTile ) void innerDirKernel( Arena A )
 {
    const int nt = A.ctl->numTiles;
    for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
       const int req = A.tileReq[ t ];
-      const ReqStatic R = A.rs[ req ];
       if( ! ( A.rd[ req ].mask & ( 1 << P_INNER ) ) )
          continue;
+      const ReqStatic R = A.rs[ req ];
       const TileCtx c = tileCtx( A, t, R );
       if( c.active ) {
          const Sys& s = A.sys[ req * kMaxRhs + c.j ];
@@ -330,9 +328,9 @@ __global__ __launch_bounds__( kTile ) void correctKernel( Arena A )
    const int nt = A.ctl->numTiles;
    for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
       const int req = A.tileReq[ t ];
-      const ReqStatic R = A.rs[ req ];
       if( ! ( A.rd[ req ].mask & ( 1 << P_CORRECT ) ) )
          continue;
+      const ReqStatic R = A.rs[ req ];
       const TileCtx c = tileCtx( A, t, R );
       if( c.active ) {
          const Sys& s = A.sys[ req * kMaxRhs + c.j ];
@@ -347,9 +345,9 @@ __global__ __launch_bounds__( kTile ) void outerResidKernel( Arena A )
    const int nt = A.ctl->numTiles;
    for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
       const int req = A.tileReq[ t ];
-      const ReqStatic R = A.rs[ req ];
       if( ! ( A.rd[ req ].mask & ( 1 << P_CORRECT ) ) )
          continue;
+      const ReqStatic R = A.rs[ req ];
       const TileCtx c = tileCtx( A, t, R );
       double v[ 2 ] = { 0, 0 };
       if( c.active && A.sys[ req * kMaxRhs + c.j ].phase == P_CORRECT ) {
@@ -362,47 +360,45 @@ __global__ __launch_bounds__( kTile ) void outerResidKernel( Arena A )
          v[ 1 ] = bb * bb;
       }
       tilePartials< 2 >( A.part, t, R.kp, v );
-      if( lastTile( A.rd[ req ], R.tiles ) ) {
-         const double* tot = requestTotals< 2 >( A.part, A.tileFirst[ req ], R.tiles, R.kp );
-         if( threadIdx.x < R.k ) {
-            Sys& s = A.sys[ req * kMaxRhs + threadIdx.x ];
-            if( s.phase == P_CORRECT ) {
-               if( s.normb < 0 ) {
-                  s.normb = sqrt( tot[ kMaxRhs + threadIdx.x ] );
-                  if( s.normb == 0 )
-                     s.normb = 1;
-               }
-               const double nr = sqrt( tot[ threadIdx.x ] );
-               const double res = nr / s.normb;
-               s.slow = ( s.outerIt > 0 && res > 0.5 * s.prevRes ) ? s.slow + 1 : 0;
-               s.res = res;
-               if( res < R.tol || nr == 0 )
-                  s.phase = P_DONE;
-               else if( ! std::isfinite( res ) || s.outerIt >= R.maxOuter
-                        || ( R.stagnation > 0 && s.slow >= R.stagnation ) )
-                  s.phase = P_FAILED;
-               else {
-                  s.phase = P_RESTART;
-                  s.scale = nr;
-                  s.prevRes = res;
-               }
-            }
-         }
-         __syncthreads();
-         publishMask( A, req, R );
-      }
       __syncthreads();
    }
 }
+
+struct StepOuter
+{
+   __device__ void operator()( Sys& s, const double* tot, int j, const ReqStatic& R ) const
+   {
+      if( s.phase != P_CORRECT )
+         return;
+      if( s.normb < 0 ) {
+         s.normb = sqrt( tot[ kMaxRhs + j ] );
+         if( s.normb == 0 )
+            s.normb = 1;
+      }
+      const double nr = sqrt( tot[ j ] );
+      const double res = nr / s.normb;
+      s.slow = ( s.outerIt > 0 && res > 0.5 * s.prevRes ) ? s.slow + 1 : 0;
+      s.res = res;
+      if( res < R.tol || nr == 0 )
+         s.phase = P_DONE;
+      else if( ! isfinite( res ) || s.outerIt >= R.maxOuter || ( R.stagnation > 0 && s.slow >= R.stagnation ) )
+         s.phase = P_FAILED;
+      else {
+         s.phase = P_RESTART;
+         s.scale = nr;
+         s.prevRes = res;
+      }
+   }
+};
 
 __global__ __launch_bounds__( kTile ) void restartKernel( Arena A )
 {
    const int nt = A.ctl->numTiles;
    for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
       const int req = A.tileReq[ t ];
-      const ReqStatic R = A.rs[ req ];
       if( ! ( A.rd[ req ].mask & ( 1 << P_RESTART ) ) )
          continue;
+      const ReqStatic R = A.rs[ req ];
       const TileCtx c = tileCtx( A, t, R );
       double v[ 1 ] = { 0 };
       if( c.active ) {
@@ -416,23 +412,22 @@ __global__ __launch_bounds__( kTile ) void restartKernel( Arena A )
          }
       }
       tilePartials< 1 >( A.part, t, R.kp, v );
-      if( lastTile( A.rd[ req ], R.tiles ) ) {
-         const double* tot = requestTotals< 1 >( A.part, A.tileFirst[ req ], R.tiles, R.kp );
-         if( threadIdx.x < R.k ) {
-            Sys& s = A.sys[ req * kMaxRhs + threadIdx.x ];
-            if( s.phase == P_RESTART ) {
-               s.rho = s.rho0 = tot[ threadIdx.x ];
-               s.innerIt = 0;
-               s.outerIt++;
-               s.phase = P_INNER;
-            }
-         }
-         __syncthreads();
-         publishMask( A, req, R );
-      }
       __syncthreads();
    }
 }
+
+struct StepRestart
+{
+   __device__ void operator()( Sys& s, const double* tot, int j, const ReqStatic& ) const
+   {
+      if( s.phase != P_RESTART )
+         return;
+      s.rho = s.rho0 = tot[ j ];
+      s.innerIt = 0;
+      s.outerIt++;
+      s.phase = P_INNER;
+   }
+};
 
 // admission: x = 0, d = 0 over the request's entries
 __global__ __launch_bounds__( 256 ) void admitKernel( int64_t E, double* x, float* d )
@@ -595,20 +590,25 @@ struct tnlhip_server_s
 
    Arena arena() const
    {
-      return Arena{ ctl.ptr, tileReq.ptr, tileFirst.ptr, rs.ptr, rd.ptr, sys.ptr, part.ptr,
-                    cols.ptr, v64.ptr, v32.ptr, slices.ptr, a64.ptr, a32.ptr };
+      return Arena{ ctl.ptr, tileReq.ptr, tileFirst.ptr, rs.ptr, rd.ptr, sys.ptr, part.ptr, cols.ptr,
+                    v64.ptr, v32.ptr, slices.ptr, a64.ptr, a32.ptr, cfg.max_requests };
    }
 
    void enqueueIterations( int iterations )
    {
       const Arena A = arena();
+      const int fg = std::min( cfg.max_requests, kServerGrid );
       for( int i = 0; i < iterations; i++ ) {
          innerMatvecKernel<<< kServerGrid, kTile, 0, stream >>>( A );
+         requestFinalKernel< 1 ><<< fg, kTile, 0, stream >>>( A, P_INNER, StepAlpha{} );
          innerUpdateKernel<<< kServerGrid, kTile, 0, stream >>>( A );
+         requestFinalKernel< 1 ><<< fg, kTile, 0, stream >>>( A, P_INNER, StepBeta{} );
          innerDirKernel<<< kServerGrid, kTile, 0, stream >>>( A );
          correctKernel<<< kServerGrid, kTile, 0, stream >>>( A );
          outerResidKernel<<< kServerGrid, kTile, 0, stream >>>( A );
+         requestFinalKernel< 2 ><<< fg, kTile, 0, stream >>>( A, P_CORRECT, StepOuter{} );
          restartKernel<<< kServerGrid, kTile, 0, stream >>>( A );
+         requestFinalKernel< 1 ><<< fg, kTile, 0, stream >>>( A, P_RESTART, StepRestart{} );
       }
    }
 
@@ -678,7 +678,7 @@ int admit( tnlhip_server_s& S )
       R.innerTol2 = q.params.inner_tolerance * q.params.inner_tolerance;
       R.stagnation = q.params.stagnation_limit;
       S.hStatic[ slot ] = R;
-      S.hDyn[ slot ] = ReqDyn{ 1 << P_CORRECT, 0u };
+      S.hDyn[ slot ] = ReqDyn{ 1 << P_CORRECT, 0 };
       for( int j = 0; j < kMaxRhs; j++ ) {
          Sys s{};
          s.phase = j < q.k ? P_CORRECT : P_IDLE;

@@ -9,32 +9,33 @@ namespace tnlhip {
 
 template< typename T, bool SQUARE >
 __global__ __launch_bounds__( kReduceBlock ) void dotKernel( int64_t n, const T* __restrict__ a,
-                                                             const T* __restrict__ b, ReduceSlot slot,
-                                                             double* __restrict__ result, int sqrtResult )
+                                                             const T* __restrict__ b, ReduceSlot slot )
 {
-   // each workgroup streams one contiguous range, 4 x 2 elements in flight per lane
+   // grid-stride, two elements in flight per lane
    double acc[ 1 ] = { 0.0 };
-   const int64_t per = ( ( n + gridDim.x - 1 ) / gridDim.x + 511 ) / 512 * 512;
-   const int64_t i0 = (int64_t) blockIdx.x * per, i1 = min( i0 + per, n );
-   int64_t i = i0 + threadIdx.x;
-   for( ; i + 3 * kReduceBlock < i1; i += 4 * kReduceBlock ) {
-      T av[ 4 ], bv[ 4 ];
-#pragma unroll
-      for( int j = 0; j < 4; j++ ) {
-         av[ j ] = __builtin_nontemporal_load( a + i + j * kReduceBlock );
-         bv[ j ] = SQUARE ? av[ j ] : __builtin_nontemporal_load( b + i + j * kReduceBlock );
-      }
-#pragma unroll
-      for( int j = 0; j < 4; j++ )
-         acc[ 0 ] += (double) av[ j ] * (double) bv[ j ];
+   const int64_t stride = (int64_t) gridDim.x * kReduceBlock;
+   int64_t i = (int64_t) blockIdx.x * kReduceBlock + threadIdx.x;
+   for( ; i + stride < n; i += 2 * stride ) {
+      const T a0 = a[ i ], a1 = a[ i + stride ];
+      const T b0 = SQUARE ? a0 : b[ i ], b1 = SQUARE ? a1 : b[ i + stride ];
+      acc[ 0 ] += (double) a0 * (double) b0;
+      acc[ 0 ] += (double) a1 * (double) b1;
    }
-   for( ; i < i1; i += kReduceBlock ) {
+   if( i < n ) {
       const double x = (double) a[ i ];
       acc[ 0 ] += SQUARE ? x * x : x * (double) b[ i ];
    }
-   double out[ 1 ];
-   if( finishReduction< 1 >( acc, slot, out ) && threadIdx.x == 0 )
-      *result = sqrtResult ? sqrt( out[ 0 ] ) : out[ 0 ];
+   storeBlockPartials< 1 >( acc, slot );
+}
+
+// result = sum of the partials (or its square root)
+__global__ __launch_bounds__( kReduceBlock ) void dotFinalKernel( ReduceSlot slot, int grid, double* __restrict__ result,
+                                                                  int sqrtResult )
+{
+   __shared__ double tot[ 1 ];
+   sumPartials< 1 >( slot, grid, tot );
+   if( threadIdx.x == 0 )
+      *result = sqrtResult ? sqrt( tot[ 0 ] ) : tot[ 0 ];
 }
 
 void deviceDot( int dtype, int64_t n, const void* a, const void* b, double* resultDevice, bool sqrtResult,
@@ -45,20 +46,17 @@ void deviceDot( int dtype, int64_t n, const void* a, const void* b, double* resu
    const bool sq = a == b;
    if( dtype == TNLHIP_F64 ) {
       if( sq )
-         dotKernel< double, true ><<< grid, kReduceBlock, 0, st >>>( n, (const double*) a, (const double*) b, slot,
-                                                                     resultDevice, sqrtResult );
+         dotKernel< double, true ><<< grid, kReduceBlock, 0, st >>>( n, (const double*) a, (const double*) b, slot );
       else
-         dotKernel< double, false ><<< grid, kReduceBlock, 0, st >>>( n, (const double*) a, (const double*) b, slot,
-                                                                      resultDevice, sqrtResult );
+         dotKernel< double, false ><<< grid, kReduceBlock, 0, st >>>( n, (const double*) a, (const double*) b, slot );
    }
    else {
       if( sq )
-         dotKernel< float, true ><<< grid, kReduceBlock, 0, st >>>( n, (const float*) a, (const float*) b, slot,
-                                                                    resultDevice, sqrtResult );
+         dotKernel< float, true ><<< grid, kReduceBlock, 0, st >>>( n, (const float*) a, (const float*) b, slot );
       else
-         dotKernel< float, false ><<< grid, kReduceBlock, 0, st >>>( n, (const float*) a, (const float*) b, slot,
-                                                                     resultDevice, sqrtResult );
+         dotKernel< float, false ><<< grid, kReduceBlock, 0, st >>>( n, (const float*) a, (const float*) b, slot );
    }
+   dotFinalKernel<<< 1, kReduceBlock, 0, st >>>( slot, grid, resultDevice, sqrtResult ? 1 : 0 );
    TNLHIP_CHECK( hipGetLastError() );
 }
 
@@ -92,27 +90,59 @@ void deviceAxpby( int dtype, int64_t n, double alpha, const void* x, double beta
 
 // Every row i is read once: w[i] and the `count` entries V[i + k*ld]; each lane keeps
 // `count` running sums (count <= 32).  Partials per workgroup and column, fixed-order finish.
-template< typename T >
+template< typename T, int C >
 __global__ __launch_bounds__( kReduceBlock ) void multidotKernel( int64_t n, int32_t count, const T* __restrict__ V,
                                                                   int64_t ld, const T* __restrict__ w,
-                                                                  ReduceSlot slot, double* __restrict__ result )
+                                                                  ReduceSlot slot )
 {
-   double acc[ kMaxMulti ];
+   // all C column loads of a row issue without branches; columns past `count` repeat the
+   // last one (a cache hit) and are multiplied by zero
+   double acc[ C ];
 #pragma unroll
-   for( int k = 0; k < kMaxMulti; k++ )
+   for( int k = 0; k < C; k++ )
       acc[ k ] = 0.0;
    const int64_t stride = (int64_t) gridDim.x * kReduceBlock;
    for( int64_t i = (int64_t) blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += stride ) {
       const double wi = (double) w[ i ];
+      T v[ C ];
 #pragma unroll
-      for( int k = 0; k < kMaxMulti; k++ )
-         if( k < count )
-            acc[ k ] += (double) V[ i + k * ld ] * wi;
+      for( int k = 0; k < C; k++ )
+         v[ k ] = V[ i + (int64_t) min( k, count - 1 ) * ld ];
+#pragma unroll
+      for( int k = 0; k < C; k++ )
+         acc[ k ] += ( k < count ? (double) v[ k ] : 0.0 ) * wi;
    }
-   double out[ kMaxMulti ];
-   if( finishReduction< kMaxMulti >( acc, slot, out ) && threadIdx.x == 0 )
-      for( int k = 0; k < count; k++ )
-         result[ k ] = out[ k ];
+   storeBlockPartials< C >( acc, slot );
+}
+
+template< int C >
+__global__ __launch_bounds__( kReduceBlock ) void multidotFinalKernel( ReduceSlot slot, int grid, int32_t count,
+                                                                       double* __restrict__ result )
+{
+   __shared__ double tot[ C ];
+   sumPartials< C >( slot, grid, tot );
+   if( threadIdx.x < count )
+      result[ threadIdx.x ] = tot[ threadIdx.x ];
+}
+
+template< typename T, int C >
+void launchMultidot( int64_t n, int32_t c, const T* V, int64_t ld, const T* w, ReduceSlot slot, double* result,
+                     hipStream_t st )
+{
+   multidotKernel< T, C ><<< kReduceGrid, kReduceBlock, 0, st >>>( n, c, V, ld, w, slot );
+   multidotFinalKernel< C ><<< 1, kReduceBlock, 0, st >>>( slot, kReduceGrid, c, result );
+}
+
+template< typename T >
+void launchMultidotBucket( int64_t n, int32_t c, const T* V, int64_t ld, const T* w, ReduceSlot slot, double* result,
+                           hipStream_t st )
+{
+   if( c <= 8 )
+      launchMultidot< T, 8 >( n, c, V, ld, w, slot, result, st );
+   else if( c <= 16 )
+      launchMultidot< T, 16 >( n, c, V, ld, w, slot, result, st );
+   else
+      launchMultidot< T, 32 >( n, c, V, ld, w, slot, result, st );
 }
 
 void deviceMultidot( int dtype, int64_t n, int32_t count, const void* V, int64_t ld, const void* w,
@@ -122,11 +152,11 @@ void deviceMultidot( int dtype, int64_t n, int32_t count, const void* V, int64_t
       const int32_t c = std::min( kMaxMulti, count - k0 );
       ReduceSlot slot = ws.slot( 0 );
       if( dtype == TNLHIP_F64 )
-         multidotKernel< double ><<< kReduceGrid, kReduceBlock, 0, st >>>(
-            n, c, (const double*) V + k0 * ld, ld, (const double*) w, slot, resultDevice + k0 );
+         launchMultidotBucket< double >( n, c, (const double*) V + k0 * ld, ld, (const double*) w, slot,
+                                         resultDevice + k0, st );
       else
-         multidotKernel< float ><<< kReduceGrid, kReduceBlock, 0, st >>>(
-            n, c, (const float*) V + k0 * ld, ld, (const float*) w, slot, resultDevice + k0 );
+         launchMultidotBucket< float >( n, c, (const float*) V + k0 * ld, ld, (const float*) w, slot,
+                                        resultDevice + k0, st );
       TNLHIP_CHECK( hipGetLastError() );
    }
 }

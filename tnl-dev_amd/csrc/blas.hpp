@@ -5,14 +5,12 @@
 
 namespace tnlhip {
 
-// Device scratch for reductions: `slots` independent reduction slots (partials + ticket
-// counter, counters start at zero and every reduction leaves them at zero), device scalars
-// and scan tile sums.  One Workspace per solver; the stand-alone C entry points use one per
+// Device scratch for reductions: `slots` independent reduction slots (partials), device
+// scalars and scan tile sums.  One Workspace per solver; the stand-alone C entry points use one per
 // host thread.
 struct Workspace
 {
    DeviceArray< double > partials;
-   DeviceArray< unsigned > counters;
    DeviceArray< double > scalars;
    DeviceArray< int32_t > scan;
    int nslots = 0;
@@ -21,8 +19,6 @@ struct Workspace
    {
       if( slots > nslots || partials.size < (int64_t) slots * width * kReduceGrid ) {
          partials.resize( (int64_t) slots * std::max( width, kMaxMulti ) * kReduceGrid );
-         counters.resize( slots );
-         TNLHIP_CHECK( hipMemset( counters.ptr, 0, sizeof( unsigned ) * slots ) );
          nslots = slots;
       }
       if( scalars.size < nscalars )
@@ -33,7 +29,7 @@ struct Workspace
       if( nslots == 0 )
          ensure( 4, kMaxMulti, 64 );
       const int64_t per = partials.size / nslots;
-      return ReduceSlot{ partials.ptr + per * i, counters.ptr + i };
+      return ReduceSlot{ partials.ptr + per * i };
    }
    int32_t* scanScratch( int64_t tiles )
    {

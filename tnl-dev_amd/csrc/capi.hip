@@ -155,14 +155,21 @@ tnlhip_matrix_s* buildEmpty( int format, int dtype, int32_t rows, int32_t cols, 
    return M.release();
 }
 
-__global__ void countKernel( int64_t n, const int32_t* __restrict__ cols, ReduceSlot slot, int64_t* result )
+__global__ __launch_bounds__( kReduceBlock ) void countKernel( int64_t n, const int32_t* __restrict__ cols,
+                                                               ReduceSlot slot )
 {
    double acc[ 1 ] = { 0.0 };
    for( int64_t i = (int64_t) blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += (int64_t) gridDim.x * kReduceBlock )
       acc[ 0 ] += cols[ i ] != -1 ? 1.0 : 0.0;
-   double out[ 1 ];
-   if( finishReduction< 1 >( acc, slot, out ) && threadIdx.x == 0 )
-      *result = (int64_t) out[ 0 ];
+   storeBlockPartials< 1 >( acc, slot );
+}
+
+__global__ __launch_bounds__( kReduceBlock ) void countFinalKernel( ReduceSlot slot, int grid, int64_t* result )
+{
+   __shared__ double tot[ 1 ];
+   sumPartials< 1 >( slot, grid, tot );
+   if( threadIdx.x == 0 )
+      *result = (int64_t) tot[ 0 ];
 }
 
 int64_t countStored( const Matrix& M, hipStream_t st )
@@ -171,7 +178,8 @@ int64_t countStored( const Matrix& M, hipStream_t st )
       return 0;
    Workspace& ws = threadWorkspace();
    int64_t* r = reinterpret_cast< int64_t* >( ws.scalars.ptr );
-   countKernel<<< kReduceGrid, kReduceBlock, 0, st >>>( M.meta.allocated, M.columns.ptr, ws.slot( 0 ), r );
+   countKernel<<< kReduceGrid, kReduceBlock, 0, st >>>( M.meta.allocated, M.columns.ptr, ws.slot( 0 ) );
+   countFinalKernel<<< 1, kReduceBlock, 0, st >>>( ws.slot( 0 ), kReduceGrid, r );
    TNLHIP_CHECK( hipGetLastError() );
    int64_t h = 0;
    TNLHIP_CHECK( hipMemcpyAsync( &h, r, sizeof( h ), hipMemcpyDeviceToHost, st ) );

@@ -32,6 +32,15 @@ void setLastError( const std::string& msg );
                                 std::string( #call ) + ": " + hipGetErrorString( e_ ) );                       \
    } while( 0 )
 
+// A call into the library's own C entry points from library code: a failure status becomes an
+// Error carrying that status and message.
+#define TNLHIP_CHECK_STATUS( call )                                           \
+   do {                                                                      \
+      const int s_ = ( call );                                               \
+      if( s_ != TNLHIP_SUCCESS )                                             \
+         throw ::tnlhip::Error( s_, tnlhip_last_error() );                   \
+   } while( 0 )
+
 #define TNLHIP_REQUIRE( cond, msg )                                     \
    do {                                                                 \
       if( ! ( cond ) )                                                  \

@@ -1,8 +1,13 @@
-// Deterministic single-launch reductions.  Each workgroup of a fixed-size grid folds its
-// grid-stride share into one partial, the last workgroup to finish (atomic ticket) sums
-// the partials in index order.  The result depends only on the grid size, never on
-// timing, so the same inputs give the same bits on every run and in every batch mix.
-// Partial sums are kept in float64 whatever the vector type.
+// Deterministic reductions in two launches.  The first kernel (a fixed grid of kReduceGrid
+// workgroups) folds each workgroup's grid-stride share into one partial per column; the
+// second, one workgroup, sums the partials in index order.  The result depends only on the
+// grid size, never on timing, so the same inputs give the same bits on every run and in
+// every batch mix.  Partial sums are kept in float64 whatever the vector type.
+//
+// Two launches rather than a last-workgroup ticket: on this GPU the eight XCDs have private
+// L2 caches, so a ticket needs a device-scope release (an L2 write-back) in every workgroup,
+// which costs more than the whole pass over a 2-4 M vector; the kernel boundary gives the
+// same ordering once per launch.
 #pragma once
 
 #include "common.hpp"
@@ -10,24 +15,21 @@
 namespace tnlhip {
 
 constexpr int kReduceBlock = 256;
-constexpr int kReduceGrid = 2048;  // partials per reduction (8 workgroups per CU on 256 CUs)
+constexpr int kReduceGrid = 1024;  // partials per column (4 workgroups per CU on 256 CUs)
 constexpr int kMaxMulti = 32;      // columns per multi-reduction pass
 
-// Scratch of one reduction slot: partials, ticket counter.
+// Scratch of one reduction slot: kReduceGrid partials per column, column-major.
 struct ReduceSlot
 {
-   double* partials;   // kReduceGrid * width
-   unsigned* counter;  // one ticket counter, zero between launches
+   double* partials;
 };
 
-// Called by every thread of the block at the end of a reduction kernel with its own
-// partial values v[0..W).  Returns true in the workgroup that performs the final sum;
-// in that workgroup thread 0 gets the totals in out[0..W).
+// Called by every thread of a kReduceBlock workgroup with its own values v[0..W); the
+// workgroup's W sums are stored at partials[k * gridDim.x + blockIdx.x].
 template< int W >
-__device__ __forceinline__ bool finishReduction( const double ( &v )[ W ], ReduceSlot slot, double ( &out )[ W ] )
+__device__ __forceinline__ void storeBlockPartials( const double ( &v )[ W ], ReduceSlot slot )
 {
    __shared__ double lds[ W ][ kReduceBlock / kWave ];
-   __shared__ bool last;
    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
 #pragma unroll
    for( int k = 0; k < W; k++ ) {
@@ -36,43 +38,64 @@ __device__ __forceinline__ bool finishReduction( const double ( &v )[ W ], Reduc
          lds[ k ][ wave ] = s;
    }
    __syncthreads();
-   if( threadIdx.x == 0 ) {
+   if( threadIdx.x < W ) {
+      const int k = threadIdx.x;
+      double s = 0;
+      for( int w = 0; w < kReduceBlock / kWave; w++ )
+         s += lds[ k ][ w ];
+      slot.partials[ (int64_t) k * gridDim.x + blockIdx.x ] = s;
+   }
+}
+
+// One kReduceBlock workgroup: out[k] = sum over the `grid` partials of column k, in a fixed
+// order; out (W doubles, usually LDS) is valid in every thread on return.  Up to 64 columns
+// get kReduceBlock / W lanes each (independent loads in flight), one column gets the block.
+template< int W >
+__device__ __forceinline__ void sumPartials( ReduceSlot slot, int grid, double* out )
+{
+   if constexpr( W >= 4 ) {
+      static_assert( ( W & ( W - 1 ) ) == 0 && W <= kReduceBlock, "column count must be a power of two" );
+      constexpr int G = kReduceBlock / W;  // lanes per column, a power of two <= 64
+      const int k = threadIdx.x / G, g = threadIdx.x % G;
+      const double* p = slot.partials + (int64_t) k * grid;
+      double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+      int j = g;
+      for( ; j + 3 * G < grid; j += 4 * G ) {
+         s0 += p[ j ];
+         s1 += p[ j + G ];
+         s2 += p[ j + 2 * G ];
+         s3 += p[ j + 3 * G ];
+      }
+      for( ; j < grid; j += G )
+         s0 += p[ j ];
+      double s = ( s0 + s1 ) + ( s2 + s3 );
+      s = groupSum< G >( s );
+      if( g == 0 )
+         out[ k ] = s;
+      __syncthreads();
+   }
+   else {
+      __shared__ double lds[ W ][ kReduceBlock / kWave ];
+      const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
 #pragma unroll
       for( int k = 0; k < W; k++ ) {
          double s = 0;
-         for( int w = 0; w < kReduceBlock / kWave; w++ )
-            s += lds[ k ][ w ];
-         slot.partials[ (int64_t) k * gridDim.x + blockIdx.x ] = s;
+         for( int j = threadIdx.x; j < grid; j += kReduceBlock )
+            s += slot.partials[ (int64_t) k * grid + j ];
+         s = groupSum< kWave >( s );
+         if( lane == 0 )
+            lds[ k ][ wave ] = s;
       }
-      __threadfence();
-      const unsigned t = atomicAdd( slot.counter, 1u );
-      last = t == gridDim.x - 1;
-   }
-   __syncthreads();
-   if( ! last )
-      return false;
-   __threadfence();
-#pragma unroll
-   for( int k = 0; k < W; k++ ) {
-      double s = 0;
-      for( int j = threadIdx.x; j < (int) gridDim.x; j += kReduceBlock )
-         s += slot.partials[ (int64_t) k * gridDim.x + j ];
-      s = groupSum< kWave >( s );
-      if( lane == 0 )
-         lds[ k ][ wave ] = s;
-   }
-   __syncthreads();
-   if( threadIdx.x == 0 ) {
-#pragma unroll
-      for( int k = 0; k < W; k++ ) {
+      __syncthreads();
+      if( threadIdx.x < W ) {
+         const int k = threadIdx.x;
          double s = 0;
          for( int w = 0; w < kReduceBlock / kWave; w++ )
             s += lds[ k ][ w ];
          out[ k ] = s;
       }
-      *slot.counter = 0;
+      __syncthreads();
    }
-   return true;
 }
 
 }  // namespace tnlhip

@@ -11,6 +11,8 @@
 // queue iterations in chunks and look at the state only between chunks.  The host decides
 // only what TNL's host decides across GMRES restart cycles (the adaptive restart length).
 #include "blas.hpp"
+
+#include <type_traits>
 #include "matrix.hpp"
 
 #include <cmath>
@@ -89,10 +91,9 @@ __global__ __launch_bounds__( 256 ) void elementwiseKernel( int64_t n, Op op, Ga
       op( i );
 }
 
-// Elementwise pass fused with W simultaneous sums; Fin runs once with the totals.
-template< int W, typename Op, typename Fin >
-__global__ __launch_bounds__( kReduceBlock ) void fusedReduceKernel( int64_t n, Op op, Fin fin, ReduceSlot slot,
-                                                                     Gate gate )
+// Elementwise pass fused with W simultaneous sums: per-workgroup partials.
+template< int W, typename Op >
+__global__ __launch_bounds__( kReduceBlock ) void fusedReduceKernel( int64_t n, Op op, ReduceSlot slot, Gate gate )
 {
    if( ! gate.open() )
       return;
@@ -104,14 +105,29 @@ __global__ __launch_bounds__( kReduceBlock ) void fusedReduceKernel( int64_t n, 
    const int64_t stride = (int64_t) gridDim.x * kReduceBlock;
    for( int64_t i = (int64_t) blockIdx.x * kReduceBlock + threadIdx.x; i < n; i += stride )
       op( i, acc );
-   double out[ W ];
-   __shared__ double tot[ W ];  // the totals, indexable by the finishing lane
-   if( finishReduction< W >( acc, slot, out ) && threadIdx.x == 0 ) {
-#pragma unroll
-      for( int k = 0; k < W; k++ )
-         tot[ k ] = out[ k ];
+   storeBlockPartials< W >( acc, slot );
+}
+
+// A Fin with `static constexpr bool kLaneParallel` is called by every lane with its index.
+template< typename F, typename = void >
+struct LaneParallel : std::false_type
+{};
+template< typename F >
+struct LaneParallel< F, std::void_t< decltype( F::kLaneParallel ) > > : std::true_type
+{};
+
+// The totals of the partials, in a fixed order; Fin runs once (lane 0) with them.
+template< int W, typename Fin >
+__global__ __launch_bounds__( kReduceBlock ) void reduceFinalKernel( Fin fin, ReduceSlot slot, int grid, Gate gate )
+{
+   if( ! gate.open() )
+      return;
+   __shared__ double tot[ W ];
+   sumPartials< W >( slot, grid, tot );
+   if constexpr( LaneParallel< Fin >::value )
+      fin( tot, (int) threadIdx.x );  // lane-parallel finish
+   else if( threadIdx.x == 0 )
       fin( tot );
-   }
 }
 
 // Serial step on one lane (Hessenberg / Givens / small triangular algebra).
@@ -136,7 +152,24 @@ void launchElementwise( int64_t n, const Op& op, Gate gate, hipStream_t st )
 template< int W, typename Op, typename Fin >
 void launchReduce( int64_t n, const Op& op, const Fin& fin, ReduceSlot slot, Gate gate, hipStream_t st )
 {
-   fusedReduceKernel< W ><<< kReduceGrid, kReduceBlock, 0, st >>>( n, op, fin, slot, gate );
+   fusedReduceKernel< W ><<< kReduceGrid, kReduceBlock, 0, st >>>( n, op, slot, gate );
+   reduceFinalKernel< W ><<< 1, kReduceBlock, 0, st >>>( fin, slot, kReduceGrid, gate );
+   TNLHIP_CHECK( hipGetLastError() );
+}
+
+// Small dense step on the 64 lanes of one wavefront: fin( lane ).
+template< typename Fin >
+__global__ void waveKernel( Fin fin, Gate gate )
+{
+   if( ! gate.open() )
+      return;
+   fin( (int) threadIdx.x );
+}
+
+template< typename Fin >
+void launchWave( const Fin& fin, Gate gate, hipStream_t st )
+{
+   waveKernel<<< 1, kWave, 0, st >>>( fin, gate );
    TNLHIP_CHECK( hipGetLastError() );
 }
 
@@ -942,19 +975,25 @@ struct FinGmresStepEnd
 };
 
 // CGS: multi-reduction acc_k = (V_k, w) fused with the previous subtraction w -= V h_prev
-template< typename T >
+template< typename T, int C >
 struct OpCgs
 {
    T* w;
    const T* V;
    int64_t ld;
-   int count;          // columns of V involved
+   int count;          // columns of V involved (1 <= count <= C)
    const double* hprev;  // subtraction coefficients (null: none)
    bool norm;          // accumulate |w|^2 into acc[0] instead of the projections
    __device__ void load() {}
-   __device__ void operator()( int64_t i, double ( &acc )[ kMaxMulti ] ) const
+   __device__ void operator()( int64_t i, double ( &acc )[ C ] ) const
    {
       double wi = (double) w[ i ];
+      T v[ C ];
+      if( ! norm ) {
+#pragma unroll
+         for( int k = 0; k < C; k++ )
+            v[ k ] = V[ i + (int64_t) min( k, count - 1 ) * ld ];
+      }
       if( hprev ) {
          double t = 0.0;
          for( int k = 0; k < count; k++ )
@@ -967,12 +1006,26 @@ struct OpCgs
          acc[ 0 ] += wi * wi;
       else {
 #pragma unroll
-         for( int k = 0; k < kMaxMulti; k++ )
-            if( k < count )
-               acc[ k ] += (double) V[ i + k * ld ] * wi;
+         for( int k = 0; k < C; k++ )
+            acc[ k ] += ( k < count ? (double) v[ k ] : 0.0 ) * wi;
       }
    }
 };
+
+// launchReduce over OpCgs with the column bucket that covers `count`
+template< typename T, typename Fin >
+void launchCgs( int64_t n, T* w, const T* V, int64_t ld, int count, const double* hprev, bool norm, const Fin& fin,
+                ReduceSlot slot, Gate gate, hipStream_t st )
+{
+   if( norm )
+      launchReduce< 1 >( n, OpCgs< T, 1 >{ w, V, ld, count, hprev, true }, fin, slot, gate, st );
+   else if( count <= 8 )
+      launchReduce< 8 >( n, OpCgs< T, 8 >{ w, V, ld, count, hprev, false }, fin, slot, gate, st );
+   else if( count <= 16 )
+      launchReduce< 16 >( n, OpCgs< T, 16 >{ w, V, ld, count, hprev, false }, fin, slot, gate, st );
+   else
+      launchReduce< 32 >( n, OpCgs< T, 32 >{ w, V, ld, count, hprev, false }, fin, slot, gate, st );
+}
 
 struct FinCgsCoef
 {
@@ -1117,14 +1170,14 @@ struct FinYL
    const T* z;
    int i;
    int64_t n;
-   __device__ void operator()() const
+   __device__ void operator()( int k ) const
    {
       const int mx = q.mx;
       double* YLi = q.YL + i * ( mx + 1 );
-      for( int k = 0; k <= mx; k++ )
+      if( k <= mx )
          YLi[ k ] = k < n ? (double) yi[ k ] : 0.0;
       double* out = i == 0 ? q.s : q.H + ( i - 1 ) * ( q.m + 1 );
-      for( int k = 0; k <= i; k++ )
+      if( k <= i )
          out[ k ] = (double) z[ k ] - (double) yi[ k ];
    }
 };
@@ -1134,11 +1187,13 @@ struct FinTUpdate
    // [T_i]_{0..i-1} = - T_{i-1} * t_i * aux
    Small q;
    int i;
-   __device__ void operator()( const double* aux ) const
+   static constexpr bool kLaneParallel = true;
+   __device__ void operator()( const double* aux, int k ) const
    {
+      // entry k of the new column, one lane each (reads columns < i only)
       const int mx = q.mx;
       const double ti = q.T[ i + i * ( mx + 1 ) ];
-      for( int k = 0; k < i; k++ ) {
+      if( k < i ) {
          double t = 0.0;
          for( int j = k; j < i; j++ )
             t -= q.T[ k + j * ( mx + 1 ) ] * ( ti * aux[ j ] );
@@ -1152,17 +1207,19 @@ struct FinCwyAux
    // aux = T_i * Y_i^T e_i  (from YL)
    Small q;
    int i;
-   __device__ void operator()() const
+   __device__ void operator()( int k ) const
    {
+      // aux_k = sum_{j >= k} T_kj (Y^T e_i)_j, one lane per k
+      __shared__ double a[ kMaxMulti ];
       const int mx = q.mx;
-      double* aux = q.aux;
-      for( int k = 0; k <= i; k++ )
-         aux[ k ] = q.YL[ i + k * ( mx + 1 ) ];
-      for( int k = 0; k <= i; k++ ) {
+      if( k <= i )
+         a[ k ] = q.YL[ i + k * ( mx + 1 ) ];
+      __syncthreads();
+      if( k <= i ) {
          double a2 = 0.0;
          for( int j = k; j <= i; j++ )
-            a2 += q.T[ k + j * ( mx + 1 ) ] * aux[ j ];
-         aux[ k ] = a2;
+            a2 += q.T[ k + j * ( mx + 1 ) ] * a[ j ];
+         q.aux[ k ] = a2;
       }
    }
 };
@@ -1190,41 +1247,57 @@ struct OpCwyV
    }
 };
 
-template< typename T >
+template< typename T, int C >
 struct OpMulti
 {
-   // acc_k = (Y_k, w), k < count
+   // acc_k = (Y_k, w), k < count <= C; lanes load all C columns without branches (columns
+   // past `count` repeat the last one, a cache hit, and are multiplied by zero)
    const T* Y;
    int64_t ld;
    const T* w;
    int count;
    __device__ void load() {}
-   __device__ void operator()( int64_t i, double ( &acc )[ kMaxMulti ] ) const
+   __device__ void operator()( int64_t i, double ( &acc )[ C ] ) const
    {
       const double wi = (double) w[ i ];
+      T y[ C ];
 #pragma unroll
-      for( int k = 0; k < kMaxMulti; k++ )
-         if( k < count )
-            acc[ k ] += (double) Y[ i + k * ld ] * wi;
+      for( int k = 0; k < C; k++ )
+         y[ k ] = Y[ i + (int64_t) min( k, count - 1 ) * ld ];
+#pragma unroll
+      for( int k = 0; k < C; k++ )
+         acc[ k ] += ( k < count ? (double) y[ k ] : 0.0 ) * wi;
    }
 };
+
+// launchReduce over OpMulti with the column bucket that covers `count` (1 <= count <= 32)
+template< typename T, typename Fin >
+void launchMulti( int64_t n, const T* Y, int64_t ld, const T* w, int count, const Fin& fin, ReduceSlot slot,
+                  Gate gate, hipStream_t st )
+{
+   if( count <= 8 )
+      launchReduce< 8 >( n, OpMulti< T, 8 >{ Y, ld, w, count }, fin, slot, gate, st );
+   else if( count <= 16 )
+      launchReduce< 16 >( n, OpMulti< T, 16 >{ Y, ld, w, count }, fin, slot, gate, st );
+   else
+      launchReduce< 32 >( n, OpMulti< T, 32 >{ Y, ld, w, count }, fin, slot, gate, st );
+}
 
 struct FinCwyTransposed
 {
    // aux = T_i^T * (Y_i^T w)
    Small q;
    int i;
-   __device__ void operator()( const double* v ) const
+   static constexpr bool kLaneParallel = true;
+   __device__ void operator()( const double* v, int k ) const
    {
+      // aux_k = sum_{j <= k} T_jk v_j, one lane per k
       const int mx = q.mx;
-      double* a = q.aux;  // in place: step k reads a[0..k], which later steps no longer need
-      for( int k = 0; k <= i; k++ )
-         a[ k ] = v[ k ];
-      for( int k = i; k >= 0; k-- ) {
+      if( k <= i ) {
          double a2 = 0.0;
          for( int j = 0; j <= k; j++ )
-            a2 += q.T[ j + k * ( mx + 1 ) ] * a[ j ];
-         a[ k ] = a2;
+            a2 += q.T[ j + k * ( mx + 1 ) ] * v[ j ];
+         q.aux[ k ] = a2;
       }
    }
 };
@@ -1353,10 +1426,10 @@ void solveGMRES( SolverImpl& S, const void* b, void* xv )
             else {
                const int passes = variant == TNLHIP_GMRES_CGSR ? 2 : 1;
                for( int l = 0; l < passes; l++ )
-                  launchReduce< kMaxMulti >( n, OpCgs< T >{ w, V, ld, i + 1, l == 0 ? nullptr : coef, false },
-                                             FinCgsCoef{ Hcol, coef, i + 1 }, S.ws.slot( 0 ), cycle, S.stream );
-               launchReduce< kMaxMulti >( n, OpCgs< T >{ w, V, ld, i + 1, coef, true }, FinCgsNorm{ st, Hcol + i + 1 },
-                                          S.ws.slot( 0 ), cycle, S.stream );
+                  launchCgs< T >( n, w, V, ld, i + 1, l == 0 ? nullptr : coef, false, FinCgsCoef{ Hcol, coef, i + 1 },
+                                  S.ws.slot( 0 ), cycle, S.stream );
+               launchCgs< T >( n, w, V, ld, i + 1, coef, true, FinCgsNorm{ st, Hcol + i + 1 }, S.ws.slot( 0 ), cycle,
+                               S.stream );
             }
             // v_{i+1} = w / |w|
             launchElementwise( n, OpScale< T >{ V + ( i + 1 ) * ld, w, &st->normw, true, 0 }, cycle, S.stream );
@@ -1374,18 +1447,16 @@ void solveGMRES( SolverImpl& S, const void* b, void* xv )
             launchReduce< 1 >( n, OpTruncate< T >{ {}, yi, z, i }, FinHouseholder< T >{ st, q, yi, i }, S.ws.slot( 0 ),
                                cycle, S.stream );
             if( i > 0 )
-               launchReduce< kMaxMulti >( n, OpMulti< T >{ V, ld, yi, i }, FinTUpdate{ q, i }, S.ws.slot( 1 ), cycle,
-                                          S.stream );
+               launchMulti< T >( n, V, ld, yi, i, FinTUpdate{ q, i }, S.ws.slot( 1 ), cycle, S.stream );
             // hauseholder_apply_trunc (s or H column i-1)
-            launchScalar( FinYL< T >{ st, q, yi, z, i, n }, cycle, S.stream );
+            launchWave( FinYL< T >{ st, q, yi, z, i, n }, cycle, S.stream );
             // hauseholder_cwy: v = e_i - Y_i T_i Y_i^T e_i
-            launchScalar( FinCwyAux{ q, i }, cycle, S.stream );
+            launchWave( FinCwyAux{ q, i }, cycle, S.stream );
             launchElementwise( n, OpCwyV< T >{ vtmp, V, ld, q.aux, i + 1, i }, cycle, S.stream );
             if( i < m ) {
                matvec( S, vtmp, w, cycle );
                // z = (I - Y_i T_i^T Y_i^T) w
-               launchReduce< kMaxMulti >( n, OpMulti< T >{ V, ld, w, i + 1 }, FinCwyTransposed{ q, i }, S.ws.slot( 2 ),
-                                          cycle, S.stream );
+               launchMulti< T >( n, V, ld, w, i + 1, FinCwyTransposed{ q, i }, S.ws.slot( 2 ), cycle, S.stream );
                launchElementwise( n, OpCwyZ< T >{ z, w, V, ld, q.aux, i + 1 }, cycle, S.stream );
             }
             launchScalar( FinGmresStepEnd{ st, q, stop, S.hist.ptr, S.histCap, i, i, i - 1, i > 0, i - 1, i > 0 },

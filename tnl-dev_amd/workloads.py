@@ -77,29 +77,34 @@ def irregular(n, mean=16.0, sigma=1.2, max_len=4096, seed=0):
     total = int(L.sum())
     cols = rng.integers(0, n - 1, total, dtype=np.int64)
     cols += cols >= rows  # skip the diagonal
-    # remove repeats within a row: redraw duplicates until none remain
+    # key = row * n + col sorted: rows stay grouped, columns ascend; redraw repeats
+    key = rows * n + cols
+    del cols
+    key.sort()
     for _ in range(64):
-        key = rows * n + cols
-        order = np.argsort(key, kind="stable")
-        ks = key[order]
-        dup = np.zeros(total, dtype=bool)
-        dup[order[1:]] = ks[1:] == ks[:-1]
-        nd = int(dup.sum())
-        if nd == 0:
+        dup = np.flatnonzero(key[1:] == key[:-1]) + 1
+        if len(dup) == 0:
             break
-        c = rng.integers(0, n - 1, nd, dtype=np.int64)
-        c += c >= rows[dup]
-        cols[dup] = c
-    vals = rng.standard_normal(total)
-    # assemble rows: off-diagonals plus the diagonal, ascending columns
-    all_rows = np.concatenate([rows, np.arange(n, dtype=np.int64)])
-    all_cols = np.concatenate([cols, np.arange(n, dtype=np.int64)])
-    diag = 1.0 + np.bincount(rows, weights=np.abs(vals), minlength=n)
-    all_vals = np.concatenate([vals, diag])
-    order = np.lexsort((all_cols, all_rows))
+        r = key[dup] // n
+        c = rng.integers(0, n - 1, len(dup), dtype=np.int64)
+        c += c >= r
+        key[dup] = r * n + c
+        key.sort()
+    del rows
+    # the diagonal joins each row in column order
+    key = np.concatenate([key, np.arange(n, dtype=np.int64) * (n + 1)])
+    key.sort()
+    all_rows = key // n
+    all_cols = (key - all_rows * n).astype(np.int32)
+    del key
+    is_diag = all_cols == all_rows
+    vals = rng.standard_normal(total + n)
+    vals[is_diag] = 0.0
+    diag = 1.0 + np.bincount(all_rows, weights=np.abs(vals), minlength=n)
+    vals[is_diag] = diag
     row_ptr = np.zeros(n + 1, dtype=np.int64)
     np.cumsum(L + 1, out=row_ptr[1:])
-    return row_ptr.astype(np.int32), all_cols[order].astype(np.int32), all_vals[order]
+    return row_ptr.astype(np.int32), all_cols, vals
 
 
 def uniform_vector(n, seed, dtype=np.float64):
