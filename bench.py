@@ -140,7 +140,7 @@ def solver_suite(tnl, torch, W, dev, args):
                 x = torch.zeros(n, dtype=torch.float64, device=dev)
                 r, t = timed(torch, lambda: tnl.refine_solve(A, b, x, A_low=A_low, tolerance=1e-10,
                                                              inner_method="gmres", inner_dtype=dt,
-                                                             convergence_residue=1e-4, max_iterations=3000,
+                                                             convergence_residue=args.inner_tol, max_iterations=3000,
                                                              restarting_min=30, restarting_max=30,
                                                              gmres_variant=variant, preconditioner=1))
                 err = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))
@@ -233,7 +233,7 @@ def serving_requests(ladder, count, seed):
     return reqs
 
 
-def serve(tnl, server, mids, ladder, reqs, pool, arrivals):
+def serve(tnl, server, mids, ladder, reqs, pool, arrivals, inner_tol=3e-6):
     """Submit each request at its arrival time (None: all at once), progress the server until
     every request finished; returns (wall seconds, latencies, results)."""
     offs = np.random.default_rng(99).integers(0, len(pool) - 16 * 10 ** 6, len(reqs))
@@ -252,7 +252,7 @@ def serve(tnl, server, mids, ladder, reqs, pool, arrivals):
             idx, shift, k = reqs[nxt]
             n = n_of[idx]
             B = pool[offs[nxt]:offs[nxt] + n * k].reshape(n, k)
-            ticket_of[server.submit(mids[idx], B, shift=shift)] = nxt
+            ticket_of[server.submit(mids[idx], B, shift=shift, inner_tolerance=inner_tol)] = nxt
             nxt += 1
         st = server.stats()
         if st["resident_requests"] == 0 and st["queued_requests"] == 0:
@@ -290,15 +290,15 @@ def serving_suite(tnl, W, args):
     rhs = sum(k for _, _, k in reqs)
     log(f"serving: {len(ladder)} resident matrices ({setup:.2f} s to upload), {count} requests, {rhs} right-hand sides")
     # warm-up: a few requests through the whole path
-    serve(tnl, server, mids, ladder, reqs[:8], pool, None)
-    wall, lat_sat, info = serve(tnl, server, mids, ladder, reqs, pool, None)
+    serve(tnl, server, mids, ladder, reqs[:8], pool, None, args.serve_inner_tol)
+    wall, lat_sat, info = serve(tnl, server, mids, ladder, reqs, pool, None, args.serve_inner_tol)
     conv = sum(1 for i in info if i["converged"])
     rate = count / wall
     log(f"serving (all at t=0): {count} requests in {wall:.2f} s = {rate:.1f} solves/s "
         f"({rhs / wall:.1f} RHS/s), converged {conv}/{count}")
     lam = 0.7 * rate
     arrivals = np.cumsum(np.random.default_rng(11).exponential(1.0 / lam, count))
-    wall2, lat, info2 = serve(tnl, server, mids, ladder, reqs, pool, arrivals)
+    wall2, lat, info2 = serve(tnl, server, mids, ladder, reqs, pool, arrivals, args.serve_inner_tol)
     conv2 = sum(1 for i in info2 if i["converged"])
     p50, p99 = np.percentile(lat, [50, 99])
     log(f"serving (Poisson, {lam:.1f} req/s offered): p50 {p50 * 1e3:.1f} ms, p99 {p99 * 1e3:.1f} ms, "
@@ -314,7 +314,7 @@ def serving_suite(tnl, W, args):
                         "p99_ms": round(p99 * 1e3, 2), "mean_ms": round(float(lat.mean()) * 1e3, 2),
                         "seconds": round(wall2, 3), "converged": conv2},
             "inner_iterations_mean": round(float(np.mean([i["inner_iterations"] for i in info])), 1),
-            "iterations_per_launch": args.serve_iterations}
+            "iterations_per_launch": args.serve_iterations, "inner_tolerance": args.serve_inner_tol}
 
 
 def main():
@@ -329,6 +329,8 @@ def main():
     ap.add_argument("--cd-grid", type=int, default=128, help="convection-diffusion grid side")
     ap.add_argument("--peclet", default="1,10,100")
     ap.add_argument("--gmres-variants", default="CWY,CGS")
+    ap.add_argument("--inner-tol", type=float, default=1e-4, help="inner relative residual of the refinement")
+    ap.add_argument("--serve-inner-tol", type=float, default=3e-6)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-irregular", action="store_true")
     ap.add_argument("--irr-rows", type=int, default=2000000)

@@ -204,6 +204,8 @@ typedef struct
    int32_t restarting_step_max;
    int32_t bicgstab_exact_residue;
    int32_t preconditioner; /* 0 none, 1 Preconditioners::Diagonal (Jacobi) */
+   int32_t use_graph;      /* 1: CG / BiCGStab iterations are captured once into a hipGraph and
+                              replayed (16 per replay); 0: launched one by one */
 } tnlhip_solver_params;
 
 void tnlhip_solver_params_default( tnlhip_solver_params* params );

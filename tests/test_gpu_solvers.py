@@ -156,3 +156,20 @@ def test_float32_solve(tnl, dev):
                                                max_iterations=1000)
     ref = O.solve("cg", rp, c, v, b, convergence_residue=1e-5, max_iterations=1000)
     assert ok and abs(its - ref["iterations"]) <= 3
+
+
+@pytest.mark.parametrize("method,precond", [("cg", 0), ("cg", 1), ("bicgstab", 1)])
+def test_graph_replay_matches_direct_launches(tnl, dev, method, precond):
+    """Iterations replayed from a captured hipGraph give the same bits as the same kernels
+    launched one by one (long enough to go past the eager batches)."""
+    rp, c, v = W.laplacian_3d(24) if method == "cg" else W.convection_diffusion_3d(20, 10.0)
+    b = W.uniform_vector(len(rp) - 1, 4)
+    runs = []
+    for g in (0, 1):
+        ok, its, res, hist, x = _run_device_solver(tnl, dev, method, rp, c, v, b, convergence_residue=1e-10,
+                                                   max_iterations=3000, preconditioner=precond, use_graph=g)
+        runs.append((ok, its, res, hist, x))
+    assert runs[0][1] > 30
+    assert runs[0][0] == runs[1][0] and runs[0][1] == runs[1][1]
+    np.testing.assert_array_equal(runs[0][4], runs[1][4])
+    np.testing.assert_array_equal(runs[0][3], runs[1][3])

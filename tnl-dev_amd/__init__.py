@@ -65,7 +65,7 @@ class SolverParams(C.Structure):
                 ("divergence_residue", C.c_double), ("gmres_variant", C.c_int32), ("restarting_min", C.c_int32),
                 ("restarting_max", C.c_int32), ("restarting_step_min", C.c_int32),
                 ("restarting_step_max", C.c_int32), ("bicgstab_exact_residue", C.c_int32),
-                ("preconditioner", C.c_int32)]
+                ("preconditioner", C.c_int32), ("use_graph", C.c_int32)]
 
 
 REFINEMENT_HISTORY = 64
@@ -553,7 +553,7 @@ class Server:
     def remove_matrix(self, mid):
         _check(lib.tnlhip_server_remove_matrix(self._h, int(mid)))
 
-    def submit(self, mid, B, shift=0.0, tolerance=1e-10, inner_tolerance=1e-4, max_outer_iterations=20,
+    def submit(self, mid, B, shift=0.0, tolerance=1e-10, inner_tolerance=3e-6, max_outer_iterations=20,
                max_inner_iterations=5000, stagnation_limit=2):
         B = np.ascontiguousarray(B, np.float64)
         if B.ndim == 1:

@@ -43,7 +43,8 @@ namespace {
 
 constexpr int kTile = 256;
 constexpr int kMaxRhs = 16;
-constexpr int kServerGrid = 1024;
+constexpr int kServerGrid = 2048;  // 8 workgroups of 256 per CU
+constexpr int kRowsPerLane = 4;     // rows per lane in a tile (independent loads in flight)
 
 enum Phase : int32_t { P_IDLE = 0, P_INNER = 1, P_CORRECT = 2, P_RESTART = 3, P_DONE = 4, P_FAILED = 5 };
 constexpr int32_t kLive = ( 1 << P_INNER ) | ( 1 << P_CORRECT ) | ( 1 << P_RESTART );
@@ -62,8 +63,8 @@ struct ReqStatic
 // device-written
 struct ReqDyn
 {
-   int32_t mask;  // bit p set when some system of the request is in phase p
-   int32_t pad;
+   int32_t mask;        // bit p set when some system of the request is in phase p
+   uint32_t bits[ 6 ];  // bits[ p ]: bit j set when system j is in phase p
 };
 
 struct Sys
@@ -96,43 +97,62 @@ struct Arena
    int maxRequests;
 };
 
+// A tile: rpt = 256 / kp rows per row group, kRowsPerLane row groups; lane (lr, j) holds
+// rows base + q * rpt + lr, right-hand side j.
 struct TileCtx
 {
-   int req, ti;
-   int32_t row, j;
-   bool active;   // this lane holds an (row, rhs) entry
-   int64_t e;     // entry index within the request
+   int32_t base, lr, j;
 };
 
-__device__ __forceinline__ TileCtx tileCtx( const Arena& A, int t, const ReqStatic& R )
+__device__ __forceinline__ TileCtx tileCtx( const Arena& A, int t, int req, const ReqStatic& R )
 {
    TileCtx c;
-   c.req = A.tileReq[ t ];
-   c.ti = t - A.tileFirst[ c.req ];
-   const int lr = threadIdx.x / R.kp;
+   const int ti = t - A.tileFirst[ req ];
+   c.lr = threadIdx.x / R.kp;
    c.j = threadIdx.x % R.kp;
-   c.row = c.ti * R.rpt + lr;
-   c.active = lr < R.rpt && c.row < R.n && c.j < R.k;
-   c.e = (int64_t) c.row * R.k + c.j;
+   c.base = ti * R.rpt * kRowsPerLane;
    return c;
 }
 
-// y = sum_k A[row, k] * v[col * K + j] over the SlicedEllpack row (padding column -1)
-template< typename V, typename X, typename Acc >
-__device__ __forceinline__ Acc rowProduct( const Arena& A, const ReqStatic& R, int32_t row, int32_t j, const V* vals,
-                                           const X* v )
+__device__ __forceinline__ bool laneOn( const TileCtx& c, const ReqStatic& R, const ReqDyn& D, int phase )
 {
-   const int32_t s = row >> 5;
-   const int32_t b0 = A.slices[ R.sliceBase + s ], b1 = A.slices[ R.sliceBase + s + 1 ];
-   const int32_t width = ( b1 - b0 ) >> 5;
-   int64_t idx = R.mat + b0 + ( row & 31 );
-   Acc sum = 0;
-   for( int32_t kk = 0; kk < width; kk++, idx += 32 ) {
-      const int32_t col = A.cols[ idx ];
-      if( col >= 0 )
-         sum += (Acc) vals[ idx ] * (Acc) v[ (int64_t) col * R.k + j ];
+   return c.lr < R.rpt && c.j < R.k && ( ( D.bits[ phase ] >> c.j ) & 1u );
+}
+
+// sum[q] = sum_k A[row_q, k] * v[col * K + j] over the SlicedEllpack rows of the lane (padding
+// column -1), left to right; the rows' loads are interleaved so kRowsPerLane are in flight.
+template< typename V, typename X, typename Acc >
+__device__ __forceinline__ void rowProducts( const Arena& A, const ReqStatic& R, const TileCtx& c, const V* vals,
+                                             const X* v, Acc ( &sum )[ kRowsPerLane ] )
+{
+   int32_t width[ kRowsPerLane ];
+   int64_t idx[ kRowsPerLane ];
+   int32_t maxw = 0;
+#pragma unroll
+   for( int q = 0; q < kRowsPerLane; q++ ) {
+      const int32_t row = c.base + q * R.rpt + c.lr;
+      sum[ q ] = 0;
+      width[ q ] = 0;
+      idx[ q ] = 0;
+      if( row < R.n ) {
+         const int32_t s = row >> 5;
+         const int32_t b0 = A.slices[ R.sliceBase + s ], b1 = A.slices[ R.sliceBase + s + 1 ];
+         width[ q ] = ( b1 - b0 ) >> 5;
+         idx[ q ] = R.mat + b0 + ( row & 31 );
+      }
+      maxw = max( maxw, width[ q ] );
    }
-   return sum;
+   for( int32_t kk = 0; kk < maxw; kk++ ) {
+#pragma unroll
+      for( int q = 0; q < kRowsPerLane; q++ ) {
+         if( kk < width[ q ] ) {
+            const int64_t g = idx[ q ] + 32 * (int64_t) kk;
+            const int32_t col = A.cols[ g ];
+            if( col >= 0 )
+               sum[ q ] += (Acc) vals[ g ] * (Acc) v[ (int64_t) col * R.k + c.j ];
+         }
+      }
+   }
 }
 
 // Per-tile sums of W values per right-hand side into part[t][w][j], in a fixed order.
@@ -159,37 +179,51 @@ __device__ __forceinline__ void tilePartials( double* part, int t, int kp, const
    }
 }
 
-// Totals over the request's tiles, tile order fixed; returns LDS holding tot[w * kMaxRhs + j],
-// valid for every thread on return.
+// Totals over the request's tiles in a fixed order (each of the 256 lanes sums a strided
+// share of one (w, j) column, then a fixed tree); returns LDS holding tot[w * kMaxRhs + j],
+// valid in every thread on return.
 template< int W >
 __device__ __forceinline__ const double* requestTotals( const double* part, int first, int tiles, int kp )
 {
+   __shared__ double red[ kTile ];
    __shared__ double out[ 2 * kMaxRhs ];
-   const int pairs = W * kp;
-   const int G = min( 64, kTile / pairs );
+   const int pairs = W * kp;  // a power of two <= 32
+   const int G = kTile / pairs;
    const int tid = threadIdx.x;
-   const int pair = tid / G, sub = tid % G;
-   double s = 0;
-   if( pair < pairs ) {
-      const int w = pair / kp, j = pair % kp;
-      for( int ti = sub; ti < tiles; ti += G )
-         s += part[ (int64_t) ( first + ti ) * 2 * kMaxRhs + w * kMaxRhs + j ];
+   const int pair = tid % pairs, g = tid / pairs;
+   const int w = pair / kp, j = pair % kp;
+   const double* p = part + (int64_t) first * 2 * kMaxRhs + w * kMaxRhs + j;
+   double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+   int ti = g;
+   for( ; ti + 3 * G < tiles; ti += 4 * G ) {
+      s0 += p[ (int64_t) ti * 2 * kMaxRhs ];
+      s1 += p[ (int64_t) ( ti + G ) * 2 * kMaxRhs ];
+      s2 += p[ (int64_t) ( ti + 2 * G ) * 2 * kMaxRhs ];
+      s3 += p[ (int64_t) ( ti + 3 * G ) * 2 * kMaxRhs ];
    }
-   for( int o = G / 2; o > 0; o >>= 1 )
-      s += __shfl_xor( s, o );
-   if( pair < pairs && sub == 0 )
-      out[ ( pair / kp ) * kMaxRhs + pair % kp ] = s;
+   for( ; ti < tiles; ti += G )
+      s0 += p[ (int64_t) ti * 2 * kMaxRhs ];
+   red[ tid ] = ( s0 + s1 ) + ( s2 + s3 );
+   __syncthreads();
+   for( int step = G / 2; step >= 1; step >>= 1 ) {
+      if( g < step )
+         red[ tid ] += red[ tid + step * pairs ];
+      __syncthreads();
+   }
+   if( tid < pairs )
+      out[ w * kMaxRhs + j ] = red[ tid ];
    __syncthreads();
    return out;
 }
 
 // Per-request finish of a reduction kernel: one workgroup per resident request whose mask
 // holds `phase` sums its tiles' partials, lanes j < k update system j, and the request's phase
-// mask is recomputed.  Step( Sys&, const double* tot, int j, const ReqStatic& ).
+// mask and bits are recomputed.  Step( Sys&, const double* tot, int j, const ReqStatic& ).
 template< int W, typename Step >
 __global__ __launch_bounds__( kTile ) void requestFinalKernel( Arena A, int phase, Step step )
 {
    __shared__ int m;
+   __shared__ uint32_t bits[ 6 ];
    for( int req = blockIdx.x; req < A.maxRequests; req += gridDim.x ) {
       if( ! ( A.rd[ req ].mask & ( 1 << phase ) ) )
          continue;
@@ -197,16 +231,23 @@ __global__ __launch_bounds__( kTile ) void requestFinalKernel( Arena A, int phas
       const double* tot = requestTotals< W >( A.part, A.tileFirst[ req ], R.tiles, R.kp );
       if( threadIdx.x == 0 )
          m = 0;
+      if( threadIdx.x < 6 )
+         bits[ threadIdx.x ] = 0;
       if( threadIdx.x < R.k ) {
          Sys& s = A.sys[ req * kMaxRhs + threadIdx.x ];
          step( s, tot, threadIdx.x, R );
       }
       __syncthreads();
-      if( threadIdx.x < R.k )
-         atomicOr( &m, 1 << A.sys[ req * kMaxRhs + threadIdx.x ].phase );
+      if( threadIdx.x < R.k ) {
+         const int ph = A.sys[ req * kMaxRhs + threadIdx.x ].phase;
+         atomicOr( &m, 1 << ph );
+         atomicOr( &bits[ ph ], 1u << threadIdx.x );
+      }
       __syncthreads();
       if( threadIdx.x == 0 )
          A.rd[ req ].mask = m;
+      if( threadIdx.x < 6 )
+         A.rd[ req ].bits[ threadIdx.x ] = bits[ threadIdx.x ];
       __syncthreads();
    }
 }
@@ -222,6 +263,13 @@ __device__ __forceinline__ double* a64( const Arena& A, const ReqStatic& R, int 
 enum { X64 = 0, B64 = 1, R64 = 2 };
 enum { D32 = 0, RR32 = 1, P32 = 2, Q32 = 3 };
 
+// entry index of row group q of the lane, or -1 past the last row
+__device__ __forceinline__ int64_t entryOf( const TileCtx& c, const ReqStatic& R, int q )
+{
+   const int32_t row = c.base + q * R.rpt + c.lr;
+   return row < R.n ? (int64_t) row * R.k + c.j : -1;
+}
+
 // ---------------------------------------------------------------- tile kernels
 
 __global__ __launch_bounds__( kTile ) void innerMatvecKernel( Arena A )
@@ -229,17 +277,27 @@ __global__ __launch_bounds__( kTile ) void innerMatvecKernel( Arena A )
    const int nt = A.ctl->numTiles;
    for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
       const int req = A.tileReq[ t ];
-      if( ! ( A.rd[ req ].mask & ( 1 << P_INNER ) ) )
+      const ReqDyn D = A.rd[ req ];
+      if( ! ( D.mask & ( 1 << P_INNER ) ) )
          continue;
       const ReqStatic R = A.rs[ req ];
-      const TileCtx c = tileCtx( A, t, R );
+      const TileCtx c = tileCtx( A, t, req, R );
       double v[ 1 ] = { 0 };
-      if( c.active && A.sys[ req * kMaxRhs + c.j ].phase == P_INNER ) {
+      if( laneOn( c, R, D, P_INNER ) ) {
          const float* p = a32( A, R, P32 );
-         float q = rowProduct< float, float, float >( A, R, c.row, c.j, A.v32, p );
-         q += (float) R.shift * p[ c.e ];
-         a32( A, R, Q32 )[ c.e ] = q;
-         v[ 0 ] = (double) p[ c.e ] * (double) q;
+         float* qv = a32( A, R, Q32 );
+         float sum[ kRowsPerLane ];
+         rowProducts< float, float, float >( A, R, c, A.v32, p, sum );
+#pragma unroll
+         for( int q = 0; q < kRowsPerLane; q++ ) {
+            const int64_t e = entryOf( c, R, q );
+            if( e >= 0 ) {
+               const float pe = p[ e ];
+               const float r = sum[ q ] + (float) R.shift * pe;
+               qv[ e ] = r;
+               v[ 0 ] += (double) pe * (double) r;
+            }
+         }
       }
       tilePartials< 1 >( A.part, t, R.kp, v );
       __syncthreads();
@@ -265,20 +323,27 @@ __global__ __launch_bounds__( kTile ) void innerUpdateKernel( Arena A )
    const int nt = A.ctl->numTiles;
    for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
       const int req = A.tileReq[ t ];
-      if( ! ( A.rd[ req ].mask & ( 1 << P_INNER ) ) )
+      const ReqDyn D = A.rd[ req ];
+      if( ! ( D.mask & ( 1 << P_INNER ) ) )
          continue;
       const ReqStatic R = A.rs[ req ];
-      const TileCtx c = tileCtx( A, t, R );
+      const TileCtx c = tileCtx( A, t, req, R );
       double v[ 1 ] = { 0 };
-      if( c.active ) {
-         const Sys& s = A.sys[ req * kMaxRhs + c.j ];
-         if( s.phase == P_INNER ) {
-            const float alpha = (float) s.alpha;
-            const float p = a32( A, R, P32 )[ c.e ];
-            a32( A, R, D32 )[ c.e ] += alpha * p;
-            float& rr = a32( A, R, RR32 )[ c.e ];
-            rr -= alpha * a32( A, R, Q32 )[ c.e ];
-            v[ 0 ] = (double) rr * (double) rr;
+      if( laneOn( c, R, D, P_INNER ) ) {
+         const float alpha = (float) A.sys[ req * kMaxRhs + c.j ].alpha;
+         float* d = a32( A, R, D32 );
+         float* rr = a32( A, R, RR32 );
+         const float* p = a32( A, R, P32 );
+         const float* qv = a32( A, R, Q32 );
+#pragma unroll
+         for( int q = 0; q < kRowsPerLane; q++ ) {
+            const int64_t e = entryOf( c, R, q );
+            if( e >= 0 ) {
+               d[ e ] += alpha * p[ e ];
+               const float r = rr[ e ] - alpha * qv[ e ];
+               rr[ e ] = r;
+               v[ 0 ] += (double) r * (double) r;
+            }
          }
       }
       tilePartials< 1 >( A.part, t, R.kp, v );
@@ -309,15 +374,20 @@ __global__ __launch_bounds__( kTile ) void innerDirKernel( Arena A )
    const int nt = A.ctl->numTiles;
    for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
       const int req = A.tileReq[ t ];
-      if( ! ( A.rd[ req ].mask & ( 1 << P_INNER ) ) )
+      const ReqDyn D = A.rd[ req ];
+      if( ! ( D.mask & ( 1 << P_INNER ) ) )
          continue;
       const ReqStatic R = A.rs[ req ];
-      const TileCtx c = tileCtx( A, t, R );
-      if( c.active ) {
-         const Sys& s = A.sys[ req * kMaxRhs + c.j ];
-         if( s.phase == P_INNER ) {
-            float& p = a32( A, R, P32 )[ c.e ];
-            p = a32( A, R, RR32 )[ c.e ] + (float) s.beta * p;
+      const TileCtx c = tileCtx( A, t, req, R );
+      if( laneOn( c, R, D, P_INNER ) ) {
+         const float beta = (float) A.sys[ req * kMaxRhs + c.j ].beta;
+         float* p = a32( A, R, P32 );
+         const float* rr = a32( A, R, RR32 );
+#pragma unroll
+         for( int q = 0; q < kRowsPerLane; q++ ) {
+            const int64_t e = entryOf( c, R, q );
+            if( e >= 0 )
+               p[ e ] = rr[ e ] + beta * p[ e ];
          }
       }
    }
@@ -328,14 +398,21 @@ __global__ __launch_bounds__( kTile ) void correctKernel( Arena A )
    const int nt = A.ctl->numTiles;
    for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
       const int req = A.tileReq[ t ];
-      if( ! ( A.rd[ req ].mask & ( 1 << P_CORRECT ) ) )
+      const ReqDyn D = A.rd[ req ];
+      if( ! ( D.mask & ( 1 << P_CORRECT ) ) )
          continue;
       const ReqStatic R = A.rs[ req ];
-      const TileCtx c = tileCtx( A, t, R );
-      if( c.active ) {
-         const Sys& s = A.sys[ req * kMaxRhs + c.j ];
-         if( s.phase == P_CORRECT )
-            a64( A, R, X64 )[ c.e ] += s.scale * (double) a32( A, R, D32 )[ c.e ];
+      const TileCtx c = tileCtx( A, t, req, R );
+      if( laneOn( c, R, D, P_CORRECT ) ) {
+         const double scale = A.sys[ req * kMaxRhs + c.j ].scale;
+         double* x = a64( A, R, X64 );
+         const float* d = a32( A, R, D32 );
+#pragma unroll
+         for( int q = 0; q < kRowsPerLane; q++ ) {
+            const int64_t e = entryOf( c, R, q );
+            if( e >= 0 )
+               x[ e ] += scale * (double) d[ e ];
+         }
       }
    }
 }
@@ -345,19 +422,29 @@ __global__ __launch_bounds__( kTile ) void outerResidKernel( Arena A )
    const int nt = A.ctl->numTiles;
    for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
       const int req = A.tileReq[ t ];
-      if( ! ( A.rd[ req ].mask & ( 1 << P_CORRECT ) ) )
+      const ReqDyn D = A.rd[ req ];
+      if( ! ( D.mask & ( 1 << P_CORRECT ) ) )
          continue;
       const ReqStatic R = A.rs[ req ];
-      const TileCtx c = tileCtx( A, t, R );
+      const TileCtx c = tileCtx( A, t, req, R );
       double v[ 2 ] = { 0, 0 };
-      if( c.active && A.sys[ req * kMaxRhs + c.j ].phase == P_CORRECT ) {
+      if( laneOn( c, R, D, P_CORRECT ) ) {
          const double* x = a64( A, R, X64 );
-         const double bb = a64( A, R, B64 )[ c.e ];
-         const double ax = rowProduct< double, double, double >( A, R, c.row, c.j, A.v64, x ) + R.shift * x[ c.e ];
-         const double r = bb - ax;
-         a64( A, R, R64 )[ c.e ] = r;
-         v[ 0 ] = r * r;
-         v[ 1 ] = bb * bb;
+         const double* b = a64( A, R, B64 );
+         double* r = a64( A, R, R64 );
+         double sum[ kRowsPerLane ];
+         rowProducts< double, double, double >( A, R, c, A.v64, x, sum );
+#pragma unroll
+         for( int q = 0; q < kRowsPerLane; q++ ) {
+            const int64_t e = entryOf( c, R, q );
+            if( e >= 0 ) {
+               const double bb = b[ e ];
+               const double re = bb - ( sum[ q ] + R.shift * x[ e ] );
+               r[ e ] = re;
+               v[ 0 ] += re * re;
+               v[ 1 ] += bb * bb;
+            }
+         }
       }
       tilePartials< 2 >( A.part, t, R.kp, v );
       __syncthreads();
@@ -396,19 +483,28 @@ __global__ __launch_bounds__( kTile ) void restartKernel( Arena A )
    const int nt = A.ctl->numTiles;
    for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
       const int req = A.tileReq[ t ];
-      if( ! ( A.rd[ req ].mask & ( 1 << P_RESTART ) ) )
+      const ReqDyn D = A.rd[ req ];
+      if( ! ( D.mask & ( 1 << P_RESTART ) ) )
          continue;
       const ReqStatic R = A.rs[ req ];
-      const TileCtx c = tileCtx( A, t, R );
+      const TileCtx c = tileCtx( A, t, req, R );
       double v[ 1 ] = { 0 };
-      if( c.active ) {
-         const Sys& s = A.sys[ req * kMaxRhs + c.j ];
-         if( s.phase == P_RESTART ) {
-            const float rr = (float) ( a64( A, R, R64 )[ c.e ] / s.scale );
-            a32( A, R, RR32 )[ c.e ] = rr;
-            a32( A, R, P32 )[ c.e ] = rr;
-            a32( A, R, D32 )[ c.e ] = 0.0f;
-            v[ 0 ] = (double) rr * (double) rr;
+      if( laneOn( c, R, D, P_RESTART ) ) {
+         const double scale = A.sys[ req * kMaxRhs + c.j ].scale;
+         const double* r = a64( A, R, R64 );
+         float* rr = a32( A, R, RR32 );
+         float* p = a32( A, R, P32 );
+         float* d = a32( A, R, D32 );
+#pragma unroll
+         for( int q = 0; q < kRowsPerLane; q++ ) {
+            const int64_t e = entryOf( c, R, q );
+            if( e >= 0 ) {
+               const float x = (float) ( r[ e ] / scale );
+               rr[ e ] = x;
+               p[ e ] = x;
+               d[ e ] = 0.0f;
+               v[ 0 ] += (double) x * (double) x;
+            }
          }
       }
       tilePartials< 1 >( A.part, t, R.kp, v );
@@ -667,7 +763,7 @@ int admit( tnlhip_server_s& S )
       R.k = q.k;
       R.kp = kpOf( q.k );
       R.rpt = kTile / R.kp;
-      R.tiles = ( M.n + R.rpt - 1 ) / R.rpt;
+      R.tiles = ( M.n + R.rpt * kRowsPerLane - 1 ) / ( R.rpt * kRowsPerLane );
       R.sliceBase = M.sliceBase;
       R.maxOuter = q.params.max_outer_iterations;
       R.maxInner = q.params.max_inner_iterations;
@@ -678,7 +774,10 @@ int admit( tnlhip_server_s& S )
       R.innerTol2 = q.params.inner_tolerance * q.params.inner_tolerance;
       R.stagnation = q.params.stagnation_limit;
       S.hStatic[ slot ] = R;
-      S.hDyn[ slot ] = ReqDyn{ 1 << P_CORRECT, 0 };
+      ReqDyn dyn{};
+      dyn.mask = 1 << P_CORRECT;
+      dyn.bits[ P_CORRECT ] = ( 1u << q.k ) - 1u;
+      S.hDyn[ slot ] = dyn;
       for( int j = 0; j < kMaxRhs; j++ ) {
          Sys s{};
          s.phase = j < q.k ? P_CORRECT : P_IDLE;
@@ -764,7 +863,7 @@ void tnlhip_request_params_default( tnlhip_request_params* p )
       return;
    std::memset( p, 0, sizeof( *p ) );
    p->tolerance = 1e-10;
-   p->inner_tolerance = 1e-4;
+   p->inner_tolerance = 3e-6;
    p->max_outer_iterations = 20;
    p->max_inner_iterations = 5000;
    p->method = TNLHIP_CG;

@@ -166,6 +166,20 @@ __global__ void waveKernel( Fin fin, Gate gate )
    fin( (int) threadIdx.x );
 }
 
+// Two lane-parallel steps in one launch, the second seeing the first's writes.
+template< typename F1, typename F2 >
+struct Then
+{
+   F1 a;
+   F2 b;
+   __device__ void operator()( int lane ) const
+   {
+      a( lane );
+      __syncthreads();
+      b( lane );
+   }
+};
+
 template< typename Fin >
 void launchWave( const Fin& fin, Gate gate, hipStream_t st )
 {
@@ -383,16 +397,65 @@ void computeNormb( SolverImpl& S, const void* b, T* tmp )
 template< typename Iter >
 void runChunks( SolverImpl& S, Iter&& iteration )
 {
-   int chunk = 4;
+   // Every kernel of an iteration is gated by the device stop flag, so iterations can be queued
+   // ahead and the host reads the state once per batch.  Short solves finish in the first
+   // eager batches (4 and 8 iterations); longer ones replay a captured graph of kGraphIters
+   // iterations (the kernels' arguments are fixed for the whole solve).
+   constexpr int kGraphIters = 16;
    int64_t queued = 0;
    const int64_t limit = (int64_t) S.params.max_iterations + 2;
-   while( true ) {
+   for( int chunk : { 4, 8 } ) {
       for( int k = 0; k < chunk && queued < limit; k++, queued++ )
          iteration();
       const State h = readState( S );
       if( h.flags[ 0 ] || queued >= limit )
-         break;
-      chunk = std::min( chunk * 2, 64 );
+         return;
+   }
+   if( ! S.params.use_graph ) {
+      int chunk = 16;
+      while( true ) {
+         for( int k = 0; k < chunk && queued < limit; k++, queued++ )
+            iteration();
+         const State h = readState( S );
+         if( h.flags[ 0 ] || queued >= limit )
+            return;
+         chunk = std::min( chunk * 2, 64 );
+      }
+   }
+   struct Graph
+   {
+      hipGraph_t g = nullptr;
+      hipGraphExec_t e = nullptr;
+      ~Graph()
+      {
+         if( e )
+            (void) hipGraphExecDestroy( e );
+         if( g )
+            (void) hipGraphDestroy( g );
+      }
+   } graph;
+   TNLHIP_CHECK( hipStreamBeginCapture( S.stream, hipStreamCaptureModeThreadLocal ) );
+   try {
+      for( int k = 0; k < kGraphIters; k++ )
+         iteration();
+   }
+   catch( ... ) {
+      hipGraph_t broken = nullptr;
+      (void) hipStreamEndCapture( S.stream, &broken );
+      if( broken )
+         (void) hipGraphDestroy( broken );
+      throw;
+   }
+   TNLHIP_CHECK( hipStreamEndCapture( S.stream, &graph.g ) );
+   TNLHIP_CHECK( hipGraphInstantiate( &graph.e, graph.g, nullptr, nullptr, 0 ) );
+   int reps = 1;
+   while( true ) {
+      for( int r = 0; r < reps && queued < limit; r++, queued += kGraphIters )
+         TNLHIP_CHECK( hipGraphLaunch( graph.e, S.stream ) );
+      const State h = readState( S );
+      if( h.flags[ 0 ] || queued >= limit )
+         return;
+      reps = std::min( reps * 2, 4 );
    }
 }
 
@@ -1322,6 +1385,105 @@ struct OpCwyZ
    }
 };
 
+// z = w - Y_i aux (as OpCwyZ), fused with the next step's hauseholder_generate inputs:
+// y_{i+1} = z with entries < i+1 zeroed (written), |y_{i+1}|^2 into acc[C-1] and the
+// projections Y_k^T y_{i+1}, k <= i, into acc[k] (the Y entries are already loaded for z).
+template< typename T, int C >
+struct OpCwyZGenerate
+{
+   T* z;
+   const T* w;
+   const T* Y;
+   int64_t ld;
+   const double* auxp;
+   int count;  // columns 0..i (count = i + 1 <= C - 1)
+   T* ynext;
+   __device__ void load() {}
+   __device__ void operator()( int64_t j, double ( &acc )[ C ] ) const
+   {
+      T y[ C - 1 ];
+#pragma unroll
+      for( int k = 0; k < C - 1; k++ )
+         y[ k ] = Y[ j + (int64_t) min( k, count - 1 ) * ld ];
+      double t = 0.0;
+#pragma unroll
+      for( int k = 0; k < C - 1; k++ )
+         if( k < count )
+            t += (double) y[ k ] * auxp[ k ];
+      const T zj = (T) ( (double) w[ j ] - t );
+      z[ j ] = zj;
+      const T yn = j < count ? (T) 0 : zj;
+      ynext[ j ] = yn;
+      const double ynd = (double) yn;
+      acc[ C - 1 ] += ynd * ynd;
+#pragma unroll
+      for( int k = 0; k < C - 1; k++ )
+         acc[ k ] += ( k < count ? (double) y[ k ] : 0.0 ) * ynd;
+   }
+};
+
+// The rest of hauseholder_generate for step i from the fused sums: y_ii +- |y|, t_i, and the
+// new column of T from Y^T y_i corrected for the changed entry i (GMRES.hpp:464-520).
+template< typename T, int C >
+struct FinGenerateFused
+{
+   Small q;
+   T* yi;
+   const T* Y;
+   int64_t ld;
+   int i;
+   static constexpr bool kLaneParallel = true;
+   __device__ void operator()( const double* v, int lane ) const
+   {
+      __shared__ double aux[ kMaxMulti ];
+      __shared__ double delta;
+      const int mx = q.mx;
+      if( lane == 0 ) {
+         const double normz = sqrt( v[ C - 1 ] );
+         const double y_ii = (double) yi[ i ];
+         const T ynew = (T) ( y_ii > 0.0 ? y_ii + normz : y_ii - normz );
+         yi[ i ] = ynew;
+         const double norm_yi_squared = 2 * ( normz * normz + fabs( y_ii ) * normz );
+         q.T[ i + i * ( mx + 1 ) ] = 2.0 / norm_yi_squared;
+         delta = (double) ynew - y_ii;
+      }
+      __syncthreads();
+      if( lane < i )
+         aux[ lane ] = v[ lane ] + (double) Y[ i + (int64_t) lane * ld ] * delta;
+      __syncthreads();
+      const double ti = q.T[ i + i * ( mx + 1 ) ];
+      if( lane < i ) {
+         double t = 0.0;
+         for( int j = lane; j < i; j++ )
+            t -= q.T[ lane + j * ( mx + 1 ) ] * ( ti * aux[ j ] );
+         q.T[ lane + i * ( mx + 1 ) ] = t;
+      }
+   }
+};
+
+template< typename T, int C >
+void launchCwyZGenerate( int64_t n, T* z, const T* w, const T* Y, int64_t ld, const double* aux, int count, Small q,
+                         ReduceSlot slot, Gate gate, hipStream_t st )
+{
+   fusedReduceKernel< C ><<< kReduceGrid, kReduceBlock, 0, st >>>(
+      n, OpCwyZGenerate< T, C >{ z, w, Y, ld, aux, count, const_cast< T* >( Y ) + (int64_t) count * ld }, slot, gate );
+   TNLHIP_CHECK( hipGetLastError() );
+}
+
+template< typename T, int C >
+void launchGenerateFinal( Small q, T* yi, const T* Y, int64_t ld, int i, ReduceSlot slot, Gate gate, hipStream_t st )
+{
+   reduceFinalKernel< C ><<< 1, kReduceBlock, 0, st >>>( FinGenerateFused< T, C >{ q, yi, Y, ld, i }, slot, kReduceGrid,
+                                                         gate );
+   TNLHIP_CHECK( hipGetLastError() );
+}
+
+// column bucket of the fused pass for `count` projections plus the norm
+inline int fusedBucket( int count )
+{
+   return count + 1 <= 8 ? 8 : count + 1 <= 16 ? 16 : 32;
+}
+
 struct FinBeta
 {
    // beta = |r|; residue = beta / normb
@@ -1443,21 +1605,36 @@ void solveGMRES( SolverImpl& S, const void* b, void* xv )
          for( int i = 0; i <= m; i++ ) {
             T* yi = V + i * ld;
             launchScalar( FinGmresStepStart{ st, q, stop, -1 }, cycle, S.stream );
-            // hauseholder_generate
-            launchReduce< 1 >( n, OpTruncate< T >{ {}, yi, z, i }, FinHouseholder< T >{ st, q, yi, i }, S.ws.slot( 0 ),
-                               cycle, S.stream );
-            if( i > 0 )
-               launchMulti< T >( n, V, ld, yi, i, FinTUpdate{ q, i }, S.ws.slot( 1 ), cycle, S.stream );
+            // hauseholder_generate (for i > 0 its passes over z and Y ran fused with the previous
+            // step's z = (I - Y T^T Y^T) w; only the small finish is left)
+            if( i == 0 )
+               launchReduce< 1 >( n, OpTruncate< T >{ {}, yi, z, i }, FinHouseholder< T >{ st, q, yi, i },
+                                  S.ws.slot( 0 ), cycle, S.stream );
+            else {
+               const int C = fusedBucket( i );
+               if( C == 8 )
+                  launchGenerateFinal< T, 8 >( q, yi, V, ld, i, S.ws.slot( 3 ), cycle, S.stream );
+               else if( C == 16 )
+                  launchGenerateFinal< T, 16 >( q, yi, V, ld, i, S.ws.slot( 3 ), cycle, S.stream );
+               else
+                  launchGenerateFinal< T, 32 >( q, yi, V, ld, i, S.ws.slot( 3 ), cycle, S.stream );
+            }
             // hauseholder_apply_trunc (s or H column i-1)
-            launchWave( FinYL< T >{ st, q, yi, z, i, n }, cycle, S.stream );
-            // hauseholder_cwy: v = e_i - Y_i T_i Y_i^T e_i
-            launchWave( FinCwyAux{ q, i }, cycle, S.stream );
+            // apply_trunc, then hauseholder_cwy's aux = T_i Y_i^T e_i (for v = e_i - Y_i aux)
+            launchWave( Then< FinYL< T >, FinCwyAux >{ FinYL< T >{ st, q, yi, z, i, n }, FinCwyAux{ q, i } }, cycle,
+                        S.stream );
             launchElementwise( n, OpCwyV< T >{ vtmp, V, ld, q.aux, i + 1, i }, cycle, S.stream );
             if( i < m ) {
                matvec( S, vtmp, w, cycle );
                // z = (I - Y_i T_i^T Y_i^T) w
                launchMulti< T >( n, V, ld, w, i + 1, FinCwyTransposed{ q, i }, S.ws.slot( 2 ), cycle, S.stream );
-               launchElementwise( n, OpCwyZ< T >{ z, w, V, ld, q.aux, i + 1 }, cycle, S.stream );
+               const int C = fusedBucket( i + 1 );
+               if( C == 8 )
+                  launchCwyZGenerate< T, 8 >( n, z, w, V, ld, q.aux, i + 1, q, S.ws.slot( 3 ), cycle, S.stream );
+               else if( C == 16 )
+                  launchCwyZGenerate< T, 16 >( n, z, w, V, ld, q.aux, i + 1, q, S.ws.slot( 3 ), cycle, S.stream );
+               else
+                  launchCwyZGenerate< T, 32 >( n, z, w, V, ld, q.aux, i + 1, q, S.ws.slot( 3 ), cycle, S.stream );
             }
             launchScalar( FinGmresStepEnd{ st, q, stop, S.hist.ptr, S.histCap, i, i, i - 1, i > 0, i - 1, i > 0 },
                           cycle, S.stream );
@@ -1555,6 +1732,7 @@ void tnlhip_solver_params_default( tnlhip_solver_params* p )
    p->restarting_step_max = 3;
    p->bicgstab_exact_residue = 0;
    p->preconditioner = 0;
+   p->use_graph = 1;
 }
 
 int tnlhip_solver_create( int method, tnlhip_solver* out )

@@ -44,7 +44,7 @@ struct Keeper
          case 3: out = beta * (A) y[ row ] + alpha * v; break;
          default: out = v / (A) diag[ row ]; break;
       }
-      y[ row ] = (Y) out;
+      __builtin_nontemporal_store( (Y) out, y + row );  // y is written once: keep L2 for x
    }
 };
 
@@ -144,6 +144,8 @@ __global__ __launch_bounds__( 256 ) void csrGroupedKernel( const int32_t* __rest
 // LONG: one row split over the waves left in the workgroup, partial sums combined in LDS in
 // a fixed order.  One descriptor per wave and no grid-stride loop, so the single workgroup
 // barrier (needed only by LONG rows) is uniform.
+constexpr int kPreSets = 4;  // STREAM blocks: offsets of the first 4 x 64 rows are preloaded
+
 template< int STREAM, typename V, typename A, typename Y >
 __global__ __launch_bounds__( 256 ) void csrAdaptiveKernel( int32_t nblocks, const int4* __restrict__ desc,
                                                             const int32_t* __restrict__ off,
@@ -170,8 +172,13 @@ __global__ __launch_bounds__( 256 ) void csrAdaptiveKernel( int32_t nblocks, con
    }
    else if( active && type == BLOCK_STREAM ) {
       const int32_t size = d.z;
-      // offsets of the first 64 rows of the block, fetched alongside the element stream
-      const int32_t rbPre = lane <= d.w ? off[ firstRow + lane ] : 0;
+      // offsets of the first kPreRows rows of the block, fetched alongside the element stream
+      int32_t rbPre[ kPreSets ];
+#pragma unroll
+      for( int q = 0; q < kPreSets; q++ ) {
+         const int32_t r = lane + q * kWave;
+         rbPre[ q ] = r <= d.w ? off[ firstRow + r ] : 0;
+      }
       int32_t c[ K ];
       V v[ K ];
 #pragma unroll
@@ -191,10 +198,25 @@ __global__ __launch_bounds__( 256 ) void csrAdaptiveKernel( int32_t nblocks, con
             lds[ wave ][ e ] = c[ k ] >= 0 ? toCompute( v[ k ], A() ) * (A) xv[ k ] : (A) 0;
       }
       waveSync();
-      const int32_t reNext = __shfl( rbPre, min( lane + 1, kWave - 1 ), kWave );
-      for( int32_t r = lane; r < d.w; r += kWave ) {
-         const int32_t rb = ( r < kWave ? rbPre : off[ firstRow + r ] ) - b;
-         const int32_t re = ( r + 1 < kWave ? reNext : off[ firstRow + r + 1 ] ) - b;
+#pragma unroll
+      for( int q = 0; q < kPreSets; q++ ) {
+         // row r = lane + 64 q ends where row r + 1 starts: the next lane's offset, or the
+         // first of the next set for lane 63
+         const int32_t nextLane = __shfl( rbPre[ q ], ( lane + 1 ) % kWave, kWave );
+         const int32_t nextSet = q + 1 < kPreSets ? __shfl( rbPre[ q + 1 < kPreSets ? q + 1 : q ], 0, kWave ) : 0;
+         const int32_t r = lane + q * kWave;
+         if( r < d.w ) {
+            const int32_t rb = rbPre[ q ] - b;
+            const int32_t re = ( lane + 1 < kWave ? nextLane : q + 1 < kPreSets ? nextSet : off[ firstRow + r + 1 ] ) - b;
+            A s = 0;
+            for( int32_t k = rb; k < re; k++ )
+               s += lds[ wave ][ k ];
+            keep( firstRow + r, s );
+         }
+      }
+      for( int32_t r = lane + kPreSets * kWave; r < d.w; r += kWave ) {
+         const int32_t rb = off[ firstRow + r ] - b;
+         const int32_t re = off[ firstRow + r + 1 ] - b;
          A s = 0;
          for( int32_t k = rb; k < re; k++ )
             s += lds[ wave ][ k ];
@@ -453,6 +475,8 @@ __global__ __launch_bounds__( 256 ) void chunkedKernel( int32_t nslices, const i
 // l % height are folded with xor-shuffles.  Lane l < height keeps rank l's running sum; at
 // the end each lane fetches the sum of its own row's rank (rowPermArray) and stores it, so
 // y is written coalesced.
+constexpr int kBiUnroll = 8;
+
 template< typename V, typename A, typename Y >
 __global__ __launch_bounds__( 256 ) void biellpackKernel( int32_t n, int32_t strips, const int32_t* __restrict__ rowPerm,
                                                           const int32_t* __restrict__ groupPointers,
@@ -477,21 +501,22 @@ __global__ __launch_bounds__( 256 ) void biellpackKernel( int32_t n, int32_t str
             const int32_t base = gp[ grp ], size = gp[ grp + 1 ] - base;
             const int h = kStrip >> grp;
             A p = 0;
-            for( int32_t e0 = l; e0 < size; e0 += 4 * kStrip ) {
-               int32_t c[ 4 ];
-               V v[ 4 ];
+            // 8 elements per lane in flight: a 32-row group up to 8 wide is one round trip
+            for( int32_t e0 = l; e0 < size; e0 += kBiUnroll * kStrip ) {
+               int32_t c[ kBiUnroll ];
+               V v[ kBiUnroll ];
 #pragma unroll
-               for( int j = 0; j < 4; j++ ) {
+               for( int j = 0; j < kBiUnroll; j++ ) {
                   const int32_t e = min( e0 + j * kStrip, size - 1 );
                   c[ j ] = ldnt( col + base + e );
                   v[ j ] = ldnt( val + base + e );
                }
-               Y xv[ 4 ];
+               Y xv[ kBiUnroll ];
 #pragma unroll
-               for( int j = 0; j < 4; j++ )
+               for( int j = 0; j < kBiUnroll; j++ )
                   xv[ j ] = x[ c[ j ] < 0 ? 0 : c[ j ] ];
 #pragma unroll
-               for( int j = 0; j < 4; j++ )
+               for( int j = 0; j < kBiUnroll; j++ )
                   if( e0 + j * kStrip < size && c[ j ] >= 0 )
                      p += toCompute( v[ j ], A() ) * (A) xv[ j ];
             }
