@@ -205,3 +205,26 @@ def test_invalid_arguments_raise(tnl, dev):
     bad[3] = 99
     with pytest.raises(tnl.TnlHipError):
         tnl.SparseMatrix.from_csr(rp, bad, v)
+
+
+@pytest.mark.parametrize("fmt,params", [("Ellpack", {"xcd_remap": 3}), ("Ellpack", {"xcd_remap": 3, "pair_rows": 1}),
+                                        ("SlicedEllpack", {"xcd_remap": 3}),
+                                        ("SlicedEllpack", {"xcd_remap": 3, "pair_rows": 1, "blocks_per_cu": 4}),
+                                        ("CSRAdaptive", {"xcd_remap": 1}), ("CSRAdaptive", {"xcd_remap": 3})])
+def test_launch_distributions_give_the_same_bits(tnl, dev, fmt, params):
+    """Row distributions over the XCDs change only which workgroup computes a row: every keeper
+    mode and row range gives the default launch's bits (a row computed twice or skipped would
+    show in the beta != 0 cases)."""
+    rng = np.random.default_rng(5)
+    for rp, c, v in (W.irregular(30001, mean=10, sigma=1.3, max_len=1500, seed=9), W.laplacian_3d(33)):
+        n = len(rp) - 1
+        x = rng.uniform(-1, 1, n)
+        y0 = rng.uniform(-1, 1, n)
+        A = tnl.SparseMatrix.from_csr(rp, c, v, format=fmt, dtype="f64")
+        B = tnl.SparseMatrix.from_csr(rp, c, v, format=fmt, dtype="f64")
+        for k, val in params.items():
+            B.set_param(k, val)
+        for alpha, beta, first, last in [(1.0, 0.0, 0, 0), (1.0, -0.5, 17, n - 5), (0.75, 3.0, 1000, 1001)]:
+            want = run_spmv(tnl, A, x, dev, y0, alpha, beta, first, last)
+            got = run_spmv(tnl, B, x, dev, y0, alpha, beta, first, last)
+            np.testing.assert_array_equal(got, want, err_msg=f"{fmt} {params} {alpha} {beta} {first} {last}")

@@ -44,7 +44,7 @@ namespace {
 constexpr int kTile = 256;
 constexpr int kMaxRhs = 16;
 constexpr int kServerGrid = 2048;  // 8 workgroups of 256 per CU
-constexpr int kRowsPerLane = 4;     // rows per lane in a tile (independent loads in flight)
+constexpr int kRowsPerLane = 8;     // rows per lane in a tile (independent loads in flight)
 
 enum Phase : int32_t { P_IDLE = 0, P_INNER = 1, P_CORRECT = 2, P_RESTART = 3, P_DONE = 4, P_FAILED = 5 };
 constexpr int32_t kLive = ( 1 << P_INNER ) | ( 1 << P_CORRECT ) | ( 1 << P_RESTART );
@@ -120,38 +120,73 @@ __device__ __forceinline__ bool laneOn( const TileCtx& c, const ReqStatic& R, co
 }
 
 // sum[q] = sum_k A[row_q, k] * v[col * K + j] over the SlicedEllpack rows of the lane (padding
-// column -1), left to right; the rows' loads are interleaved so kRowsPerLane are in flight.
+// column -1), left to right.  Rows go in pairs, eight slots of both rows per round trip (loads
+// past a row's width are masked off), so 16 column, value and x loads are in flight.
+constexpr int kSlotChunk = 8;
+
 template< typename V, typename X, typename Acc >
 __device__ __forceinline__ void rowProducts( const Arena& A, const ReqStatic& R, const TileCtx& c, const V* vals,
                                              const X* v, Acc ( &sum )[ kRowsPerLane ] )
 {
-   int32_t width[ kRowsPerLane ];
-   int64_t idx[ kRowsPerLane ];
-   int32_t maxw = 0;
 #pragma unroll
-   for( int q = 0; q < kRowsPerLane; q++ ) {
-      const int32_t row = c.base + q * R.rpt + c.lr;
-      sum[ q ] = 0;
-      width[ q ] = 0;
-      idx[ q ] = 0;
-      if( row < R.n ) {
-         const int32_t s = row >> 5;
-         const int32_t b0 = A.slices[ R.sliceBase + s ], b1 = A.slices[ R.sliceBase + s + 1 ];
-         width[ q ] = ( b1 - b0 ) >> 5;
-         idx[ q ] = R.mat + b0 + ( row & 31 );
-      }
-      maxw = max( maxw, width[ q ] );
-   }
-   for( int32_t kk = 0; kk < maxw; kk++ ) {
+   for( int q0 = 0; q0 < kRowsPerLane; q0 += 2 ) {
+      int32_t width[ 2 ];
+      int64_t idx[ 2 ];
 #pragma unroll
-      for( int q = 0; q < kRowsPerLane; q++ ) {
-         if( kk < width[ q ] ) {
-            const int64_t g = idx[ q ] + 32 * (int64_t) kk;
-            const int32_t col = A.cols[ g ];
-            if( col >= 0 )
-               sum[ q ] += (Acc) vals[ g ] * (Acc) v[ (int64_t) col * R.k + c.j ];
+      for( int u = 0; u < 2; u++ ) {
+         const int32_t row = c.base + ( q0 + u ) * R.rpt + c.lr;
+         width[ u ] = 0;
+         idx[ u ] = 0;
+         if( row < R.n ) {
+            const int32_t s = row >> 5;
+            const int32_t b0 = A.slices[ R.sliceBase + s ], b1 = A.slices[ R.sliceBase + s + 1 ];
+            width[ u ] = ( b1 - b0 ) >> 5;
+            idx[ u ] = R.mat + b0 + ( row & 31 );
          }
       }
+      Acc acc[ 2 ] = { 0, 0 };
+      const int32_t maxw = max( width[ 0 ], width[ 1 ] );
+      for( int32_t k0 = 0; k0 < maxw; k0 += kSlotChunk ) {
+         int32_t col[ 2 ][ kSlotChunk ];
+#pragma unroll
+         for( int u = 0; u < 2; u++ )
+#pragma unroll
+            for( int k = 0; k < kSlotChunk; k++ )
+               col[ u ][ k ] = k0 + k < width[ u ] ? A.cols[ idx[ u ] + 32 * (int64_t) ( k0 + k ) ] : -1;
+         Acc term[ 2 ][ kSlotChunk ];
+#pragma unroll
+         for( int u = 0; u < 2; u++ )
+#pragma unroll
+            for( int k = 0; k < kSlotChunk; k++ )
+               term[ u ][ k ] = col[ u ][ k ] >= 0
+                                   ? (Acc) vals[ idx[ u ] + 32 * (int64_t) ( k0 + k ) ]
+                                        * (Acc) v[ (int64_t) col[ u ][ k ] * R.k + c.j ]
+                                   : (Acc) 0;
+#pragma unroll
+         for( int u = 0; u < 2; u++ )
+#pragma unroll
+            for( int k = 0; k < kSlotChunk; k++ )
+               if( col[ u ][ k ] >= 0 )
+                  acc[ u ] += term[ u ][ k ];
+      }
+      sum[ q0 ] = acc[ 0 ];
+      sum[ q0 + 1 ] = acc[ 1 ];
+   }
+}
+
+// Walks the tile table, fetching the next tile's request one step ahead; f( t, req ).
+template< typename F >
+__device__ __forceinline__ void forTiles( const Arena& A, F&& f )
+{
+   const int nt = A.ctl->numTiles;
+   int t = blockIdx.x;
+   int req = t < nt ? A.tileReq[ t ] : 0;
+   while( t < nt ) {
+      const int tn = t + gridDim.x;
+      const int rn = tn < nt ? A.tileReq[ tn ] : 0;
+      f( t, req );
+      t = tn;
+      req = rn;
    }
 }
 
@@ -274,12 +309,10 @@ __device__ __forceinline__ int64_t entryOf( const TileCtx& c, const ReqStatic& R
 
 __global__ __launch_bounds__( kTile ) void innerMatvecKernel( Arena A )
 {
-   const int nt = A.ctl->numTiles;
-   for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
-      const int req = A.tileReq[ t ];
+   forTiles( A, [ & ]( int t, int req ) {
       const ReqDyn D = A.rd[ req ];
       if( ! ( D.mask & ( 1 << P_INNER ) ) )
-         continue;
+         return;
       const ReqStatic R = A.rs[ req ];
       const TileCtx c = tileCtx( A, t, req, R );
       double v[ 1 ] = { 0 };
@@ -301,7 +334,7 @@ __global__ __launch_bounds__( kTile ) void innerMatvecKernel( Arena A )
       }
       tilePartials< 1 >( A.part, t, R.kp, v );
       __syncthreads();
-   }
+   } );
 }
 
 struct StepAlpha
@@ -320,12 +353,10 @@ struct StepAlpha
 
 __global__ __launch_bounds__( kTile ) void innerUpdateKernel( Arena A )
 {
-   const int nt = A.ctl->numTiles;
-   for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
-      const int req = A.tileReq[ t ];
+   forTiles( A, [ & ]( int t, int req ) {
       const ReqDyn D = A.rd[ req ];
       if( ! ( D.mask & ( 1 << P_INNER ) ) )
-         continue;
+         return;
       const ReqStatic R = A.rs[ req ];
       const TileCtx c = tileCtx( A, t, req, R );
       double v[ 1 ] = { 0 };
@@ -348,7 +379,7 @@ __global__ __launch_bounds__( kTile ) void innerUpdateKernel( Arena A )
       }
       tilePartials< 1 >( A.part, t, R.kp, v );
       __syncthreads();
-   }
+   } );
 }
 
 struct StepBeta
@@ -371,12 +402,10 @@ struct StepBeta
 
 __global__ __launch_bounds__( kTile ) void innerDirKernel( Arena A )
 {
-   const int nt = A.ctl->numTiles;
-   for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
-      const int req = A.tileReq[ t ];
+   forTiles( A, [ & ]( int t, int req ) {
       const ReqDyn D = A.rd[ req ];
       if( ! ( D.mask & ( 1 << P_INNER ) ) )
-         continue;
+         return;
       const ReqStatic R = A.rs[ req ];
       const TileCtx c = tileCtx( A, t, req, R );
       if( laneOn( c, R, D, P_INNER ) ) {
@@ -390,17 +419,15 @@ __global__ __launch_bounds__( kTile ) void innerDirKernel( Arena A )
                p[ e ] = rr[ e ] + beta * p[ e ];
          }
       }
-   }
+   } );
 }
 
 __global__ __launch_bounds__( kTile ) void correctKernel( Arena A )
 {
-   const int nt = A.ctl->numTiles;
-   for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
-      const int req = A.tileReq[ t ];
+   forTiles( A, [ & ]( int t, int req ) {
       const ReqDyn D = A.rd[ req ];
       if( ! ( D.mask & ( 1 << P_CORRECT ) ) )
-         continue;
+         return;
       const ReqStatic R = A.rs[ req ];
       const TileCtx c = tileCtx( A, t, req, R );
       if( laneOn( c, R, D, P_CORRECT ) ) {
@@ -414,17 +441,15 @@ __global__ __launch_bounds__( kTile ) void correctKernel( Arena A )
                x[ e ] += scale * (double) d[ e ];
          }
       }
-   }
+   } );
 }
 
 __global__ __launch_bounds__( kTile ) void outerResidKernel( Arena A )
 {
-   const int nt = A.ctl->numTiles;
-   for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
-      const int req = A.tileReq[ t ];
+   forTiles( A, [ & ]( int t, int req ) {
       const ReqDyn D = A.rd[ req ];
       if( ! ( D.mask & ( 1 << P_CORRECT ) ) )
-         continue;
+         return;
       const ReqStatic R = A.rs[ req ];
       const TileCtx c = tileCtx( A, t, req, R );
       double v[ 2 ] = { 0, 0 };
@@ -448,7 +473,7 @@ __global__ __launch_bounds__( kTile ) void outerResidKernel( Arena A )
       }
       tilePartials< 2 >( A.part, t, R.kp, v );
       __syncthreads();
-   }
+   } );
 }
 
 struct StepOuter
@@ -480,12 +505,10 @@ struct StepOuter
 
 __global__ __launch_bounds__( kTile ) void restartKernel( Arena A )
 {
-   const int nt = A.ctl->numTiles;
-   for( int t = blockIdx.x; t < nt; t += gridDim.x ) {
-      const int req = A.tileReq[ t ];
+   forTiles( A, [ & ]( int t, int req ) {
       const ReqDyn D = A.rd[ req ];
       if( ! ( D.mask & ( 1 << P_RESTART ) ) )
-         continue;
+         return;
       const ReqStatic R = A.rs[ req ];
       const TileCtx c = tileCtx( A, t, req, R );
       double v[ 1 ] = { 0 };
@@ -509,7 +532,7 @@ __global__ __launch_bounds__( kTile ) void restartKernel( Arena A )
       }
       tilePartials< 1 >( A.part, t, R.kp, v );
       __syncthreads();
-   }
+   } );
 }
 
 struct StepRestart

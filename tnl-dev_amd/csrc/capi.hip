@@ -464,7 +464,8 @@ int tnlhip_matrix_set_param( tnlhip_matrix A, const char* name_c, int32_t value 
       else if( name == "xcd_remap" ) {
          // 2 (contiguous blocks without the XCD remap) is switched off: a run of it ended in a
          // GPU memory fault whose cause is not established
-         TNLHIP_REQUIRE( value == 0 || value == 1, "xcd_remap is 0 (grid-stride) or 1 (XCD-remapped blocks)" );
+         TNLHIP_REQUIRE( value == 0 || value == 1 || value == 3,
+                         "xcd_remap is 0 (grid-stride), 1 (XCD-remapped blocks) or 3 (per-XCD grid-stride)" );
          A->xcdRemap = value;
       }
       else if( name == "pair_rows" )

@@ -81,7 +81,7 @@ __device__ __forceinline__ int64_t logicalBlock( int remap )
 
 // Row sequence of a lane taking P consecutive rows at a time.  dist 0: grid-stride over the
 // whole range; 1: each workgroup owns `per` consecutive rows, logical blocks remapped so an
-// XCD owns a contiguous eighth.
+// XCD owns a contiguous eighth; 3: each XCD grid-strides over its own contiguous eighth.
 struct RowSeq
 {
    int64_t start, end, step;
@@ -92,6 +92,19 @@ __device__ __forceinline__ RowSeq rowSequence( int64_t base, int64_t last, int64
    if( dist == 0 )
       return RowSeq{ base + ( (int64_t) blockIdx.x * blockDim.x + threadIdx.x ) * P, last,
                      (int64_t) gridDim.x * blockDim.x * P };
+   if( dist == 3 ) {
+      // workgroups are dispatched to the XCDs round-robin: XCD x (blockIdx % 8) sweeps the
+      // x-th eighth of the rows with a grid-stride loop of its own, so the x entries a row
+      // band gathers stay in that XCD's L2 while the sweep passes
+      const int64_t perXcd = gridDim.x / kXcds;
+      const int64_t pos = blockIdx.x / kXcds;
+      if( pos >= perXcd )
+         return RowSeq{ last, last, 1 };
+      const int64_t chunk = ( ( last - base + kXcds - 1 ) / kXcds + P - 1 ) / P * P;
+      const int64_t r0 = base + (int64_t) ( blockIdx.x % kXcds ) * chunk;
+      return RowSeq{ r0 + ( pos * blockDim.x + threadIdx.x ) * P, min( r0 + chunk, last ),
+                     perXcd * blockDim.x * P };
+   }
    const int64_t r0 = base + logicalBlock( 1 ) * per;
    return RowSeq{ r0 + (int64_t) threadIdx.x * P, min( r0 + per, last ), (int64_t) blockDim.x * P };
 }
@@ -151,7 +164,7 @@ __global__ __launch_bounds__( 256 ) void csrAdaptiveKernel( int32_t nblocks, con
                                                             const int32_t* __restrict__ off,
                                                             const int32_t* __restrict__ col,
                                                             const V* __restrict__ val, const Y* __restrict__ x,
-                                                            Keeper< A, Y > keep )
+                                                            Keeper< A, Y > keep, int remap )
 {
    if( keep.closed() )
       return;
@@ -159,7 +172,9 @@ __global__ __launch_bounds__( 256 ) void csrAdaptiveKernel( int32_t nblocks, con
    __shared__ A lds[ 4 ][ STREAM ];
    __shared__ A longPart[ 4 ];
    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-   const int32_t di = blockIdx.x * 4 + wave;
+   // remap: XCD x (blockIdx % 8) takes the x-th eighth of the descriptors, in order
+   const int64_t lb = remap ? logicalBlock( 1 ) : blockIdx.x;
+   const int64_t di = lb * 4 + wave;
    const bool active = di < nblocks;
    const int4 d = active ? desc[ di ] : make_int4( BLOCK_VECTOR, 0, 0, 0 );
    const int type = d.x & 0xff;
@@ -432,6 +447,8 @@ __global__ __launch_bounds__( 256 ) void chunkedKernel( int32_t nslices, const i
       const int4 s = slices[ sl ];  // size, chunkSize, firstSegment, pointer
       if( s.z >= last || s.z + s.x <= first )
          continue;  // uniform over the workgroup
+      // every row owns exactly one chunk when the slice uses as many chunks as it has rows
+      const bool onePerRow = s.x > 0 && rowToChunk[ s.z + s.x - 1 ] == s.x;
       A p = 0;
       for( int32_t k0 = 0; k0 < s.y; k0 += kChunk ) {
          int32_t c[ kChunk ];
@@ -450,6 +467,12 @@ __global__ __launch_bounds__( 256 ) void chunkedKernel( int32_t nslices, const i
          for( int j = 0; j < kChunk; j++ )
             if( k0 + j < s.y && c[ j ] >= 0 )
                p += toCompute( v[ j ], A() ) * (A) xv[ j ];
+      }
+      if( onePerRow ) {  // uniform: chunk t is row s.z + t, no exchange through LDS
+         const int32_t row = s.z + t;
+         if( t < s.x && row >= first && row < last )
+            keep( row, (A) 0 + p );
+         continue;
       }
       part[ t ] = p;
       __syncthreads();
@@ -489,12 +512,24 @@ __global__ __launch_bounds__( 256 ) void biellpackKernel( int32_t n, int32_t str
    const int lane = threadIdx.x % kWave, half = lane / 32, l = lane % 32;
    const int32_t wave = ( blockIdx.x * blockDim.x + threadIdx.x ) / kWave;
    const int32_t waves = gridDim.x * ( blockDim.x / kWave );
+   // group pointers of the half-wave's strip are fetched one iteration ahead
+   int32_t gp[ kLogStrip + 1 ];
+   auto fetch = [ & ]( int32_t strip, int32_t( &g )[ kLogStrip + 1 ] ) {
+#pragma unroll
+      for( int k = 0; k <= kLogStrip; k++ )
+         g[ k ] = strip < strips ? groupPointers[ (int64_t) strip * kGroupsPerStrip + k ] : 0;
+   };
+   fetch( 2 * wave + half, gp );
    for( int32_t pair = wave; 2 * pair < strips; pair += waves ) {
       const int32_t strip = 2 * pair + half;
       const bool valid = strip < strips;
+      const int64_t row = (int64_t) strip * kStrip + l;
+      const bool mine = valid && row < n;
+      const int rank = mine ? rowPerm[ row ] - strip * kStrip : 0;
+      int32_t gpNext[ kLogStrip + 1 ];
+      fetch( 2 * ( pair + waves ) + half, gpNext );
       A acc = 0;
       if( valid ) {
-         const int32_t* gp = groupPointers + (int64_t) strip * kGroupsPerStrip;
          // group 5 (height 1) is always empty: the widths are taken at ranks 15, 7, 3, 1, 0
 #pragma unroll
          for( int grp = 0; grp < kLogStrip; grp++ ) {
@@ -526,12 +561,12 @@ __global__ __launch_bounds__( 256 ) void biellpackKernel( int32_t n, int32_t str
                acc += p;
          }
       }
-      const int64_t row = (int64_t) strip * kStrip + l;
-      const bool mine = valid && row < n;
-      const int rank = mine ? rowPerm[ row ] - strip * kStrip : 0;
       const A v = __shfl( acc, half * 32 + rank, kWave );
       if( mine && row >= first && row < last )
          keep( row, v );
+#pragma unroll
+      for( int k = 0; k <= kLogStrip; k++ )
+         gp[ k ] = gpNext[ k ];
    }
 }
 
@@ -577,9 +612,12 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
       case TNLHIP_CSR_ADAPTIVE: {
          if( M.format == TNLHIP_CSR_ADAPTIVE && first == 0 && last == M.rows && M.adaptiveBlocks > 0 ) {
             constexpr int STREAM = kAdaptiveStreamBytesPerWave / sizeof( A );
-            const unsigned groups = ( M.adaptiveBlocks + 3 ) / 4;
+            unsigned groups = ( M.adaptiveBlocks + 3 ) / 4;
+            if( M.xcdRemap )
+               groups = ( groups + kXcds - 1 ) / kXcds * kXcds;  // logicalBlock needs a multiple of 8
             csrAdaptiveKernel< STREAM, V, A, Y ><<< groups, 256, 0, st >>>(
-               M.adaptiveBlocks, reinterpret_cast< const int4* >( M.adaptive.ptr ), M.offsets.ptr, col, val, x, keep );
+               M.adaptiveBlocks, reinterpret_cast< const int4* >( M.adaptive.ptr ), M.offsets.ptr, col, val, x, keep,
+               M.xcdRemap ? 1 : 0 );
             break;
          }
          const int G = M.format == TNLHIP_CSR_VECTOR ? 64 : M.lanes;
