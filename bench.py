@@ -33,7 +33,7 @@ MALL_BYTES = 256 << 20
 MEASURED_TRAFFIC = {"CSRAdaptive": 488.8e6, "Ellpack": 484.8e6, "ChunkedEllpack": 507.0e6}
 FORMATS = ["CSRScalar", "CSRVector", "CSRLight", "CSRHybrid", "CSRAdaptive", "Ellpack", "SlicedEllpack",
            "ChunkedEllpack", "BiEllpack"]
-HEADLINE = "CSRAdaptive"
+HEADLINE = "SlicedEllpack"
 
 
 def log(*a):

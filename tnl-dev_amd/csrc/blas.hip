@@ -129,8 +129,9 @@ template< typename T, int C >
 void launchMultidot( int64_t n, int32_t c, const T* V, int64_t ld, const T* w, ReduceSlot slot, double* result,
                      hipStream_t st )
 {
-   multidotKernel< T, C ><<< kReduceGrid, kReduceBlock, 0, st >>>( n, c, V, ld, w, slot );
-   multidotFinalKernel< C ><<< 1, kReduceBlock, 0, st >>>( slot, kReduceGrid, c, result );
+   const int grid = reduceGridFor( C );
+   multidotKernel< T, C ><<< grid, kReduceBlock, 0, st >>>( n, c, V, ld, w, slot );
+   multidotFinalKernel< C ><<< 1, kReduceBlock, 0, st >>>( slot, grid, c, result );
 }
 
 template< typename T >

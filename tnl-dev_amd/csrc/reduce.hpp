@@ -12,11 +12,21 @@
 
 #include "common.hpp"
 
+#include <algorithm>
+
 namespace tnlhip {
 
 constexpr int kReduceBlock = 256;
 constexpr int kReduceGrid = 1024;  // partials per column (4 workgroups per CU on 256 CUs)
 constexpr int kMaxMulti = 32;      // columns per multi-reduction pass
+
+// Workgroups of a pass keeping W running sums per lane.  The 32-column passes need about 140
+// VGPRs (3 resident workgroups per CU), so their grid is one resident round instead of 1024
+// workgroups in a full and a one-third round; the grid (hence the sum order) is fixed per W.
+inline int reduceGridFor( int W )
+{
+   return W >= 32 ? std::min( kReduceGrid, 3 * deviceProps().cus ) : kReduceGrid;
+}
 
 // Scratch of one reduction slot: kReduceGrid partials per column, column-major.
 struct ReduceSlot

@@ -152,8 +152,9 @@ void launchElementwise( int64_t n, const Op& op, Gate gate, hipStream_t st )
 template< int W, typename Op, typename Fin >
 void launchReduce( int64_t n, const Op& op, const Fin& fin, ReduceSlot slot, Gate gate, hipStream_t st )
 {
-   fusedReduceKernel< W ><<< kReduceGrid, kReduceBlock, 0, st >>>( n, op, slot, gate );
-   reduceFinalKernel< W ><<< 1, kReduceBlock, 0, st >>>( fin, slot, kReduceGrid, gate );
+   const int grid = reduceGridFor( W );
+   fusedReduceKernel< W ><<< grid, kReduceBlock, 0, st >>>( n, op, slot, gate );
+   reduceFinalKernel< W ><<< 1, kReduceBlock, 0, st >>>( fin, slot, grid, gate );
    TNLHIP_CHECK( hipGetLastError() );
 }
 
@@ -1465,7 +1466,7 @@ template< typename T, int C >
 void launchCwyZGenerate( int64_t n, T* z, const T* w, const T* Y, int64_t ld, const double* aux, int count, Small q,
                          ReduceSlot slot, Gate gate, hipStream_t st )
 {
-   fusedReduceKernel< C ><<< kReduceGrid, kReduceBlock, 0, st >>>(
+   fusedReduceKernel< C ><<< reduceGridFor( C ), kReduceBlock, 0, st >>>(
       n, OpCwyZGenerate< T, C >{ z, w, Y, ld, aux, count, const_cast< T* >( Y ) + (int64_t) count * ld }, slot, gate );
    TNLHIP_CHECK( hipGetLastError() );
 }
@@ -1473,7 +1474,8 @@ void launchCwyZGenerate( int64_t n, T* z, const T* w, const T* Y, int64_t ld, co
 template< typename T, int C >
 void launchGenerateFinal( Small q, T* yi, const T* Y, int64_t ld, int i, ReduceSlot slot, Gate gate, hipStream_t st )
 {
-   reduceFinalKernel< C ><<< 1, kReduceBlock, 0, st >>>( FinGenerateFused< T, C >{ q, yi, Y, ld, i }, slot, kReduceGrid,
+   reduceFinalKernel< C ><<< 1, kReduceBlock, 0, st >>>( FinGenerateFused< T, C >{ q, yi, Y, ld, i }, slot,
+                                                         reduceGridFor( C ),
                                                          gate );
    TNLHIP_CHECK( hipGetLastError() );
 }
