@@ -30,7 +30,8 @@ HBM_PEAK_GBS = 8000.0
 MALL_BYTES = 256 << 20
 # HBM bytes per launch at 160^3 FP64 from rocprofv3 --pmc FETCH_SIZE (x2, the factor a plain
 # 512 MiB stream reads) and WRITE_SIZE runs: profiles/spmv_laplacian160_traffic_pmc.md
-MEASURED_TRAFFIC = {"CSRAdaptive": 488.8e6, "Ellpack": 484.8e6, "ChunkedEllpack": 507.0e6}
+MEASURED_TRAFFIC = {"SlicedEllpack": 487.3e6, "Ellpack": 484.8e6, "ChunkedEllpack": 490.5e6, "CSRAdaptive": 493.2e6,
+                    "BiEllpack": 508.5e6}
 FORMATS = ["CSRScalar", "CSRVector", "CSRLight", "CSRHybrid", "CSRAdaptive", "Ellpack", "SlicedEllpack",
            "ChunkedEllpack", "BiEllpack"]
 HEADLINE = "SlicedEllpack"
@@ -156,6 +157,27 @@ def solver_suite(tnl, torch, W, dev, args):
         del A, lows
         torch.cuda.empty_cache()
     return res
+
+
+def gmres_step_suite(tnl, torch, W, dev, args):
+    """Per-iteration cost of Jacobi GMRES(30) on the cd_grid^3 convection-diffusion matrix in
+    CSRAdaptive: two solves capped at 60 and 240 iterations, in the --gmres-dtype precision."""
+    pe = float(args.peclet.split(",")[0])
+    rp, c, v = W.convection_diffusion_3d(args.cd_grid, pe)
+    n = len(rp) - 1
+    A = tnl.SparseMatrix.from_csr(rp, c, v, format="CSRAdaptive", dtype=args.gmres_dtype)
+    b = torch.tensor(W.uniform_vector(n, 77), device=dev).to(A.vector_dtype)
+    out = {"workload": f"convection-diffusion {args.cd_grid}^3 Pe={pe:g}, GMRES(30) CWY + Jacobi, {args.gmres_dtype}"}
+    for its in (60, 240):
+        s = tnl.GMRES()
+        s.setMatrix(A)
+        s.setup(convergence_residue=1e-12, max_iterations=its, restarting_min=30, restarting_max=30,
+                gmres_variant=args.gmres_variants.split(",")[0], preconditioner=1)
+        x = torch.zeros(n, dtype=A.vector_dtype, device=dev)
+        _, secs = timed(torch, lambda: s.solve(b, x))
+        out[f"ms_per_iteration_{its}"] = round(1e3 * secs / s.getIterations(), 4)
+        log(f"GMRES {args.gmres_dtype}: {s.getIterations()} iterations, {1e3 * secs / s.getIterations():.3f} ms/it")
+    return out
 
 
 def irregular_suite(tnl, torch, W, dev, args):
@@ -330,6 +352,7 @@ def main():
     ap.add_argument("--peclet", default="1,10,100")
     ap.add_argument("--gmres-variants", default="CWY,CGS")
     ap.add_argument("--inner-tol", type=float, default=1e-4, help="inner relative residual of the refinement")
+    ap.add_argument("--gmres-dtype", default="f64")
     ap.add_argument("--serve-inner-tol", type=float, default=3e-6)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-irregular", action="store_true")
@@ -351,6 +374,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.only == "serving":
         print(json.dumps(serving_suite(tnl, W, args)))
+        return
+    if args.only == "gmres":
+        print(json.dumps(gmres_step_suite(tnl, torch, W, dev, args)))
         return
     if args.only == "irregular":
         print(json.dumps(irregular_suite(tnl, torch, W, dev, args)))
