@@ -102,6 +102,22 @@ int tnlhip_matrix_create_csr( int format, int value_dtype, int32_t rows, int32_t
 /* B = A across formats / value types on the device (SparseMatrix.hpp:1093-1107). */
 int tnlhip_matrix_convert( tnlhip_matrix source, int format, int value_dtype, tnlhip_matrix* out );
 
+/* Matrix Market files (Matrices/MatrixReader.hpp, MatrixWriter.hpp:105-130).
+ * tnlhip_mtx_read_csr: MatrixReader::readMtx into a host CSR matrix -- header "%%MatrixMarket
+ *   matrix coordinate real general|symmetric", row capacities counted from the file (padding
+ *   column -1 where entries repeat), setElement semantics (sorted, last repeated value kept,
+ *   explicit zeros stored).  Pass NULL arrays to query rows / columns / storage first.
+ * tnlhip_mtx_read: the same into a matrix of any format on the device (readMtx for a
+ *   Devices::Hip matrix: read on the host, then copied).
+ * tnlhip_mtx_write: MatrixWriter::writeMtx (entries whose value is not zero, 6 digits). */
+int tnlhip_mtx_read_csr( const char* path, int32_t* rows, int32_t* columns, int64_t* storage, int32_t* row_pointers,
+                         int32_t* column_indexes, double* values );
+int tnlhip_mtx_read( const char* path, int format, int value_dtype, tnlhip_matrix* out );
+int tnlhip_mtx_write( tnlhip_matrix A, const char* path );
+/* writeMtx of a host CSR matrix (padding column -1 allowed). */
+int tnlhip_mtx_write_csr( const char* path, int32_t rows, int32_t columns, const int32_t* row_pointers,
+                          const int32_t* column_indexes, const double* values );
+
 int tnlhip_matrix_destroy( tnlhip_matrix A );
 
 int tnlhip_matrix_get_info( tnlhip_matrix A, tnlhip_matrix_info* info );

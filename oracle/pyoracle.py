@@ -125,8 +125,35 @@ def ref():
         L.ref_solve.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_void_p,
                                 C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_void_p, C.c_int,
                                 C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_double)]
+        L.ref_read_mtx.restype = C.c_int
+        L.ref_read_mtx.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong),
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
+        L.ref_write_mtx.restype = C.c_int
+        L.ref_write_mtx.argtypes = [C.c_char_p, C.c_int, C.c_int, _i32p, _i32p, _f64p]
         _ref = L
     return _ref
+
+
+def ref_read_mtx(path):
+    """TNL's MatrixReader::readMtx into a Host CSR matrix: (rows, cols, row_ptr, cols, vals), or
+    raises ValueError with TNL's message."""
+    L = ref()
+    rows, cols, st = C.c_int(), C.c_int(), C.c_longlong()
+    err = C.create_string_buffer(1024)
+    if L.ref_read_mtx(str(path).encode(), C.byref(rows), C.byref(cols), C.byref(st), None, None, None, err, 1024):
+        raise ValueError(err.value.decode(errors="replace"))
+    rp = np.zeros(rows.value + 1, np.int32)
+    ci = np.zeros(max(st.value, 1), np.int32)
+    v = np.zeros(max(st.value, 1), np.float64)
+    L.ref_read_mtx(str(path).encode(), C.byref(rows), C.byref(cols), C.byref(st), rp.ctypes.data, ci.ctypes.data,
+                   v.ctypes.data, err, 1024)
+    return rows.value, cols.value, rp, ci[:st.value], v[:st.value]
+
+
+def ref_write_mtx(path, rows, cols, row_ptr, col_idx, vals):
+    """TNL's MatrixWriter::writeMtx of a Host CSR matrix."""
+    ref().ref_write_mtx(str(path).encode(), int(rows), int(cols), np.ascontiguousarray(row_ptr, np.int32),
+                        np.ascontiguousarray(col_idx, np.int32), np.ascontiguousarray(vals, np.float64))
 
 
 # ------------------------------------------------------------------ segments

@@ -14,6 +14,8 @@
 #include <TNL/Algorithms/Segments/Kernels/CSRAdaptiveKernel.h>
 #include <TNL/Containers/Vector.h>
 #include <TNL/Matrices/SparseMatrix.h>
+#include <TNL/Matrices/MatrixReader.h>
+#include <TNL/Matrices/MatrixWriter.h>
 #include <TNL/Solvers/Linear/BICGStab.h>
 #include <TNL/Solvers/Linear/CG.h>
 #include <TNL/Solvers/Linear/GMRES.h>
@@ -440,6 +442,52 @@ int ref_solve( int method, int n, const int* rp, const int* cols, const double* 
       *history_len = count;
    std::remove( path );
    return rc;
+}
+
+
+// MatrixReader::readMtx into a Host CSR matrix; returns 0, or -1 with the exception text in err.
+int ref_read_mtx( const char* path, int* rows, int* cols, long long* storage, int* rp, int* ci, double* v, char* err,
+                  int errlen )
+{
+   using M = CsrMatrix;
+   try {
+      M A;
+      TNL::Matrices::MatrixReader< M >::readMtx( std::string( path ), A );
+      *rows = A.getRows();
+      *cols = A.getColumns();
+      const auto& off = A.getSegments().getOffsets();
+      *storage = A.getValues().getSize();
+      if( rp )
+         for( int i = 0; i <= A.getRows(); i++ )
+            rp[ i ] = off.getElement( i );
+      if( ci )
+         for( long long k = 0; k < *storage; k++ ) {
+            ci[ k ] = A.getColumnIndexes().getElement( k );
+            v[ k ] = A.getValues().getElement( k );
+         }
+      return 0;
+   }
+   catch( const std::exception& e ) {
+      std::snprintf( err, errlen, "%s", e.what() );
+      return -1;
+   }
+}
+
+// MatrixWriter::writeMtx of a Host CSR matrix given by its arrays.
+int ref_write_mtx( const char* path, int rows, int cols, const int* rp, const int* ci, const double* v )
+{
+   using M = CsrMatrix;
+   M A( rows, cols );
+   IVec caps( rows );
+   for( int i = 0; i < rows; i++ )
+      caps[ i ] = rp[ i + 1 ] - rp[ i ];
+   A.setRowCapacities( caps );
+   for( int k = 0; k < rp[ rows ]; k++ ) {
+      A.getColumnIndexes()[ k ] = ci[ k ];
+      A.getValues()[ k ] = v[ k ];
+   }
+   TNL::Matrices::MatrixWriter< M >::writeMtx( std::string( path ), A );
+   return 0;
 }
 
 }  // extern "C"

@@ -4,6 +4,8 @@
 // and a CG / GMRES solve through LinearSolver::setMatrix / solve.  Prints one line per check;
 // exits non-zero on the first failure.
 #include <TNL/Matrices/SparseMatrix.h>
+#include <TNL/Matrices/MatrixReader.h>
+#include <TNL/Matrices/MatrixWriter.h>
 #include <TNL/Solvers/Linear/CG.h>
 #include <TNL/Solvers/Linear/GMRES.h>
 
@@ -228,6 +230,38 @@ int main()
       std::printf( "GMRES converged=%d iterations=%d residue=%.3e\n", (int) ok2, gmres.getIterations(),
                    gmres.getResidue() );
       EXPECT_EQ( ok2, true );
+   }
+   // Matrix Market: readMtx into a SlicedEllpack matrix on the device, writeMtx back
+   {
+      using Matrix = Matrices::SparseMatrix< double, Devices::Hip, int, Matrices::GeneralMatrix, Seg::SlicedEllpack >;
+      const char* path = "/tmp/tnlhip_front_test.mtx";
+      {
+         std::FILE* f = std::fopen( path, "w" );
+         std::fprintf( f, "%%%%MatrixMarket matrix coordinate real symmetric\n%% test\n3 3 4\n1 1 2\n2 1 -1\n2 2 2\n"
+                          "3 3 5\n" );
+         std::fclose( f );
+      }
+      Matrix A;
+      Matrices::MatrixReader< Matrix >::readMtx( path, A );
+      EXPECT_EQ( A.getRows(), 3 );
+      EXPECT_EQ( A.getColumns(), 3 );
+      // A = [[2,-1,0],[-1,2,0],[0,0,5]]: A (1,2,3) = (0,3,15)
+      Containers::Vector< double, Devices::Hip, int > xv( std::vector< double >{ 1, 2, 3 } ), yv( 3, 0.0 );
+      A.vectorProduct( xv, yv );
+      const auto yh = yv.toHost();
+      EXPECT_EQ( yh[ 0 ], 0.0 );
+      EXPECT_EQ( yh[ 1 ], 3.0 );
+      EXPECT_EQ( yh[ 2 ], 15.0 );
+      Matrices::MatrixWriter< Matrix >::writeMtx( "/tmp/tnlhip_front_test.out.mtx", A );
+      std::FILE* f = std::fopen( "/tmp/tnlhip_front_test.out.mtx", "r" );
+      int lines = 0;
+      for( int ch; ( ch = std::fgetc( f ) ) != EOF; )
+         lines += ch == '\n';
+      std::fclose( f );
+      EXPECT_EQ( lines, 5 + 5 );  // header block of 5 lines, 5 stored entries
+      std::remove( path );
+      std::remove( "/tmp/tnlhip_front_test.out.mtx" );
+      std::printf( "%s MatrixMarket\n", failures ? "FAIL" : "PASS" );
    }
    std::printf( failures ? "FAILURES %d\n" : "ALL PASSED\n", failures );
    return failures ? 1 : 0;

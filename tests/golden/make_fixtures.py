@@ -51,6 +51,78 @@ def cases():
     return out
 
 
+def mtx_cases():
+    """Matrix Market texts covering what MatrixReader accepts and rejects."""
+    rng = np.random.default_rng(21)
+    cases = {
+        "general": "%%MatrixMarket matrix coordinate real general\n% comment\n%\n4 5 6\n1 1 1.5\n4 5 -2e-3\n"
+                   "2 3 7\n1 4 0.25\n3 2 1e10\n4 1 -0.125\n",
+        "unsorted_duplicates_zeros": "%%MatrixMarket matrix coordinate real general\n3 4 7\n2 3 1\n2 1 2\n2 3 5\n"
+                                     "1 1 0\n3 3 4\n3 3 0\n2 2 -1\n",
+        "err_row_over_capacity": "%%MatrixMarket matrix coordinate real general\n3 3 4\n2 3 1\n2 1 2\n2 3 5\n"
+                                 "2 2 -1\n",
+        "symmetric": "%%MatrixMarket matrix coordinate real symmetric\n% lower triangle\n4 4 6\n1 1 4\n2 1 -1\n"
+                     "2 2 4\n3 2 -1\n4 1 0.5\n4 4 4\n",
+        "coordinates_keyword_blank_lines": "%%MatrixMarket matrix coordinates real general\n2 2 2\n\n1 2 3.0\n\n"
+                                           "2 1 -3.0\n",
+        "extra_spaces": "%%MatrixMarket  matrix coordinate  real general\n  3   2   2\n 1  2   8.5\n3 1    -4\n",
+        "integer_tokens_truncate": "%%MatrixMarket matrix coordinate real general\n3 3 2\n2.7 1.2 5\n3 3 1e2\n",
+        "empty_rows": "%%MatrixMarket matrix coordinate real general\n5 5 2\n2 2 1\n5 1 2\n",
+        "err_header": "%%MatrixMarket matrix array real general\n2 2\n1\n2\n3\n4\n",
+        "err_complex": "%%MatrixMarket matrix coordinate complex general\n1 1 1\n1 1 1 0\n",
+        "err_skew": "%%MatrixMarket matrix coordinate real skew-symmetric\n2 2 1\n2 1 1\n",
+        "err_not_mm": "MatrixMarket matrix coordinate real general\n1 1 1\n1 1 1\n",
+        "err_dims": "%%MatrixMarket matrix coordinate real general\n2 2\n1 1 1\n",
+        "err_tokens": "%%MatrixMarket matrix coordinate real general\n2 2 1\n1 1\n",
+        "err_tabs": "%%MatrixMarket matrix coordinate real general\n2 2 1\n1\t1\t1.0\n",
+        "err_out_of_range": "%%MatrixMarket matrix coordinate real general\n2 2 1\n3 1 1.0\n",
+        "err_zero_dims": "%%MatrixMarket matrix coordinate real general\n0 2 0\n",
+    }
+    # a larger random general matrix and a symmetric Laplacian, written with full precision
+    n = 300
+    lines = []
+    for _ in range(2000):
+        r, c = rng.integers(1, n + 1, 2)
+        lines.append(f"{r} {c} {rng.standard_normal():.17g}")
+    cases["random_300"] = (f"%%MatrixMarket matrix coordinate real general\n{n} {n} {len(lines)}\n" +
+                           "\n".join(lines) + "\n")
+    rp, c, v = W.laplacian_2d(12)
+    low = [(i + 1, c[k] + 1, v[k]) for i in range(len(rp) - 1) for k in range(rp[i], rp[i + 1]) if c[k] <= i]
+    m = len(rp) - 1
+    cases["laplacian_12_symmetric"] = (f"%%MatrixMarket matrix coordinate real symmetric\n{m} {m} {len(low)}\n" +
+                                       "".join(f"{a} {b} {val:.17g}\n" for a, b, val in low))
+    return cases
+
+
+def write_mtx_fixtures():
+    import tempfile
+    out = {}
+    names = sorted(mtx_cases())
+    out["names"] = np.array(names)
+    with tempfile.TemporaryDirectory() as tmp:
+        for name in names:
+            text = mtx_cases()[name]
+            out[f"{name}/text"] = np.array(text)
+            path = os.path.join(tmp, name + ".mtx")
+            with open(path, "w") as f:
+                f.write(text)
+            try:
+                rows, cols, rp, ci, v = O.ref_read_mtx(path)
+            except ValueError as e:
+                out[f"{name}/error"] = np.array(str(e))
+                continue
+            out[f"{name}/dims"] = np.array([rows, cols], np.int32)
+            out[f"{name}/row_ptr"] = rp
+            out[f"{name}/cols"] = ci
+            out[f"{name}/vals"] = v
+            wpath = os.path.join(tmp, name + ".out.mtx")
+            O.ref_write_mtx(wpath, rows, cols, rp, ci, v)
+            with open(wpath) as f:
+                out[f"{name}/written"] = np.array(f.read())
+    np.savez_compressed(os.path.join(HERE, "mtx.npz"), **out)
+    print("mtx.npz", os.path.getsize(os.path.join(HERE, "mtx.npz")))
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("oracle/_ref/libtnlref.so is missing: run make -C oracle in the reference container")
@@ -132,4 +204,10 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "mtx":
+        if not O.ref_available():
+            raise SystemExit("oracle/_ref/libtnlref.so is missing: run make -C oracle in the reference container")
+        write_mtx_fixtures()
+    else:
+        main()
+        write_mtx_fixtures()

@@ -2,6 +2,7 @@
 (golden fixtures written from the reference build) and the oracle; products bit-exact for the
 one-lane-per-row kernels (same summation order, no contraction) and within
 |dy_i| <= 1e-13 * sum_j |a_ij x_j| for the cooperative kernels (FP64)."""
+import os
 import numpy as np
 import pytest
 import torch
@@ -228,3 +229,36 @@ def test_launch_distributions_give_the_same_bits(tnl, dev, fmt, params):
             want = run_spmv(tnl, A, x, dev, y0, alpha, beta, first, last)
             got = run_spmv(tnl, B, x, dev, y0, alpha, beta, first, last)
             np.testing.assert_array_equal(got, want, err_msg=f"{fmt} {params} {alpha} {beta} {first} {last}")
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_matrix_market_read_into_every_format(tnl, dev, fmt, tmp_path):
+    """readMtx into a device matrix of each format: TNL's capacities (metadata of the format for
+    the file's row capacities), explicit zeros stored, y = A x as the dense product, and
+    writeMtx from the device gives TNL's text."""
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mtx.npz"))
+    for name in (str(n) for n in d["names"]):
+        if f"{name}/error" in d.files:
+            continue
+        path = tmp_path / f"{name}.mtx"
+        path.write_text(str(d[f"{name}/text"]))
+        A = tnl.SparseMatrix.from_mtx(path, format=fmt, dtype="f64")
+        rows, cols = (int(x) for x in d[f"{name}/dims"])
+        rp, ci, v = d[f"{name}/row_ptr"], d[f"{name}/cols"], d[f"{name}/vals"]
+        assert (A.info.rows, A.info.columns) == (rows, cols)
+        assert A.info.nonzeros == int(np.sum(ci != -1)), name
+        seg = O.Segments(KIND[fmt], np.diff(rp).astype(np.int32))
+        for nm in O.METADATA_NAMES[seg.kind]:
+            np.testing.assert_array_equal(A.metadata(nm), seg.get(nm), err_msg=f"{name} {nm}")
+        x = np.random.default_rng(1).uniform(-1, 1, cols)
+        dense = np.zeros((rows, cols))
+        for r in range(rows):
+            for k in range(rp[r], rp[r + 1]):
+                if ci[k] >= 0:
+                    dense[r, ci[k]] = v[k]
+        got = run_spmv(tnl, A, x, dev)
+        want = dense @ x
+        assert np.all(np.abs(got - want) <= 1e-13 * (np.abs(dense) @ np.abs(x)) + 1e-300), name
+        out = tmp_path / f"{name}.out.mtx"
+        A.write_mtx(out)
+        assert out.read_text() == str(d[f"{name}/written"]), name

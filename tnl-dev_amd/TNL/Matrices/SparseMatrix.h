@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "../Algorithms/Segments/Segments.h"
@@ -130,6 +131,19 @@ public:
                                                &handle ) );
       staged = false;
    }
+
+   // MatrixReader::readMtx / MatrixWriter::writeMtx (Matrices/MatrixReader.hpp, MatrixWriter.hpp)
+   void readMtxFile( const std::string& path )
+   {
+      release();
+      detail::check( tnlhip_mtx_read( path.c_str(), format, valueDtype, &handle ) );
+      tnlhip_matrix_info info;
+      detail::check( tnlhip_matrix_get_info( handle, &info ) );
+      rows = info.rows;
+      columns = info.columns;
+      staged = false;
+   }
+   void writeMtxFile( const std::string& path ) const { detail::check( tnlhip_mtx_write( getHandle(), path.c_str() ) ); }
 
    // B = A across formats and value types, on the device (SparseMatrix.hpp:1093-1107)
    template< typename R2, template< typename, typename, typename > class S2, typename C2 >

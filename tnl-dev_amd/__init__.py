@@ -36,7 +36,8 @@ EXPORTS = [
     "tnlhip_refine_solve", "tnlhip_server_config_default", "tnlhip_request_params_default",
     "tnlhip_server_footprint", "tnlhip_server_create", "tnlhip_server_destroy", "tnlhip_server_add_matrix",
     "tnlhip_server_remove_matrix", "tnlhip_server_submit", "tnlhip_server_progress", "tnlhip_server_result",
-    "tnlhip_server_stats", "tnlhip_autotune_csr",
+    "tnlhip_server_stats", "tnlhip_autotune_csr", "tnlhip_mtx_read_csr", "tnlhip_mtx_read", "tnlhip_mtx_write",
+    "tnlhip_mtx_write_csr",
 ]
 
 
@@ -155,6 +156,10 @@ def _load():
         "tnlhip_server_progress": (C.c_int, [vp, vp, i32, C.POINTER(i32)]),
         "tnlhip_server_result": (C.c_int, [vp, i64, C.POINTER(RequestResult)]),
         "tnlhip_server_stats": (C.c_int, [vp, C.POINTER(ServerStats)]),
+        "tnlhip_mtx_read_csr": (C.c_int, [C.c_char_p, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64), vp, vp, vp]),
+        "tnlhip_mtx_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.POINTER(vp)]),
+        "tnlhip_mtx_write": (C.c_int, [vp, C.c_char_p]),
+        "tnlhip_mtx_write_csr": (C.c_int, [C.c_char_p, i32, i32, vp, vp, vp]),
         "tnlhip_autotune_csr": (C.c_int, [C.c_int, i32, i32, vp, vp, vp, i32, C.c_uint32, i32, C.POINTER(Tuning),
                                           C.POINTER(Tuning), i32, C.POINTER(i32), C.POINTER(vp)]),
     }
@@ -236,6 +241,18 @@ class SparseMatrix:
             _check(lib.tnlhip_matrix_create_csr(fmt, dt, rows, cols, _ptr(row_pointers), _ptr(column_indexes),
                                                 _ptr(values), 0, C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def from_mtx(cls, path, format="CSRScalar", dtype="f64"):
+        """MatrixReader< SparseMatrix< Real, Devices::Hip, ... > >::readMtx( path, A )."""
+        fmt = FORMATS[format] if isinstance(format, str) else format
+        h = C.c_void_p(0)
+        _check(lib.tnlhip_mtx_read(str(path).encode(), fmt, _dtype_id(dtype), C.byref(h)))
+        return cls(h)
+
+    def write_mtx(self, path):
+        """MatrixWriter::writeMtx( path, A )."""
+        _check(lib.tnlhip_mtx_write(self._h, str(path).encode()))
 
     def __del__(self):
         try:
@@ -627,6 +644,29 @@ def autotune(row_pointers, column_indexes, values, dtype="f64", formats=None, re
                 "pair_rows": t.pair_rows, "seconds": t.seconds}
 
     return SparseMatrix(h), as_dict(best), [as_dict(allt[i]) for i in range(min(cnt.value, cap))]
+
+
+def mtx_read_csr(path):
+    """MatrixReader::readMtx into a host CSR matrix (Matrices/MatrixReader.hpp): returns
+    (rows, columns, row_pointers, column_indexes, values) with TNL's capacities and padding."""
+    rows, cols, st = C.c_int32(), C.c_int32(), C.c_int64()
+    p = str(path).encode()
+    _check(lib.tnlhip_mtx_read_csr(p, C.byref(rows), C.byref(cols), C.byref(st), None, None, None))
+    rp = np.zeros(rows.value + 1, np.int32)
+    ci = np.zeros(max(st.value, 1), np.int32)
+    v = np.zeros(max(st.value, 1), np.float64)
+    _check(lib.tnlhip_mtx_read_csr(p, C.byref(rows), C.byref(cols), C.byref(st), rp.ctypes.data, ci.ctypes.data,
+                                   v.ctypes.data))
+    return rows.value, cols.value, rp, ci[:st.value], v[:st.value]
+
+
+def mtx_write_csr(path, rows, columns, row_pointers, column_indexes, values):
+    """MatrixWriter::writeMtx of a host CSR matrix (Matrices/MatrixWriter.hpp:105-130)."""
+    rp = np.ascontiguousarray(row_pointers, np.int32)
+    ci = np.ascontiguousarray(column_indexes, np.int32)
+    v = np.ascontiguousarray(values, np.float64)
+    _check(lib.tnlhip_mtx_write_csr(str(path).encode(), int(rows), int(columns), rp.ctypes.data, ci.ctypes.data,
+                                    v.ctypes.data))
 
 
 def server_footprint(rows, nnz, max_row, nrhs):

@@ -308,6 +308,80 @@ int tnlhip_matrix_convert( tnlhip_matrix source, int format, int value_dtype, tn
    } );
 }
 
+int tnlhip_mtx_read_csr( const char* path, int32_t* rows, int32_t* columns, int64_t* storage, int32_t* row_pointers,
+                         int32_t* column_indexes, double* values )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( path, "NULL path" );
+      const MtxCsr m = readMtxCsr( path );
+      if( rows )
+         *rows = m.rows;
+      if( columns )
+         *columns = m.columns;
+      if( storage )
+         *storage = (int64_t) m.columnIndexes.size();
+      if( row_pointers )
+         std::memcpy( row_pointers, m.rowPointers.data(), sizeof( int32_t ) * m.rowPointers.size() );
+      if( column_indexes )
+         std::memcpy( column_indexes, m.columnIndexes.data(), sizeof( int32_t ) * m.columnIndexes.size() );
+      if( values )
+         std::memcpy( values, m.values.data(), sizeof( double ) * m.values.size() );
+   } );
+}
+
+int tnlhip_mtx_read( const char* path, int format, int value_dtype, tnlhip_matrix* out )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( path && out, "NULL argument" );
+      const MtxCsr m = readMtxCsr( path );
+      std::vector< int32_t > caps( m.rows );
+      for( int32_t i = 0; i < m.rows; i++ )
+         caps[ i ] = m.rowPointers[ i + 1 ] - m.rowPointers[ i ];
+      std::unique_ptr< tnlhip_matrix_s > M( buildEmpty( format, value_dtype, m.rows, m.columns, caps ) );
+      DeviceArray< int32_t > drp, dcol;
+      DeviceArray< double > dval;
+      drp.upload( m.rowPointers );
+      dcol.upload( m.columnIndexes );
+      dval.upload( m.values );
+      // setElement keeps explicit zeros (SparseMatrixView.hpp:270-345)
+      convertFromCsrDevice( *M, drp.ptr, dcol.ptr, dval.ptr, nullptr, true );
+      M->nnz = countStored( *M, nullptr );
+      *out = M.release();
+   } );
+}
+
+int tnlhip_mtx_write_csr( const char* path, int32_t rows, int32_t columns, const int32_t* row_pointers,
+                          const int32_t* column_indexes, const double* values )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( path && rows >= 0 && columns >= 0 && ( rows == 0 || row_pointers ), "bad argument" );
+      std::vector< int32_t > rp( row_pointers, row_pointers + rows + 1 );
+      const int64_t nnz = rows > 0 ? rp[ rows ] : 0;
+      std::vector< int32_t > ci( column_indexes, column_indexes + nnz );
+      std::vector< double > v( values, values + nnz );
+      writeMtx( path, rows, columns, rp, ci, v );
+   } );
+}
+
+int tnlhip_mtx_write( tnlhip_matrix A, const char* path )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( A && path, "NULL argument" );
+      const Matrix& S = *A;
+      DeviceArray< int32_t > dcaps( S.rows );
+      capacitiesOf( S, dcaps.ptr, nullptr );
+      std::vector< int32_t > caps = dcaps.download();
+      std::vector< int32_t > rp( (size_t) S.rows + 1, 0 );
+      for( int32_t i = 0; i < S.rows; i++ )
+         rp[ i + 1 ] = rp[ i ] + caps[ i ];
+      DeviceArray< int32_t > drp, dcol( rp[ S.rows ] );
+      DeviceArray< double > dval( rp[ S.rows ] );
+      drp.upload( rp );
+      gatherToCsr( S, drp.ptr, dcol.ptr, dval.ptr, nullptr, true );  // stored zeros count in the header
+      writeMtx( path, S.rows, S.cols, rp, dcol.download(), dval.download() );
+   } );
+}
+
 int tnlhip_matrix_destroy( tnlhip_matrix A )
 {
    return guarded( [ & ] { delete A; } );

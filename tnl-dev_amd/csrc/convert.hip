@@ -57,7 +57,7 @@ __global__ void fillPaddingKernel( int64_t n, int32_t* cols, V* vals )
 template< typename View, typename V >
 __global__ void scatterFromCsrKernel( int32_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ scol,
                                       const double* __restrict__ sval, View dst, int32_t* __restrict__ dcol,
-                                      V* __restrict__ dval, int32_t* __restrict__ overflow )
+                                      V* __restrict__ dval, int32_t* __restrict__ overflow, int keepZeros )
 {
    const int64_t stride = (int64_t) gridDim.x * blockDim.x;
    for( int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride ) {
@@ -66,7 +66,7 @@ __global__ void scatterFromCsrKernel( int32_t n, const int32_t* __restrict__ rp,
       for( int32_t k = rp[ r ]; k < rp[ r + 1 ]; k++ ) {
          const double v = sval[ k ];
          const int32_t c = scol[ k ];
-         if( v != 0.0 && c != -1 ) {
+         if( ( keepZeros || v != 0.0 ) && c != -1 ) {
             if( local >= cap ) {
                *overflow = 1;
                break;
@@ -83,7 +83,7 @@ __global__ void scatterFromCsrKernel( int32_t n, const int32_t* __restrict__ rp,
 template< typename View, typename V >
 __global__ void gatherToCsrKernel( int32_t n, View src, const int32_t* __restrict__ scol, const V* __restrict__ sval,
                                    const int32_t* __restrict__ rp, int32_t* __restrict__ ocol,
-                                   double* __restrict__ oval )
+                                   double* __restrict__ oval, int keepZeros )
 {
    const int64_t stride = (int64_t) gridDim.x * blockDim.x;
    for( int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride ) {
@@ -93,7 +93,7 @@ __global__ void gatherToCsrKernel( int32_t n, View src, const int32_t* __restric
          const int64_t g = src.global( (int32_t) r, l );
          const int32_t c = scol[ g ];
          const double v = toDouble( sval[ g ] );
-         if( c != -1 && v != 0.0 ) {
+         if( c != -1 && ( keepZeros || v != 0.0 ) ) {
             ocol[ o ] = c;
             oval[ o ] = v;
             o++;
@@ -136,7 +136,7 @@ __global__ void diagonalKernel( int32_t n, View v, const int32_t* __restrict__ c
 }
 
 void convertFromCsrDevice( Matrix& A, const int32_t* rowPtr, const int32_t* cols, const double* vals,
-                           hipStream_t stream )
+                           hipStream_t stream, bool keepZeros )
 {
    DeviceArray< int32_t > overflow( 1 );
    TNLHIP_CHECK( hipMemsetAsync( overflow.ptr, 0, sizeof( int32_t ), stream ) );
@@ -148,8 +148,8 @@ void convertFromCsrDevice( Matrix& A, const int32_t* rowPtr, const int32_t* cols
                                                                                           A.columns.ptr, dval );
       if( A.rows > 0 )
          withView( A, [ & ]( auto view ) {
-            scatterFromCsrKernel<<< gridFor( A.rows, 256 ), 256, 0, stream >>>( A.rows, rowPtr, cols, vals, view,
-                                                                              A.columns.ptr, dval, overflow.ptr );
+            scatterFromCsrKernel<<< gridFor( A.rows, 256 ), 256, 0, stream >>>(
+               A.rows, rowPtr, cols, vals, view, A.columns.ptr, dval, overflow.ptr, keepZeros ? 1 : 0 );
          } );
    } );
    TNLHIP_CHECK( hipGetLastError() );
@@ -171,7 +171,7 @@ void capacitiesOf( const Matrix& A, int32_t* capsDevice, hipStream_t stream )
 }
 
 void gatherToCsr( const Matrix& A, const int32_t* rpDevice, int32_t* colsDevice, double* valsDevice,
-                  hipStream_t stream )
+                  hipStream_t stream, bool keepZeros )
 {
    if( A.rows == 0 )
       return;
@@ -180,7 +180,8 @@ void gatherToCsr( const Matrix& A, const int32_t* rpDevice, int32_t* colsDevice,
       const V* sval = reinterpret_cast< const V* >( A.values.ptr );
       withView( A, [ & ]( auto view ) {
          gatherToCsrKernel<<< gridFor( A.rows, 256 ), 256, 0, stream >>>( A.rows, view, A.columns.ptr, sval,
-                                                                           rpDevice, colsDevice, valsDevice );
+                                                                           rpDevice, colsDevice, valsDevice,
+                                                                           keepZeros ? 1 : 0 );
       } );
    } );
    TNLHIP_CHECK( hipGetLastError() );

@@ -106,10 +106,24 @@ void spmvLaunch( const Matrix& A, const void* x, void* y, double alpha, double b
 void spmvLaunchGated( const Matrix& A, const void* x, void* y, double alpha, double beta, const void* diag,
                       const int32_t* gate, int gateMode, hipStream_t stream, int32_t first = 0, int32_t last = -1 );
 void convertFromCsrDevice( Matrix& A, const int32_t* rowPtr, const int32_t* cols, const double* vals,
-                           hipStream_t stream );
+                           hipStream_t stream,
+                           bool keepZeros = false );
 void gatherToCsr( const Matrix& A, const int32_t* rpDevice, int32_t* colsDevice, double* valsDevice,
-                  hipStream_t stream );
+                  hipStream_t stream,
+                  bool keepZeros = false );
 void extractDiagonal( const Matrix& A, void* diag, hipStream_t stream );
 void capacitiesOf( const Matrix& A, int32_t* capsDevice, hipStream_t stream );
+
+// Matrix Market files with TNL's MatrixReader / MatrixWriter semantics (mtx.cpp).  The CSR
+// keeps TNL's reading result: row capacities from the file, padding -1, explicit zeros kept.
+struct MtxCsr
+{
+   int32_t rows = 0, columns = 0;
+   std::vector< int32_t > rowPointers, columnIndexes;
+   std::vector< double > values;
+};
+MtxCsr readMtxCsr( const std::string& path );
+void writeMtx( const std::string& path, int32_t rows, int32_t columns, const std::vector< int32_t >& rowPointers,
+               const std::vector< int32_t >& columnIndexes, const std::vector< double >& values );
 
 }  // namespace tnlhip
