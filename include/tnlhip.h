@@ -193,7 +193,8 @@ typedef enum
 {
    TNLHIP_CG = 0,       /* Solvers/Linear/CG.hpp */
    TNLHIP_BICGSTAB = 1, /* Solvers/Linear/BICGStab.hpp */
-   TNLHIP_GMRES = 2     /* Solvers/Linear/GMRES.hpp */
+   TNLHIP_GMRES = 2,    /* Solvers/Linear/GMRES.hpp */
+   TNLHIP_TFQMR = 3     /* Solvers/Linear/TFQMR.hpp */
 } tnlhip_method;
 
 typedef enum

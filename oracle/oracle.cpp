@@ -23,6 +23,7 @@
  *   stopping rules           src/TNL/Solvers/IterativeSolver.hpp:103-150
  *   CG                       src/TNL/Solvers/Linear/CG.hpp:17-107
  *   BiCGStab                 src/TNL/Solvers/Linear/BICGStab.hpp:39-147
+ *   TFQMR                    src/TNL/Solvers/Linear/TFQMR.hpp:17-125
  *   GMRES                    src/TNL/Solvers/Linear/GMRES.hpp:79-801
  *   Jacobi                   src/TNL/Solvers/Linear/Preconditioners/Diagonal.hpp:22-47
  */
@@ -1084,6 +1085,90 @@ bool solveBiCGStab( const Csr& A, const Jacobi& M, const Vec& b, Vec& x, Iter& i
    return it.checkConvergence();
 }
 
+bool solveTFQMR( const Csr& A, const Jacobi& M, const Vec& b, Vec& x, Iter& it )
+{
+   // TFQMR.hpp:17-125: odd iterations take alpha = rho / (v, r_ast), even ones recompute
+   // A u; the residue is the quasi-residual bound tau sqrt(iter + 1) / |b|
+   const I n = A.n;
+   Vec r( n ), w( n ), u( n ), v( n ), d( n ), r_ast( n ), Au( n ), Mt( n );
+   auto matvec = [ & ]( const Vec& src, Vec& dst )
+   {
+      if( M.on ) {
+         A.spmv( src, Mt );
+         M.solve( Mt, dst );
+      }
+      else
+         A.spmv( src, dst );
+   };
+   double tau, theta, eta, rho, alpha, b_norm, w_norm;
+   if( M.on ) {
+      M.solve( b, Mt );
+      b_norm = norm2( Mt );
+      A.spmv( x, Mt );
+      for( I i = 0; i < n; i++ )
+         Mt[ i ] = b[ i ] - Mt[ i ];
+      M.solve( Mt, r );
+   }
+   else {
+      b_norm = norm2( b );
+      A.spmv( x, r );
+      for( I i = 0; i < n; i++ )
+         r[ i ] = b[ i ] - r[ i ];
+   }
+   w = r;
+   u = r;
+   matvec( u, Au );
+   v = Au;
+   std::fill( d.begin(), d.end(), 0.0 );
+   tau = norm2( r );
+   theta = eta = 0.0;
+   r_ast = r;
+   rho = dot( r_ast, r );
+   alpha = 0.0;
+   if( b_norm == 0.0 )
+      b_norm = 1.0;
+   it.it = 0;
+   it.setResidue( tau / b_norm );
+   while( it.next() ) {
+      const I iter = it.it;
+      if( iter % 2 == 1 )
+         alpha = rho / dot( v, r_ast );
+      else
+         matvec( u, Au );
+      for( I i = 0; i < n; i++ )
+         w[ i ] -= alpha * Au[ i ];
+      const double dc = theta * theta * eta / alpha;
+      for( I i = 0; i < n; i++ )
+         d[ i ] = u[ i ] + dc * d[ i ];
+      w_norm = norm2( w );
+      theta = w_norm / tau;
+      const double c = 1.0 / std::sqrt( 1.0 + theta * theta );
+      tau = tau * theta * c;
+      eta = c * c * alpha;
+      for( I i = 0; i < n; i++ )
+         x[ i ] += eta * d[ i ];
+      it.setResidue( tau * std::sqrt( iter + 1 ) / b_norm );
+      if( iter > it.minIt && it.res < it.conv )
+         break;
+      if( iter % 2 == 0 ) {
+         const double rho_new = dot( w, r_ast );
+         const double beta = rho_new / rho;
+         rho = rho_new;
+         for( I i = 0; i < n; i++ )
+            u[ i ] = w[ i ] + beta * u[ i ];
+         for( I i = 0; i < n; i++ )
+            v[ i ] = beta * Au[ i ] + ( beta * beta ) * v[ i ];
+         matvec( u, Au );
+         for( I i = 0; i < n; i++ )
+            v[ i ] += Au[ i ];
+      }
+      else
+         for( I i = 0; i < n; i++ )
+            u[ i ] -= alpha * v[ i ];
+   }
+   return it.checkConvergence();
+}
+
 struct Gmres
 {
    // GMRES.hpp:79-801, Devices::Host instantiation (localOffset = 0, no MPI)
@@ -1503,6 +1588,8 @@ orc_solve( int32_t method, int32_t n, const int32_t* row_ptr, const int32_t* col
       Gmres g( A, M, it, *p );
       rc = g.solve( bb, xx );
    }
+   else if( method == 3 )
+      rc = solveTFQMR( A, M, bb, xx, it ) ? 1 : 0;
    std::copy( xx.begin(), xx.end(), x );
    *iterations = it.it;
    *residue = it.res;

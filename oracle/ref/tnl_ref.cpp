@@ -19,6 +19,7 @@
 #include <TNL/Solvers/Linear/BICGStab.h>
 #include <TNL/Solvers/Linear/CG.h>
 #include <TNL/Solvers/Linear/GMRES.h>
+#include <TNL/Solvers/Linear/TFQMR.h>
 #include <TNL/Solvers/Linear/Preconditioners/Diagonal.h>
 
 #include <algorithm>
@@ -412,6 +413,10 @@ int ref_solve( int method, int n, const int* rp, const int* cols, const double* 
       Config::ParameterContainer prm;
       prm.addParameter< bool >( "bicgstab-exact-residue", p->bicgstab_exact_residue != 0 );
       s.setup( prm );
+      run( s );
+   }
+   else if( method == 3 ) {
+      WithHistory< Solvers::Linear::TFQMR< CsrMatrix > > s;
       run( s );
    }
    else {

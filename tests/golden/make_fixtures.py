@@ -123,9 +123,11 @@ def write_mtx_fixtures():
     print("mtx.npz", os.path.getsize(os.path.join(HERE, "mtx.npz")))
 
 
-def main():
+def main(only_solvers=False):
     if not O.ref_available():
         raise SystemExit("oracle/_ref/libtnlref.so is missing: run make -C oracle in the reference container")
+    if only_solvers:
+        return write_solver_fixtures()
     seg = {}
     for name, (rp, c, v) in cases().items():
         n = len(rp) - 1
@@ -162,7 +164,10 @@ def main():
         for vs in (4, 8):
             ad[f"{name}/{vs}"] = O.ref_csr_adaptive_blocks(rp, vs)
     np.savez_compressed(os.path.join(HERE, "adaptive.npz"), **ad)
+    write_solver_fixtures()
 
+
+def write_solver_fixtures():
     sol = {}
     systems = {
         "lap2d_24": W.laplacian_2d(24),
@@ -179,6 +184,9 @@ def main():
     runs += [("convdiff_8_pe10", "gmres", {"gmres_variant": var, "preconditioner": 1, "restarting_min": 30,
                                            "restarting_max": 30, "convergence_residue": 1e-10})
              for var in (3, 4)]
+    runs += [("lap2d_24", "tfqmr", {}), ("lap2d_24", "tfqmr", {"preconditioner": 1}),
+             ("convdiff_8_pe10", "tfqmr", {"preconditioner": 1, "convergence_residue": 1e-10}),
+             ("irr_2000", "tfqmr", {"preconditioner": 1, "convergence_residue": 1e-10, "min_iterations": 3})]
     for name, (rp, c, v) in systems.items():
         sol[f"{name}/row_ptr"] = rp
         sol[f"{name}/cols"] = c
@@ -199,12 +207,13 @@ def main():
         sol[f"{key}/converged"] = np.array([r["converged"]])
         sol[f"{key}/x"] = r["x"]
     np.savez_compressed(os.path.join(HERE, "solvers.npz"), **sol)
-    for f in ("segments.npz", "adaptive.npz", "solvers.npz"):
-        print(f, os.path.getsize(os.path.join(HERE, f)))
+    print("solvers.npz", os.path.getsize(os.path.join(HERE, "solvers.npz")))
 
 
 if __name__ == "__main__":
-    if len(sys.argv) > 1 and sys.argv[1] == "mtx":
+    if len(sys.argv) > 1 and sys.argv[1] == "solvers":
+        main(only_solvers=True)
+    elif len(sys.argv) > 1 and sys.argv[1] == "mtx":
         if not O.ref_available():
             raise SystemExit("oracle/_ref/libtnlref.so is missing: run make -C oracle in the reference container")
         write_mtx_fixtures()

@@ -158,11 +158,11 @@ def test_float32_solve(tnl, dev):
     assert ok and abs(its - ref["iterations"]) <= 3
 
 
-@pytest.mark.parametrize("method,precond", [("cg", 0), ("cg", 1), ("bicgstab", 1)])
+@pytest.mark.parametrize("method,precond", [("cg", 0), ("cg", 1), ("bicgstab", 1), ("tfqmr", 0), ("tfqmr", 1)])
 def test_graph_replay_matches_direct_launches(tnl, dev, method, precond):
     """Iterations replayed from a captured hipGraph give the same bits as the same kernels
     launched one by one (long enough to go past the eager batches)."""
-    rp, c, v = W.laplacian_3d(24) if method == "cg" else W.convection_diffusion_3d(20, 10.0)
+    rp, c, v = W.laplacian_3d(24) if method in ("cg", "tfqmr") else W.convection_diffusion_3d(20, 10.0)
     b = W.uniform_vector(len(rp) - 1, 4)
     runs = []
     for g in (0, 1):

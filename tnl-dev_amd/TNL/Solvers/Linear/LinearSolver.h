@@ -109,6 +109,11 @@ template< typename Matrix >
 class CG : public LinearSolverBase< Matrix, TNLHIP_CG >
 {};
 
+// TFQMR.hpp:17-125
+template< typename Matrix >
+class TFQMR : public LinearSolverBase< Matrix, TNLHIP_TFQMR >
+{};
+
 template< typename Matrix >
 class BICGStab : public LinearSolverBase< Matrix, TNLHIP_BICGSTAB >
 {

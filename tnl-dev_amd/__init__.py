@@ -21,7 +21,7 @@ FORMATS = {
 }
 FORMAT_NAMES = {v: k for k, v in FORMATS.items()}
 DTYPES = {"f64": 0, "f32": 1, "bf16": 2}
-METHODS = {"cg": 0, "bicgstab": 1, "gmres": 2}
+METHODS = {"cg": 0, "bicgstab": 1, "gmres": 2, "tfqmr": 3}
 GMRES_VARIANTS = {"CGS": 0, "CGSR": 1, "MGS": 2, "MGSR": 3, "CWY": 4}
 
 # every function declared in include/tnlhip.h
@@ -503,6 +503,10 @@ class GMRES(LinearSolver):
     method = "gmres"
 
 
+class TFQMR(LinearSolver):
+    method = "tfqmr"
+
+
 def refine_solve(A, b, x, A_low=None, tolerance=1e-10, inner_method="gmres", inner_dtype="f32",
                  max_outer_iterations=50, stagnation_limit=2, stream=None, **inner):
     """FP64 iterative refinement around a low-precision inner solve (no TNL counterpart).
@@ -677,4 +681,4 @@ def server_footprint(rows, nnz, max_row, nrhs):
 
 def getLinearSolver(name):
     """LinearSolverTypeResolver::getLinearSolver by name (Solvers/LinearSolverTypeResolver.h:36-150)."""
-    return {"cg": CG, "bicgstab": BICGStab, "gmres": GMRES}[name]()
+    return {"cg": CG, "bicgstab": BICGStab, "gmres": GMRES, "tfqmr": TFQMR}[name]()

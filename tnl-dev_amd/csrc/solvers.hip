@@ -863,6 +863,216 @@ void solveBiCGStab( SolverImpl& S, const void* b, void* xv )
    } );
 }
 
+// ================================================================== TFQMR
+
+// TFQMR.hpp:17-125.  State: tau -> s1, theta -> s2, eta -> omega, rho, alpha, beta; `stop`
+// marks the in-loop convergence break (or a failed next-iteration check), which still lets
+// x += eta d run before the gate closes.
+
+template< typename T >
+struct OpTfqmrInit : NoScalars
+{
+   // w = u = r_ast = r; d = 0; acc = (r, r)
+   T* w;
+   T* u;
+   T* rast;
+   T* d;
+   const T* r;
+   __device__ void operator()( int64_t i, double ( &acc )[ 1 ] ) const
+   {
+      const T ri = r[ i ];
+      w[ i ] = ri;
+      u[ i ] = ri;
+      rast[ i ] = ri;
+      d[ i ] = 0;
+      acc[ 0 ] += (double) ri * (double) ri;
+   }
+};
+
+struct FinTfqmrInit
+{
+   // tau = |r|, rho = (r_ast, r), theta = eta = alpha = 0; first nextIteration
+   State* st;
+   double* hist;
+   int64_t cap;
+   Stop stop;
+   __device__ void operator()( const double* v ) const
+   {
+      State& s = *st;
+      s.s1 = sqrt( v[ 0 ] );
+      s.rho = v[ 0 ];
+      s.s2 = s.omega = s.alpha = 0.0;
+      s.stop = 0;
+      setResidue( s, s.s1 / s.normb, hist, cap );
+      if( ! checkNext( s, stop ) )
+         s.flags[ 0 ] = 1;
+      s.it++;
+   }
+};
+
+struct FinTfqmrAlpha
+{
+   State* st;
+   __device__ void operator()( const double* v ) const { st->alpha = st->rho / v[ 0 ]; }
+};
+
+template< typename T >
+struct OpTfqmrW
+{
+   // w -= alpha Au; d = u + (theta^2 eta / alpha) d; acc = |w|^2
+   T* w;
+   T* d;
+   const T* u;
+   const T* Au;
+   const State* st;
+   double alpha, dc;
+   __device__ void load()
+   {
+      alpha = st->alpha;
+      dc = st->s2 * st->s2 * st->omega / alpha;
+   }
+   __device__ void operator()( int64_t i, double ( &acc )[ 1 ] ) const
+   {
+      const T wi = (T) ( (double) w[ i ] - alpha * (double) Au[ i ] );
+      w[ i ] = wi;
+      d[ i ] = (T) ( (double) u[ i ] + dc * (double) d[ i ] );
+      acc[ 0 ] += (double) wi * (double) wi;
+   }
+};
+
+struct FinTfqmrTheta
+{
+   // theta, c, tau, eta; residue tau sqrt(iter + 1) / |b|; the in-loop break, else the next
+   // iteration's check
+   State* st;
+   double* hist;
+   int64_t cap;
+   Stop stop;
+   __device__ void operator()( const double* v ) const
+   {
+      State& s = *st;
+      const double w_norm = sqrt( v[ 0 ] );
+      s.s2 = w_norm / s.s1;
+      const double c = 1.0 / sqrt( 1.0 + s.s2 * s.s2 );
+      s.s1 = s.s1 * s.s2 * c;
+      s.omega = c * c * s.alpha;
+      setResidue( s, s.s1 * sqrt( (double) ( s.it + 1 ) ) / s.normb, hist, cap );
+      if( s.it > stop.minIt && s.residue < stop.conv )
+         s.stop = 1;
+      else {
+         if( ! checkNext( s, stop ) )
+            s.stop = 1;
+         s.it++;
+      }
+   }
+};
+
+template< typename T >
+struct OpTfqmrX
+{
+   // x += eta d
+   T* x;
+   const T* d;
+   const State* st;
+   double eta;
+   __device__ void load() { eta = st->omega; }
+   __device__ void operator()( int64_t i ) const { x[ i ] = (T) ( (double) x[ i ] + eta * (double) d[ i ] ); }
+};
+
+template< typename T >
+struct OpTfqmrUOdd
+{
+   // u -= alpha v
+   T* u;
+   const T* v;
+   const State* st;
+   double alpha;
+   __device__ void load() { alpha = st->alpha; }
+   __device__ void operator()( int64_t i ) const { u[ i ] = (T) ( (double) u[ i ] - alpha * (double) v[ i ] ); }
+};
+
+struct FinTfqmrBeta
+{
+   State* st;
+   __device__ void operator()( const double* v ) const
+   {
+      st->beta = v[ 0 ] / st->rho;
+      st->rho = v[ 0 ];
+   }
+};
+
+template< typename T >
+struct OpTfqmrUV
+{
+   // u = w + beta u; v = beta Au + beta^2 v
+   T* u;
+   T* v;
+   const T* w;
+   const T* Au;
+   const State* st;
+   double beta;
+   __device__ void load() { beta = st->beta; }
+   __device__ void operator()( int64_t i ) const
+   {
+      u[ i ] = (T) ( (double) w[ i ] + beta * (double) u[ i ] );
+      v[ i ] = (T) ( beta * (double) Au[ i ] + ( beta * beta ) * (double) v[ i ] );
+   }
+};
+
+template< typename T >
+struct OpAddTo : NoScalars
+{
+   // y += x
+   T* y;
+   const T* x;
+   __device__ void operator()( int64_t i ) const { y[ i ] = (T) ( (double) y[ i ] + (double) x[ i ] ); }
+};
+
+template< typename T >
+void solveTFQMR( SolverImpl& S, const void* b, void* xv )
+{
+   // vectors: r, w, u, v, d, r_ast, Au, M_tmp (TFQMR.hpp:128-139)
+   T* r = (T*) S.vec( 0 );
+   T* w = (T*) S.vec( 1 );
+   T* u = (T*) S.vec( 2 );
+   T* v = (T*) S.vec( 3 );
+   T* d = (T*) S.vec( 4 );
+   T* rast = (T*) S.vec( 5 );
+   T* Au = (T*) S.vec( 6 );
+   T* Mt = (T*) S.vec( 7 );
+   T* x = (T*) xv;
+   const Gate always{ nullptr, 0 }, running{ S.state.ptr->flags, 1 };
+   const Stop stop = S.stopRule();
+   State* st = S.state.ptr;
+   computeNormb< T >( S, b, Mt );
+   residual< T >( S, b, x, r, always );
+   launchReduce< 1 >( S.n, OpTfqmrInit< T >{ {}, w, u, rast, d, r }, FinTfqmrInit{ st, S.hist.ptr, S.histCap, stop },
+                      S.ws.slot( 0 ), always, S.stream );
+   matvec( S, u, Au, always );
+   launchElementwise( S.n, OpCopy< T >{ {}, v, Au }, always, S.stream );
+   int64_t iter = 1;  // the iteration the next queued body belongs to (its parity is fixed)
+   runChunks( S, [ & ] {
+      const bool odd = iter % 2 == 1;
+      iter++;
+      if( odd )
+         launchReduce< 1 >( S.n, OpDot< T >{ {}, v, rast }, FinTfqmrAlpha{ st }, S.ws.slot( 0 ), running, S.stream );
+      else
+         matvec( S, u, Au, running );
+      launchReduce< 1 >( S.n, OpTfqmrW< T >{ w, d, u, Au, st, 0, 0 },
+                         FinTfqmrTheta{ st, S.hist.ptr, S.histCap, stop }, S.ws.slot( 1 ), running, S.stream );
+      launchElementwise( S.n, OpTfqmrX< T >{ x, d, st, 0 }, running, S.stream );
+      launchScalar( FinPromoteStop{ st }, running, S.stream );
+      if( odd )
+         launchElementwise( S.n, OpTfqmrUOdd< T >{ u, v, st, 0 }, running, S.stream );
+      else {
+         launchReduce< 1 >( S.n, OpDot< T >{ {}, w, rast }, FinTfqmrBeta{ st }, S.ws.slot( 2 ), running, S.stream );
+         launchElementwise( S.n, OpTfqmrUV< T >{ u, v, w, Au, st, 0 }, running, S.stream );
+         matvec( S, u, Au, running );
+         launchElementwise( S.n, OpAddTo< T >{ {}, v, Au }, running, S.stream );
+      }
+   } );
+}
+
 // ================================================================== GMRES
 
 // Small dense state of one restart cycle, in device memory (m = restarting_max):
@@ -1685,6 +1895,7 @@ void solveTyped( SolverImpl& S, const void* b, void* x )
       case TNLHIP_CG: solveCG< T >( S, b, x ); break;
       case TNLHIP_BICGSTAB: solveBiCGStab< T >( S, b, x ); break;
       case TNLHIP_GMRES: solveGMRES< T >( S, b, x ); break;
+      case TNLHIP_TFQMR: solveTFQMR< T >( S, b, x ); break;
    }
 }
 
@@ -1703,6 +1914,8 @@ void prepare( SolverImpl& S )
    int nvec = 4;
    if( S.method == TNLHIP_BICGSTAB )
       nvec = 7;
+   if( S.method == TNLHIP_TFQMR )
+      nvec = 8;
    if( S.method == TNLHIP_GMRES )
       nvec = 4 + ( S.params.restarting_max + 1 ) + 1;
    S.vecs.resize( (int64_t) nvec * S.ld * (int64_t) S.vbytes );
@@ -1741,7 +1954,7 @@ int tnlhip_solver_create( int method, tnlhip_solver* out )
 {
    return guarded( [ & ] {
       TNLHIP_REQUIRE( out, "NULL out" );
-      TNLHIP_REQUIRE( method >= TNLHIP_CG && method <= TNLHIP_GMRES, "unknown method" );
+      TNLHIP_REQUIRE( method >= TNLHIP_CG && method <= TNLHIP_TFQMR, "unknown method" );
       auto* s = new tnlhip_solver_s;
       s->method = method;
       *out = s;
