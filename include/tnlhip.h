@@ -118,6 +118,44 @@ int tnlhip_mtx_write( tnlhip_matrix A, const char* path );
 int tnlhip_mtx_write_csr( const char* path, int32_t rows, int32_t columns, const int32_t* row_pointers,
                           const int32_t* column_indexes, const double* values );
 
+/* TNL binary files (File / Object format, Object.hpp:17-75): "TNLMN", the serialization type
+ * string, then SparseMatrix::save -- rows, columns, the values and column-index arrays and the
+ * segments' own fields (Matrix.hpp:129-150, SparseMatrixView.hpp:911-920; CSRView.hpp:221-224,
+ * EllpackView.hpp:404-410, SlicedEllpackView.hpp:310-318, ChunkedEllpackView.hpp:369-378,
+ * BiEllpackView.hpp:345-351).  The element order of the Ellpack family depends on the
+ * organization TNL was built with, which the file does not record, hence the layout:
+ *   TNLHIP_FILE_CUDA_LAYOUT  column-major, as TNL's Devices::Cuda matrices write it
+ *                            (CSR kernel names carry TNL::Devices::Cuda)
+ *   TNLHIP_FILE_HOST_LAYOUT  row-major, as TNL's Devices::Host matrices write it (Ellpack's
+ *                            alignedSize = rows; CSR kernel names carry TNL::Devices::Host) */
+typedef enum
+{
+   TNLHIP_FILE_CUDA_LAYOUT = 0,
+   TNLHIP_FILE_HOST_LAYOUT = 1
+} tnlhip_file_layout;
+
+typedef struct
+{
+   int32_t format;      /* TNLHIP_CSR_* from the CSR kernel name, or the Ellpack-family format */
+   int32_t value_dtype; /* TNLHIP_F64 ("double") or TNLHIP_F32 ("float") */
+   int32_t rows;
+   int32_t columns;
+   int64_t storage_size; /* length of the stored values array */
+} tnlhip_file_info;
+
+/* SparseMatrix::save( fileName ) (SparseMatrix.hpp:1266-1270).  f64 and f32 matrices only: TNL
+ * has no bfloat16 Real. */
+int tnlhip_matrix_save( tnlhip_matrix A, const char* path, int layout );
+/* SparseMatrix::load( fileName ) (SparseMatrix.hpp:1248-1255, 1282-1286).  format and value_dtype
+ * -1 take them from the file; otherwise the file's type must match them, where any CSR kernel
+ * name matches every CSR format (the stored arrays do not depend on the kernel).  The metadata
+ * is checked to be the one TNL builds (every slot in range, every column index in [-1,
+ * columns)) before anything reaches the device. */
+int tnlhip_matrix_load( const char* path, int format, int value_dtype, int layout, tnlhip_matrix* out );
+/* getObjectType( fileName ) (Object.hpp:64-84) and the checks of tnlhip_matrix_load, on the host
+ * only; type (may be NULL) receives the serialization type string. */
+int tnlhip_file_inspect( const char* path, int layout, tnlhip_file_info* info, char* type, int type_len );
+
 int tnlhip_matrix_destroy( tnlhip_matrix A );
 
 int tnlhip_matrix_get_info( tnlhip_matrix A, tnlhip_matrix_info* info );

@@ -130,6 +130,12 @@ def ref():
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
         L.ref_write_mtx.restype = C.c_int
         L.ref_write_mtx.argtypes = [C.c_char_p, C.c_int, C.c_int, _i32p, _i32p, _f64p]
+        L.ref_save_tnl.restype = C.c_int
+        L.ref_save_tnl.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f64p, C.c_char_p,
+                                   C.c_char_p, C.c_int]
+        L.ref_load_tnl.restype = C.c_int
+        L.ref_load_tnl.argtypes = [C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_void_p, C.c_void_p,
+                                   C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_int]
         _ref = L
     return _ref
 
@@ -154,6 +160,44 @@ def ref_write_mtx(path, rows, cols, row_ptr, col_idx, vals):
     """TNL's MatrixWriter::writeMtx of a Host CSR matrix."""
     ref().ref_write_mtx(str(path).encode(), int(rows), int(cols), np.ascontiguousarray(row_ptr, np.int32),
                         np.ascontiguousarray(col_idx, np.int32), np.ascontiguousarray(vals, np.float64))
+
+
+# TNL matrix types of ref_save_tnl / ref_load_tnl: CSR with each kernel, then the Ellpack family
+TNL_FILE_VARIANTS = ["CSRScalar", "CSRVector", "CSRHybrid", "CSRLight", "CSRAdaptive", "Ellpack", "SlicedEllpack",
+                     "ChunkedEllpack", "BiEllpack"]
+
+
+def ref_save_tnl(variant, row_ptr, col_idx, vals, ncols, path, column_major=True, single=False):
+    """TNL's SparseMatrix< double or float, Host, int, General, variant >::operator=( CSR ) then
+    save( path ); column_major selects the organization of the Ellpack family."""
+    v = TNL_FILE_VARIANTS.index(variant) if isinstance(variant, str) else variant
+    rp = np.ascontiguousarray(row_ptr, np.int32)
+    err = C.create_string_buffer(1024)
+    if ref().ref_save_tnl(v, 1 if column_major else 0, 1 if single else 0, len(rp) - 1, int(ncols), rp,
+                          np.ascontiguousarray(col_idx, np.int32), np.ascontiguousarray(vals, np.float64),
+                          str(path).encode(), err, 1024):
+        raise ValueError(err.value.decode(errors="replace"))
+
+
+def ref_load_tnl(variant, path, x=None, column_major=True, single=False):
+    """TNL's SparseMatrix::load( path ) into the given type, then y = A x when x is given:
+    (rows, columns, y); raises ValueError with TNL's message."""
+    v = TNL_FILE_VARIANTS.index(variant) if isinstance(variant, str) else variant
+    rows, cols = C.c_int(), C.c_int()
+    err = C.create_string_buffer(1024)
+    xs = None if x is None else np.ascontiguousarray(x, np.float64)
+    y = None
+    if xs is not None:
+        # the row count is known only after loading: load once without the product
+        if ref().ref_load_tnl(v, 1 if column_major else 0, 1 if single else 0, str(path).encode(), None, None,
+                              C.byref(rows), C.byref(cols), err, 1024):
+            raise ValueError(err.value.decode(errors="replace"))
+        y = np.zeros(max(rows.value, 1), np.float64)
+    if ref().ref_load_tnl(v, 1 if column_major else 0, 1 if single else 0, str(path).encode(),
+                          None if xs is None else xs.ctypes.data, None if y is None else y.ctypes.data,
+                          C.byref(rows), C.byref(cols), err, 1024):
+        raise ValueError(err.value.decode(errors="replace"))
+    return rows.value, cols.value, None if y is None else y[:rows.value]
 
 
 # ------------------------------------------------------------------ segments

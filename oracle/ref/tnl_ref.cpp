@@ -241,6 +241,83 @@ struct WithHistory : Solver
    void closeHistory() { this->residualHistoryFile.close(); }
 };
 
+
+// SparseMatrix< Real, Host, int, General, S >::operator=( CSR ) then save( path ): TNL's
+// File/Object format (Object.hpp:17-75, SparseMatrixView.hpp:911-920).
+template< typename Real, template< typename, typename, typename > class S >
+int saveAs( int n, int ncols, const int* rp, const int* cols, const double* vals, const char* path )
+{
+   CsrMatrix A;
+   fillCSR( A, n, ncols, rp, cols, vals );
+   Matrices::SparseMatrix< Real, Host, int, Matrices::GeneralMatrix, S > B;
+   B = A;
+   B.save( std::string( path ) );
+   return 0;
+}
+
+// SparseMatrix< Real, Host, int, General, S >::load( path ) then vectorProduct( x, y ).
+template< typename Real, template< typename, typename, typename > class S >
+int loadAs( const char* path, const double* x, double* y, int* rows, int* cols )
+{
+   Matrices::SparseMatrix< Real, Host, int, Matrices::GeneralMatrix, S > B;
+   B.load( std::string( path ) );
+   *rows = B.getRows();
+   *cols = B.getColumns();
+   if( x && y ) {
+      Containers::Vector< Real, Host, int > xv( B.getColumns() ), yv( B.getRows() );
+      for( int i = 0; i < B.getColumns(); i++ )
+         xv[ i ] = (Real) x[ i ];
+      yv = 0;
+      B.vectorProduct( xv, yv );
+      for( int i = 0; i < B.getRows(); i++ )
+         y[ i ] = yv[ i ];
+   }
+   return 0;
+}
+
+// variant: 0..4 CSR with the Scalar, Vector, Hybrid, Light, Adaptive kernel, 5 Ellpack,
+// 6 SlicedEllpack, 7 ChunkedEllpack, 8 BiEllpack; colmajor selects the organization of 5..8.
+template< typename Real, typename F >
+int byVariant( int variant, int colmajor, F&& f )
+{
+   switch( variant * 2 + ( colmajor ? 1 : 0 ) ) {
+      case 0: case 1: return f.template operator()< Real, Seg::CSRScalar >();
+      case 2: case 3: return f.template operator()< Real, Seg::CSRVector >();
+      case 4: case 5: return f.template operator()< Real, Seg::CSRHybrid >();
+      case 6: case 7: return f.template operator()< Real, Seg::CSRLight >();
+      case 8: case 9: return f.template operator()< Real, Seg::CSRAdaptive >();
+      case 10: return f.template operator()< Real, EllRow >();
+      case 11: return f.template operator()< Real, EllC >();
+      case 12: return f.template operator()< Real, SlR >();
+      case 13: return f.template operator()< Real, SlC >();
+      case 14: return f.template operator()< Real, ChR >();
+      case 15: return f.template operator()< Real, ChC >();
+      case 16: return f.template operator()< Real, BiR >();
+      case 17: return f.template operator()< Real, BiC >();
+   }
+   return -2;
+}
+
+struct SaveFn
+{
+   int n, ncols;
+   const int *rp, *cols;
+   const double* vals;
+   const char* path;
+   template< typename Real, template< typename, typename, typename > class S >
+   int operator()() const { return saveAs< Real, S >( n, ncols, rp, cols, vals, path ); }
+};
+
+struct LoadFn
+{
+   const char* path;
+   const double* x;
+   double* y;
+   int *rows, *cols;
+   template< typename Real, template< typename, typename, typename > class S >
+   int operator()() const { return loadAs< Real, S >( path, x, y, rows, cols ); }
+};
+
 }  // namespace
 
 extern "C" {
@@ -493,6 +570,34 @@ int ref_write_mtx( const char* path, int rows, int cols, const int* rp, const in
    }
    TNL::Matrices::MatrixWriter< M >::writeMtx( std::string( path ), A );
    return 0;
+}
+
+// TNL's binary save of a matrix converted from CSR (single = 1: float values).
+int ref_save_tnl( int variant, int colmajor, int single, int n, int ncols, const int* rp, const int* cols,
+                  const double* vals, const char* path, char* err, int errlen )
+{
+   SaveFn f{ n, ncols, rp, cols, vals, path };
+   try {
+      return single ? byVariant< float >( variant, colmajor, f ) : byVariant< double >( variant, colmajor, f );
+   }
+   catch( const std::exception& e ) {
+      std::snprintf( err, errlen, "%s", e.what() );
+      return -1;
+   }
+}
+
+// TNL's binary load into the given matrix type, then y = A x when x and y are given.
+int ref_load_tnl( int variant, int colmajor, int single, const char* path, const double* x, double* y, int* rows,
+                  int* cols, char* err, int errlen )
+{
+   LoadFn f{ path, x, y, rows, cols };
+   try {
+      return single ? byVariant< float >( variant, colmajor, f ) : byVariant< double >( variant, colmajor, f );
+   }
+   catch( const std::exception& e ) {
+      std::snprintf( err, errlen, "%s", e.what() );
+      return -1;
+   }
 }
 
 }  // extern "C"

@@ -262,6 +262,30 @@ int main()
       std::remove( path );
       std::remove( "/tmp/tnlhip_front_test.out.mtx" );
       std::printf( "%s MatrixMarket\n", failures ? "FAIL" : "PASS" );
+      // binary save / load: the loaded matrix gives the same product
+      A.save( "/tmp/tnlhip_front_test.tnl" );
+      Matrix B;
+      B.load( "/tmp/tnlhip_front_test.tnl" );
+      Containers::Vector< double, Devices::Hip, int > zv( 3, 0.0 );
+      B.vectorProduct( xv, zv );
+      const auto zh = zv.toHost();
+      EXPECT_EQ( zh[ 0 ], 0.0 );
+      EXPECT_EQ( zh[ 1 ], 3.0 );
+      EXPECT_EQ( zh[ 2 ], 15.0 );
+      std::remove( "/tmp/tnlhip_front_test.tnl" );
+      // a file of another segments type is refused, as SparseMatrix::load does
+      A.save( "/tmp/tnlhip_front_test.tnl" );
+      bool refused = false;
+      try {
+         Matrices::SparseMatrix< double, Devices::Hip, int, Matrices::GeneralMatrix, Seg::Ellpack > E;
+         E.load( "/tmp/tnlhip_front_test.tnl" );
+      }
+      catch( const std::exception& ) {
+         refused = true;
+      }
+      EXPECT_EQ( refused, true );
+      std::remove( "/tmp/tnlhip_front_test.tnl" );
+      std::printf( "%s save/load\n", failures ? "FAIL" : "PASS" );
    }
    std::printf( failures ? "FAILURES %d\n" : "ALL PASSED\n", failures );
    return failures ? 1 : 0;

@@ -11,6 +11,8 @@ Outputs (numpy .npz, data only):
                 the converted storage (values, columns) and y = A x (column-major
                 organization, TNL's GPU default; metadata also for row-major)
   adaptive.npz  CSR-Adaptive block descriptors per case and value size
+  tnlfile.npz   TNL's binary SparseMatrix::save of small matrices in every matrix type, both
+                organizations and both Real types (file bytes)
   solvers.npz   residual histories, iteration counts, convergence flags and solutions of
                 CG / BiCGStab / GMRES (all variants, with and without Jacobi) at one thread
 """
@@ -210,9 +212,63 @@ def write_solver_fixtures():
     print("solvers.npz", os.path.getsize(os.path.join(HERE, "solvers.npz")))
 
 
+def tnlfile_cases():
+    """Matrices for TNL binary file fixtures: no explicit zeros, so that every conversion keeps
+    the same entries."""
+    rng = np.random.default_rng(31)
+
+    def random_rows(n, ncols, lengths):
+        rp = np.zeros(n + 1, np.int32)
+        rp[1:] = np.cumsum(lengths)
+        ci = np.concatenate([np.sort(rng.choice(ncols, size=L, replace=False)) for L in lengths] or [[]])
+        # eighths keep the fixtures compressible; exact in float as in double
+        v = rng.integers(1, 17, rp[-1]) / 8.0 * rng.choice([-1.0, 1.0], rp[-1])
+        return rp, ci.astype(np.int32), v, ncols
+
+    out = {}
+    out["tiny_5x5"] = (np.array([0, 2, 3, 6, 6, 7], np.int32), np.array([0, 3, 1, 0, 2, 4, 3], np.int32),
+                       np.arange(1, 8, dtype=np.float64), 5)
+    lengths = rng.integers(0, 26, 70)
+    lengths[::9] = 0
+    out["ragged_70x60"] = random_rows(70, 60, lengths)
+    out["many_slices_600x400"] = random_rows(600, 400, rng.integers(1, 21, 600))
+    out["empty_rows_40x40"] = random_rows(40, 40, np.zeros(40, np.int64))
+    return out
+
+
+def write_tnlfile_fixtures():
+    """TNL's own SparseMatrix::save of every case, matrix type, organization and Real type."""
+    import tempfile
+    out = {}
+    cases = tnlfile_cases()
+    out["names"] = np.array(sorted(cases))
+    with tempfile.TemporaryDirectory() as tmp:
+        for name in sorted(cases):
+            rp, ci, v, ncols = cases[name]
+            out[f"{name}/row_ptr"] = rp
+            out[f"{name}/cols"] = ci
+            out[f"{name}/vals"] = v
+            out[f"{name}/ncols"] = np.array([ncols], np.int32)
+            for variant in O.TNL_FILE_VARIANTS:
+                # a CSR file does not depend on the organization
+                orgs = (("cm", True),) if variant.startswith("CSR") else (("cm", True), ("rm", False))
+                for org, cm in orgs:
+                    for dt, single in (("f64", False), ("f32", True)):
+                        path = os.path.join(tmp, f"{name}_{variant}_{org}_{dt}.tnl")
+                        O.ref_save_tnl(variant, rp, ci, v, ncols, path, column_major=cm, single=single)
+                        with open(path, "rb") as f:
+                            out[f"{name}/{variant}/{org}/{dt}"] = np.frombuffer(f.read(), np.uint8)
+    np.savez_compressed(os.path.join(HERE, "tnlfile.npz"), **out)
+    print("tnlfile.npz", os.path.getsize(os.path.join(HERE, "tnlfile.npz")))
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "solvers":
         main(only_solvers=True)
+    elif len(sys.argv) > 1 and sys.argv[1] == "tnlfile":
+        if not O.ref_available():
+            raise SystemExit("oracle/_ref/libtnlref.so is missing: run make -C oracle in the reference container")
+        write_tnlfile_fixtures()
     elif len(sys.argv) > 1 and sys.argv[1] == "mtx":
         if not O.ref_available():
             raise SystemExit("oracle/_ref/libtnlref.so is missing: run make -C oracle in the reference container")
@@ -220,3 +276,4 @@ if __name__ == "__main__":
     else:
         main()
         write_mtx_fixtures()
+        write_tnlfile_fixtures()

@@ -1,0 +1,121 @@
+"""TNL binary matrix files through the device.  A matrix built on the GPU and saved gives the
+bytes TNL's own SparseMatrix::save writes for the same matrix (tests/golden/tnlfile.npz, made
+from the reference build), in both layouts; loading TNL's files gives, element for element, the
+matrix the library builds from the same CSR arrays, and the same product."""
+import os
+
+import numpy as np
+import pytest
+
+import tnl_files as TF
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = ["CSRScalar", "CSRVector", "CSRHybrid", "CSRLight", "CSRAdaptive", "Ellpack", "SlicedEllpack",
+            "ChunkedEllpack", "BiEllpack"]
+METADATA = {"CSR": ["offsets"], "Ellpack": ["ellpack"], "SlicedEllpack": ["sliceOffsets", "sliceSegmentSizes", "sliced"],
+            "ChunkedEllpack": ["rowToChunkMapping", "rowToSliceMapping", "chunksToSegmentsMapping", "rowPointers",
+                               "slices", "chunked"],
+            "BiEllpack": ["rowPermArray", "groupPointers", "biellpack"]}
+
+
+@pytest.fixture(scope="module")
+def files():
+    return np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tnlfile.npz"))
+
+
+def names(files):
+    return [str(n) for n in files["names"]]
+
+
+def case(files, name):
+    return (files[f"{name}/row_ptr"], files[f"{name}/cols"], files[f"{name}/vals"], int(files[f"{name}/ncols"][0]))
+
+
+def tnl_bytes(files, name, variant, layout, dt):
+    """TNL's file: the row-major one for the host layout; for the CUDA layout the column-major one,
+    whose CSR kernel names TNL's CUDA matrices write with Devices::Cuda."""
+    org = "cm" if layout == "cuda" or variant.startswith("CSR") else "rm"
+    data = files[f"{name}/{variant}/{org}/{dt}"].tobytes()
+    if layout == "cuda" and variant.startswith("CSR"):
+        t = TF.fields(data)["type"]
+        enc = t.replace("TNL::Devices::Host", "TNL::Devices::Cuda").encode()
+        data = b"TNLMN" + len(enc).to_bytes(4, "little", signed=True) + enc + data[9 + len(t):]
+    return data
+
+
+def family(variant):
+    return "CSR" if variant.startswith("CSR") else variant
+
+
+def product(tnl, A, x):
+    import torch
+    y = torch.zeros(A.getRows(), dtype=A.vector_dtype, device="cuda:0")
+    A.vectorProduct(x.to(A.vector_dtype), y)
+    torch.cuda.synchronize()
+    return y.cpu().numpy()
+
+
+@pytest.mark.parametrize("layout", ["cuda", "host"])
+def test_save_writes_tnls_bytes(tnl, dev, files, tmp_path, layout):
+    for name in names(files):
+        rp, ci, v, nc = case(files, name)
+        for variant in VARIANTS:
+            for dt in ("f64", "f32"):
+                print(f"save {name} {variant} {dt} {layout}", flush=True)
+                A = tnl.SparseMatrix.from_csr(rp, ci, v, columns=nc, format=variant, dtype=dt)
+                p = tmp_path / "m.tnl"
+                A.save(p, layout)
+                assert TF.masked(p.read_bytes()) == TF.masked(tnl_bytes(files, name, variant, layout, dt)), \
+                    (name, variant, dt, layout)
+
+
+@pytest.mark.parametrize("layout", ["cuda", "host"])
+def test_load_gives_the_matrix_built_from_csr(tnl, dev, files, tmp_path, layout):
+    import torch
+    for name in names(files):
+        rp, ci, v, nc = case(files, name)
+        x = torch.linspace(-1.0, 1.0, max(nc, 1), dtype=torch.float64, device=dev)[:nc]
+        for variant in VARIANTS:
+            for dt in ("f64", "f32"):
+                print(f"load {name} {variant} {dt} {layout}", flush=True)
+                p = tmp_path / "m.tnl"
+                p.write_bytes(tnl_bytes(files, name, variant, layout, dt))
+                B = tnl.SparseMatrix.load(p, layout=layout)
+                A = tnl.SparseMatrix.from_csr(rp, ci, v, columns=nc, format=variant, dtype=dt)
+                key = (name, variant, dt, layout)
+                assert (B.format, B.value_dtype, B.getRows(), B.getColumns()) == (variant, dt, A.getRows(), nc), key
+                for field in ("storage_size", "allocated", "nonzeros", "lanes_per_row"):
+                    assert getattr(B.info, field) == getattr(A.info, field), (key, field)
+                for got, want in zip(B.storage(), A.storage()):
+                    np.testing.assert_array_equal(got, want, err_msg=str(key))
+                for m in METADATA[family(variant)]:
+                    np.testing.assert_array_equal(B.metadata(m), A.metadata(m), err_msg=str((key, m)))
+                if variant == "CSRAdaptive":
+                    np.testing.assert_array_equal(B.metadata("adaptiveBlocks"), A.metadata("adaptiveBlocks"))
+                np.testing.assert_array_equal(product(tnl, B, x), product(tnl, A, x), err_msg=str(key))
+
+
+def test_round_trip_and_type_checks(tnl, dev, files, tmp_path):
+    import torch
+    rp, ci, v, nc = case(files, "many_slices_600x400")
+    x = torch.linspace(-1.0, 1.0, nc, dtype=torch.float64, device=dev)
+    for variant in VARIANTS:
+        A = tnl.SparseMatrix.from_csr(rp, ci, v, columns=nc, format=variant, dtype="f64")
+        for layout in ("cuda", "host"):
+            p = tmp_path / f"{variant}_{layout}.tnl"
+            A.save(p, layout)
+            B = tnl.SparseMatrix.load(p, format=variant, dtype="f64", layout=layout)
+            np.testing.assert_array_equal(product(tnl, B, x), product(tnl, A, x), err_msg=f"{variant} {layout}")
+    # any CSR kernel reads a CSR file; another segments type or Real type is refused
+    p = tmp_path / "csr.tnl"
+    tnl.SparseMatrix.from_csr(rp, ci, v, columns=nc, format="CSRScalar").save(p)
+    B = tnl.SparseMatrix.load(p, format="CSRAdaptive")
+    np.testing.assert_array_equal(product(tnl, B, x), product(tnl, tnl.SparseMatrix.from_csr(
+        rp, ci, v, columns=nc, format="CSRAdaptive"), x))
+    with pytest.raises(tnl.TnlHipError, match="object type does not match"):
+        tnl.SparseMatrix.load(p, format="Ellpack")
+    with pytest.raises(tnl.TnlHipError, match="object type does not match"):
+        tnl.SparseMatrix.load(p, dtype="f32")
+    with pytest.raises(tnl.TnlHipError, match="bfloat16"):
+        tnl.SparseMatrix.from_csr(rp, ci, v, columns=nc, format="SlicedEllpack", dtype="bf16").save(p)

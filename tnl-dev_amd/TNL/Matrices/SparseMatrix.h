@@ -145,6 +145,23 @@ public:
    }
    void writeMtxFile( const std::string& path ) const { detail::check( tnlhip_mtx_write( getHandle(), path.c_str() ) ); }
 
+   // SparseMatrix::save / load( fileName ) in TNL's binary File / Object format
+   // (SparseMatrix.hpp:1248-1286), with the column-major layout of TNL's GPU matrices
+   void save( const std::string& fileName ) const
+   {
+      detail::check( tnlhip_matrix_save( getHandle(), fileName.c_str(), TNLHIP_FILE_CUDA_LAYOUT ) );
+   }
+   void load( const std::string& fileName )
+   {
+      release();
+      detail::check( tnlhip_matrix_load( fileName.c_str(), format, valueDtype, TNLHIP_FILE_CUDA_LAYOUT, &handle ) );
+      tnlhip_matrix_info info;
+      detail::check( tnlhip_matrix_get_info( handle, &info ) );
+      rows = info.rows;
+      columns = info.columns;
+      staged = false;
+   }
+
    // B = A across formats and value types, on the device (SparseMatrix.hpp:1093-1107)
    template< typename R2, template< typename, typename, typename > class S2, typename C2 >
    SparseMatrix& operator=( const SparseMatrix< R2, Device, Index, MatrixType, S2, C2 >& other )

@@ -37,8 +37,10 @@ EXPORTS = [
     "tnlhip_server_footprint", "tnlhip_server_create", "tnlhip_server_destroy", "tnlhip_server_add_matrix",
     "tnlhip_server_remove_matrix", "tnlhip_server_submit", "tnlhip_server_progress", "tnlhip_server_result",
     "tnlhip_server_stats", "tnlhip_autotune_csr", "tnlhip_mtx_read_csr", "tnlhip_mtx_read", "tnlhip_mtx_write",
-    "tnlhip_mtx_write_csr",
+    "tnlhip_mtx_write_csr", "tnlhip_matrix_save", "tnlhip_matrix_load", "tnlhip_file_inspect",
 ]
+# element order of TNL binary files: "cuda" = column-major (TNL's GPU matrices), "host" = row-major
+FILE_LAYOUTS = {"cuda": 0, "host": 1}
 
 
 class TnlHipError(RuntimeError):
@@ -53,6 +55,11 @@ class CapacityError(TnlHipError):
 
 class BreakdownError(TnlHipError):
     """GMRES hit a zero Hessenberg diagonal (TNL throws 1)."""
+
+
+class FileInfo(C.Structure):
+    _fields_ = [("format", C.c_int32), ("value_dtype", C.c_int32), ("rows", C.c_int32), ("columns", C.c_int32),
+                ("storage_size", C.c_int64)]
 
 
 class MatrixInfo(C.Structure):
@@ -112,6 +119,13 @@ class ServerStats(C.Structure):
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"libtnlhip.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+    # torch's wheel ships its own HIP runtime under the same soname (libamdhip64.so.7).  Loaded
+    # first, it is the one runtime of the process and libtnlhip.so binds to it; loaded after the
+    # library, the two runtimes see no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, i32, i64, f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double
     sig = {
@@ -160,6 +174,9 @@ def _load():
         "tnlhip_mtx_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.POINTER(vp)]),
         "tnlhip_mtx_write": (C.c_int, [vp, C.c_char_p]),
         "tnlhip_mtx_write_csr": (C.c_int, [C.c_char_p, i32, i32, vp, vp, vp]),
+        "tnlhip_matrix_save": (C.c_int, [vp, C.c_char_p, C.c_int]),
+        "tnlhip_matrix_load": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
+        "tnlhip_file_inspect": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(FileInfo), C.c_char_p, C.c_int]),
         "tnlhip_autotune_csr": (C.c_int, [C.c_int, i32, i32, vp, vp, vp, i32, C.c_uint32, i32, C.POINTER(Tuning),
                                           C.POINTER(Tuning), i32, C.POINTER(i32), C.POINTER(vp)]),
     }
@@ -253,6 +270,19 @@ class SparseMatrix:
     def write_mtx(self, path):
         """MatrixWriter::writeMtx( path, A )."""
         _check(lib.tnlhip_mtx_write(self._h, str(path).encode()))
+
+    def save(self, path, layout="cuda"):
+        """SparseMatrix::save( fileName ): TNL's binary File / Object format (SparseMatrix.hpp:1266-1270)."""
+        _check(lib.tnlhip_matrix_save(self._h, str(path).encode(), FILE_LAYOUTS[layout]))
+
+    @classmethod
+    def load(cls, path, format=None, dtype=None, layout="cuda"):
+        """SparseMatrix::load( fileName ) (SparseMatrix.hpp:1282-1286); format / dtype None take the file's."""
+        fmt = -1 if format is None else (FORMATS[format] if isinstance(format, str) else format)
+        dt = -1 if dtype is None else _dtype_id(dtype)
+        h = C.c_void_p(0)
+        _check(lib.tnlhip_matrix_load(str(path).encode(), fmt, dt, FILE_LAYOUTS[layout], C.byref(h)))
+        return cls(h)
 
     def __del__(self):
         try:
@@ -671,6 +701,17 @@ def mtx_write_csr(path, rows, columns, row_pointers, column_indexes, values):
     v = np.ascontiguousarray(values, np.float64)
     _check(lib.tnlhip_mtx_write_csr(str(path).encode(), int(rows), int(columns), rp.ctypes.data, ci.ctypes.data,
                                     v.ctypes.data))
+
+
+def file_inspect(path, layout="cuda"):
+    """getObjectType( fileName ) of a TNL binary matrix file plus the checks of SparseMatrix.load,
+    on the host: {"type", "format", "dtype", "rows", "columns", "storage_size"}."""
+    info = FileInfo()
+    t = C.create_string_buffer(4096)
+    _check(lib.tnlhip_file_inspect(str(path).encode(), FILE_LAYOUTS[layout], C.byref(info), t, 4096))
+    return {"type": t.value.decode(), "format": FORMAT_NAMES[info.format],
+            "dtype": {v: k for k, v in DTYPES.items()}[info.value_dtype], "rows": info.rows, "columns": info.columns,
+            "storage_size": info.storage_size}
 
 
 def server_footprint(rows, nnz, max_row, nrhs):

@@ -124,25 +124,22 @@ int hybridLanes( int64_t nnz, int32_t rows )
    return g;
 }
 
-tnlhip_matrix_s* buildEmpty( int format, int dtype, int32_t rows, int32_t cols, const std::vector< int32_t >& caps )
+// A matrix of the given metadata with uninitialized storage; capTotal = the sum of the row
+// capacities, which sets the lanes per row of the CSR kernels.
+tnlhip_matrix_s* buildFromSegments( int format, int dtype, int32_t rows, int32_t cols, HostSegments meta,
+                                    int64_t capTotal )
 {
-   TNLHIP_REQUIRE( format >= TNLHIP_CSR_SCALAR && format <= TNLHIP_BIELLPACK, "unknown format" );
-   TNLHIP_REQUIRE( dtype >= TNLHIP_F64 && dtype <= TNLHIP_BF16, "unknown value type" );
-   TNLHIP_REQUIRE( rows >= 0 && cols >= 0, "negative dimensions" );
    auto M = std::make_unique< tnlhip_matrix_s >();
    M->format = format;
    M->dtype = dtype;
    M->rows = rows;
    M->cols = cols;
-   buildSegments( M->meta, kindOf( format ), rows, caps.data() );
+   M->meta = std::move( meta );
    if( format == TNLHIP_CSR_ADAPTIVE ) {
       const int vsize = dtype == TNLHIP_F64 ? 8 : 4;  // LDS holds products in the compute type
       M->adaptiveHost = buildAdaptiveBlocks( rows, M->meta.offsets.data(), vsize, kWave, 256, kAdaptiveStreamBytes,
                                              kAdaptiveMaxElements );
    }
-   int64_t capTotal = 0;
-   for( int32_t c : caps )
-      capTotal += c;
    if( format == TNLHIP_CSR_LIGHT )
       M->lanes = lightLanes( capTotal, rows );
    else if( format == TNLHIP_CSR_HYBRID || format == TNLHIP_CSR_ADAPTIVE )
@@ -153,6 +150,19 @@ tnlhip_matrix_s* buildEmpty( int format, int dtype, int32_t rows, int32_t cols, 
    M->columns.resize( M->meta.allocated );
    M->uploadMetadata();
    return M.release();
+}
+
+tnlhip_matrix_s* buildEmpty( int format, int dtype, int32_t rows, int32_t cols, const std::vector< int32_t >& caps )
+{
+   TNLHIP_REQUIRE( format >= TNLHIP_CSR_SCALAR && format <= TNLHIP_BIELLPACK, "unknown format" );
+   TNLHIP_REQUIRE( dtype >= TNLHIP_F64 && dtype <= TNLHIP_BF16, "unknown value type" );
+   TNLHIP_REQUIRE( rows >= 0 && cols >= 0, "negative dimensions" );
+   HostSegments meta;
+   buildSegments( meta, kindOf( format ), rows, caps.data() );
+   int64_t capTotal = 0;
+   for( int32_t c : caps )
+      capTotal += c;
+   return buildFromSegments( format, dtype, rows, cols, std::move( meta ), capTotal );
 }
 
 __global__ __launch_bounds__( kReduceBlock ) void countKernel( int64_t n, const int32_t* __restrict__ cols,
@@ -379,6 +389,69 @@ int tnlhip_mtx_write( tnlhip_matrix A, const char* path )
       drp.upload( rp );
       gatherToCsr( S, drp.ptr, dcol.ptr, dval.ptr, nullptr, true );  // stored zeros count in the header
       writeMtx( path, S.rows, S.cols, rp, dcol.download(), dval.download() );
+   } );
+}
+
+int tnlhip_matrix_save( tnlhip_matrix A, const char* path, int layout )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( A && path, "NULL argument" );
+      TNLHIP_REQUIRE( layout == TNLHIP_FILE_CUDA_LAYOUT || layout == TNLHIP_FILE_HOST_LAYOUT, "unknown file layout" );
+      if( A->dtype == TNLHIP_BF16 )
+         throw Error( TNLHIP_ERROR_UNSUPPORTED, "TNL files hold double or float values, not bfloat16" );
+      const int64_t n = A->meta.allocated;
+      std::vector< int32_t > cols( (size_t) n );
+      std::vector< uint8_t > vals( (size_t) n * dtypeBytes( A->dtype ) );
+      if( n > 0 ) {
+         TNLHIP_CHECK( hipMemcpy( cols.data(), A->columns.ptr, sizeof( int32_t ) * n, hipMemcpyDeviceToHost ) );
+         TNLHIP_CHECK( hipMemcpy( vals.data(), A->values.ptr, vals.size(), hipMemcpyDeviceToHost ) );
+      }
+      writeTnlFile( path, layout == TNLHIP_FILE_HOST_LAYOUT, A->format, A->dtype, A->rows, A->cols, A->meta, cols.data(),
+                    vals.data() );
+   } );
+}
+
+int tnlhip_matrix_load( const char* path, int format, int value_dtype, int layout, tnlhip_matrix* out )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( path && out, "NULL argument" );
+      TNLHIP_REQUIRE( layout == TNLHIP_FILE_CUDA_LAYOUT || layout == TNLHIP_FILE_HOST_LAYOUT, "unknown file layout" );
+      TNLHIP_REQUIRE( format >= -1 && format <= TNLHIP_BIELLPACK, "unknown format" );
+      TNLHIP_REQUIRE( value_dtype >= -1 && value_dtype <= TNLHIP_BF16, "unknown value type" );
+      TnlFileMatrix f = readTnlFile( path, layout == TNLHIP_FILE_HOST_LAYOUT );
+      const int fmt = format < 0 ? f.format : format;
+      const int dtype = value_dtype < 0 ? f.dtype : value_dtype;
+      // SparseMatrix::load requires the file's type (Object.hpp:42-50); the CSR kernel is free
+      if( kindOf( fmt ) != kindOf( f.format ) || dtype != f.dtype )
+         throw Error( TNLHIP_ERROR_INVALID_ARGUMENT,
+                      "Failed to deserialize an object from the file '" + std::string( path ) +
+                         "': object type does not match (expected format " + std::to_string( fmt ) + ", value type " +
+                         std::to_string( dtype ) + ", found " + f.type + ")." );
+      const int64_t capTotal = kindOf( fmt ) == KIND_CSR ? f.meta.allocated : 0;
+      std::unique_ptr< tnlhip_matrix_s > M(
+         buildFromSegments( fmt, dtype, f.rows, f.columns, std::move( f.meta ), capTotal ) );
+      M->values.upload( f.values.data(), (int64_t) f.values.size() );
+      M->columns.upload( f.columnIndexes );
+      M->nnz = countStored( *M, nullptr );
+      *out = M.release();
+   } );
+}
+
+int tnlhip_file_inspect( const char* path, int layout, tnlhip_file_info* info, char* type, int type_len )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( path, "NULL path" );
+      TNLHIP_REQUIRE( layout == TNLHIP_FILE_CUDA_LAYOUT || layout == TNLHIP_FILE_HOST_LAYOUT, "unknown file layout" );
+      const TnlFileMatrix f = readTnlFile( path, layout == TNLHIP_FILE_HOST_LAYOUT );
+      if( info ) {
+         info->format = f.format;
+         info->value_dtype = f.dtype;
+         info->rows = f.rows;
+         info->columns = f.columns;
+         info->storage_size = f.fileStorage;
+      }
+      if( type && type_len > 0 )
+         std::snprintf( type, (size_t) type_len, "%s", f.type.c_str() );
    } );
 }
 

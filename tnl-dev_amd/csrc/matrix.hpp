@@ -126,4 +126,21 @@ MtxCsr readMtxCsr( const std::string& path );
 void writeMtx( const std::string& path, int32_t rows, int32_t columns, const std::vector< int32_t >& rowPointers,
                const std::vector< int32_t >& columnIndexes, const std::vector< double >& values );
 
+// TNL binary files (File / Object format) of SparseMatrix, tnlfile.cpp.  host = the row-major
+// organization and Devices::Host kernel names of TNL's Host matrices; otherwise column-major
+// with Devices::Cuda names, the layout the device arrays already have.
+struct TnlFileMatrix
+{
+   std::string type;
+   int format = TNLHIP_CSR_SCALAR, dtype = TNLHIP_F64;
+   int32_t rows = 0, columns = 0;
+   int64_t fileStorage = 0;               // length of the stored arrays in the file
+   HostSegments meta;                     // checked, column-major
+   std::vector< int32_t > columnIndexes;  // meta.allocated, device order
+   std::vector< uint8_t > values;         // meta.allocated elements of dtype, device order
+};
+TnlFileMatrix readTnlFile( const std::string& path, bool host );
+void writeTnlFile( const std::string& path, bool host, int format, int dtype, int32_t rows, int32_t columns,
+                   const HostSegments& meta, const int32_t* columnIndexes, const void* values );
+
 }  // namespace tnlhip
