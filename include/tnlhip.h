@@ -258,7 +258,7 @@ typedef struct
    int32_t restarting_step_min;
    int32_t restarting_step_max;
    int32_t bicgstab_exact_residue;
-   int32_t preconditioner; /* 0 none, 1 Preconditioners::Diagonal (Jacobi) */
+   int32_t preconditioner; /* 0 none, 1 Preconditioners::Diagonal (Jacobi), 2 Preconditioners::ILU0 */
    int32_t use_graph;      /* 1: CG / BiCGStab iterations are captured once into a hipGraph and
                               replayed (16 per replay); 0: launched one by one */
 } tnlhip_solver_params;
@@ -276,7 +276,11 @@ int tnlhip_solver_set_matrix( tnlhip_solver solver, tnlhip_matrix A );
 
 /* setup( parameters ) / setMaxIterations, setConvergenceResidue, ...; when
  * params->preconditioner == 1 this is setPreconditioner( Diagonal ) followed by its
- * update( matrix ) (Preconditioners/Diagonal.hpp:22-41). */
+ * update( matrix ) (Preconditioners/Diagonal.hpp:22-41); with 2, setPreconditioner( ILU0 ) and
+ * its update (Preconditioners/ILU0.hpp:17-117: factors with A's sparsity, TNL's Host arithmetic
+ * bit for bit), done at the next solve.  ILU(0) needs every row to store its columns below the
+ * diagonal before the others and an entry on or above the diagonal (TNLHIP_ERROR_UNSUPPORTED /
+ * INVALID_ARGUMENT otherwise). */
 int tnlhip_solver_set_params( tnlhip_solver solver, const tnlhip_solver_params* params );
 
 /* bool LinearSolver::solve( b, x ), x holds the initial guess on entry; *converged is

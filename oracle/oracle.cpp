@@ -26,6 +26,8 @@
  *   TFQMR                    src/TNL/Solvers/Linear/TFQMR.hpp:17-125
  *   GMRES                    src/TNL/Solvers/Linear/GMRES.hpp:79-801
  *   Jacobi                   src/TNL/Solvers/Linear/Preconditioners/Diagonal.hpp:22-47
+ *   ILU(0)                   src/TNL/Solvers/Linear/Preconditioners/ILU0.hpp:17-137,
+ *                            src/TNL/Solvers/Linear/Utils/TriangularSolve.h:25-111
  */
 #include "oracle.h"
 
@@ -928,13 +930,11 @@ struct Iter
 struct Jacobi
 {
    std::vector< double > d;
-   bool on = false;
    void update( const Csr& A )
    {
       d.resize( A.n );
       for( I i = 0; i < A.n; i++ )
          d[ i ] = A.element( i, i );
-      on = true;
    }
    void solve( const std::vector< double >& b, std::vector< double >& x ) const
    {
@@ -943,9 +943,119 @@ struct Jacobi
    }
 };
 
+// ILU(0), Preconditioners/ILU0.hpp:17-117 (Host): L (unit diagonal implied) and U kept as two
+// CSR matrices with the sparsity of A's lower and upper parts.  Row i of A is read in storage
+// order; padding (column < 0) is skipped when counting, entries with column < i go to L, the
+// rest up to column N to U.  The factorization runs row by row: for each L entry k (ascending),
+// L_ik /= U_kk, then every later L entry and every U entry j of the row loses L_ik * U_kj,
+// where U_kj is 0 when row k of U has no column j.  TNL stores the rows of U in reverse order;
+// the row order of storage changes no value and is not kept here.
+// solve: triangularSolveLower< true > then triangularSolveUpper< true, true >
+// (Utils/TriangularSolve.h:25-111): x_i = b_i - sum L_ij x_j in row order, then
+// x_i = ( x_i - sum_{c >= 1} U_ij x_j ) / U_i[ 0 ], the first stored entry of U's row.
+struct Ilu0
+{
+   I n = 0;
+   std::vector< I > Lp, Lc, Up, Uc;
+   std::vector< double > Lv, Uv;
+   double uElement( I k, I j ) const
+   {
+      for( I c = Up[ k ]; c < Up[ k + 1 ]; c++ )
+         if( Uc[ c ] == j )
+            return Uv[ c ];
+      return 0.0;
+   }
+   void update( const Csr& A )
+   {
+      n = A.n;
+      Lp.assign( (size_t) n + 1, 0 );
+      Up.assign( (size_t) n + 1, 0 );
+      Lc.clear(), Uc.clear(), Lv.clear(), Uv.clear();
+      for( I i = 0; i < n; i++ ) {
+         // the counting pass of ILU0.hpp:36-50, then the copy of the first entries
+         I nl = 0, nu = 0;
+         for( I g = A.rp[ i ]; g < A.rp[ i + 1 ]; g++ ) {
+            const I col = A.c[ g ];
+            if( col < 0 )
+               continue;
+            if( col < i )
+               nl++;
+            else if( col < n )
+               nu++;
+            else
+               break;
+         }
+         I g = A.rp[ i ];
+         while( g < A.rp[ i + 1 ] && A.c[ g ] < 0 )
+            g++;
+         for( I c = 0; c < nl; c++, g++ ) {
+            Lc.push_back( A.c[ g ] );
+            Lv.push_back( A.v[ g ] );
+         }
+         for( I c = 0; c < nu; c++, g++ ) {
+            Uc.push_back( A.c[ g ] );
+            Uv.push_back( A.v[ g ] );
+         }
+         Lp[ i + 1 ] = (I) Lc.size();
+         Up[ i + 1 ] = (I) Uc.size();
+      }
+      for( I i = 0; i < n; i++ )
+         for( I ck = Lp[ i ]; ck < Lp[ i + 1 ]; ck++ ) {
+            const I k = Lc[ ck ];
+            const double lik = Lv[ ck ] / uElement( k, k );
+            Lv[ ck ] = lik;
+            for( I cj = ck + 1; cj < Lp[ i + 1 ]; cj++ )
+               Lv[ cj ] = Lv[ cj ] - lik * uElement( k, Lc[ cj ] );
+            for( I cj = Up[ i ]; cj < Up[ i + 1 ]; cj++ )
+               Uv[ cj ] = Uv[ cj ] - lik * uElement( k, Uc[ cj ] );
+         }
+   }
+   void solve( const std::vector< double >& b, std::vector< double >& x ) const
+   {
+      for( I i = 0; i < n; i++ ) {
+         double xi = b[ i ];
+         for( I c = Lp[ i ]; c < Lp[ i + 1 ]; c++ )
+            xi -= Lv[ c ] * x[ Lc[ c ] ];
+         x[ i ] = xi;
+      }
+      for( I k = 0; k < n; k++ ) {
+         const I i = n - 1 - k;
+         double xi = x[ i ];
+         for( I c = Up[ i ] + 1; c < Up[ i + 1 ]; c++ )
+            xi -= Uv[ c ] * x[ Uc[ c ] ];
+         x[ i ] = xi / Uv[ Up[ i ] ];
+      }
+   }
+};
+
+// The solvers' preconditioner: none, Jacobi (Diagonal) or ILU(0).
+struct Precond
+{
+   int kind = 0;
+   Jacobi jacobi;
+   Ilu0 ilu;
+   bool on = false;
+   void update( const Csr& A, int k )
+   {
+      kind = k;
+      on = k != 0;
+      if( k == 1 )
+         jacobi.update( A );
+      else if( k == 2 )
+         ilu.update( A );
+   }
+   void solve( const std::vector< double >& b, std::vector< double >& x ) const
+   {
+      if( kind == 1 )
+         jacobi.solve( b, x );
+      else
+         ilu.solve( b, x );
+   }
+};
+
 using Vec = std::vector< double >;
 
-bool solveCG( const Csr& A, const Jacobi& M, const Vec& b, Vec& x, Iter& it )
+bool solveCG( const Csr& A, const Precond& M, const Vec& b, Vec& x, Iter& it )
 {
    // CG.hpp:17-107
    const I n = A.n;
@@ -1007,7 +1117,7 @@ bool solveCG( const Csr& A, const Jacobi& M, const Vec& b, Vec& x, Iter& it )
    return it.checkConvergence();
 }
 
-bool solveBiCGStab( const Csr& A, const Jacobi& M, const Vec& b, Vec& x, Iter& it, bool exact )
+bool solveBiCGStab( const Csr& A, const Precond& M, const Vec& b, Vec& x, Iter& it, bool exact )
 {
    // BICGStab.hpp:39-147
    const I n = A.n;
@@ -1085,7 +1195,7 @@ bool solveBiCGStab( const Csr& A, const Jacobi& M, const Vec& b, Vec& x, Iter& i
    return it.checkConvergence();
 }
 
-bool solveTFQMR( const Csr& A, const Jacobi& M, const Vec& b, Vec& x, Iter& it )
+bool solveTFQMR( const Csr& A, const Precond& M, const Vec& b, Vec& x, Iter& it )
 {
    // TFQMR.hpp:17-125: odd iterations take alpha = rho / (v, r_ast), even ones recompute
    // A u; the residue is the quasi-residual bound tau sqrt(iter + 1) / |b|
@@ -1174,13 +1284,13 @@ struct Gmres
    // GMRES.hpp:79-801, Devices::Host instantiation (localOffset = 0, no MPI)
    enum { CGS = 0, CGSR = 1, MGS = 2, MGSR = 3, CWY = 4 };
    const Csr& A;
-   const Jacobi& M;
+   const Precond& M;
    Iter& it;
    int variant, rmin, rmax, smin, smax;
    I size = 0, ldSize = 0;
    Vec r, w, Mt, z, V, Y, cs, sn, H, s, T, YL;
 
-   Gmres( const Csr& A_, const Jacobi& M_, Iter& it_, const orc_solver_params& p )
+   Gmres( const Csr& A_, const Precond& M_, Iter& it_, const orc_solver_params& p )
    : A( A_ ), M( M_ ), it( it_ ), variant( p.gmres_variant ), rmin( p.restarting_min ), rmax( p.restarting_max ),
      smin( p.restarting_step_min ), smax( p.restarting_step_max )
    {}
@@ -1568,14 +1678,45 @@ orc_solver_params_default( orc_solver_params* p )
 }
 
 extern "C" int32_t
+orc_ilu0( int32_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, int32_t* nnzL, int32_t* nnzU,
+          int32_t* Lrp, int32_t* Lc, double* Lv, int32_t* Urp, int32_t* Uc, double* Uv, const double* b, double* x )
+{
+   Csr A{ n, row_ptr, cols, vals };
+   Ilu0 f;
+   f.update( A );
+   *nnzL = (int32_t) f.Lc.size();
+   *nnzU = (int32_t) f.Uc.size();
+   if( Lrp ) {
+      std::copy( f.Lp.begin(), f.Lp.end(), Lrp );
+      std::copy( f.Lc.begin(), f.Lc.end(), Lc );
+      std::copy( f.Lv.begin(), f.Lv.end(), Lv );
+   }
+   if( Urp ) {
+      std::copy( f.Up.begin(), f.Up.end(), Urp );
+      std::copy( f.Uc.begin(), f.Uc.end(), Uc );
+      std::copy( f.Uv.begin(), f.Uv.end(), Uv );
+   }
+   if( b && x ) {
+      Vec xx( n );
+      f.solve( Vec( b, b + n ), xx );
+      std::copy( xx.begin(), xx.end(), x );
+   }
+   return 0;
+}
+
+extern "C" int32_t
 orc_solve( int32_t method, int32_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, const double* b,
            double* x, const orc_solver_params* p, int32_t* iterations, double* residue, double* history,
            int32_t history_cap, int32_t* history_len )
 {
    Csr A{ n, row_ptr, cols, vals };
-   Jacobi M;
-   if( p->preconditioner == 1 )
-      M.update( A );
+   Precond M;
+   if( p->preconditioner == 1 || p->preconditioner == 2 )
+      M.update( A, p->preconditioner );
+   if( p->preconditioner == 2 )
+      for( I i = 0; i < n; i++ )
+         if( M.ilu.Up[ i ] == M.ilu.Up[ i + 1 ] )
+            return -1;  // a row without entries in U: no U_ii to divide by
    std::vector< double > hist;
    Iter it{ p->max_iterations, p->min_iterations, p->convergence_residue, p->divergence_residue, 0, 0.0, &hist };
    Vec bb( b, b + n ), xx( x, x + n );

@@ -82,6 +82,12 @@ void orc_solver_params_default( orc_solver_params* p );
 
 /* Returns 1 when converged (checkConvergence), 0 otherwise, -1 on GMRES breakdown (TNL throws).
  * history receives every setResidue value (up to history_cap). */
+/* ILU(0) factors of a square CSR matrix (rows of U in natural order, U's first entry of a row is
+ * the one divided by) and x = (LU)^-1 b when b and x are given; NULL arrays query the sizes. */
+int32_t orc_ilu0( int32_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, int32_t* nnzL,
+                  int32_t* nnzU, int32_t* Lrp, int32_t* Lc, double* Lv, int32_t* Urp, int32_t* Uc, double* Uv,
+                  const double* b, double* x );
+
 int32_t orc_solve( int32_t method, int32_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals,
                    const double* b, double* x, const orc_solver_params* params, int32_t* iterations, double* residue,
                    double* history, int32_t history_cap, int32_t* history_len );

@@ -53,7 +53,7 @@ def make_params(lib_default, **kw):
         if k == "gmres_variant" and isinstance(v, str):
             v = GMRES_VARIANTS[v]
         if k == "preconditioner" and isinstance(v, str):
-            v = {"none": 0, "diagonal": 1}[v]
+            v = {"none": 0, "diagonal": 1, "ilu0": 2}[v]
         setattr(p, k, v)
     return p
 
@@ -89,6 +89,9 @@ def lib():
         L.orc_dot_f32.restype = C.c_float
         L.orc_dot_f32.argtypes = [C.c_int64, _f32p, _f32p]
         L.orc_solver_params_default.argtypes = [C.c_void_p]
+        L.orc_ilu0.restype = C.c_int32
+        L.orc_ilu0.argtypes = [C.c_int32, _i32p, _i32p, _f64p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)] + \
+            [C.c_void_p] * 8
         L.orc_solve.restype = C.c_int32
         L.orc_solve.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_void_p,
                                 C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_void_p, C.c_int32,
@@ -136,6 +139,9 @@ def ref():
         L.ref_load_tnl.restype = C.c_int
         L.ref_load_tnl.argtypes = [C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_void_p, C.c_void_p,
                                    C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_int]
+        L.ref_ilu0.restype = C.c_int
+        L.ref_ilu0.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.POINTER(C.c_int), C.POINTER(C.c_int)] + \
+            [C.c_void_p] * 8
         _ref = L
     return _ref
 
@@ -331,6 +337,35 @@ def dot(a, b):
 
 
 # ------------------------------------------------------------------ solvers
+
+
+def ilu0(row_ptr, cols, vals, b=None, backend="oracle"):
+    """ILU(0) factors ((Lrp, Lc, Lv), (Urp, Uc, Uv)) and (LU)^-1 b.  The oracle keeps U's rows in
+    natural order; TNL stores them reversed (row N-1-i of its U is row i), so backend="ref"
+    returns them reordered to natural order."""
+    rp = np.ascontiguousarray(row_ptr, np.int32)
+    c = np.ascontiguousarray(cols, np.int32)
+    v = np.ascontiguousarray(vals, np.float64)
+    n = len(rp) - 1
+    f = lib().orc_ilu0 if backend == "oracle" else ref().ref_ilu0
+    nl, nu = C.c_int32(), C.c_int32()
+    f(n, rp, c, v, C.byref(nl), C.byref(nu), *([None] * 8))
+    Lrp, Urp = np.zeros(n + 1, np.int32), np.zeros(n + 1, np.int32)
+    Lc, Uc = np.zeros(max(nl.value, 1), np.int32), np.zeros(max(nu.value, 1), np.int32)
+    Lv, Uv = np.zeros(max(nl.value, 1)), np.zeros(max(nu.value, 1))
+    x = np.zeros(n)
+    bb = None if b is None else np.ascontiguousarray(b, np.float64)
+    f(n, rp, c, v, C.byref(nl), C.byref(nu), Lrp.ctypes.data, Lc.ctypes.data, Lv.ctypes.data, Urp.ctypes.data,
+      Uc.ctypes.data, Uv.ctypes.data, None if bb is None else bb.ctypes.data, None if bb is None else x.ctypes.data)
+    Lc, Lv, Uc, Uv = Lc[:nl.value], Lv[:nl.value], Uc[:nu.value], Uv[:nu.value]
+    if backend != "oracle":
+        # TNL's U row k holds matrix row n-1-k
+        lens = np.diff(Urp)[::-1]
+        starts = Urp[:-1][::-1]
+        order = np.concatenate([np.arange(s, s + L) for s, L in zip(starts, lens)] or [np.zeros(0, np.int64)])
+        Uc, Uv = Uc[order], Uv[order]
+        Urp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    return (Lrp, Lc, Lv), (Urp, Uc, Uv), (None if b is None else x)
 
 
 def solve(method, row_ptr, cols, vals, b, x0=None, backend="oracle", threads=1, history_cap=100000, **params):

@@ -21,6 +21,7 @@
 #include <TNL/Solvers/Linear/GMRES.h>
 #include <TNL/Solvers/Linear/TFQMR.h>
 #include <TNL/Solvers/Linear/Preconditioners/Diagonal.h>
+#include <TNL/Solvers/Linear/Preconditioners/ILU0.h>
 
 #include <algorithm>
 #include <chrono>
@@ -453,7 +454,9 @@ int ref_solve( int method, int n, const int* rp, const int* cols, const double* 
    if( fd >= 0 )
       close( fd );
    using Pre = Solvers::Linear::Preconditioners::Diagonal< CsrMatrix >;
+   using Ilu = Solvers::Linear::Preconditioners::ILU0< CsrMatrix >;
    auto pre = std::make_shared< Pre >();
+   auto ilu = std::make_shared< Ilu >();
    int rc = 0;
    auto run = [ & ]( auto& solver )
    {
@@ -461,6 +464,10 @@ int ref_solve( int method, int n, const int* rp, const int* cols, const double* 
       if( p->preconditioner == 1 ) {
          pre->update( A );
          solver.setPreconditioner( pre );
+      }
+      else if( p->preconditioner == 2 ) {
+         ilu->update( A );
+         solver.setPreconditioner( ilu );
       }
       solver.setMaxIterations( p->max_iterations );
       solver.setMinIterations( p->min_iterations );
@@ -598,6 +605,46 @@ int ref_load_tnl( int variant, int colmajor, int single, const char* path, const
       std::snprintf( err, errlen, "%s", e.what() );
       return -1;
    }
+}
+
+// TNL's ILU0< CSR >::update of a CSR matrix: the factors as CSR arrays (rows of U in TNL's
+// reversed order, as stored), then solve( b, x ).  Pass NULL arrays to query the sizes.
+int ref_ilu0( int n, const int* rp, const int* cols, const double* vals, int* nnzL, int* nnzU, int* Lrp, int* Lc,
+              double* Lv, int* Urp, int* Uc, double* Uv, const double* b, double* x )
+{
+   struct Peek : Solvers::Linear::Preconditioners::ILU0< CsrMatrix >
+   {
+      using Solvers::Linear::Preconditioners::ILU0< CsrMatrix >::L;
+      using Solvers::Linear::Preconditioners::ILU0< CsrMatrix >::U;
+   };
+   auto A = std::make_shared< CsrMatrix >();
+   fillCSR( *A, n, n, rp, cols, vals );
+   Peek ilu;
+   ilu.update( A );
+   *nnzL = ilu.L.getValues().getSize();
+   *nnzU = ilu.U.getValues().getSize();
+   auto dump = []( const auto& M, int* p, int* c, double* v ) {
+      if( ! p )
+         return;
+      const auto& off = M.getSegments().getOffsets();
+      for( int i = 0; i <= M.getRows(); i++ )
+         p[ i ] = off[ i ];
+      for( int k = 0; k < M.getValues().getSize(); k++ ) {
+         c[ k ] = M.getColumnIndexes()[ k ];
+         v[ k ] = M.getValues()[ k ];
+      }
+   };
+   dump( ilu.L, Lrp, Lc, Lv );
+   dump( ilu.U, Urp, Uc, Uv );
+   if( b && x ) {
+      DVec bv( n ), xv( n );
+      for( int i = 0; i < n; i++ )
+         bv[ i ] = b[ i ];
+      ilu.solve( bv.getConstView(), xv.getView() );
+      for( int i = 0; i < n; i++ )
+         x[ i ] = xv[ i ];
+   }
+   return 0;
 }
 
 }  // extern "C"

@@ -8,6 +8,7 @@
 #include <TNL/Matrices/MatrixWriter.h>
 #include <TNL/Solvers/Linear/CG.h>
 #include <TNL/Solvers/Linear/GMRES.h>
+#include <TNL/Solvers/LinearSolverTypeResolver.h>
 
 #include <cmath>
 #include <cstdio>
@@ -230,6 +231,22 @@ int main()
       std::printf( "GMRES converged=%d iterations=%d residue=%.3e\n", (int) ok2, gmres.getIterations(),
                    gmres.getResidue() );
       EXPECT_EQ( ok2, true );
+
+      // run-time choice by name (LinearSolverTypeResolver.h): BiCGStab with ILU(0)
+      auto solver = Solvers::getLinearSolver< Matrix >( "bicgstab" );
+      auto pre = Solvers::getPreconditioner< Matrix >( "ilu0" );
+      EXPECT_EQ( solver != nullptr && pre != nullptr, true );
+      EXPECT_EQ( Solvers::getLinearSolver< Matrix >( "sor" ) == nullptr, true );
+      solver->setMatrix( A );
+      solver->setPreconditioner( pre );
+      solver->setConvergenceResidue( 1e-8 );
+      solver->setMaxIterations( 10000 );
+      Vector x3( n, 0.0 );
+      const bool ok3 = solver->solve( b, x3 );
+      std::printf( "BiCGStab+ILU0 converged=%d iterations=%d residue=%.3e\n", (int) ok3, solver->getIterations(),
+                   solver->getResidue() );
+      EXPECT_EQ( ok3, true );
+      EXPECT_EQ( solver->getIterations() < cg.getIterations(), true );
    }
    // Matrix Market: readMtx into a SlicedEllpack matrix on the device, writeMtx back
    {

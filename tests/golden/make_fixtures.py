@@ -189,6 +189,13 @@ def write_solver_fixtures():
     runs += [("lap2d_24", "tfqmr", {}), ("lap2d_24", "tfqmr", {"preconditioner": 1}),
              ("convdiff_8_pe10", "tfqmr", {"preconditioner": 1, "convergence_residue": 1e-10}),
              ("irr_2000", "tfqmr", {"preconditioner": 1, "convergence_residue": 1e-10, "min_iterations": 3})]
+    # ILU(0)
+    runs += [("lap2d_24", "cg", {"preconditioner": 2}), ("lap2d_24", "bicgstab", {"preconditioner": 2}),
+             ("lap2d_24", "gmres", {"gmres_variant": 4, "preconditioner": 2}),
+             ("convdiff_8_pe10", "gmres", {"gmres_variant": 3, "preconditioner": 2, "restarting_min": 30,
+                                           "restarting_max": 30, "convergence_residue": 1e-10}),
+             ("convdiff_8_pe10", "tfqmr", {"preconditioner": 2, "convergence_residue": 1e-10}),
+             ("irr_2000", "bicgstab", {"preconditioner": 2, "convergence_residue": 1e-10})]
     for name, (rp, c, v) in systems.items():
         sol[f"{name}/row_ptr"] = rp
         sol[f"{name}/cols"] = c

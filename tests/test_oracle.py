@@ -211,3 +211,76 @@ def test_oracle_matches_live_reference_build(kind):
         ov, oc = a.convert(rp, c, v)
         np.testing.assert_array_equal(ov, vals[:len(ov)])
         np.testing.assert_array_equal(a.spmv(ov, oc, x), y)
+
+
+def _ilu_cases():
+    return {"lap2d_12": W.laplacian_2d(12), "irr_500": W.irregular(500, mean=8, sigma=1.0, max_len=64, seed=5),
+            "convdiff_5": W.convection_diffusion_3d(5, 10.0)}
+
+
+def test_ilu0_factors_match_tnl():
+    """The oracle's ILU(0) factors and solve equal TNL's ILU0 (Host) bit for bit."""
+    if not O.ref_available():
+        pytest.skip("TNL reference build not present")
+    for name, (rp, c, v) in _ilu_cases().items():
+        b = W.uniform_vector(len(rp) - 1, 1)
+        Lo, Uo, xo = O.ilu0(rp, c, v, b)
+        Lr, Ur, xr = O.ilu0(rp, c, v, b, backend="ref")
+        for got, want in zip(Lo + Uo + (xo,), Lr + Ur + (xr,)):
+            np.testing.assert_array_equal(got, want, err_msg=name)
+
+
+def _dense_lu(L, U, n):
+    Ld = np.eye(n)
+    Ud = np.zeros((n, n))
+    for (rp, c, v), D in ((L, Ld), (U, Ud)):
+        for i in range(n):
+            for k in range(rp[i], rp[i + 1]):
+                D[i, c[k]] += v[k]
+    return Ld, Ud
+
+
+def test_ilu0_reproduces_a_on_its_pattern():
+    """The defining property of ILU(0): (L U)_ij = a_ij wherever a_ij is stored, L unit lower
+    and U upper triangular with A's sparsity; (L U) x = b for the solve."""
+    for name, (rp, c, v) in _ilu_cases().items():
+        n = len(rp) - 1
+        b = W.uniform_vector(n, 2)
+        L, U, x = O.ilu0(rp, c, v, b)
+        Ld, Ud = _dense_lu(L, U, n)
+        assert np.all(np.tril(Ud, -1) == 0) and np.all(np.triu(Ld, 1) == 0), name
+        A = W.csr_to_dense(rp, c, v, n)
+        LU = Ld @ Ud
+        rows = np.repeat(np.arange(n), np.diff(rp))
+        scale = np.abs(Ld) @ np.abs(Ud)
+        err = np.abs(LU[rows, c] - A[rows, c]) / np.maximum(scale[rows, c], 1e-300)
+        assert err.max() < 1e-13, (name, err.max())
+        np.testing.assert_allclose(LU @ x, b, rtol=0, atol=1e-10 * np.abs(b).max(), err_msg=name)
+
+
+def test_ilu0_is_the_exact_lu_without_fill():
+    """A tridiagonal matrix has no fill-in: ILU(0) is its LU factorization (Thomas algorithm)."""
+    n = 50
+    rng = np.random.default_rng(3)
+    lo, di, up = rng.uniform(-1, 1, n), rng.uniform(3, 4, n), rng.uniform(-1, 1, n)
+    rp = [0]
+    cols, vals = [], []
+    for i in range(n):
+        for j, val in ((i - 1, lo[i]), (i, di[i]), (i + 1, up[i])):
+            if 0 <= j < n:
+                cols.append(j)
+                vals.append(val)
+        rp.append(len(cols))
+    rp, cols, vals = np.array(rp, np.int32), np.array(cols, np.int32), np.array(vals)
+    b = rng.uniform(-1, 1, n)
+    L, U, x = O.ilu0(rp, cols, vals, b)
+    # Thomas: u_0 = d_0, l_i = a_i / u_{i-1}, u_i = d_i - l_i c_{i-1}
+    u = np.zeros(n)
+    l = np.zeros(n)
+    u[0] = di[0]
+    for i in range(1, n):
+        l[i] = lo[i] / u[i - 1]
+        u[i] = di[i] - l[i] * up[i - 1]
+    np.testing.assert_allclose(L[2], l[1:], rtol=1e-15)
+    np.testing.assert_allclose(U[2][U[0][:-1]], u, rtol=1e-15)
+    np.testing.assert_allclose(x, np.linalg.solve(W.csr_to_dense(rp, cols, vals, n), b), rtol=1e-12)

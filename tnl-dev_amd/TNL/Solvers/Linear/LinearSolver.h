@@ -1,5 +1,5 @@
 // LinearSolver< Matrix > with IterativeSolver's setters (src/TNL/Solvers/Linear/LinearSolver.h:115-175,
-// src/TNL/Solvers/IterativeSolver.h:70-245) and the CG / BICGStab / GMRES front classes, over
+// src/TNL/Solvers/IterativeSolver.h:70-245) and the CG / BICGStab / GMRES / TFQMR front classes, over
 // the tnlhip solver entry points.  solve() returns checkConvergence() like TNL's.
 #pragma once
 
@@ -10,13 +10,16 @@
 #include "../../Config/ParameterContainer.h"
 #include "../../Containers/Vector.h"
 #include "Preconditioners/Diagonal.h"
+#include "Preconditioners/ILU0.h"
 
 namespace TNL {
 namespace Solvers {
 namespace Linear {
 
-template< typename Matrix, int Method >
-class LinearSolverBase
+// The polymorphic base every solver shares (what getLinearSolver returns); the method is fixed
+// at construction.
+template< typename Matrix >
+class LinearSolver
 {
 public:
    using MatrixType = Matrix;
@@ -24,14 +27,14 @@ public:
    using PreconditionerPointer = std::shared_ptr< Preconditioners::Preconditioner< Matrix > >;
    using VectorType = Containers::Vector< typename Matrix::VectorReal, Devices::Hip, int >;
 
-   LinearSolverBase()
+   explicit LinearSolver( int method )
    {
-      detail::check( tnlhip_solver_create( Method, &handle ) );
+      detail::check( tnlhip_solver_create( method, &handle ) );
       tnlhip_solver_params_default( &params );
    }
-   virtual ~LinearSolverBase() { tnlhip_solver_destroy( handle ); }
-   LinearSolverBase( const LinearSolverBase& ) = delete;
-   LinearSolverBase& operator=( const LinearSolverBase& ) = delete;
+   virtual ~LinearSolver() { tnlhip_solver_destroy( handle ); }
+   LinearSolver( const LinearSolver& ) = delete;
+   LinearSolver& operator=( const LinearSolver& ) = delete;
 
    void setMatrix( const MatrixPointer& m )
    {
@@ -103,6 +106,13 @@ protected:
    tnlhip_solver_params params;
    MatrixPointer matrix;
    PreconditionerPointer preconditioner;
+};
+
+template< typename Matrix, int Method >
+class LinearSolverBase : public LinearSolver< Matrix >
+{
+public:
+   LinearSolverBase() : LinearSolver< Matrix >( Method ) {}
 };
 
 template< typename Matrix >

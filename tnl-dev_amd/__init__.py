@@ -442,6 +442,13 @@ def exclusive_scan(inp, out, total=True, stream=None):
 
 class Diagonal:
     """Preconditioners::Diagonal (Jacobi), Solvers/Linear/Preconditioners/Diagonal.h."""
+    kind = 1
+
+
+class ILU0:
+    """Preconditioners::ILU0, Solvers/Linear/Preconditioners/ILU0.h: factors with A's sparsity,
+    factorized and applied level by level on the device."""
+    kind = 2
 
 
 class LinearSolver:
@@ -469,7 +476,7 @@ class LinearSolver:
         _check(lib.tnlhip_solver_set_matrix(self._h, matrix._h))
 
     def setPreconditioner(self, preconditioner):
-        self.params.preconditioner = 1 if preconditioner is not None else 0
+        self.params.preconditioner = 0 if preconditioner is None else getattr(preconditioner, "kind", 1)
 
     def setMaxIterations(self, n):
         self.params.max_iterations = int(n)
@@ -490,7 +497,7 @@ class LinearSolver:
             if k == "gmres_variant" and isinstance(v, str):
                 v = GMRES_VARIANTS[v]
             if k == "preconditioner" and isinstance(v, str):
-                v = {"none": 0, "diagonal": 1}[v]
+                v = {"none": 0, "diagonal": 1, "ilu0": 2}[v]
             setattr(self.params, k, v)
         return True
 
@@ -721,5 +728,11 @@ def server_footprint(rows, nnz, max_row, nrhs):
 
 
 def getLinearSolver(name):
-    """LinearSolverTypeResolver::getLinearSolver by name (Solvers/LinearSolverTypeResolver.h:36-150)."""
+    """LinearSolverTypeResolver::getLinearSolver by name (Solvers/LinearSolverTypeResolver.h:59-90)."""
     return {"cg": CG, "bicgstab": BICGStab, "gmres": GMRES, "tfqmr": TFQMR}[name]()
+
+
+def getPreconditioner(name):
+    """LinearSolverTypeResolver::getPreconditioner by name (Solvers/LinearSolverTypeResolver.h:92-115):
+    None for "none"."""
+    return {"none": lambda: None, "diagonal": Diagonal, "ilu0": ILU0}[name]()

@@ -177,6 +177,37 @@ __device__ __forceinline__ float mulAddExact( float acc, float a, float b )
    return acc + a * b;
 }
 
+// Device-flag gate of the solver kernels.
+// Gate: 0 always run; 1 run while !done; 2 run while !done && aux; 3 run while !done && !aux
+struct Gate
+{
+   const int32_t* flags;
+   int mode;
+   __device__ __forceinline__ bool open() const
+   {
+      if( mode == 0 )
+         return true;
+      const int32_t done = __builtin_nontemporal_load( flags );
+      if( done )
+         return false;
+      if( mode == 1 )
+         return true;
+      const int32_t aux = __builtin_nontemporal_load( flags + 1 );
+      return mode == 2 ? aux != 0 : aux == 0;
+   }
+};
+
+__device__ __forceinline__ double mulSubExact( double acc, double a, double b )
+{
+#pragma clang fp contract( off )
+   return acc - a * b;
+}
+__device__ __forceinline__ float mulSubExact( float acc, float a, float b )
+{
+#pragma clang fp contract( off )
+   return acc - a * b;
+}
+
 // ------------------------------------------------------------------ wavefront reductions
 
 constexpr int kWave = 64;

@@ -1,5 +1,5 @@
-// Krylov solvers CG, BiCGStab and GMRES (CGS / CGSR / MGS / MGSR / CWY) with the Diagonal
-// (Jacobi) preconditioner, following TNL's algorithms step for step:
+// Krylov solvers CG, BiCGStab, TFQMR and GMRES (CGS / CGSR / MGS / MGSR / CWY) with the Diagonal
+// (Jacobi) or ILU(0) preconditioner, following TNL's algorithms step for step:
 //   Solvers/Linear/CG.hpp:17-107, BICGStab.hpp:39-147, GMRES.hpp:79-801,
 //   Preconditioners/Diagonal.hpp:22-47, stopping rules Solvers/IterativeSolver.hpp:103-150.
 //
@@ -12,7 +12,9 @@
 // only what TNL's host decides across GMRES restart cycles (the adaptive restart length).
 #include "blas.hpp"
 
+#include <memory>
 #include <type_traits>
+#include "ilu.hpp"
 #include "matrix.hpp"
 
 #include <cmath>
@@ -58,24 +60,6 @@ __device__ __forceinline__ void setResidue( State& s, double r, double* hist, in
    s.histLen++;
 }
 
-// Gate: 0 always run; 1 run while !done; 2 run while !done && aux; 3 run while !done && !aux
-struct Gate
-{
-   const int32_t* flags;
-   int mode;
-   __device__ __forceinline__ bool open() const
-   {
-      if( mode == 0 )
-         return true;
-      const int32_t done = __builtin_nontemporal_load( flags );
-      if( done )
-         return false;
-      if( mode == 1 )
-         return true;
-      const int32_t aux = __builtin_nontemporal_load( flags + 1 );
-      return mode == 2 ? aux != 0 : aux == 0;
-   }
-};
 
 // ------------------------------------------------------------------ generic kernels
 
@@ -267,6 +251,7 @@ struct SolverImpl
    Workspace ws;
    DeviceArray< uint8_t > diag;
    bool diagValid = false;
+   std::unique_ptr< Ilu0 > ilu;  // preconditioner 2, built on first use
    DeviceArray< State > state;
    DeviceArray< double > hist;
    int64_t histCap = 0;
@@ -296,6 +281,12 @@ namespace {
 
 void ensureDiagonal( SolverImpl& S )
 {
+   if( S.params.preconditioner == 2 && ! S.ilu ) {
+      // ILU0::update (ILU0.hpp:17-117), in the vector type of the solve
+      auto f = std::make_unique< Ilu0 >();
+      f->build( *S.A, S.vdtype == TNLHIP_F64 ? TNLHIP_F64 : TNLHIP_F32, S.stream );
+      S.ilu = std::move( f );
+   }
    if( S.params.preconditioner != 1 || S.diagValid )
       return;
    S.diag.resize( S.n * (int64_t) S.vbytes );
@@ -356,6 +347,8 @@ void matvec( SolverImpl& S, const void* x, void* y, Gate gate )
 {
    spmvLaunchGated( *S.A, x, y, 1.0, 0.0, S.params.preconditioner == 1 ? S.diag.ptr : nullptr, gate.flags, gate.mode,
                     S.stream );
+   if( S.params.preconditioner == 2 )
+      S.ilu->solve( y, y, gate, S.stream );
 }
 
 // r = M^{-1} ( b - A x ) (compute_residue of BiCGStab / GMRES) or, with precondition =
@@ -367,6 +360,8 @@ void residual( SolverImpl& S, const void* b, const void* x, T* r, Gate gate, boo
    spmvLaunchGated( *S.A, x, r, -1.0, 1.0, nullptr, gate.flags, gate.mode, S.stream );
    if( precondition && S.params.preconditioner == 1 )
       launchElementwise( S.n, OpDivide< T >{ {}, r, r, (const T*) S.diag.ptr }, gate, S.stream );
+   if( precondition && S.params.preconditioner == 2 )
+      S.ilu->solve( r, r, gate, S.stream );
 }
 
 // normb = || M^{-1} b || (1 when zero), written to state.normb
@@ -388,6 +383,10 @@ void computeNormb( SolverImpl& S, const void* b, T* tmp )
    if( S.params.preconditioner == 1 ) {
       launchElementwise( S.n, OpDivide< T >{ {}, tmp, (const T*) b, (const T*) S.diag.ptr }, Gate{ nullptr, 0 },
                          S.stream );
+      src = tmp;
+   }
+   if( S.params.preconditioner == 2 ) {
+      S.ilu->solve( b, tmp, Gate{ nullptr, 0 }, S.stream );
       src = tmp;
    }
    launchReduce< 1 >( S.n, OpSquares< T >{ {}, src }, FinNormb< T >{ S.state.ptr }, S.ws.slot( 0 ), Gate{ nullptr, 0 },
@@ -579,8 +578,74 @@ struct CgDirectionOp
 };
 
 template< typename T >
+struct CgInitFromZOp : NoScalars
+{
+   // p = z (z = M^-1 r given); acc += r * z
+   T* p;
+   const T* z;
+   const T* r;
+   __device__ void operator()( int64_t i, double ( &acc )[ 1 ] ) const
+   {
+      const T zi = z[ i ];
+      p[ i ] = zi;
+      acc[ 0 ] += (double) r[ i ] * (double) zi;
+   }
+};
+
+template< typename T >
+struct CgStepOp
+{
+   // x += alpha p; r -= alpha Ap
+   T* x;
+   T* r;
+   const T* p;
+   const T* Ap;
+   const State* st;
+   double alpha;
+   __device__ void load() { alpha = st->alpha; }
+   __device__ void operator()( int64_t i ) const
+   {
+      x[ i ] = (T) ( (double) x[ i ] + alpha * (double) p[ i ] );
+      r[ i ] = (T) ( (double) r[ i ] - alpha * (double) Ap[ i ] );
+   }
+};
+
+// CG with a preconditioner applied as a separate solve (ILU(0)): z = M^-1 r after every
+// residual update (CG.hpp:44-46, 88-90)
+template< typename T >
+void solveCGSeparate( SolverImpl& S, const void* b, void* xv )
+{
+   T* r = (T*) S.vec( 0 );
+   T* p = (T*) S.vec( 1 );
+   T* Ap = (T*) S.vec( 2 );
+   T* z = (T*) S.vec( 3 );
+   T* x = (T*) xv;
+   const Gate always{ nullptr, 0 }, running{ S.state.ptr->flags, 1 };
+   const Stop stop = S.stopRule();
+   computeNormb< T >( S, b, z );
+   residual< T >( S, b, x, r, always, false );
+   S.ilu->solve( r, z, always, S.stream );
+   launchReduce< 1 >( S.n, CgInitFromZOp< T >{ {}, p, z, r }, FinCgInit{ S.state.ptr, S.hist.ptr, S.histCap, stop },
+                      S.ws.slot( 0 ), always, S.stream );
+   runChunks( S, [ & ] {
+      spmvLaunchGated( *S.A, p, Ap, 1.0, 0.0, nullptr, running.flags, running.mode, S.stream );
+      launchReduce< 1 >( S.n, OpDot< T >{ {}, Ap, p }, FinCgAlpha{ S.state.ptr, S.hist.ptr, S.histCap },
+                         S.ws.slot( 1 ), running, S.stream );
+      launchElementwise( S.n, CgStepOp< T >{ x, r, p, Ap, S.state.ptr, 0 }, running, S.stream );
+      S.ilu->solve( r, z, running, S.stream );
+      launchReduce< 1 >( S.n, OpDot< T >{ {}, r, z }, FinCgBeta{ S.state.ptr, S.hist.ptr, S.histCap, stop },
+                         S.ws.slot( 2 ), running, S.stream );
+      launchElementwise( S.n, CgDirectionOp< T >{ p, z, S.state.ptr, 0 }, running, S.stream );
+   } );
+}
+
+template< typename T >
 void solveCG( SolverImpl& S, const void* b, void* xv )
 {
+   if( S.params.preconditioner == 2 ) {
+      solveCGSeparate< T >( S, b, xv );
+      return;
+   }
    // vectors: r, p, Ap, z
    T* r = (T*) S.vec( 0 );
    T* p = (T*) S.vec( 1 );
@@ -1972,6 +2037,7 @@ int tnlhip_solver_set_matrix( tnlhip_solver s, tnlhip_matrix A )
       TNLHIP_REQUIRE( s && A, "NULL argument" );
       s->A = reinterpret_cast< Matrix* >( A );
       s->diagValid = false;
+      s->ilu.reset();
    } );
 }
 
@@ -1979,12 +2045,14 @@ int tnlhip_solver_set_params( tnlhip_solver s, const tnlhip_solver_params* p )
 {
    return guarded( [ & ] {
       TNLHIP_REQUIRE( s && p, "NULL argument" );
-      TNLHIP_REQUIRE( p->preconditioner == 0 || p->preconditioner == 1, "unknown preconditioner" );
+      TNLHIP_REQUIRE( p->preconditioner >= 0 && p->preconditioner <= 2, "unknown preconditioner" );
       TNLHIP_REQUIRE( p->gmres_variant >= 0 && p->gmres_variant <= 4, "unknown GMRES variant" );
       const bool pre = p->preconditioner != s->params.preconditioner;
       s->params = *p;
-      if( pre )
+      if( pre ) {
          s->diagValid = false;
+         s->ilu.reset();
+      }
    } );
 }
 
