@@ -85,7 +85,7 @@ typedef struct
    int64_t allocated;     /* elements held in HBM (BiEllpack: groupPointers[last], not 32x) */
    int64_t nonzeros;      /* getNonzeroElementsCount() */
    int32_t lanes_per_row; /* CSR light / hybrid / vector kernel parameter (0 otherwise) */
-   int32_t reserved;
+   int32_t symmetric;     /* 1 for a SymmetricMatrix (the lower triangle is stored) */
 } tnlhip_matrix_info;
 
 /* SparseMatrix<Real, Devices::Hip, int, GeneralMatrix, Segments> A;  A = csr;
@@ -98,6 +98,16 @@ typedef struct
 int tnlhip_matrix_create_csr( int format, int value_dtype, int32_t rows, int32_t columns, const int32_t* row_pointers,
                               const int32_t* column_indexes, const double* values, int host_pointers,
                               tnlhip_matrix* out );
+
+/* SparseMatrix<Real, Devices::Hip, int, SymmetricMatrix, Segments> A;  A = csr: the lower
+ *   triangle, diagonal included, of a square matrix (an entry above the diagonal is an error).
+ *   Products scale y by beta when beta != 1, then add alpha a_rc x_r to y_c for every stored
+ *   a_rc with c < r and the row sums alpha sum_c a_rc x_c to y_r, with floating-point atomics
+ *   (SparseMatrixView.hpp:390-491): y matches TNL's to rounding, not bit for bit.  Conversions,
+ *   solvers (with Jacobi), save / load keep the matrix type; ILU(0) refuses it. */
+int tnlhip_matrix_create_symmetric_csr( int format, int value_dtype, int32_t rows, const int32_t* row_pointers,
+                                        const int32_t* column_indexes, const double* values, int host_pointers,
+                                        tnlhip_matrix* out );
 
 /* B = A across formats / value types on the device (SparseMatrix.hpp:1093-1107). */
 int tnlhip_matrix_convert( tnlhip_matrix source, int format, int value_dtype, tnlhip_matrix* out );
@@ -141,6 +151,8 @@ typedef struct
    int32_t rows;
    int32_t columns;
    int64_t storage_size; /* length of the stored values array */
+   int32_t symmetric;    /* 1 when the type names a Symmetric matrix */
+   int32_t reserved;
 } tnlhip_file_info;
 
 /* SparseMatrix::save( fileName ) (SparseMatrix.hpp:1266-1270).  f64 and f32 matrices only: TNL

@@ -16,7 +16,7 @@
  *   CSR-Adaptive blocks      src/TNL/Algorithms/Segments/Kernels/CSRAdaptiveKernel.hpp:95-164,
  *                            detail/CSRAdaptiveKernelParameters.h:16-112
  *   conversion               src/TNL/Matrices/SparseMatrix.hpp:1070-1180
- *   vectorProduct keepers    src/TNL/Matrices/SparseMatrixView.hpp:390-491
+ *   vectorProduct keepers    src/TNL/Matrices/SparseMatrixView.hpp:390-491 (general and symmetric)
  *   reduce (sequential)      src/TNL/Algorithms/detail/Reduction.hpp:37-80
  *   multireduce (sequential) src/TNL/Algorithms/Multireduction.hpp:30-100
  *   gemv (host)              src/TNL/Matrices/MatrixOperations.h:46-150
@@ -1675,6 +1675,33 @@ orc_solver_params_default( orc_solver_params* p )
    p->restarting_step_max = 3;
    p->bicgstab_exact_residue = 0;
    p->preconditioner = 0;
+}
+
+// SparseMatrixView::vectorProduct for MatrixType::isSymmetric() (SparseMatrixView.hpp:390-491):
+// outVector *= beta when beta != 1, then symmetricFetch adds the transposed entry of every
+// stored a_rc with c < r to y_c (AtomicOperations, sequential on one Host thread) and
+// keeperGeneral adds alpha * (row sum) to y_r.
+extern "C" void
+orc_symmetric_spmv( int32_t n, const int32_t* rp, const int32_t* cols, const double* vals, const double* x, double* y,
+                    double alpha, double beta, int32_t first, int32_t last )
+{
+   if( last == 0 )
+      last = n;
+   if( beta != 1.0 )
+      for( I i = 0; i < n; i++ )
+         y[ i ] *= beta;
+   for( I r = first; r < last; r++ ) {
+      double s = 0;
+      for( I g = rp[ r ]; g < rp[ r + 1 ]; g++ ) {
+         const I c = cols[ g ];
+         if( c == -1 )
+            continue;
+         if( c < r )
+            y[ c ] += alpha * vals[ g ] * x[ r ];
+         s = s + vals[ g ] * x[ c ];
+      }
+      y[ r ] += alpha * s;
+   }
 }
 
 extern "C" int32_t

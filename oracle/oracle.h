@@ -82,6 +82,12 @@ void orc_solver_params_default( orc_solver_params* p );
 
 /* Returns 1 when converged (checkConvergence), 0 otherwise, -1 on GMRES breakdown (TNL throws).
  * history receives every setResidue value (up to history_cap). */
+/* SymmetricMatrix product of a lower-triangle CSR matrix (TNL's Host order, one thread):
+ * y *= beta when beta != 1, then per row r in [first, last): y_c += (alpha a_rc) x_r for c < r,
+ * y_r += alpha * sum_c a_rc x_c. */
+void orc_symmetric_spmv( int32_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, const double* x,
+                         double* y, double alpha, double beta, int32_t first, int32_t last );
+
 /* ILU(0) factors of a square CSR matrix (rows of U in natural order, U's first entry of a row is
  * the one divided by) and x = (LU)^-1 b when b and x are given; NULL arrays query the sizes. */
 int32_t orc_ilu0( int32_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, int32_t* nnzL,

@@ -92,6 +92,9 @@ def lib():
         L.orc_ilu0.restype = C.c_int32
         L.orc_ilu0.argtypes = [C.c_int32, _i32p, _i32p, _f64p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)] + \
             [C.c_void_p] * 8
+        L.orc_symmetric_spmv.restype = None
+        L.orc_symmetric_spmv.argtypes = [C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
+                                         C.c_int32, C.c_int32]
         L.orc_solve.restype = C.c_int32
         L.orc_solve.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_void_p,
                                 C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_void_p, C.c_int32,
@@ -139,6 +142,9 @@ def ref():
         L.ref_load_tnl.restype = C.c_int
         L.ref_load_tnl.argtypes = [C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_void_p, C.c_void_p,
                                    C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_int]
+        L.ref_symmetric_spmv.restype = C.c_int
+        L.ref_symmetric_spmv.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
+                                         C.c_int, C.c_int]
         L.ref_ilu0.restype = C.c_int
         L.ref_ilu0.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.POINTER(C.c_int), C.POINTER(C.c_int)] + \
             [C.c_void_p] * 8
@@ -337,6 +343,18 @@ def dot(a, b):
 
 
 # ------------------------------------------------------------------ solvers
+
+
+def symmetric_spmv(row_ptr, cols, vals, x, y0=None, alpha=1.0, beta=0.0, first=0, last=0, backend="oracle"):
+    """The SymmetricMatrix product of a lower-triangle CSR matrix: y = beta y0 (scaled only when
+    beta != 1) plus alpha A x with A = L + L^T - D."""
+    rp = np.ascontiguousarray(row_ptr, np.int32)
+    n = len(rp) - 1
+    y = np.zeros(n) if y0 is None else np.array(y0, np.float64)
+    f = lib().orc_symmetric_spmv if backend == "oracle" else ref().ref_symmetric_spmv
+    f(n, rp, np.ascontiguousarray(cols, np.int32), np.ascontiguousarray(vals, np.float64),
+      np.ascontiguousarray(x, np.float64), y, float(alpha), float(beta), int(first), int(last))
+    return y
 
 
 def ilu0(row_ptr, cols, vals, b=None, backend="oracle"):

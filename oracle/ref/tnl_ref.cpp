@@ -647,4 +647,23 @@ int ref_ilu0( int n, const int* rp, const int* cols, const double* vals, int* nn
    return 0;
 }
 
+// SparseMatrix< double, Host, int, SymmetricMatrix, CSR >::vectorProduct on one thread, the
+// matrix given by its lower triangle in CSR.
+int ref_symmetric_spmv( int n, const int* rp, const int* cols, const double* vals, const double* x, double* y,
+                        double alpha, double beta, int first, int last )
+{
+   omp_set_num_threads( 1 );
+   Matrices::SparseMatrix< double, Host, int, Matrices::SymmetricMatrix, Seg::CSRDefault > A;
+   fillCSR( A, n, n, rp, cols, vals );
+   DVec xv( n ), yv( n );
+   for( int i = 0; i < n; i++ ) {
+      xv[ i ] = x[ i ];
+      yv[ i ] = y[ i ];
+   }
+   A.vectorProduct( xv, yv, alpha, beta, first, last );
+   for( int i = 0; i < n; i++ )
+      y[ i ] = yv[ i ];
+   return 0;
+}
+
 }  // extern "C"

@@ -304,6 +304,41 @@ int main()
       std::remove( "/tmp/tnlhip_front_test.tnl" );
       std::printf( "%s save/load\n", failures ? "FAIL" : "PASS" );
    }
+   // SymmetricMatrix: setElement above the diagonal lands on the mirror (SymmetricSparseMatrixTest.hpp:679-712)
+   {
+      using Sym = Matrices::SparseMatrix< double, Devices::Hip, int, Matrices::SymmetricMatrix, Seg::CSRDefault >;
+      Sym S( 4, 4 );
+      S.setRowCapacities( std::vector< int >{ 1, 1, 2, 2 } );
+      S.setElement( 0, 0, 1 );
+      S.setElement( 1, 1, 2 );
+      S.setElement( 1, 2, 3 );  // stored as (2, 1)
+      S.setElement( 3, 1, 4 );
+      S.setElement( 3, 2, 5 );
+      Containers::Vector< double, Devices::Hip, int > xs( 4, 2.0 ), ys( 4, 0.0 );
+      S.vectorProduct( xs, ys );
+      const auto yh = ys.toHost();
+      EXPECT_EQ( yh[ 0 ], 2.0 );
+      EXPECT_EQ( yh[ 1 ], 18.0 );
+      EXPECT_EQ( yh[ 2 ], 16.0 );
+      EXPECT_EQ( yh[ 3 ], 18.0 );
+      S.save( "/tmp/tnlhip_front_sym.tnl" );
+      Sym T;
+      T.load( "/tmp/tnlhip_front_sym.tnl" );
+      Containers::Vector< double, Devices::Hip, int > zs( 4, 0.0 );
+      T.vectorProduct( xs, zs );
+      EXPECT_EQ( zs.toHost()[ 1 ], 18.0 );
+      bool refused = false;
+      try {
+         Matrices::SparseMatrix< double, Devices::Hip, int, Matrices::GeneralMatrix, Seg::CSRDefault > G;
+         G.load( "/tmp/tnlhip_front_sym.tnl" );
+      }
+      catch( const std::exception& ) {
+         refused = true;
+      }
+      EXPECT_EQ( refused, true );
+      std::remove( "/tmp/tnlhip_front_sym.tnl" );
+      std::printf( "%s SymmetricMatrix\n", failures ? "FAIL" : "PASS" );
+   }
    std::printf( failures ? "FAILURES %d\n" : "ALL PASSED\n", failures );
    return failures ? 1 : 0;
 }

@@ -262,3 +262,46 @@ def test_matrix_market_read_into_every_format(tnl, dev, fmt, tmp_path):
         out = tmp_path / f"{name}.out.mtx"
         A.write_mtx(out)
         assert out.read_text() == str(d[f"{name}/written"]), name
+
+
+def _symmetric_random(n, seed):
+    """Lower triangle of A + A^T for an irregular A."""
+    rp, c, v = W.irregular(n, mean=10, sigma=1.0, max_len=60, seed=seed)
+    A = W.csr_to_dense(rp, c, v, n)
+    L = np.tril(A + A.T)
+    rows, cols = np.nonzero(L)
+    rp2 = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=n))]).astype(np.int32)
+    return rp2, cols.astype(np.int32), L[rows, cols]
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_symmetric_products(tnl, dev, fmt):
+    """SymmetricMatrix products (transposed entries added with atomics): TNL's unit-test values
+    exactly, and the oracle (TNL's Host order) to |dy_i| <= 1e-13 * (|alpha| sum_j |a_ij x_j| +
+    |beta y_i|) over the full matrix, for the keepers and a row range."""
+    for (rp, c, v), x, y in tnl_cases.symmetric_matrices():
+        A = tnl.SparseMatrix.symmetric_from_csr(rp, c, v, format=fmt)
+        assert A.symmetric
+        np.testing.assert_array_equal(run_spmv(tnl, A, x, dev), np.array(y, np.float64))
+    n = 700
+    rp, c, v = _symmetric_random(n, 13)
+    L = W.csr_to_dense(rp, c, v, n)
+    full = L + L.T - np.diag(np.diag(L))
+    x = W.uniform_vector(n, 7)
+    y0 = W.uniform_vector(n, 8)
+    A = tnl.SparseMatrix.symmetric_from_csr(rp, c, v, format=fmt)
+    for alpha, beta, first, last in ((1.0, 0.0, 0, 0), (-1.0, 1.0, 0, 0), (0.5, 2.0, 0, 0), (1.0, 0.0, 200, 450)):
+        print(f"symmetric {fmt} {alpha} {beta} {first} {last}", flush=True)
+        got = run_spmv(tnl, A, x, dev, y0, alpha, beta, first, last)
+        want = O.symmetric_spmv(rp, c, v, x, y0, alpha, beta, first, last)
+        scale = abs(alpha) * np.abs(full) @ np.abs(x) + abs(beta) * np.abs(y0)
+        assert np.all(np.abs(got - want) <= 1e-13 * scale + 1e-300), (fmt, alpha, beta, first, last)
+    A32 = tnl.SparseMatrix.symmetric_from_csr(rp, c, v, format=fmt, dtype="f32")
+    got32 = run_spmv(tnl, A32, x, dev)
+    assert np.all(np.abs(got32 - full @ x) <= 1e-5 * (np.abs(full) @ np.abs(x)))
+    bad_c = c.copy()
+    bad_c[np.argmax(np.diff(rp) > 0)] = n - 1
+    with pytest.raises(tnl.TnlHipError, match="lower triangle"):
+        tnl.SparseMatrix.symmetric_from_csr(rp, bad_c, v, format=fmt)
+    B = A.convert("SlicedEllpack")
+    assert B.symmetric

@@ -117,5 +117,18 @@ def test_round_trip_and_type_checks(tnl, dev, files, tmp_path):
         tnl.SparseMatrix.load(p, format="Ellpack")
     with pytest.raises(tnl.TnlHipError, match="object type does not match"):
         tnl.SparseMatrix.load(p, dtype="f32")
+    # a SymmetricMatrix keeps its type through the file ("Symmetric" in the type string)
+    n = 300
+    lrp = np.concatenate([[0, 1], np.arange(3, 2 * n, 2)]).astype(np.int32)
+    lc = np.concatenate([[0], np.stack([np.arange(n - 1), np.arange(1, n)], 1).ravel()]).astype(np.int32)
+    lv = np.concatenate([[4.0], np.tile([-1.0, 4.0], n - 1)])
+    xs = torch.linspace(-1.0, 1.0, n, dtype=torch.float64, device=dev)
+    S = tnl.SparseMatrix.symmetric_from_csr(lrp, lc, lv, format="BiEllpack")
+    ps = tmp_path / "sym.tnl"
+    S.save(ps)
+    assert tnl.file_inspect(ps)["symmetric"] and "Symmetric" in tnl.file_inspect(ps)["type"]
+    T = tnl.SparseMatrix.load(ps)
+    assert T.symmetric
+    np.testing.assert_array_equal(product(tnl, T, xs), product(tnl, S, xs))
     with pytest.raises(tnl.TnlHipError, match="bfloat16"):
         tnl.SparseMatrix.from_csr(rp, ci, v, columns=nc, format="SlicedEllpack", dtype="bf16").save(p)

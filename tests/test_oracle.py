@@ -284,3 +284,58 @@ def test_ilu0_is_the_exact_lu_without_fill():
     np.testing.assert_allclose(L[2], l[1:], rtol=1e-15)
     np.testing.assert_allclose(U[2][U[0][:-1]], u, rtol=1e-15)
     np.testing.assert_allclose(x, np.linalg.solve(W.csr_to_dense(rp, cols, vals, n), b), rtol=1e-12)
+
+
+def _lower(rp, c, v):
+    """The lower triangle (diagonal included) of a CSR matrix."""
+    keep = c <= np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    lens = np.add.reduceat(keep.astype(np.int64), rp[:-1]) * (np.diff(rp) > 0)
+    return np.concatenate([[0], np.cumsum(lens)]).astype(np.int32), c[keep], v[keep]
+
+
+def _symmetric_cases():
+    out = {"lap3d_8": W.laplacian_3d(8)}
+    rp, c, v = W.irregular(400, mean=10, sigma=1.0, max_len=60, seed=9)
+    A = W.csr_to_dense(rp, c, v, 400)
+    S = A + A.T
+    rows, cols = np.nonzero(S)
+    rp2 = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=400))]).astype(np.int32)
+    out["irr_sym_400"] = (rp2, cols.astype(np.int32), S[rows, cols])
+    return {k: _lower(*m) for k, m in out.items()}
+
+
+def test_symmetric_product_matches_tnl():
+    """The oracle's SymmetricMatrix product equals TNL's Host product (one thread) bit for bit,
+    for the keeper variants and row ranges."""
+    if not O.ref_available():
+        pytest.skip("TNL reference build not present")
+    for name, (rp, c, v) in _symmetric_cases().items():
+        n = len(rp) - 1
+        x = W.uniform_vector(n, 3)
+        y0 = W.uniform_vector(n, 4)
+        for alpha, beta, first, last in ((1.0, 0.0, 0, 0), (-1.0, 1.0, 0, 0), (0.5, 2.0, 0, 0), (1.0, 0.0, n // 3, n // 2)):
+            got = O.symmetric_spmv(rp, c, v, x, y0, alpha, beta, first, last)
+            want = O.symmetric_spmv(rp, c, v, x, y0, alpha, beta, first, last, backend="ref")
+            np.testing.assert_array_equal(got, want, err_msg=f"{name} {alpha} {beta} {first} {last}")
+
+
+def test_symmetric_product_is_the_full_product():
+    """L + L^T - D applied to x, to rounding; beta scales y0 (NaN in y0 survives beta = 0, as
+    TNL multiplies rather than clears)."""
+    for name, (rp, c, v) in _symmetric_cases().items():
+        n = len(rp) - 1
+        L = W.csr_to_dense(rp, c, v, n)
+        full = L + L.T - np.diag(np.diag(L))
+        x = W.uniform_vector(n, 5)
+        y0 = W.uniform_vector(n, 6)
+        got = O.symmetric_spmv(rp, c, v, x, y0, 0.5, 2.0)
+        scale = 0.5 * np.abs(full) @ np.abs(x) + 2.0 * np.abs(y0)
+        assert np.all(np.abs(got - (0.5 * full @ x + 2.0 * y0)) <= 1e-14 * scale), name
+    y0 = np.full(n, np.nan)
+    assert np.all(np.isnan(O.symmetric_spmv(rp, c, v, x, y0, 1.0, 0.0)))
+
+
+def test_symmetric_product_golden_values():
+    """TNL's own expected products of its symmetric-matrix unit test."""
+    for (rp, c, v), x, y in tnl_cases.symmetric_matrices():
+        np.testing.assert_array_equal(O.symmetric_spmv(rp, c, v, x), np.array(y, np.float64))

@@ -102,7 +102,8 @@ def retyped(data, old, new):
 
 def test_unsupported_types_are_refused(tnl, files, tmp_path):
     data = files["tiny_5x5/BiEllpack/cm/f64"].tobytes()
-    for old, new, match in (("General", "Symmetric", "only General matrices"),
+    assert tnl.file_inspect(write(tmp_path, retyped(data, "General", "Symmetric")))["symmetric"]
+    for old, new, match in (("General", "Hermitian", "only General and Symmetric matrices"),
                             (", int, General", ", long int, General", "unsupported Index type"),
                             ("BiEllpack<", "SlicedEllpackX<", "unknown segments type")):
         with pytest.raises(tnl.TnlHipError, match=match):

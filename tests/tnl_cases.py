@@ -103,3 +103,39 @@ def to_csr(case):
             cols[k] = 0
             vals[k] = 0.0
     return rp.astype(np.int32), cols, vals
+
+
+def symmetric_matrices():
+    """SymmetricSparseMatrixTest.hpp:679-851 (test_VectorProduct): the lower triangle (TNL's
+    setElement moves an entry above the diagonal to its mirror), the input vector and TNL's
+    expected product."""
+    def csr(n, entries):
+        entries = sorted((r, c, v) if c <= r else (c, r, v) for r, c, v in entries)
+        merged = {}
+        for r, c, v in entries:
+            merged[(r, c)] = v
+        rp = np.zeros(n + 1, np.int32)
+        for r, _ in merged:
+            rp[r + 1] += 1
+        rp = np.cumsum(rp).astype(np.int32)
+        cols = np.array([c for _, c in merged], np.int32)
+        vals = np.array(list(merged.values()), np.float64)
+        return rp, cols, vals
+
+    out = []
+    out.append((csr(4, [(0, 0, 1), (1, 1, 2), (2, 1, 3), (3, 1, 4), (3, 2, 5)]), np.full(4, 2.0),
+                [2, 18, 16, 18]))
+    out.append((csr(4, [(0, 0, 1), (1, 0, 2), (2, 0, 3), (2, 1, 6), (2, 2, 7), (3, 1, 8), (3, 3, 9)]),
+                np.full(4, 2.0), [12, 32, 32, 34]))
+    out.append((csr(4, [(0, 0, 1), (0, 1, 2), (0, 2, 3), (0, 3, 4), (1, 0, 2), (1, 1, 5), (2, 0, 3), (2, 2, 6),
+                        (3, 0, 4), (3, 3, 7)]), np.array([0.0, 1, 2, 3]), [20, 5, 12, 21]))
+    out.append((csr(8, [(0, 0, 1), (1, 1, 2), (2, 0, 3), (2, 2, 5), (3, 1, 6), (3, 3, 8), (4, 0, 9), (4, 2, 10),
+                        (4, 4, 11), (5, 1, 12), (5, 3, 13), (5, 5, 14), (6, 0, 15), (6, 2, 16), (6, 4, 17),
+                        (6, 6, 18), (7, 1, 19), (7, 3, 20), (7, 5, 21), (7, 7, 22)]),
+                np.array([1.0, 2, 1, 2, 1, 2, 1, 2]), [28, 78, 34, 94, 47, 120, 66, 164]))
+    out.append((csr(8, [(0, 0, 1), (1, 1, 2), (2, 2, 3), (3, 2, 4), (3, 3, 5), (4, 2, 6), (4, 3, 7), (4, 4, 8),
+                        (5, 2, 9), (5, 3, 10), (5, 4, 11), (5, 5, 12), (6, 6, 13), (7, 7, 14)]),
+                np.arange(1.0, 9.0),
+                [1, 4, 3 * 3 + 4 * 4 + 5 * 6 + 6 * 9, 3 * 4 + 4 * 5 + 5 * 7 + 6 * 10, 3 * 6 + 4 * 7 + 5 * 8 + 6 * 11,
+                 3 * 9 + 4 * 10 + 5 * 11 + 6 * 12, 7 * 13, 8 * 14]))
+    return out

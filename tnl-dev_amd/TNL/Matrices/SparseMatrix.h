@@ -8,6 +8,8 @@
 // solvers then run on the GPU.
 #pragma once
 
+#include <utility>
+
 #include <algorithm>
 #include <cstdint>
 #include <memory>
@@ -33,6 +35,12 @@ struct GeneralMatrix
    static constexpr bool isSymmetric() { return false; }
 };
 
+// Matrices/MatrixType.h: a symmetric matrix stores its lower triangle, diagonal included
+struct SymmetricMatrix
+{
+   static constexpr bool isSymmetric() { return true; }
+};
+
 template< typename Real,
           typename Device = Devices::Hip,
           typename Index = int,
@@ -43,7 +51,6 @@ class SparseMatrix
 {
    static_assert( std::is_same< Device, Devices::Hip >::value, "this SparseMatrix is the Devices::Hip one" );
    static_assert( std::is_same< Index, int >::value, "Index is int (int32)" );
-   static_assert( ! MatrixType::isSymmetric(), "GeneralMatrix only" );
 
 public:
    using RealType = Real;
@@ -53,6 +60,7 @@ public:
    using VectorReal = typename std::conditional< std::is_same< Real, double >::value, double, float >::type;
    using SegmentsTag = Segments< Device, Index, void >;
    static constexpr int format = SegmentsTag::format;
+   static constexpr bool symmetric = MatrixType::isSymmetric();
    static constexpr int valueDtype =
       std::is_same< Real, double >::value ? TNLHIP_F64 : std::is_same< Real, float >::value ? TNLHIP_F32 : TNLHIP_BF16;
 
@@ -91,6 +99,8 @@ public:
    void addElement( Index row, Index column, double value, double thisElementMultiplicator = 1.0 )
    {
       stage();
+      if( symmetric && row < column )
+         std::swap( row, column );  // SparseMatrixView.hpp: the upper entry is its mirror
       const Index b = offsets[ row ], e = b + used[ row ];
       Index k = b;
       while( k < e && cols[ k ] < column )
@@ -113,6 +123,8 @@ public:
    double getElement( Index row, Index column ) const
    {
       const_cast< SparseMatrix* >( this )->stage();
+      if( symmetric && row < column )
+         std::swap( row, column );
       for( Index k = offsets[ row ]; k < offsets[ row ] + used[ row ]; k++ )
          if( cols[ k ] == column )
             return vals[ k ];
@@ -127,14 +139,20 @@ public:
       release();
       this->rows = rows;
       this->columns = columns;
-      detail::check( tnlhip_matrix_create_csr( format, valueDtype, rows, columns, rowPointers, columnIndexes, values, 1,
-                                               &handle ) );
+      if( symmetric )
+         detail::check( tnlhip_matrix_create_symmetric_csr( format, valueDtype, rows, rowPointers, columnIndexes, values,
+                                                            1, &handle ) );
+      else
+         detail::check( tnlhip_matrix_create_csr( format, valueDtype, rows, columns, rowPointers, columnIndexes,
+                                                  values, 1, &handle ) );
       staged = false;
    }
 
    // MatrixReader::readMtx / MatrixWriter::writeMtx (Matrices/MatrixReader.hpp, MatrixWriter.hpp)
    void readMtxFile( const std::string& path )
    {
+      if( symmetric )
+         throw std::logic_error( "readMtx into a SymmetricMatrix is not supported" );
       release();
       detail::check( tnlhip_mtx_read( path.c_str(), format, valueDtype, &handle ) );
       tnlhip_matrix_info info;
@@ -157,6 +175,11 @@ public:
       detail::check( tnlhip_matrix_load( fileName.c_str(), format, valueDtype, TNLHIP_FILE_CUDA_LAYOUT, &handle ) );
       tnlhip_matrix_info info;
       detail::check( tnlhip_matrix_get_info( handle, &info ) );
+      if( ( info.symmetric != 0 ) != symmetric ) {
+         release();
+         throw std::runtime_error( "object type does not match: the file holds a " +
+                                   std::string( symmetric ? "General" : "Symmetric" ) + " matrix" );
+      }
       rows = info.rows;
       columns = info.columns;
       staged = false;
@@ -225,9 +248,15 @@ public:
          if( rows > 0 && offsets.size() != (size_t) rows + 1 )
             throw std::logic_error( "matrix dimensions were not set" );
          auto* self = const_cast< SparseMatrix* >( this );
-         detail::check( tnlhip_matrix_create_csr( format, valueDtype, rows, columns, offsets.data(),
-                                                  cols.empty() ? nullptr : cols.data(),
-                                                  vals.empty() ? nullptr : vals.data(), 1, &self->handle ) );
+         if( symmetric )
+            detail::check( tnlhip_matrix_create_symmetric_csr( format, valueDtype, rows, offsets.data(),
+                                                               cols.empty() ? nullptr : cols.data(),
+                                                               vals.empty() ? nullptr : vals.data(), 1,
+                                                               &self->handle ) );
+         else
+            detail::check( tnlhip_matrix_create_csr( format, valueDtype, rows, columns, offsets.data(),
+                                                     cols.empty() ? nullptr : cols.data(),
+                                                     vals.empty() ? nullptr : vals.data(), 1, &self->handle ) );
       }
       return handle;
    }

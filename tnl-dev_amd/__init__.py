@@ -38,6 +38,7 @@ EXPORTS = [
     "tnlhip_server_remove_matrix", "tnlhip_server_submit", "tnlhip_server_progress", "tnlhip_server_result",
     "tnlhip_server_stats", "tnlhip_autotune_csr", "tnlhip_mtx_read_csr", "tnlhip_mtx_read", "tnlhip_mtx_write",
     "tnlhip_mtx_write_csr", "tnlhip_matrix_save", "tnlhip_matrix_load", "tnlhip_file_inspect",
+    "tnlhip_matrix_create_symmetric_csr",
 ]
 # element order of TNL binary files: "cuda" = column-major (TNL's GPU matrices), "host" = row-major
 FILE_LAYOUTS = {"cuda": 0, "host": 1}
@@ -59,13 +60,13 @@ class BreakdownError(TnlHipError):
 
 class FileInfo(C.Structure):
     _fields_ = [("format", C.c_int32), ("value_dtype", C.c_int32), ("rows", C.c_int32), ("columns", C.c_int32),
-                ("storage_size", C.c_int64)]
+                ("storage_size", C.c_int64), ("symmetric", C.c_int32), ("reserved", C.c_int32)]
 
 
 class MatrixInfo(C.Structure):
     _fields_ = [("rows", C.c_int32), ("columns", C.c_int32), ("format", C.c_int32), ("value_dtype", C.c_int32),
                 ("storage_size", C.c_int64), ("allocated", C.c_int64), ("nonzeros", C.c_int64),
-                ("lanes_per_row", C.c_int32), ("reserved", C.c_int32)]
+                ("lanes_per_row", C.c_int32), ("symmetric", C.c_int32)]
 
 
 class SolverParams(C.Structure):
@@ -134,6 +135,7 @@ def _load():
         "tnlhip_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(i64)]),
         "tnlhip_matrix_create_csr": (C.c_int, [C.c_int, C.c_int, i32, i32, vp, vp, vp, C.c_int, C.POINTER(vp)]),
         "tnlhip_matrix_convert": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(vp)]),
+        "tnlhip_matrix_create_symmetric_csr": (C.c_int, [C.c_int, C.c_int, i32, vp, vp, vp, C.c_int, C.POINTER(vp)]),
         "tnlhip_matrix_destroy": (C.c_int, [vp]),
         "tnlhip_matrix_get_info": (C.c_int, [vp, C.POINTER(MatrixInfo)]),
         "tnlhip_matrix_get_metadata": (i64, [vp, C.c_char_p, vp]),
@@ -258,6 +260,25 @@ class SparseMatrix:
             _check(lib.tnlhip_matrix_create_csr(fmt, dt, rows, cols, _ptr(row_pointers), _ptr(column_indexes),
                                                 _ptr(values), 0, C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def symmetric_from_csr(cls, row_pointers, column_indexes, values, format="CSRScalar", dtype="f64"):
+        """SparseMatrix< ..., SymmetricMatrix, ... > = csr of the lower triangle (diagonal included);
+        host numpy arrays."""
+        import numpy as np
+        rp = np.ascontiguousarray(row_pointers, np.int32)
+        ci = np.ascontiguousarray(column_indexes, np.int32)
+        va = np.ascontiguousarray(values, np.float64)
+        h = C.c_void_p(0)
+        _check(lib.tnlhip_matrix_create_symmetric_csr(FORMATS[format] if isinstance(format, str) else format,
+                                                      _dtype_id(dtype), len(rp) - 1, rp.ctypes.data_as(C.c_void_p),
+                                                      ci.ctypes.data_as(C.c_void_p), va.ctypes.data_as(C.c_void_p),
+                                                      1, C.byref(h)))
+        return cls(h)
+
+    @property
+    def symmetric(self):
+        return bool(self.info.symmetric)
 
     @classmethod
     def from_mtx(cls, path, format="CSRScalar", dtype="f64"):
@@ -718,7 +739,7 @@ def file_inspect(path, layout="cuda"):
     _check(lib.tnlhip_file_inspect(str(path).encode(), FILE_LAYOUTS[layout], C.byref(info), t, 4096))
     return {"type": t.value.decode(), "format": FORMAT_NAMES[info.format],
             "dtype": {v: k for k, v in DTYPES.items()}[info.value_dtype], "rows": info.rows, "columns": info.columns,
-            "storage_size": info.storage_size}
+            "storage_size": info.storage_size, "symmetric": bool(info.symmetric)}
 
 
 def server_footprint(rows, nnz, max_row, nrhs):

@@ -296,6 +296,42 @@ int tnlhip_matrix_create_csr( int format, int value_dtype, int32_t rows, int32_t
    } );
 }
 
+int tnlhip_matrix_create_symmetric_csr( int format, int value_dtype, int32_t rows, const int32_t* row_pointers,
+                                        const int32_t* column_indexes, const double* values, int host_pointers,
+                                        tnlhip_matrix* out )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( out != nullptr && rows >= 0, "bad argument" );
+      TNLHIP_REQUIRE( row_pointers != nullptr || rows == 0, "row_pointers is NULL" );
+      std::vector< int32_t > rp( (size_t) rows + 1, 0 );
+      if( rows > 0 ) {
+         if( host_pointers )
+            std::memcpy( rp.data(), row_pointers, sizeof( int32_t ) * rp.size() );
+         else
+            TNLHIP_CHECK( hipMemcpy( rp.data(), row_pointers, sizeof( int32_t ) * rp.size(), hipMemcpyDeviceToHost ) );
+      }
+      const int64_t nnz = rp[ rows ];
+      std::vector< int32_t > ci( (size_t) std::max< int64_t >( nnz, 0 ) );
+      if( nnz > 0 ) {
+         if( host_pointers )
+            std::memcpy( ci.data(), column_indexes, sizeof( int32_t ) * ci.size() );
+         else
+            TNLHIP_CHECK( hipMemcpy( ci.data(), column_indexes, sizeof( int32_t ) * ci.size(), hipMemcpyDeviceToHost ) );
+      }
+      for( int32_t i = 0; i < rows; i++ )
+         for( int32_t k = rp[ i ]; k < rp[ i + 1 ] && k < nnz; k++ )
+            if( ci[ k ] > i )
+               throw Error( TNLHIP_ERROR_INVALID_ARGUMENT,
+                            "a symmetric matrix stores its lower triangle: entry (" + std::to_string( i ) + ", " +
+                               std::to_string( ci[ k ] ) + ") lies above the diagonal" );
+      tnlhip_matrix M = nullptr;
+      TNLHIP_CHECK_STATUS( tnlhip_matrix_create_csr( format, value_dtype, rows, rows, row_pointers, column_indexes,
+                                                     values, host_pointers, &M ) );
+      M->symmetric = true;
+      *out = M;
+   } );
+}
+
 int tnlhip_matrix_convert( tnlhip_matrix source, int format, int value_dtype, tnlhip_matrix* out )
 {
    return guarded( [ & ] {
@@ -314,6 +350,7 @@ int tnlhip_matrix_convert( tnlhip_matrix source, int format, int value_dtype, tn
       gatherToCsr( S, drp.ptr, dcol.ptr, dval.ptr, nullptr );
       convertFromCsrDevice( *M, drp.ptr, dcol.ptr, dval.ptr, nullptr );
       M->nnz = countStored( *M, nullptr );
+      M->symmetric = S.symmetric;
       *out = M.release();
    } );
 }
@@ -406,8 +443,8 @@ int tnlhip_matrix_save( tnlhip_matrix A, const char* path, int layout )
          TNLHIP_CHECK( hipMemcpy( cols.data(), A->columns.ptr, sizeof( int32_t ) * n, hipMemcpyDeviceToHost ) );
          TNLHIP_CHECK( hipMemcpy( vals.data(), A->values.ptr, vals.size(), hipMemcpyDeviceToHost ) );
       }
-      writeTnlFile( path, layout == TNLHIP_FILE_HOST_LAYOUT, A->format, A->dtype, A->rows, A->cols, A->meta, cols.data(),
-                    vals.data() );
+      writeTnlFile( path, layout == TNLHIP_FILE_HOST_LAYOUT, A->format, A->dtype, A->symmetric, A->rows, A->cols,
+                    A->meta, cols.data(), vals.data() );
    } );
 }
 
@@ -433,6 +470,7 @@ int tnlhip_matrix_load( const char* path, int format, int value_dtype, int layou
       M->values.upload( f.values.data(), (int64_t) f.values.size() );
       M->columns.upload( f.columnIndexes );
       M->nnz = countStored( *M, nullptr );
+      M->symmetric = f.symmetric;
       *out = M.release();
    } );
 }
@@ -449,6 +487,8 @@ int tnlhip_file_inspect( const char* path, int layout, tnlhip_file_info* info, c
          info->rows = f.rows;
          info->columns = f.columns;
          info->storage_size = f.fileStorage;
+         info->symmetric = f.symmetric ? 1 : 0;
+         info->reserved = 0;
       }
       if( type && type_len > 0 )
          std::snprintf( type, (size_t) type_len, "%s", f.type.c_str() );
@@ -472,7 +512,7 @@ int tnlhip_matrix_get_info( tnlhip_matrix A, tnlhip_matrix_info* info )
       info->allocated = A->meta.allocated;
       info->nonzeros = A->nnz;
       info->lanes_per_row = A->lanes;
-      info->reserved = 0;
+      info->symmetric = A->symmetric ? 1 : 0;
    } );
 }
 

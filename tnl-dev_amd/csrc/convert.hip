@@ -7,29 +7,6 @@
 
 namespace tnlhip {
 
-template< typename F >
-void withView( const Matrix& M, F&& f )
-{
-   switch( kindOf( M.format ) ) {
-      case KIND_CSR: f( CsrView{ M.offsets.ptr } ); break;
-      case KIND_ELLPACK: f( EllpackView{ M.meta.segmentSize, M.meta.alignedSize } ); break;
-      case KIND_SLICED: f( SlicedView{ M.sliceOffsets.ptr, M.sliceSizes.ptr } ); break;
-      case KIND_CHUNKED: f( ChunkedView{ M.rowToChunk.ptr, M.rowToSlice.ptr, M.slices.ptr } ); break;
-      case KIND_BIELLPACK: f( BiView{ M.rowPerm.ptr, M.groupPointers.ptr } ); break;
-   }
-}
-
-template< typename F >
-void withValueType( int dtype, F&& f )
-{
-   switch( dtype ) {
-      case TNLHIP_F64: f( (double*) nullptr ); break;
-      case TNLHIP_F32: f( (float*) nullptr ); break;
-      case TNLHIP_BF16: f( (bf16*) nullptr ); break;
-      default: throw Error( TNLHIP_ERROR_INVALID_ARGUMENT, "unknown value type" );
-   }
-}
-
 __device__ __forceinline__ double toDouble( double v ) { return v; }
 __device__ __forceinline__ double toDouble( float v ) { return (double) v; }
 __device__ __forceinline__ double toDouble( bf16 v ) { return (double) __bfloat162float( v ); }

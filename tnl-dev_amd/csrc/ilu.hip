@@ -184,6 +184,8 @@ void uploadValues( DeviceArray< uint8_t >& dst, const std::vector< double >& v, 
 void Ilu0::build( const Matrix& A, int vectorDtype, hipStream_t st )
 {
    TNLHIP_REQUIRE( A.rows == A.cols, "ILU(0) needs a square matrix" );
+   if( A.symmetric )
+      throw Error( TNLHIP_ERROR_UNSUPPORTED, "ILU(0) of a symmetric matrix (lower triangle stored) is not supported" );
    TNLHIP_REQUIRE( vectorDtype == TNLHIP_F64 || vectorDtype == TNLHIP_F32, "ILU(0) computes in double or float" );
    n = A.rows;
    dtype = vectorDtype;

@@ -92,6 +92,7 @@ struct Matrix
    int32_t xcdRemap = 0;      // row distribution of the padded kernels: 0 grid-stride, 1 XCD-remapped blocks
    int32_t pairRows = 0;      // padded kernels: two adjacent rows per lane with 8 / 16-byte loads
    int32_t blocksPerCu = 16;  // workgroups per CU the grid of the padded kernels is sized for
+   bool symmetric = false;    // SymmetricMatrix: the lower triangle is stored
    DeviceArray< uint8_t > diagonal;  // cached Jacobi diagonal (vector type)
    bool diagonalValid = false;
 
@@ -105,6 +106,9 @@ void spmvLaunch( const Matrix& A, const void* x, void* y, double alpha, double b
 // with gateMode 2, unless gate[1] != 0.  last < 0 means all rows.
 void spmvLaunchGated( const Matrix& A, const void* x, void* y, double alpha, double beta, const void* diag,
                       const int32_t* gate, int gateMode, hipStream_t stream, int32_t first = 0, int32_t last = -1 );
+// SymmetricMatrix product (symmetric.hip), used by spmvLaunchGated for symmetric matrices
+void spmvSymmetric( const Matrix& A, const void* x, void* y, double alpha, double beta, int32_t first, int32_t last,
+                    const void* diag, Gate gate, hipStream_t stream );
 void convertFromCsrDevice( Matrix& A, const int32_t* rowPtr, const int32_t* cols, const double* vals,
                            hipStream_t stream,
                            bool keepZeros = false );
@@ -133,6 +137,7 @@ struct TnlFileMatrix
 {
    std::string type;
    int format = TNLHIP_CSR_SCALAR, dtype = TNLHIP_F64;
+   bool symmetric = false;
    int32_t rows = 0, columns = 0;
    int64_t fileStorage = 0;               // length of the stored arrays in the file
    HostSegments meta;                     // checked, column-major
@@ -140,7 +145,7 @@ struct TnlFileMatrix
    std::vector< uint8_t > values;         // meta.allocated elements of dtype, device order
 };
 TnlFileMatrix readTnlFile( const std::string& path, bool host );
-void writeTnlFile( const std::string& path, bool host, int format, int dtype, int32_t rows, int32_t columns,
-                   const HostSegments& meta, const int32_t* columnIndexes, const void* values );
+void writeTnlFile( const std::string& path, bool host, int format, int dtype, bool symmetric, int32_t rows,
+                   int32_t columns, const HostSegments& meta, const int32_t* columnIndexes, const void* values );
 
 }  // namespace tnlhip

@@ -708,6 +708,10 @@ void spmvLaunchGated( const Matrix& A, const void* x, void* y, double alpha, dou
 {
    if( last < 0 )
       last = A.rows;
+   if( A.symmetric ) {
+      spmvSymmetric( A, x, y, alpha, beta, first, last, diag, Gate{ gate, gate ? gateMode : 0 }, stream );
+      return;
+   }
    switch( A.dtype ) {
       case TNLHIP_F64:
          launchTyped< double, double, double >( A, (const double*) x, (double*) y, alpha, beta, first, last,

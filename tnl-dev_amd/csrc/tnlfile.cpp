@@ -4,7 +4,7 @@
 // then the characters) and its fields in native byte order (Object.hpp:17-75, File.hpp:246-289).
 // An array is an object of type "Containers::Array< T, [any_device], int, [any_allocator] >"
 // followed by its int32 size and the raw elements (Containers/Array.hpp:563-601).  A sparse matrix
-// writes its type "Matrices::SparseMatrix< Real, Segments, [any_device], int, General,
+// writes its type "Matrices::SparseMatrix< Real, Segments, [any_device], int, General | Symmetric,
 // [any_allocator], [any_allocator] >", rows, columns, the values array, the column-index array
 // and then the segments' fields (SparseMatrixView.hpp:911-920, MatrixView.hpp:113-121):
 //   CSR             offsets                                       (CSRView.hpp:221-224)
@@ -79,10 +79,11 @@ std::string segmentsType( int format, bool host )
    }
 }
 
-std::string matrixType( int format, int dtype, bool host )
+std::string matrixType( int format, int dtype, bool host, bool symmetric )
 {
    return std::string( "Matrices::SparseMatrix< " ) + realName( dtype ) + ", " + segmentsType( format, host ) + ", "
-        + kAnyDevice + ", int, General, " + kAnyAllocator + ", " + kAnyAllocator + " >";
+        + kAnyDevice + ", int, " + ( symmetric ? "Symmetric" : "General" ) + ", " + kAnyAllocator + ", "
+        + kAnyAllocator + " >";
 }
 
 // "name< p1, p2< q >, p3 >" -> { "name", "p1", "p2< q >", "p3" }: parameters split at the
@@ -498,10 +499,10 @@ void readSegments( Reader& in, int kind, int32_t n, bool host, HostSegments& s, 
 
 }  // namespace
 
-void writeTnlFile( const std::string& path, bool host, int format, int dtype, int32_t rows, int32_t columns,
-                   const HostSegments& s, const int32_t* cols, const void* vals )
+void writeTnlFile( const std::string& path, bool host, int format, int dtype, bool symmetric, int32_t rows,
+                   int32_t columns, const HostSegments& s, const int32_t* cols, const void* vals )
 {
-   const std::string type = matrixType( format, dtype, host );
+   const std::string type = matrixType( format, dtype, host, symmetric );
    const size_t eb = dtypeBytes( dtype );
    const int64_t stored = fileStorage( s, host );
    // the element order of the file
@@ -584,8 +585,10 @@ TnlFileMatrix readTnlFile( const std::string& path, bool host )
       throw Error( TNLHIP_ERROR_UNSUPPORTED, "unsupported Real type " + p[ 1 ] + " in " + path );
    if( p[ 4 ] != "int" )
       throw Error( TNLHIP_ERROR_UNSUPPORTED, "unsupported Index type " + p[ 4 ] + " in " + path );
-   if( p[ 5 ] != "General" )
-      throw Error( TNLHIP_ERROR_UNSUPPORTED, "only General matrices are supported, not " + p[ 5 ] );
+   if( p[ 5 ] == "Symmetric" )
+      m.symmetric = true;
+   else if( p[ 5 ] != "General" )
+      throw Error( TNLHIP_ERROR_UNSUPPORTED, "only General and Symmetric matrices are supported, not " + p[ 5 ] );
    const std::vector< std::string > seg = splitType( p[ 2 ] );
    if( seg.size() < 3 || seg[ 1 ] != kAnyDevice || seg[ 2 ] != "int" )
       in.fail( "unknown segments type " + p[ 2 ] );
