@@ -244,7 +244,8 @@ typedef enum
    TNLHIP_CG = 0,       /* Solvers/Linear/CG.hpp */
    TNLHIP_BICGSTAB = 1, /* Solvers/Linear/BICGStab.hpp */
    TNLHIP_GMRES = 2,    /* Solvers/Linear/GMRES.hpp */
-   TNLHIP_TFQMR = 3     /* Solvers/Linear/TFQMR.hpp */
+   TNLHIP_TFQMR = 3,    /* Solvers/Linear/TFQMR.hpp */
+   TNLHIP_BICGSTABL = 4 /* Solvers/Linear/BICGStabL.hpp: BiCGStab(ell) */
 } tnlhip_method;
 
 typedef enum
@@ -273,6 +274,8 @@ typedef struct
    int32_t preconditioner; /* 0 none, 1 Preconditioners::Diagonal (Jacobi), 2 Preconditioners::ILU0 */
    int32_t use_graph;      /* 1: CG / BiCGStab iterations are captured once into a hipGraph and
                               replayed (16 per replay); 0: launched one by one */
+   int32_t bicgstab_ell;   /* BiCGStab(ell): Bi-CG steps per minimal-residual update, 1..16
+                              ("bicgstab-ell", default 1) */
 } tnlhip_solver_params;
 
 void tnlhip_solver_params_default( tnlhip_solver_params* params );

@@ -24,6 +24,7 @@
  *   CG                       src/TNL/Solvers/Linear/CG.hpp:17-107
  *   BiCGStab                 src/TNL/Solvers/Linear/BICGStab.hpp:39-147
  *   TFQMR                    src/TNL/Solvers/Linear/TFQMR.hpp:17-125
+ *   BiCGStab(ell)            src/TNL/Solvers/Linear/BICGStabL.hpp:37-284
  *   GMRES                    src/TNL/Solvers/Linear/GMRES.hpp:79-801
  *   Jacobi                   src/TNL/Solvers/Linear/Preconditioners/Diagonal.hpp:22-47
  *   ILU(0)                   src/TNL/Solvers/Linear/Preconditioners/ILU0.hpp:17-137,
@@ -1195,6 +1196,127 @@ bool solveBiCGStab( const Csr& A, const Precond& M, const Vec& b, Vec& x, Iter& 
    return it.checkConvergence();
 }
 
+// BiCGStab(ell), BICGStabL.hpp:37-253: ell Bi-CG steps on the columns of R and U, then the
+// minimal-residual part (MGS of R's columns, the small triangular systems for g_0 / g_2) and three
+// gemv updates.  Inside the Bi-CG part nextIteration() is called only to count (its check is
+// ignored); the stopping rule is checked once per outer iteration.  Returns -1 when the initial
+// residue is NaN (TNL throws).
+int solveBiCGStabL( const Csr& A, const Precond& M, const Vec& b, Vec& x, Iter& it, int ell, bool exact )
+{
+   const I n = A.n;
+   const size_t ld = n;
+   Vec R( ( ell + 1 ) * ld ), U( ( ell + 1 ) * ld ), r_ast( n ), Mt( n ), res( n );
+   Vec sigma( ell + 1 ), g0( ell + 1 ), g1( ell + 1 ), g2( ell + 1 ), T( ell * ell );
+   auto col = [ & ]( Vec& V, int j ) { return V.data() + j * ld; };
+   auto matvec = [ & ]( const double* src, double* dst )
+   {
+      Vec in( src, src + n ), out( n );
+      if( M.on ) {
+         A.spmv( in, Mt );
+         M.solve( Mt, out );
+      }
+      else
+         A.spmv( in, out );
+      std::copy( out.begin(), out.end(), dst );
+   };
+   auto compute_residue = [ & ]( double* r )
+   {
+      Vec out( n );
+      if( M.on ) {
+         A.spmv( x, Mt );
+         for( I i = 0; i < n; i++ )
+            Mt[ i ] = b[ i ] - Mt[ i ];
+         M.solve( Mt, out );
+      }
+      else {
+         A.spmv( x, out );
+         for( I i = 0; i < n; i++ )
+            out[ i ] = b[ i ] - out[ i ];
+      }
+      std::copy( out.begin(), out.end(), r );
+   };
+   auto dotp = [ & ]( const double* a, const double* c ) { return dot( Vec( a, a + n ), Vec( c, c + n ) ); };
+   double b_norm;
+   if( M.on ) {
+      M.solve( b, Mt );
+      b_norm = norm2( Mt );
+   }
+   else
+      b_norm = norm2( b );
+   if( b_norm == 0.0 )
+      b_norm = 1.0;
+   double* r0 = col( R, 0 );
+   double* u0 = col( U, 0 );
+   compute_residue( r0 );
+   sigma[ 0 ] = std::sqrt( dotp( r0, r0 ) );
+   if( std::isnan( sigma[ 0 ] ) )
+      return -1;
+   for( I i = 0; i < n; i++ )
+      r_ast[ i ] = r0[ i ] / sigma[ 0 ];
+   double rho_0 = 1.0, alpha = 0.0, omega = 1.0;
+   std::fill( u0, u0 + n, 0.0 );
+   it.it = 0;
+   it.setResidue( sigma[ 0 ] / b_norm );
+   while( it.checkNext() ) {
+      rho_0 = -omega * rho_0;
+      for( int j = 0; j < ell; j++ ) {
+         it.it++;
+         const double rho_1 = dotp( r_ast.data(), col( R, j ) );
+         const double beta = alpha * rho_1 / rho_0;
+         rho_0 = rho_1;
+         // U_[0:j] := R_[0:j] - beta U_[0:j]  (geam: 1.0 * R + (-beta) * U)
+         for( int c = 0; c <= j; c++ )
+            for( I i = 0; i < n; i++ )
+               col( U, c )[ i ] = 1.0 * col( R, c )[ i ] + -beta * col( U, c )[ i ];
+         matvec( col( U, j ), col( U, j + 1 ) );
+         const double gamma = dotp( r_ast.data(), col( U, j + 1 ) );
+         alpha = rho_0 / gamma;
+         // R_[0:j] := R_[0:j] - alpha U_[1:j+1]
+         for( int c = 0; c <= j; c++ )
+            for( I i = 0; i < n; i++ )
+               col( R, c )[ i ] = 1.0 * col( R, c )[ i ] + -alpha * col( U, c + 1 )[ i ];
+         matvec( col( R, j ), col( R, j + 1 ) );
+         for( I i = 0; i < n; i++ )
+            x[ i ] = x[ i ] + alpha * u0[ i ];
+      }
+      for( int j = 1; j <= ell; j++ ) {
+         double* rj = col( R, j );
+         for( int i = 1; i < j; i++ ) {
+            const double* ri = col( R, i );
+            const int ij = ( i - 1 ) + ( j - 1 ) * ell;
+            T[ ij ] = dotp( ri, rj ) / sigma[ i ];
+            for( I k = 0; k < n; k++ )
+               rj[ k ] = rj[ k ] - T[ ij ] * ri[ k ];
+         }
+         sigma[ j ] = dotp( rj, rj );
+         g1[ j ] = dotp( r0, rj ) / sigma[ j ];
+      }
+      omega = g1[ ell ];
+      for( int j = ell; j >= 1; j-- ) {
+         g0[ j ] = g1[ j ];
+         for( int i = j + 1; i <= ell; i++ )
+            g0[ j ] -= T[ ( j - 1 ) + ( i - 1 ) * ell ] * g0[ i ];
+      }
+      for( int j = 1; j < ell; j++ ) {
+         g2[ j ] = g0[ j + 1 ];
+         for( int i = j + 1; i < ell; i++ )
+            g2[ j ] += T[ ( j - 1 ) + ( i - 1 ) * ell ] * g0[ i + 1 ];
+      }
+      g2[ 0 ] = g0[ 1 ];
+      gemv< double >( n, ell, 1.0, R.data(), (I) ld, g2.data(), 1.0, x.data() );
+      gemv< double >( n, ell, -1.0, R.data() + ld, (I) ld, &g1[ 1 ], 1.0, r0 );
+      gemv< double >( n, ell, -1.0, U.data() + ld, (I) ld, &g0[ 1 ], 1.0, u0 );
+      if( exact ) {
+         compute_residue( res.data() );
+         sigma[ 0 ] = norm2( res );
+      }
+      else
+         sigma[ 0 ] = std::sqrt( dotp( r0, r0 ) );
+      it.setResidue( sigma[ 0 ] / b_norm );
+   }
+   return it.checkConvergence() ? 1 : 0;
+}
+
 bool solveTFQMR( const Csr& A, const Precond& M, const Vec& b, Vec& x, Iter& it )
 {
    // TFQMR.hpp:17-125: odd iterations take alpha = rho / (v, r_ast), even ones recompute
@@ -1675,6 +1797,7 @@ orc_solver_params_default( orc_solver_params* p )
    p->restarting_step_max = 3;
    p->bicgstab_exact_residue = 0;
    p->preconditioner = 0;
+   p->bicgstab_ell = 1;  // BICGStabL.hpp:21
 }
 
 // SparseMatrixView::vectorProduct for MatrixType::isSymmetric() (SparseMatrixView.hpp:390-491):
@@ -1758,6 +1881,8 @@ orc_solve( int32_t method, int32_t n, const int32_t* row_ptr, const int32_t* col
    }
    else if( method == 3 )
       rc = solveTFQMR( A, M, bb, xx, it ) ? 1 : 0;
+   else if( method == 4 )
+      rc = solveBiCGStabL( A, M, bb, xx, it, p->bicgstab_ell, p->bicgstab_exact_residue != 0 );
    std::copy( xx.begin(), xx.end(), x );
    *iterations = it.it;
    *residue = it.res;

@@ -62,8 +62,8 @@ int64_t orc_csr_adaptive_blocks( int32_t n, const int32_t* offsets, int32_t valu
 double orc_dot_f64( int64_t n, const double* a, const double* b );
 float orc_dot_f32( int64_t n, const float* a, const float* b );
 
-/* Krylov solvers on a host CSR matrix. method: 0 CG, 1 BiCGStab, 2 GMRES.
- * variant (GMRES): 0 CGS, 1 CGSR, 2 MGS, 3 MGSR, 4 CWY.  precond: 0 none, 1 diagonal. */
+/* Krylov solvers on a host CSR matrix. method: 0 CG, 1 BiCGStab, 2 GMRES, 3 TFQMR, 4 BiCGStab(ell).
+ * variant (GMRES): 0 CGS, 1 CGSR, 2 MGS, 3 MGSR, 4 CWY.  precond: 0 none, 1 diagonal, 2 ILU(0). */
 typedef struct {
    int32_t max_iterations;
    int32_t min_iterations;
@@ -76,6 +76,7 @@ typedef struct {
    int32_t restarting_step_max;
    int32_t bicgstab_exact_residue;
    int32_t preconditioner;
+   int32_t bicgstab_ell;
 } orc_solver_params;
 
 void orc_solver_params_default( orc_solver_params* p );

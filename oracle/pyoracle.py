@@ -14,7 +14,7 @@ ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libtnlref.so")
 
 KINDS = {"CSR": 0, "Ellpack": 1, "SlicedEllpack": 2, "ChunkedEllpack": 3, "BiEllpack": 4}
-METHODS = {"cg": 0, "bicgstab": 1, "gmres": 2, "tfqmr": 3}
+METHODS = {"cg": 0, "bicgstab": 1, "gmres": 2, "tfqmr": 3, "bicgstabl": 4}
 GMRES_VARIANTS = {"CGS": 0, "CGSR": 1, "MGS": 2, "MGSR": 3, "CWY": 4}
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -43,6 +43,7 @@ class SolverParams(C.Structure):
         ("restarting_step_max", C.c_int32),
         ("bicgstab_exact_residue", C.c_int32),
         ("preconditioner", C.c_int32),
+        ("bicgstab_ell", C.c_int32),
     ]
 
 

@@ -17,6 +17,7 @@
 #include <TNL/Matrices/MatrixReader.h>
 #include <TNL/Matrices/MatrixWriter.h>
 #include <TNL/Solvers/Linear/BICGStab.h>
+#include <TNL/Solvers/Linear/BICGStabL.h>
 #include <TNL/Solvers/Linear/CG.h>
 #include <TNL/Solvers/Linear/GMRES.h>
 #include <TNL/Solvers/Linear/TFQMR.h>
@@ -431,7 +432,7 @@ typedef struct
    int max_iterations, min_iterations;
    double convergence_residue, divergence_residue;
    int gmres_variant, restarting_min, restarting_max, restarting_step_min, restarting_step_max;
-   int bicgstab_exact_residue, preconditioner;
+   int bicgstab_exact_residue, preconditioner, bicgstab_ell;
 } ref_solver_params;
 
 // Solve on the Host with TNL's solvers; history via residual-history-file.  Returns
@@ -502,6 +503,19 @@ int ref_solve( int method, int n, const int* rp, const int* cols, const double* 
    else if( method == 3 ) {
       WithHistory< Solvers::Linear::TFQMR< CsrMatrix > > s;
       run( s );
+   }
+   else if( method == 4 ) {
+      WithHistory< Solvers::Linear::BICGStabL< CsrMatrix > > s;
+      Config::ParameterContainer prm;
+      prm.addParameter< int >( "bicgstab-ell", p->bicgstab_ell );
+      prm.addParameter< bool >( "bicgstab-exact-residue", p->bicgstab_exact_residue != 0 );
+      s.setup( prm );
+      try {
+         run( s );
+      }
+      catch( const std::runtime_error& ) {
+         rc = -1;  // initial residue NaN
+      }
    }
    else {
       WithHistory< Solvers::Linear::GMRES< CsrMatrix > > s;

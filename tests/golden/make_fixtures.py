@@ -196,6 +196,13 @@ def write_solver_fixtures():
                                            "restarting_max": 30, "convergence_residue": 1e-10}),
              ("convdiff_8_pe10", "tfqmr", {"preconditioner": 2, "convergence_residue": 1e-10}),
              ("irr_2000", "bicgstab", {"preconditioner": 2, "convergence_residue": 1e-10})]
+    # BiCGStab(ell)
+    runs += [("lap2d_24", "bicgstabl", {"bicgstab_ell": 1}), ("lap2d_24", "bicgstabl", {"bicgstab_ell": 2}),
+             ("lap2d_24", "bicgstabl", {"bicgstab_ell": 4, "preconditioner": 1}),
+             ("convdiff_8_pe10", "bicgstabl", {"bicgstab_ell": 2, "preconditioner": 1, "convergence_residue": 1e-10}),
+             ("convdiff_8_pe10", "bicgstabl", {"bicgstab_ell": 3, "preconditioner": 2, "bicgstab_exact_residue": 1,
+                                               "convergence_residue": 1e-10}),
+             ("irr_2000", "bicgstabl", {"bicgstab_ell": 2, "preconditioner": 1, "convergence_residue": 1e-10})]
     for name, (rp, c, v) in systems.items():
         sol[f"{name}/row_ptr"] = rp
         sol[f"{name}/cols"] = c

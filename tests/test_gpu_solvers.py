@@ -98,10 +98,12 @@ def _check_run(tnl, d, dev, run):
     ok, its, res, hist, x = _run_device_solver(tnl, dev, method, rp, c, v, b, **params)
     ref_its = int(d[f"{run}/iterations"][0])
     assert ok == bool(d[f"{run}/converged"][0]), run
-    assert abs(its - ref_its) <= 2, (run, its, ref_its)
+    assert abs(its - ref_its) <= max(2, params.get("bicgstab_ell", 1)), (run, its, ref_its)
     h_ref = d[f"{run}/history"]
     m = min(len(hist), len(h_ref), 20)
-    np.testing.assert_allclose(hist[:m], h_ref[:m], rtol=1e-6, err_msg=run)
+    # residues near 1e-12 and below are exact residues b - A x at the rounding floor of A x
+    # (about 1e-16 |A| |x| / |b|), hence the absolute 1e-14
+    np.testing.assert_allclose(hist[:m], h_ref[:m], rtol=1e-6, atol=1e-14, err_msg=run)
     A = W.csr_to_dense(rp, c, v, len(b))
     true_res = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
     tol = params.get("convergence_residue", 1e-6)
@@ -159,7 +161,7 @@ def test_float32_solve(tnl, dev):
 
 
 @pytest.mark.parametrize("method,precond", [("cg", 0), ("cg", 1), ("bicgstab", 1), ("tfqmr", 0), ("tfqmr", 1),
-                                            ("cg", 2), ("bicgstab", 2)])
+                                            ("cg", 2), ("bicgstab", 2), ("bicgstabl", 1)])
 def test_graph_replay_matches_direct_launches(tnl, dev, method, precond):
     """Iterations replayed from a captured hipGraph give the same bits as the same kernels
     launched one by one (long enough to go past the eager batches)."""

@@ -136,6 +136,21 @@ public:
    }
 };
 
+// BICGStabL.hpp:17-35: bicgstab-ell, bicgstab-exact-residue
+template< typename Matrix >
+class BICGStabL : public LinearSolverBase< Matrix, TNLHIP_BICGSTABL >
+{
+public:
+   bool setup( const Config::ParameterContainer& p, const std::string& prefix = "" ) override
+   {
+      if( p.checkParameter( prefix + "bicgstab-ell" ) )
+         this->params.bicgstab_ell = p.getInt( prefix + "bicgstab-ell" );
+      if( p.checkParameter( prefix + "bicgstab-exact-residue" ) )
+         this->params.bicgstab_exact_residue = p.getInt( prefix + "bicgstab-exact-residue" );
+      return LinearSolverBase< Matrix, TNLHIP_BICGSTABL >::setup( p, prefix );
+   }
+};
+
 template< typename Matrix >
 class GMRES : public LinearSolverBase< Matrix, TNLHIP_GMRES >
 {

@@ -1,7 +1,7 @@
 // Run-time choice of a linear solver and a preconditioner by name,
 // src/TNL/Solvers/LinearSolverTypeResolver.h:30-150.  Names without a Devices::Hip
-// implementation ("jacobi", "sor", "bicgstabl", "ilut") give nullptr with the same message on
-// std::cerr as an unknown name.
+// implementation ("jacobi", "sor", "ilut") give nullptr with the same message on std::cerr as
+// an unknown name.
 #pragma once
 
 #include <iostream>
@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "Linear/BICGStab.h"
+#include "Linear/BICGStabL.h"
 #include "Linear/CG.h"
 #include "Linear/GMRES.h"
 #include "Linear/Preconditioners/Diagonal.h"
@@ -22,7 +23,7 @@ namespace Solvers {
 inline std::vector< std::string >
 getLinearSolverOptions()
 {
-   return { "cg", "bicgstab", "gmres", "tfqmr" };
+   return { "cg", "bicgstab", "bicgstabl", "gmres", "tfqmr" };
 }
 
 inline std::vector< std::string >
@@ -50,6 +51,8 @@ getLinearSolver( const std::string& name )
       return std::make_shared< Linear::CG< MatrixType > >();
    if( name == "bicgstab" )
       return std::make_shared< Linear::BICGStab< MatrixType > >();
+   if( name == "bicgstabl" )
+      return std::make_shared< Linear::BICGStabL< MatrixType > >();
    if( name == "gmres" )
       return std::make_shared< Linear::GMRES< MatrixType > >();
    if( name == "tfqmr" )

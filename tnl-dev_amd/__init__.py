@@ -21,7 +21,7 @@ FORMATS = {
 }
 FORMAT_NAMES = {v: k for k, v in FORMATS.items()}
 DTYPES = {"f64": 0, "f32": 1, "bf16": 2}
-METHODS = {"cg": 0, "bicgstab": 1, "gmres": 2, "tfqmr": 3}
+METHODS = {"cg": 0, "bicgstab": 1, "gmres": 2, "tfqmr": 3, "bicgstabl": 4}
 GMRES_VARIANTS = {"CGS": 0, "CGSR": 1, "MGS": 2, "MGSR": 3, "CWY": 4}
 
 # every function declared in include/tnlhip.h
@@ -74,7 +74,7 @@ class SolverParams(C.Structure):
                 ("divergence_residue", C.c_double), ("gmres_variant", C.c_int32), ("restarting_min", C.c_int32),
                 ("restarting_max", C.c_int32), ("restarting_step_min", C.c_int32),
                 ("restarting_step_max", C.c_int32), ("bicgstab_exact_residue", C.c_int32),
-                ("preconditioner", C.c_int32), ("use_graph", C.c_int32)]
+                ("preconditioner", C.c_int32), ("use_graph", C.c_int32), ("bicgstab_ell", C.c_int32)]
 
 
 REFINEMENT_HISTORY = 64
@@ -565,6 +565,11 @@ class TFQMR(LinearSolver):
     method = "tfqmr"
 
 
+class BICGStabL(LinearSolver):
+    """BiCGStab(ell), Solvers/Linear/BICGStabL.h; setup(bicgstab_ell=..., bicgstab_exact_residue=...)."""
+    method = "bicgstabl"
+
+
 def refine_solve(A, b, x, A_low=None, tolerance=1e-10, inner_method="gmres", inner_dtype="f32",
                  max_outer_iterations=50, stagnation_limit=2, stream=None, **inner):
     """FP64 iterative refinement around a low-precision inner solve (no TNL counterpart).
@@ -750,7 +755,7 @@ def server_footprint(rows, nnz, max_row, nrhs):
 
 def getLinearSolver(name):
     """LinearSolverTypeResolver::getLinearSolver by name (Solvers/LinearSolverTypeResolver.h:59-90)."""
-    return {"cg": CG, "bicgstab": BICGStab, "gmres": GMRES, "tfqmr": TFQMR}[name]()
+    return {"cg": CG, "bicgstab": BICGStab, "gmres": GMRES, "tfqmr": TFQMR, "bicgstabl": BICGStabL}[name]()
 
 
 def getPreconditioner(name):

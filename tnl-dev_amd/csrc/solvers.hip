@@ -1138,6 +1138,298 @@ void solveTFQMR( SolverImpl& S, const void* b, void* xv )
    } );
 }
 
+// ================================================================== BiCGStab(ell)
+
+// BICGStabL.hpp:37-253.  State: rho -> rho_0, s1 -> rho_1, s2 -> gamma, alpha, beta, omega;
+// the small arrays sigma, g_0, g_1, g_2 (ell + 1 each) and T (ell x ell) live in S.small.
+// Within an outer iteration the ell Bi-CG steps only count iterations; the stopping rule is
+// checked once per outer iteration, after the residue of the minimal-residual update.
+struct BlSmall
+{
+   double* sigma;
+   double* g0;
+   double* g1;
+   double* g2;
+   double* T;
+   int ell;
+};
+
+inline BlSmall blSmall( SolverImpl& S, int ell )
+{
+   double* p = S.small.ptr;
+   return BlSmall{ p, p + ( ell + 1 ), p + 2 * ( ell + 1 ), p + 3 * ( ell + 1 ), p + 4 * ( ell + 1 ), ell };
+}
+
+struct FinBlInit
+{
+   // sigma_0 = || r_0 ||; NaN -> breakdown 2 (TNL throws); residue sigma_0 / b_norm; first check
+   State* st;
+   BlSmall sm;
+   double* hist;
+   int64_t cap;
+   Stop stop;
+   __device__ void operator()( const double* v ) const
+   {
+      State& s = *st;
+      const double sigma0 = sqrt( v[ 0 ] );
+      sm.sigma[ 0 ] = sigma0;
+      s.rho = 1.0;
+      s.alpha = 0.0;
+      s.omega = 1.0;
+      if( isnan( sigma0 ) ) {
+         s.breakdown = 2;
+         s.flags[ 0 ] = 1;
+         return;
+      }
+      setResidue( s, sigma0 / s.normb, hist, cap );
+      if( ! checkNext( s, stop ) )
+         s.flags[ 0 ] = 1;
+   }
+};
+
+template< typename T >
+struct OpBlInit
+{
+   // r_ast = r_0 / sigma_0; u_0 = 0
+   T* rast;
+   T* u0;
+   const T* r0;
+   const double* sigma;
+   double s0;
+   __device__ void load() { s0 = sigma[ 0 ]; }
+   __device__ void operator()( int64_t i ) const
+   {
+      rast[ i ] = (T) ( (double) r0[ i ] / s0 );
+      u0[ i ] = 0;
+   }
+};
+
+struct FinBlRho0
+{
+   State* st;
+   __device__ void operator()() const { st->rho = -st->omega * st->rho; }
+};
+
+struct FinBlBeta
+{
+   // nextIteration() counts; rho_1 = (r_ast, r_j); beta = alpha rho_1 / rho_0; rho_0 = rho_1
+   State* st;
+   __device__ void operator()( const double* v ) const
+   {
+      State& s = *st;
+      s.it++;
+      const double rho1 = v[ 0 ];
+      s.beta = s.alpha * rho1 / s.rho;
+      s.rho = rho1;
+   }
+};
+
+struct FinBlAlpha
+{
+   State* st;
+   __device__ void operator()( const double* v ) const { st->alpha = st->rho / v[ 0 ]; }
+};
+
+// Dst_c = Src_c + coef * Add_{c + shift} for c = 0..count-1 (geam with alpha = 1, beta = coef)
+template< typename T >
+struct OpBlGeam
+{
+   T* dst;
+   const T* src;
+   const T* add;
+   int64_t ld;
+   int count;
+   const double* coefPtr;
+   bool negate;
+   double coef;
+   __device__ void load() { coef = negate ? -*coefPtr : *coefPtr; }
+   __device__ void operator()( int64_t i ) const
+   {
+      for( int c = 0; c < count; c++ ) {
+         const int64_t k = (int64_t) c * ld + i;
+         dst[ k ] = (T) ( (double) src[ k ] + coef * (double) add[ k ] );
+      }
+   }
+};
+
+template< typename T >
+struct OpBlAxpy
+{
+   // y += coef x
+   T* y;
+   const T* x;
+   const double* coefPtr;
+   bool negate;
+   double coef;
+   __device__ void load() { coef = negate ? -*coefPtr : *coefPtr; }
+   __device__ void operator()( int64_t i ) const { y[ i ] = (T) ( (double) y[ i ] + coef * (double) x[ i ] ); }
+};
+
+struct FinBlT
+{
+   // T_ij = (r_i, r_j) / sigma_i
+   BlSmall sm;
+   int i, j;
+   __device__ void operator()( const double* v ) const
+   {
+      sm.T[ ( i - 1 ) + ( j - 1 ) * sm.ell ] = v[ 0 ] / sm.sigma[ i ];
+   }
+};
+
+template< typename T >
+struct OpBlSigma : NoScalars
+{
+   // (r_j, r_j), (r_0, r_j)
+   const T* rj;
+   const T* r0;
+   __device__ void operator()( int64_t i, double ( &acc )[ 2 ] ) const
+   {
+      const double a = (double) rj[ i ];
+      acc[ 0 ] += a * a;
+      acc[ 1 ] += (double) r0[ i ] * a;
+   }
+};
+
+struct FinBlSigma
+{
+   BlSmall sm;
+   int j;
+   __device__ void operator()( const double* v ) const
+   {
+      sm.sigma[ j ] = v[ 0 ];
+      sm.g1[ j ] = v[ 1 ] / v[ 0 ];
+   }
+};
+
+struct FinBlSolveG
+{
+   // omega = g_1[ell]; g_0 = T^-1 g_1; g_2 = T S g_0; g_2[0] = g_0[1]
+   State* st;
+   BlSmall sm;
+   __device__ void operator()() const
+   {
+      const int ell = sm.ell;
+      st->omega = sm.g1[ ell ];
+      for( int j = ell; j >= 1; j-- ) {
+         double g = sm.g1[ j ];
+         for( int i = j + 1; i <= ell; i++ )
+            g -= sm.T[ ( j - 1 ) + ( i - 1 ) * ell ] * sm.g0[ i ];
+         sm.g0[ j ] = g;
+      }
+      for( int j = 1; j < ell; j++ ) {
+         double g = sm.g0[ j + 1 ];
+         for( int i = j + 1; i < ell; i++ )
+            g += sm.T[ ( j - 1 ) + ( i - 1 ) * ell ] * sm.g0[ i + 1 ];
+         sm.g2[ j ] = g;
+      }
+      sm.g2[ 0 ] = sm.g0[ 1 ];
+   }
+};
+
+template< typename T >
+struct OpBlUpdate
+{
+   // x += R_[0:ell) g_2;  r_0 -= R_[1:ell] g_1[1:];  u_0 -= U_[1:ell] g_0[1:]  (gemv, MatrixOperations.h:46-150)
+   T* x;
+   T* R;
+   T* U;
+   int64_t ld;
+   BlSmall sm;
+   __device__ void load() {}
+   __device__ void operator()( int64_t i ) const
+   {
+      const int ell = sm.ell;
+      double ax = 0.0, ar = 0.0, au = 0.0;
+      for( int k = 0; k < ell; k++ ) {
+         ax += (double) R[ (int64_t) k * ld + i ] * sm.g2[ k ];
+         ar += (double) R[ (int64_t) ( k + 1 ) * ld + i ] * -sm.g1[ k + 1 ];
+         au += (double) U[ (int64_t) ( k + 1 ) * ld + i ] * -sm.g0[ k + 1 ];
+      }
+      x[ i ] = (T) ( ax + (double) x[ i ] );
+      R[ i ] = (T) ( ar + (double) R[ i ] );
+      U[ i ] = (T) ( au + (double) U[ i ] );
+   }
+};
+
+struct FinBlResidue
+{
+   State* st;
+   BlSmall sm;
+   double* hist;
+   int64_t cap;
+   Stop stop;
+   __device__ void operator()( const double* v ) const
+   {
+      State& s = *st;
+      sm.sigma[ 0 ] = sqrt( v[ 0 ] );
+      setResidue( s, sm.sigma[ 0 ] / s.normb, hist, cap );
+      if( ! checkNext( s, stop ) )
+         s.flags[ 0 ] = 1;
+   }
+};
+
+template< typename T >
+void solveBiCGStabL( SolverImpl& S, const void* b, void* xv )
+{
+   const int ell = S.params.bicgstab_ell;
+   const int64_t ld = S.ld;
+   // vectors: R (ell + 1 columns), U (ell + 1), r_ast, M_tmp, res
+   T* R = (T*) S.vec( 0 );
+   T* U = (T*) S.vec( ell + 1 );
+   T* rast = (T*) S.vec( 2 * ell + 2 );
+   T* Mt = (T*) S.vec( 2 * ell + 3 );
+   T* res = (T*) S.vec( 2 * ell + 4 );
+   T* x = (T*) xv;
+   auto Rc = [ & ]( int j ) { return R + (int64_t) j * ld; };
+   auto Uc = [ & ]( int j ) { return U + (int64_t) j * ld; };
+   const BlSmall sm = blSmall( S, ell );
+   State* st = S.state.ptr;
+   const Gate always{ nullptr, 0 }, running{ st->flags, 1 };
+   const Stop stop = S.stopRule();
+   const bool exact = S.params.bicgstab_exact_residue != 0;
+   computeNormb< T >( S, b, Mt );
+   residual< T >( S, b, x, Rc( 0 ), always );
+   launchReduce< 1 >( S.n, OpSquares< T >{ {}, Rc( 0 ) }, FinBlInit{ st, sm, S.hist.ptr, S.histCap, stop },
+                      S.ws.slot( 0 ), always, S.stream );
+   launchElementwise( S.n, OpBlInit< T >{ rast, Uc( 0 ), Rc( 0 ), sm.sigma, 0 }, running, S.stream );
+   runChunks( S, [ & ] {
+      launchScalar( FinBlRho0{ st }, running, S.stream );
+      for( int j = 0; j < ell; j++ ) {
+         launchReduce< 1 >( S.n, OpDot< T >{ {}, rast, Rc( j ) }, FinBlBeta{ st }, S.ws.slot( 0 ), running, S.stream );
+         launchElementwise( S.n, OpBlGeam< T >{ U, R, U, ld, j + 1, &st->beta, true, 0 }, running, S.stream );
+         // U_c = R_c - beta U_c: the add operand is U itself, the source R
+         matvec( S, Uc( j ), Uc( j + 1 ), running );
+         launchReduce< 1 >( S.n, OpDot< T >{ {}, rast, Uc( j + 1 ) }, FinBlAlpha{ st }, S.ws.slot( 0 ), running,
+                            S.stream );
+         launchElementwise( S.n, OpBlGeam< T >{ R, R, Uc( 1 ), ld, j + 1, &st->alpha, true, 0 }, running, S.stream );
+         matvec( S, Rc( j ), Rc( j + 1 ), running );
+         launchElementwise( S.n, OpBlAxpy< T >{ x, Uc( 0 ), &st->alpha, false, 0 }, running, S.stream );
+      }
+      for( int j = 1; j <= ell; j++ ) {
+         for( int i = 1; i < j; i++ ) {
+            launchReduce< 1 >( S.n, OpDot< T >{ {}, Rc( i ), Rc( j ) }, FinBlT{ sm, i, j }, S.ws.slot( 0 ), running,
+                               S.stream );
+            launchElementwise( S.n, OpBlAxpy< T >{ Rc( j ), Rc( i ), &sm.T[ ( i - 1 ) + ( j - 1 ) * ell ], true, 0 },
+                               running, S.stream );
+         }
+         launchReduce< 2 >( S.n, OpBlSigma< T >{ {}, Rc( j ), Rc( 0 ) }, FinBlSigma{ sm, j }, S.ws.slot( 1 ), running,
+                            S.stream );
+      }
+      launchScalar( FinBlSolveG{ st, sm }, running, S.stream );
+      launchElementwise( S.n, OpBlUpdate< T >{ x, R, U, ld, sm }, running, S.stream );
+      if( exact ) {
+         residual< T >( S, b, x, res, running );
+         launchReduce< 1 >( S.n, OpSquares< T >{ {}, res }, FinBlResidue{ st, sm, S.hist.ptr, S.histCap, stop },
+                            S.ws.slot( 0 ), running, S.stream );
+      }
+      else
+         launchReduce< 1 >( S.n, OpSquares< T >{ {}, Rc( 0 ) }, FinBlResidue{ st, sm, S.hist.ptr, S.histCap, stop },
+                            S.ws.slot( 0 ), running, S.stream );
+   } );
+   if( readState( S ).breakdown == 2 )
+      throw Error( TNLHIP_ERROR_RUNTIME, "BiCGstab(ell): initial residue is NAN" );
+}
+
 // ================================================================== GMRES
 
 // Small dense state of one restart cycle, in device memory (m = restarting_max):
@@ -1961,6 +2253,7 @@ void solveTyped( SolverImpl& S, const void* b, void* x )
       case TNLHIP_BICGSTAB: solveBiCGStab< T >( S, b, x ); break;
       case TNLHIP_GMRES: solveGMRES< T >( S, b, x ); break;
       case TNLHIP_TFQMR: solveTFQMR< T >( S, b, x ); break;
+      case TNLHIP_BICGSTABL: solveBiCGStabL< T >( S, b, x ); break;
    }
 }
 
@@ -1981,6 +2274,11 @@ void prepare( SolverImpl& S )
       nvec = 7;
    if( S.method == TNLHIP_TFQMR )
       nvec = 8;
+   if( S.method == TNLHIP_BICGSTABL ) {
+      nvec = 2 * ( S.params.bicgstab_ell + 1 ) + 3;
+      const int64_t l1 = S.params.bicgstab_ell + 1;
+      S.small.resize( 4 * l1 + l1 * l1 + 8 );
+   }
    if( S.method == TNLHIP_GMRES )
       nvec = 4 + ( S.params.restarting_max + 1 ) + 1;
    S.vecs.resize( (int64_t) nvec * S.ld * (int64_t) S.vbytes );
@@ -2013,13 +2311,14 @@ void tnlhip_solver_params_default( tnlhip_solver_params* p )
    p->bicgstab_exact_residue = 0;
    p->preconditioner = 0;
    p->use_graph = 1;
+   p->bicgstab_ell = 1;  // BICGStabL.hpp:21
 }
 
 int tnlhip_solver_create( int method, tnlhip_solver* out )
 {
    return guarded( [ & ] {
       TNLHIP_REQUIRE( out, "NULL out" );
-      TNLHIP_REQUIRE( method >= TNLHIP_CG && method <= TNLHIP_TFQMR, "unknown method" );
+      TNLHIP_REQUIRE( method >= TNLHIP_CG && method <= TNLHIP_BICGSTABL, "unknown method" );
       auto* s = new tnlhip_solver_s;
       s->method = method;
       *out = s;
@@ -2047,6 +2346,7 @@ int tnlhip_solver_set_params( tnlhip_solver s, const tnlhip_solver_params* p )
       TNLHIP_REQUIRE( s && p, "NULL argument" );
       TNLHIP_REQUIRE( p->preconditioner >= 0 && p->preconditioner <= 2, "unknown preconditioner" );
       TNLHIP_REQUIRE( p->gmres_variant >= 0 && p->gmres_variant <= 4, "unknown GMRES variant" );
+      TNLHIP_REQUIRE( p->bicgstab_ell >= 1 && p->bicgstab_ell <= 16, "bicgstab_ell must be in [1, 16]" );
       const bool pre = p->preconditioner != s->params.preconditioner;
       s->params = *p;
       if( pre ) {
