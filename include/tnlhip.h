@@ -13,7 +13,8 @@
  * Conventions
  *  - Index type is int32 (TNL's default `int`).
  *  - Device pointers are HIP device pointers; `stream` is a hipStream_t (NULL = default).
- *  - Vectors are float64 for F64 matrices and float32 for F32 and BF16 matrices.
+ *  - Vectors are float64 for F64 matrices, float32 for F32 and BF16 matrices and int32 for
+ *    BINARY ones.
  *  - Every call returns a tnlhip_status; tnlhip_last_error() describes the last failure
  *    on the calling thread.
  *  - Calls that take a stream are asynchronous on it; calls that return host values
@@ -70,7 +71,10 @@ typedef enum
 {
    TNLHIP_F64 = 0,
    TNLHIP_F32 = 1,
-   TNLHIP_BF16 = 2 /* values stored as bfloat16, ComputeReal = float (SparseMatrix.h:40-49) */
+   TNLHIP_BF16 = 2, /* values stored as bfloat16, ComputeReal = float (SparseMatrix.h:40-49) */
+   TNLHIP_BINARY = 3 /* Real = bool: no values stored, every stored entry is 1; ComputeReal = Index
+                        (SparseMatrixView.h:20-29), so vectors are int32 and a product sums x over
+                        the stored columns in int32 */
 } tnlhip_dtype;
 
 typedef struct tnlhip_matrix_s* tnlhip_matrix;

@@ -1827,6 +1827,24 @@ orc_symmetric_spmv( int32_t n, const int32_t* rp, const int32_t* cols, const dou
    }
 }
 
+// isBinary() fetch (SparseMatrixView.hpp:413-437): x_c itself, summed in ComputeReal = int; the
+// general keepers (SparseMatrixView.hpp:439-491) with int multiplicators.  Sums wrap as uint32.
+extern "C" void
+orc_binary_spmv( int32_t n, const int32_t* rp, const int32_t* cols, const int32_t* x, int32_t* y, int32_t alpha,
+                 int32_t beta, int32_t first, int32_t last )
+{
+   if( last == 0 )
+      last = n;
+   for( I r = first; r < last; r++ ) {
+      uint32_t s = 0;
+      for( I g = rp[ r ]; g < rp[ r + 1 ]; g++ )
+         if( cols[ g ] != -1 )
+            s += (uint32_t) x[ cols[ g ] ];
+      const uint32_t v = alpha == 1 ? s : (uint32_t) alpha * s;
+      y[ r ] = (int32_t) ( beta == 0 ? v : (uint32_t) beta * (uint32_t) y[ r ] + v );
+   }
+}
+
 extern "C" int32_t
 orc_ilu0( int32_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, int32_t* nnzL, int32_t* nnzU,
           int32_t* Lrp, int32_t* Lc, double* Lv, int32_t* Urp, int32_t* Uc, double* Uv, const double* b, double* x )

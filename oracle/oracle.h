@@ -89,6 +89,11 @@ void orc_solver_params_default( orc_solver_params* p );
 void orc_symmetric_spmv( int32_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, const double* x,
                          double* y, double alpha, double beta, int32_t first, int32_t last );
 
+/* Binary matrix product (Real = bool, ComputeReal = int): y_r = alpha * sum_c x_c + beta * y_r over
+ * the stored columns, in int32 with TNL's keeper choice by alpha / beta. */
+void orc_binary_spmv( int32_t n, const int32_t* row_ptr, const int32_t* cols, const int32_t* x, int32_t* y,
+                      int32_t alpha, int32_t beta, int32_t first, int32_t last );
+
 /* ILU(0) factors of a square CSR matrix (rows of U in natural order, U's first entry of a row is
  * the one divided by) and x = (LU)^-1 b when b and x are given; NULL arrays query the sizes. */
 int32_t orc_ilu0( int32_t n, const int32_t* row_ptr, const int32_t* cols, const double* vals, int32_t* nnzL,

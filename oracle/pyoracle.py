@@ -96,6 +96,9 @@ def lib():
         L.orc_symmetric_spmv.restype = None
         L.orc_symmetric_spmv.argtypes = [C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
                                          C.c_int32, C.c_int32]
+        L.orc_binary_spmv.restype = None
+        L.orc_binary_spmv.argtypes = [C.c_int32, _i32p, _i32p, _i32p, _i32p, C.c_int32, C.c_int32, C.c_int32,
+                                      C.c_int32]
         L.orc_solve.restype = C.c_int32
         L.orc_solve.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_void_p,
                                 C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_void_p, C.c_int32,
@@ -146,6 +149,9 @@ def ref():
         L.ref_symmetric_spmv.restype = C.c_int
         L.ref_symmetric_spmv.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
                                          C.c_int, C.c_int]
+        L.ref_binary_spmv.restype = C.c_int
+        L.ref_binary_spmv.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _i32p, _i32p, C.c_int, C.c_int, C.c_int,
+                                      C.c_int]
         L.ref_ilu0.restype = C.c_int
         L.ref_ilu0.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.POINTER(C.c_int), C.POINTER(C.c_int)] + \
             [C.c_void_p] * 8
@@ -355,6 +361,21 @@ def symmetric_spmv(row_ptr, cols, vals, x, y0=None, alpha=1.0, beta=0.0, first=0
     f = lib().orc_symmetric_spmv if backend == "oracle" else ref().ref_symmetric_spmv
     f(n, rp, np.ascontiguousarray(cols, np.int32), np.ascontiguousarray(vals, np.float64),
       np.ascontiguousarray(x, np.float64), y, float(alpha), float(beta), int(first), int(last))
+    return y
+
+
+def binary_spmv(row_ptr, cols, x, y0=None, alpha=1, beta=0, first=0, last=0, ncols=None, backend="oracle"):
+    """Binary matrix product in int32: y = alpha * (sum of x over the stored columns) + beta * y0."""
+    rp = np.ascontiguousarray(row_ptr, np.int32)
+    n = len(rp) - 1
+    xs = np.ascontiguousarray(x, np.int32)
+    y = np.zeros(n, np.int32) if y0 is None else np.array(y0, np.int32)
+    c = np.ascontiguousarray(cols, np.int32)
+    if backend == "oracle":
+        lib().orc_binary_spmv(n, rp, c, xs, y, int(alpha), int(beta), int(first), int(last))
+    else:
+        ref().ref_binary_spmv(n, len(xs) if ncols is None else ncols, rp, c, xs, y, int(alpha), int(beta),
+                              int(first), int(last))
     return y
 
 

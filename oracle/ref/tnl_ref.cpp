@@ -680,4 +680,28 @@ int ref_symmetric_spmv( int n, const int* rp, const int* cols, const double* val
    return 0;
 }
 
+// SparseMatrix< bool, Host, int, General, CSR, int >::vectorProduct (a binary matrix), one thread.
+int ref_binary_spmv( int n, int ncols, const int* rp, const int* cols, const int* x, int* y, int alpha, int beta,
+                     int first, int last )
+{
+   omp_set_num_threads( 1 );
+   Matrices::SparseMatrix< bool, Host, int, Matrices::GeneralMatrix, Seg::CSRDefault, int > A( n, ncols );
+   IVec caps( n );
+   for( int i = 0; i < n; i++ )
+      caps[ i ] = rp[ i + 1 ] - rp[ i ];
+   A.setRowCapacities( caps );
+   for( int i = 0; i < n; i++ )
+      for( int k = rp[ i ]; k < rp[ i + 1 ]; k++ )
+         A.setElement( i, cols[ k ], true );
+   IVec xv( ncols ), yv( n );
+   for( int i = 0; i < ncols; i++ )
+      xv[ i ] = x[ i ];
+   for( int i = 0; i < n; i++ )
+      yv[ i ] = y[ i ];
+   A.vectorProduct( xv, yv, alpha, beta, first, last );
+   for( int i = 0; i < n; i++ )
+      y[ i ] = yv[ i ];
+   return 0;
+}
+
 }  // extern "C"

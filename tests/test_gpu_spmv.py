@@ -305,3 +305,43 @@ def test_symmetric_products(tnl, dev, fmt):
         tnl.SparseMatrix.symmetric_from_csr(rp, bad_c, v, format=fmt)
     B = A.convert("SlicedEllpack")
     assert B.symmetric
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_binary_products(tnl, dev, fmt):
+    """Binary matrices (no values, int32 vectors): TNL's unit-test values and the oracle (TNL's
+    product) exactly, for the keepers and a row range; explicit zeros of the source are dropped,
+    the stored entries read back as ones, and the Krylov solvers refuse the type."""
+    for (rp, c), xval, y in tnl_cases.binary_matrices():
+        n = len(rp) - 1
+        A = tnl.SparseMatrix.from_csr(rp, c, None, format=fmt, dtype="binary")
+        xd = torch.full((n,), xval, dtype=torch.int32, device=dev)
+        yd = torch.zeros(n, dtype=torch.int32, device=dev)
+        A.vectorProduct(xd, yd)
+        np.testing.assert_array_equal(yd.cpu().numpy(), np.array(y, np.int32))
+    rng = np.random.default_rng(5)
+    rp, c, v = W.irregular(500, mean=8, sigma=1.0, max_len=40, seed=6)
+    v = v.copy()
+    v[::7] = 0.0  # dropped by the conversion, as operator= drops explicit zeros
+    keep = v != 0
+    lens = np.add.reduceat(keep.astype(np.int64), rp[:-1]) * (np.diff(rp) > 0)
+    rp_k = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    A = tnl.SparseMatrix.from_csr(rp, c, v, format=fmt, dtype="binary")
+    assert A.value_dtype == "binary" and A.getNonzeroElementsCount() == int(keep.sum())
+    vals, cols = A.storage()
+    assert np.all(vals[cols >= 0] == 1.0)
+    x = rng.integers(-1000, 1000, 500).astype(np.int32)
+    y0 = rng.integers(-1000, 1000, 500).astype(np.int32)
+    for alpha, beta, first, last in ((1, 0, 0, 0), (3, 0, 0, 0), (1, 2, 0, 0), (-2, 5, 50, 200)):
+        print(f"binary {fmt} {alpha} {beta} {first} {last}", flush=True)
+        xd = torch.tensor(x, device=dev)
+        yd = torch.tensor(y0, device=dev)
+        A.vectorProduct(xd, yd, alpha, beta, first, last)
+        want = O.binary_spmv(rp_k, c[keep], x, y0, alpha, beta, first, last)
+        np.testing.assert_array_equal(yd.cpu().numpy(), want)
+    B = A.convert("BiEllpack")
+    assert B.value_dtype == "binary"
+    s = tnl.getLinearSolver("cg")
+    s.setMatrix(A)
+    with pytest.raises(tnl.TnlHipError, match="binary"):
+        s.solve(torch.zeros(500, dtype=torch.float32, device=dev), torch.zeros(500, dtype=torch.float32, device=dev))

@@ -339,3 +339,20 @@ def test_symmetric_product_golden_values():
     """TNL's own expected products of its symmetric-matrix unit test."""
     for (rp, c, v), x, y in tnl_cases.symmetric_matrices():
         np.testing.assert_array_equal(O.symmetric_spmv(rp, c, v, x), np.array(y, np.float64))
+
+
+def test_binary_product_golden_values_and_tnl():
+    """Binary matrices (Real = bool, ComputeReal = int): TNL's unit-test values, and TNL's own
+    product for random patterns, keepers and row ranges."""
+    for (rp, c), xval, y in tnl_cases.binary_matrices():
+        n = len(rp) - 1
+        np.testing.assert_array_equal(O.binary_spmv(rp, c, np.full(n, xval)), np.array(y, np.int32))
+    if not O.ref_available():
+        return
+    rng = np.random.default_rng(4)
+    rp, c, _ = W.irregular(300, mean=8, sigma=1.0, max_len=40, seed=4)
+    x = rng.integers(-1000, 1000, 300).astype(np.int32)
+    y0 = rng.integers(-1000, 1000, 300).astype(np.int32)
+    for alpha, beta, first, last in ((1, 0, 0, 0), (3, 0, 0, 0), (1, 2, 0, 0), (-2, 5, 50, 200)):
+        np.testing.assert_array_equal(O.binary_spmv(rp, c, x, y0, alpha, beta, first, last),
+                                      O.binary_spmv(rp, c, x, y0, alpha, beta, first, last, backend="ref"))

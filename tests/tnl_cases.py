@@ -139,3 +139,18 @@ def symmetric_matrices():
                 [1, 4, 3 * 3 + 4 * 4 + 5 * 6 + 6 * 9, 3 * 4 + 4 * 5 + 5 * 7 + 6 * 10, 3 * 6 + 4 * 7 + 5 * 8 + 6 * 11,
                  3 * 9 + 4 * 10 + 5 * 11 + 6 * 12, 7 * 13, 8 * 14]))
     return out
+
+
+def binary_matrices():
+    """BinarySparseMatrixTest.hpp:584-790 (test_VectorProduct): the column pattern per row, the
+    constant input vector and TNL's expected product (ComputeReal = int)."""
+    def csr(rows):
+        rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+        return rp, np.array([c for r in rows for c in r], np.int32)
+
+    m4 = [[0, 1, 2, 5], [1, 2, 3, 4], [0, 1, 2, 3, 4], [1, 2, 3, 4], [2, 3, 4, 5], [3, 4, 5, 6], [0, 1, 2, 3, 4],
+          [0, 1, 2, 3, 4]]
+    return [(csr([[0], [1, 3], [1], [2]]), 2, [2, 4, 2, 2]),
+            (csr([[0, 1, 2], [3], [0, 1, 2], [1]]), 2, [6, 2, 6, 2]),
+            (csr([[0, 1, 2], [1, 2, 3], [0, 1, 2], [1, 2, 3]]), 2, [6, 6, 6, 6]),
+            (csr(m4), 2, [8, 8, 10, 8, 8, 8, 10, 10])]

@@ -20,7 +20,7 @@ FORMATS = {
     "Ellpack": 5, "SlicedEllpack": 6, "ChunkedEllpack": 7, "BiEllpack": 8,
 }
 FORMAT_NAMES = {v: k for k, v in FORMATS.items()}
-DTYPES = {"f64": 0, "f32": 1, "bf16": 2}
+DTYPES = {"f64": 0, "f32": 1, "bf16": 2, "binary": 3}
 METHODS = {"cg": 0, "bicgstab": 1, "gmres": 2, "tfqmr": 3, "bicgstabl": 4}
 GMRES_VARIANTS = {"CGS": 0, "CGSR": 1, "MGS": 2, "MGSR": 3, "CWY": 4}
 
@@ -248,12 +248,12 @@ class SparseMatrix:
         if isinstance(row_pointers, np.ndarray):
             rp = np.ascontiguousarray(row_pointers, np.int32)
             ci = np.ascontiguousarray(column_indexes, np.int32)
-            va = np.ascontiguousarray(values, np.float64)
+            va = None if values is None else np.ascontiguousarray(values, np.float64)  # None: a binary matrix of ones
             rows = len(rp) - 1
             cols = rows if columns is None else columns
             _check(lib.tnlhip_matrix_create_csr(fmt, dt, rows, cols, rp.ctypes.data_as(C.c_void_p),
-                                                ci.ctypes.data_as(C.c_void_p), va.ctypes.data_as(C.c_void_p),
-                                                1, C.byref(h)))
+                                                ci.ctypes.data_as(C.c_void_p),
+                                                None if va is None else va.ctypes.data_as(C.c_void_p), 1, C.byref(h)))
         else:
             rows = row_pointers.numel() - 1
             cols = rows if columns is None else columns
@@ -332,12 +332,12 @@ class SparseMatrix:
 
     @property
     def value_dtype(self):
-        return {0: "f64", 1: "f32", 2: "bf16"}[self.info.value_dtype]
+        return {0: "f64", 1: "f32", 2: "bf16", 3: "binary"}[self.info.value_dtype]
 
     @property
     def vector_dtype(self):
         import torch
-        return torch.float64 if self.info.value_dtype == 0 else torch.float32
+        return {0: torch.float64, 3: torch.int32}.get(self.info.value_dtype, torch.float32)
 
     def convert(self, format, dtype=None):
         """B = A into another format / value type, on the device."""

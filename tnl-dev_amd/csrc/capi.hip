@@ -155,7 +155,7 @@ tnlhip_matrix_s* buildFromSegments( int format, int dtype, int32_t rows, int32_t
 tnlhip_matrix_s* buildEmpty( int format, int dtype, int32_t rows, int32_t cols, const std::vector< int32_t >& caps )
 {
    TNLHIP_REQUIRE( format >= TNLHIP_CSR_SCALAR && format <= TNLHIP_BIELLPACK, "unknown format" );
-   TNLHIP_REQUIRE( dtype >= TNLHIP_F64 && dtype <= TNLHIP_BF16, "unknown value type" );
+   TNLHIP_REQUIRE( dtype >= TNLHIP_F64 && dtype <= TNLHIP_BINARY, "unknown value type" );
    TNLHIP_REQUIRE( rows >= 0 && cols >= 0, "negative dimensions" );
    HostSegments meta;
    buildSegments( meta, kindOf( format ), rows, caps.data() );
@@ -281,6 +281,16 @@ int tnlhip_matrix_create_csr( int format, int value_dtype, int32_t rows, int32_t
       DeviceArray< double > dval;
       const int32_t* prp = row_pointers;
       const int32_t* pcol = column_indexes;
+      // a binary matrix may come without values: every entry is then a one
+      std::vector< double > ones;
+      if( value_dtype == TNLHIP_BINARY && values == nullptr ) {
+         ones.assign( (size_t) std::max< int64_t >( nnzIn, 0 ), 1.0 );
+         values = ones.data();
+         if( ! host_pointers ) {
+            dval.upload( ones );
+            values = dval.ptr;
+         }
+      }
       const double* pval = values;
       if( host_pointers ) {
          drp.upload( rp );
@@ -303,6 +313,8 @@ int tnlhip_matrix_create_symmetric_csr( int format, int value_dtype, int32_t row
    return guarded( [ & ] {
       TNLHIP_REQUIRE( out != nullptr && rows >= 0, "bad argument" );
       TNLHIP_REQUIRE( row_pointers != nullptr || rows == 0, "row_pointers is NULL" );
+      if( value_dtype == TNLHIP_BINARY )
+         throw Error( TNLHIP_ERROR_UNSUPPORTED, "symmetric binary matrices are not supported" );
       std::vector< int32_t > rp( (size_t) rows + 1, 0 );
       if( rows > 0 ) {
          if( host_pointers )
@@ -434,8 +446,8 @@ int tnlhip_matrix_save( tnlhip_matrix A, const char* path, int layout )
    return guarded( [ & ] {
       TNLHIP_REQUIRE( A && path, "NULL argument" );
       TNLHIP_REQUIRE( layout == TNLHIP_FILE_CUDA_LAYOUT || layout == TNLHIP_FILE_HOST_LAYOUT, "unknown file layout" );
-      if( A->dtype == TNLHIP_BF16 )
-         throw Error( TNLHIP_ERROR_UNSUPPORTED, "TNL files hold double or float values, not bfloat16" );
+      if( A->dtype == TNLHIP_BF16 || A->dtype == TNLHIP_BINARY )
+         throw Error( TNLHIP_ERROR_UNSUPPORTED, "TNL files are written here for double or float values only" );
       const int64_t n = A->meta.allocated;
       std::vector< int32_t > cols( (size_t) n );
       std::vector< uint8_t > vals( (size_t) n * dtypeBytes( A->dtype ) );
@@ -610,7 +622,13 @@ int tnlhip_matrix_get_storage( tnlhip_matrix A, double* values, int32_t* columns
       if( columns )
          TNLHIP_CHECK( hipMemcpy( columns, A->columns.ptr, sizeof( int32_t ) * n, hipMemcpyDeviceToHost ) );
       if( values ) {
-         if( A->dtype == TNLHIP_F64 )
+         if( A->dtype == TNLHIP_BINARY ) {
+            std::vector< int32_t > c( n );
+            TNLHIP_CHECK( hipMemcpy( c.data(), A->columns.ptr, sizeof( int32_t ) * n, hipMemcpyDeviceToHost ) );
+            for( int64_t i = 0; i < n; i++ )
+               values[ i ] = c[ i ] != -1 ? 1.0 : 0.0;
+         }
+         else if( A->dtype == TNLHIP_F64 )
             TNLHIP_CHECK( hipMemcpy( values, A->values.ptr, sizeof( double ) * n, hipMemcpyDeviceToHost ) );
          else if( A->dtype == TNLHIP_F32 ) {
             std::vector< float > t( n );

@@ -153,9 +153,10 @@ __device__ __forceinline__ float toCompute( float v, float ) { return v; }
 __device__ __forceinline__ double toCompute( float v, double ) { return (double) v; }
 __device__ __forceinline__ float toCompute( bf16 v, float ) { return __bfloat162float( v ); }
 
+// bytes of one stored value (BINARY matrices store none)
 inline size_t dtypeBytes( int dtype )
 {
-   return dtype == TNLHIP_F64 ? 8 : dtype == TNLHIP_F32 ? 4 : 2;
+   return dtype == TNLHIP_F64 ? 8 : dtype == TNLHIP_F32 ? 4 : dtype == TNLHIP_BF16 ? 2 : 0;
 }
 
 // Vector type used with matrices of the given value dtype.

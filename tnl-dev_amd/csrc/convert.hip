@@ -10,6 +10,7 @@ namespace tnlhip {
 __device__ __forceinline__ double toDouble( double v ) { return v; }
 __device__ __forceinline__ double toDouble( float v ) { return (double) v; }
 __device__ __forceinline__ double toDouble( bf16 v ) { return (double) __bfloat162float( v ); }
+__device__ __forceinline__ double toDouble( uint8_t v ) { return (double) v; }
 
 template< typename V >
 __device__ __forceinline__ V fromDouble( double v );
@@ -19,6 +20,8 @@ template<>
 __device__ __forceinline__ float fromDouble< float >( double v ) { return (float) v; }
 template<>
 __device__ __forceinline__ bf16 fromDouble< bf16 >( double v ) { return __float2bfloat16( (float) v ); }
+template<>
+__device__ __forceinline__ uint8_t fromDouble< uint8_t >( double v ) { return v != 0.0; }
 
 template< typename V >
 __global__ void fillPaddingKernel( int64_t n, int32_t* cols, V* vals )
@@ -26,7 +29,8 @@ __global__ void fillPaddingKernel( int64_t n, int32_t* cols, V* vals )
    const int64_t stride = (int64_t) gridDim.x * blockDim.x;
    for( int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride ) {
       cols[ i ] = -1;
-      vals[ i ] = fromDouble< V >( 0.0 );
+      if( vals )
+         vals[ i ] = fromDouble< V >( 0.0 );
    }
 }
 
@@ -50,7 +54,8 @@ __global__ void scatterFromCsrKernel( int32_t n, const int32_t* __restrict__ rp,
             }
             const int64_t g = dst.global( (int32_t) r, local++ );
             dcol[ g ] = c;
-            dval[ g ] = fromDouble< V >( v );
+            if( dval )
+               dval[ g ] = fromDouble< V >( v );
          }
       }
    }
@@ -69,7 +74,7 @@ __global__ void gatherToCsrKernel( int32_t n, View src, const int32_t* __restric
       for( int32_t l = 0; l < cap; l++ ) {
          const int64_t g = src.global( (int32_t) r, l );
          const int32_t c = scol[ g ];
-         const double v = toDouble( sval[ g ] );
+         const double v = sval ? toDouble( sval[ g ] ) : 1.0;  // a binary matrix stores ones
          if( c != -1 && ( keepZeros || v != 0.0 ) ) {
             ocol[ o ] = c;
             oval[ o ] = v;
@@ -166,6 +171,8 @@ void gatherToCsr( const Matrix& A, const int32_t* rpDevice, int32_t* colsDevice,
 
 void extractDiagonal( const Matrix& A, void* diag, hipStream_t stream )
 {
+   if( A.dtype == TNLHIP_BINARY )
+      throw Error( TNLHIP_ERROR_UNSUPPORTED, "the diagonal of a binary matrix is not extracted" );
    if( A.rows == 0 )
       return;
    withValueType( A.dtype, [ & ]( auto* tag ) {

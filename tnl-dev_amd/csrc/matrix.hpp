@@ -109,6 +109,9 @@ void spmvLaunchGated( const Matrix& A, const void* x, void* y, double alpha, dou
 // SymmetricMatrix product (symmetric.hip), used by spmvLaunchGated for symmetric matrices
 void spmvSymmetric( const Matrix& A, const void* x, void* y, double alpha, double beta, int32_t first, int32_t last,
                     const void* diag, Gate gate, hipStream_t stream );
+// Binary matrix product (symmetric.hip): int32 vectors, alpha / beta truncated to int32
+void spmvBinary( const Matrix& A, const void* x, void* y, double alpha, double beta, int32_t first, int32_t last,
+                 Gate gate, hipStream_t stream );
 void convertFromCsrDevice( Matrix& A, const int32_t* rowPtr, const int32_t* cols, const double* vals,
                            hipStream_t stream,
                            bool keepZeros = false );

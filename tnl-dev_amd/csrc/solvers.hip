@@ -2261,6 +2261,8 @@ void prepare( SolverImpl& S )
 {
    TNLHIP_REQUIRE( S.A, "no matrix was set; call tnlhip_solver_set_matrix first" );
    TNLHIP_REQUIRE( S.A->rows == S.A->cols, "the matrix must be square" );
+   if( S.A->dtype == TNLHIP_BINARY )
+      throw Error( TNLHIP_ERROR_UNSUPPORTED, "the Krylov solvers work on real-valued matrices, not binary ones" );
    S.vdtype = vectorDtype( S.A->dtype );
    S.vbytes = dtypeBytes( S.vdtype );
    S.n = S.A->rows;

@@ -708,6 +708,12 @@ void spmvLaunchGated( const Matrix& A, const void* x, void* y, double alpha, dou
 {
    if( last < 0 )
       last = A.rows;
+   if( A.dtype == TNLHIP_BINARY ) {
+      if( A.symmetric || diag )
+         throw Error( TNLHIP_ERROR_UNSUPPORTED, "symmetric binary matrices and preconditioned binary products" );
+      spmvBinary( A, x, y, alpha, beta, first, last, Gate{ gate, gate ? gateMode : 0 }, stream );
+      return;
+   }
    if( A.symmetric ) {
       spmvSymmetric( A, x, y, alpha, beta, first, last, diag, Gate{ gate, gate ? gateMode : 0 }, stream );
       return;

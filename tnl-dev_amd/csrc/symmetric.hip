@@ -1,4 +1,6 @@
-// Products of symmetric matrices: SparseMatrix< ..., SymmetricMatrix, ... >::vectorProduct
+// Products of symmetric and of binary matrices, one lane per row over the format's index map.
+//
+// Symmetric: SparseMatrix< ..., SymmetricMatrix, ... >::vectorProduct
 // (Matrices/SparseMatrixView.hpp:390-491 with MatrixType::isSymmetric()).  The matrix stores its
 // lower triangle, diagonal included.  When beta != 1 the whole output vector is scaled by beta
 // first (TNL multiplies rather than clears, so a NaN in y survives beta = 0).  Then every stored
@@ -77,7 +79,48 @@ void launchSymmetric( const Matrix& M, const Y* x, Y* y, double alpha, double be
    TNLHIP_CHECK( hipGetLastError() );
 }
 
+// Binary matrices (Real = bool, ComputeReal = Index): the fetch of a stored entry is x_c itself
+// (SparseMatrixView.hpp:413-437), summed in int32; the keepers are those of a general matrix with
+// alpha and beta as int32 (SparseMatrixView.hpp:439-491).
+template< typename View >
+__global__ __launch_bounds__( 256 ) void binaryKernel( int32_t first, int32_t last, View view,
+                                                       const int32_t* __restrict__ col, const int32_t* __restrict__ x,
+                                                       int32_t* y, int32_t alpha, int32_t beta, Gate gate )
+{
+   if( ! gate.open() )
+      return;
+   const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+   for( int64_t r = first + (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < last; r += stride ) {
+      const int32_t size = view.size( (int32_t) r );
+      uint32_t s = 0;  // int32 sum with wrap-around
+      for( int32_t l = 0; l < size; l++ ) {
+         const int32_t c = col[ view.global( (int32_t) r, l ) ];
+         if( c >= 0 )
+            s += (uint32_t) x[ c ];
+      }
+      const int32_t v = (int32_t) s;
+      if( beta == 0 )
+         y[ r ] = alpha == 1 ? v : (int32_t) ( (uint32_t) alpha * (uint32_t) v );
+      else
+         y[ r ] = (int32_t) ( (uint32_t) beta * (uint32_t) y[ r ] + ( alpha == 1 ? (uint32_t) v
+                                                                                  : (uint32_t) alpha * (uint32_t) v ) );
+   }
+}
+
 }  // namespace
+
+void spmvBinary( const Matrix& M, const void* x, void* y, double alpha, double beta, int32_t first, int32_t last,
+                 Gate gate, hipStream_t st )
+{
+   if( last <= first )
+      return;
+   withView( M, [ & ]( auto view ) {
+      binaryKernel<<< gridFor( last - first, 256 ), 256, 0, st >>>( first, last, view, M.columns.ptr,
+                                                                     (const int32_t*) x, (int32_t*) y,
+                                                                     (int32_t) alpha, (int32_t) beta, gate );
+   } );
+   TNLHIP_CHECK( hipGetLastError() );
+}
 
 void spmvSymmetric( const Matrix& M, const void* x, void* y, double alpha, double beta, int32_t first, int32_t last,
                     const void* diag, Gate gate, hipStream_t st )

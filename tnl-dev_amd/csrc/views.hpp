@@ -121,6 +121,7 @@ void withValueType( int dtype, F&& f )
       case TNLHIP_F64: f( (double*) nullptr ); break;
       case TNLHIP_F32: f( (float*) nullptr ); break;
       case TNLHIP_BF16: f( (bf16*) nullptr ); break;
+      case TNLHIP_BINARY: f( (uint8_t*) nullptr ); break;  // no values array: pointers are NULL
       default: throw Error( TNLHIP_ERROR_INVALID_ARGUMENT, "unknown value type" );
    }
 }
