@@ -121,8 +121,12 @@ __device__ __forceinline__ void waveSync()
 
 // G = 1: CSRScalar (one lane per row, exact Host order); G = 2..32: CSRLight / CSRHybrid;
 // G = 64: CSRVector (a whole wavefront per row).  Groups of G lanes stride over the row
-// with contiguous loads, then reduce across the group with cross-lane shuffles.
-template< int G, bool EXACT, typename V, typename A, typename Y >
+// with contiguous loads, then reduce across the group with cross-lane shuffles.  Every group
+// keeps R rows in flight (rows r, r + stride, ...): their offsets, first chunk of columns /
+// values and the gathers of x are issued together, so short rows do not serialize on the
+// offsets -> columns -> x latency chain.  Each lane still sums its elements of a row in
+// increasing order, so the result does not depend on R.
+template< int G, int R, bool EXACT, typename V, typename A, typename Y >
 __global__ __launch_bounds__( 256 ) void csrGroupedKernel( const int32_t* __restrict__ off,
                                                            const int32_t* __restrict__ col,
                                                            const V* __restrict__ val, const Y* __restrict__ x,
@@ -133,18 +137,65 @@ __global__ __launch_bounds__( 256 ) void csrGroupedKernel( const int32_t* __rest
    constexpr int GPB = 256 / G;
    const int gl = threadIdx.x % G;
    const int64_t stride = (int64_t) gridDim.x * GPB;
-   for( int64_t r = first + (int64_t) blockIdx.x * GPB + threadIdx.x / G; r < last; r += stride ) {
-      const int32_t b = off[ r ], e = off[ r + 1 ];
-      A s = 0;
-      for( int32_t k = b + gl; k < e; k += G ) {
-         const int32_t c = col[ k ];
-         if( c >= 0 )
-            s = madd< EXACT >( s, toCompute( val[ k ], A() ), (A) x[ c ] );
+   if constexpr( R == 1 ) {
+      for( int64_t r = first + (int64_t) blockIdx.x * GPB + threadIdx.x / G; r < last; r += stride ) {
+         const int32_t b = off[ r ], e = off[ r + 1 ];
+         A s = 0;
+         for( int32_t k = b + gl; k < e; k += G ) {
+            const int32_t c = col[ k ];
+            if( c >= 0 )
+               s = madd< EXACT >( s, toCompute( val[ k ], A() ), (A) x[ c ] );
+         }
+         if constexpr( G > 1 )
+            s = groupSum< G >( s );
+         if( gl == 0 )
+            keep( r, s );
       }
-      if constexpr( G > 1 )
-         s = groupSum< G >( s );
-      if( gl == 0 )
-         keep( r, s );
+      return;
+   }
+   for( int64_t r0 = first + (int64_t) blockIdx.x * GPB + threadIdx.x / G; r0 < last; r0 += stride * R ) {
+      int32_t b[ R ], e[ R ];
+#pragma unroll
+      for( int j = 0; j < R; j++ ) {
+         const int64_t r = r0 + j * stride;
+         const bool valid = r < last;
+         b[ j ] = valid ? off[ r ] : 0;
+         e[ j ] = valid ? off[ r + 1 ] : 0;
+      }
+      int32_t c[ R ];
+      V v[ R ];
+#pragma unroll
+      for( int j = 0; j < R; j++ ) {
+         const int32_t k = b[ j ] + gl;
+         const bool in = k < e[ j ];
+         c[ j ] = in ? col[ k ] : -1;
+         v[ j ] = in ? val[ k ] : V( 0 );
+      }
+      Y xv[ R ];
+#pragma unroll
+      for( int j = 0; j < R; j++ )
+         xv[ j ] = x[ c[ j ] < 0 ? 0 : c[ j ] ];
+      A s[ R ];
+#pragma unroll
+      for( int j = 0; j < R; j++ ) {
+         s[ j ] = 0;
+         if( c[ j ] >= 0 )
+            s[ j ] = madd< EXACT >( s[ j ], toCompute( v[ j ], A() ), (A) xv[ j ] );
+         for( int32_t k = b[ j ] + gl + G; k < e[ j ]; k += G ) {
+            const int32_t ck = col[ k ];
+            if( ck >= 0 )
+               s[ j ] = madd< EXACT >( s[ j ], toCompute( val[ k ], A() ), (A) x[ ck ] );
+         }
+      }
+#pragma unroll
+      for( int j = 0; j < R; j++ ) {
+         A t = s[ j ];
+         if constexpr( G > 1 )
+            t = groupSum< G >( t );
+         const int64_t r = r0 + j * stride;
+         if( gl == 0 && r < last )
+            keep( r, t );
+      }
    }
 }
 
@@ -603,7 +654,7 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
       return;
    switch( M.format ) {
       case TNLHIP_CSR_SCALAR:
-         csrGroupedKernel< 1, true, V, A, Y >
+         csrGroupedKernel< 1, 1, true, V, A, Y >
             <<< gridFor( rows, 256 ), 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last );
          break;
       case TNLHIP_CSR_VECTOR:
@@ -625,25 +676,25 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
          const unsigned grid = gridFor( rows, gpb );
          switch( G ) {
             case 1:
-               csrGroupedKernel< 1, false, V, A, Y ><<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last );
+               csrGroupedKernel< 1, 1, false, V, A, Y ><<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last );
                break;
             case 2:
-               csrGroupedKernel< 2, false, V, A, Y ><<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last );
+               csrGroupedKernel< 2, 4, false, V, A, Y ><<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last );
                break;
             case 4:
-               csrGroupedKernel< 4, false, V, A, Y ><<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last );
+               csrGroupedKernel< 4, 4, false, V, A, Y ><<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last );
                break;
             case 8:
-               csrGroupedKernel< 8, false, V, A, Y ><<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last );
+               csrGroupedKernel< 8, 4, false, V, A, Y ><<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last );
                break;
             case 16:
-               csrGroupedKernel< 16, false, V, A, Y ><<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last );
+               csrGroupedKernel< 16, 2, false, V, A, Y ><<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last );
                break;
             case 32:
-               csrGroupedKernel< 32, false, V, A, Y ><<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last );
+               csrGroupedKernel< 32, 2, false, V, A, Y ><<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last );
                break;
             default:
-               csrGroupedKernel< 64, false, V, A, Y ><<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last );
+               csrGroupedKernel< 64, 1, false, V, A, Y ><<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last );
                break;
          }
          break;
