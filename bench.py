@@ -230,6 +230,66 @@ def irregular_suite(tnl, torch, W, dev, args):
     return out
 
 
+def next_suite(tnl, torch, W, dev, args, general_us=None):
+    """The components next to the path, measured on their own: a SymmetricMatrix SpMV (lower
+    triangle, SlicedEllpack, FP64) and a binary SpMV (SlicedEllpack, int32) on the grid^3
+    Laplacian; on the convection-diffusion cd_grid^3 box (Pe = 10) BiCGStab to 1e-10 with
+    Jacobi and with ILU(0), and BiCGStab(2) with Jacobi (iterations, seconds, ILU(0) setup)."""
+    out = {}
+    N = args.grid
+    rp, c, v = W.laplacian_3d(N)
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    keep = c <= rows
+    lrp = np.concatenate([[0], np.cumsum(np.bincount(rows[keep], minlength=n))]).astype(np.int32)
+    x = torch.tensor(W.uniform_vector(n, 2024), device=dev)
+    y = torch.zeros(n, dtype=torch.float64, device=dev)
+    flush = torch.zeros(64 << 20, dtype=torch.float64, device=dev)
+    S = tnl.SparseMatrix.symmetric_from_csr(lrp, c[keep], v[keep], format="SlicedEllpack")
+    steady, cold = time_spmv(tnl, torch, S, x, y, args.steps, args.warmup, flush)
+    out["symmetric_spmv"] = {"workload": f"lower triangle of the {N}^3 Laplacian, SlicedEllpack, FP64",
+                             "stored_entries": int(keep.sum()), "us": round(steady * 1e6, 2),
+                             "us_cold": round(cold * 1e6, 2),
+                             "vs_general_us": None if general_us is None else round(general_us, 2)}
+    log(f"symmetric SlicedEllpack {N}^3: {steady * 1e6:.2f} us (general {general_us} us)")
+    del S
+    Bm = tnl.SparseMatrix.from_csr(rp, c, None, format="SlicedEllpack", dtype="binary")
+    xi = torch.randint(-1000, 1000, (n,), dtype=torch.int32, device=dev)
+    yi = torch.zeros(n, dtype=torch.int32, device=dev)
+    steady, cold = time_spmv(tnl, torch, Bm, xi, yi, args.steps, args.warmup, flush)
+    bb = len(c) * 4 + 4 * n + 4 * n
+    out["binary_spmv"] = {"workload": f"pattern of the {N}^3 Laplacian, SlicedEllpack, int32 vectors",
+                          "us": round(steady * 1e6, 2), "GB/s": round(bb / steady / 1e9, 1),
+                          "frac_hbm": round(bb / steady / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"binary SlicedEllpack {N}^3: {steady * 1e6:.2f} us, {bb / steady / 1e9:.0f} GB/s")
+    del Bm, x, y, xi, yi, flush
+    torch.cuda.empty_cache()
+    M = min(args.cd_grid, 64)
+    rp, c, v = W.convection_diffusion_3d(M, 10.0)
+    n = len(rp) - 1
+    A = tnl.SparseMatrix.from_csr(rp, c, v, format="SlicedEllpack")
+    b = torch.tensor(W.uniform_vector(n, 7), device=dev)
+    runs = {}
+    for name, method, kw in (("bicgstab_jacobi", "bicgstab", {"preconditioner": 1}),
+                             ("bicgstab_ilu0", "bicgstab", {"preconditioner": 2}),
+                             ("bicgstabl2_jacobi", "bicgstabl", {"preconditioner": 1, "bicgstab_ell": 2})):
+        s = tnl.getLinearSolver(method)
+        s.setMatrix(A)
+        s.setup(convergence_residue=1e-10, max_iterations=20000, **kw)
+        xs = torch.zeros(n, dtype=torch.float64, device=dev)
+        ok, first = timed(torch, lambda: s.solve(b, xs))  # includes the ILU(0) factorization
+        xs.zero_()
+        ok, secs = timed(torch, lambda: s.solve(b, xs))
+        runs[name] = {"converged": bool(ok), "iterations": s.getIterations(), "seconds": round(secs, 4),
+                      "first_solve_seconds": round(first, 4)}
+        log(f"  {name}: {s.getIterations()} iterations, {secs * 1e3:.1f} ms (first solve {first * 1e3:.1f} ms)")
+    out["convdiff_pe10"] = {"workload": f"convection-diffusion {M}^3, Pe = 10, SlicedEllpack, x0 = 0, to 1e-10",
+                            "runs": runs}
+    del A
+    torch.cuda.empty_cache()
+    return out
+
+
 def serving_ladder(W):
     """Resident matrices of the serving workload: 2D 5-point and 3D 7-point Laplacians whose
     sizes step geometrically from about 1e3 to 1e6 rows (8 sizes per dimension)."""
@@ -356,10 +416,11 @@ def main():
     ap.add_argument("--serve-inner-tol", type=float, default=3e-6)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-irregular", action="store_true")
+    ap.add_argument("--no-next", action="store_true", help="skip the symmetric / binary / ILU(0) suite")
     ap.add_argument("--irr-rows", type=int, default=2000000)
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-iterations", type=int, default=8)
-    ap.add_argument("--only", default="", help="run one suite: spmv | solvers | serving")
+    ap.add_argument("--only", default="", help="run one suite: solvers | serving | gmres | irregular | next")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -380,6 +441,9 @@ def main():
         return
     if args.only == "irregular":
         print(json.dumps(irregular_suite(tnl, torch, W, dev, args)))
+        return
+    if args.only == "next":
+        print(json.dumps(next_suite(tnl, torch, W, dev, args)))
         return
     if args.only == "solvers":
         print(json.dumps(solver_suite(tnl, torch, W, dev, args)))
@@ -473,9 +537,11 @@ def main():
         "formats": results,
         "calibration": {"stream_read_GB/s": round(read_gbs, 1), "copy_GB/s": round(copy_gbs, 1)},
     }
+    del x, y, flush
+    torch.cuda.empty_cache()
+    if not args.no_next:
+        out["next"] = next_suite(tnl, torch, W, dev, args, results.get("SlicedEllpack", {}).get("us"))
     if not args.no_solvers:
-        del x, y, flush
-        torch.cuda.empty_cache()
         out["solvers"] = solver_suite(tnl, torch, W, dev, args)
     if not args.no_irregular:
         torch.cuda.empty_cache()
