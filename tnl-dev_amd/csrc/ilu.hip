@@ -15,7 +15,8 @@
 // x_i = b_i - sum L_ij x_j; backward x_i = (x_i - sum_{c >= 1} U_ij x_j) / U_i[0] -- each
 // product rounded before the subtraction, so the factors and the solves equal TNL's Host results
 // bit for bit.  A level with more than kNarrowLevel rows is one launch over the device; runs of
-// consecutive smaller levels are one single-workgroup launch with a barrier between levels.
+// consecutive smaller levels are one launch of a single 1024-lane workgroup with a barrier
+// between levels.
 #include "ilu.hpp"
 
 #include <algorithm>
@@ -25,7 +26,8 @@ namespace tnlhip {
 namespace {
 
 constexpr int kIluBlock = 256;
-constexpr int32_t kNarrowLevel = 2048;
+constexpr int kRunBlock = 1024;          // one workgroup of 16 waves for runs of small levels
+constexpr int32_t kNarrowLevel = 1024;   // a level of at most this many rows: one row per lane
 
 template< typename T >
 struct IluView
@@ -99,7 +101,7 @@ __global__ __launch_bounds__( kIluBlock ) void levelKernel( IluView< T > f, cons
 // consecutive small levels in one workgroup; the barrier makes each level's results visible
 // to the next (same CU)
 template< int STAGE, typename T >
-__global__ __launch_bounds__( kIluBlock ) void levelRunKernel( IluView< T > f, const int32_t* __restrict__ order,
+__global__ __launch_bounds__( kRunBlock ) void levelRunKernel( IluView< T > f, const int32_t* __restrict__ order,
                                                                const int32_t* __restrict__ levels, int32_t first,
                                                                int32_t last, const T* b, T* x, Gate gate )
 {
@@ -107,7 +109,7 @@ __global__ __launch_bounds__( kIluBlock ) void levelRunKernel( IluView< T > f, c
       return;
    for( int32_t lv = first; lv < last; lv++ ) {
       const int32_t end = levels[ lv + 1 ];
-      for( int32_t k = levels[ lv ] + threadIdx.x; k < end; k += kIluBlock )
+      for( int32_t k = levels[ lv ] + threadIdx.x; k < end; k += kRunBlock )
          processRow< STAGE >( f, order[ k ], b, x );
       __syncthreads();
    }
@@ -155,7 +157,7 @@ void runSchedule( const Ilu0::Schedule& s, const IluView< T >& f, const T* b, T*
                                                                                              b, x, gate );
       }
       else
-         levelRunKernel< STAGE, T ><<< 1, kIluBlock, 0, st >>>( f, s.order.ptr, s.levelsDevice.ptr, L.firstLevel,
+         levelRunKernel< STAGE, T ><<< 1, kRunBlock, 0, st >>>( f, s.order.ptr, s.levelsDevice.ptr, L.firstLevel,
                                                                 L.lastLevel, b, x, gate );
    }
    TNLHIP_CHECK( hipGetLastError() );
