@@ -339,6 +339,25 @@ int main()
       std::remove( "/tmp/tnlhip_front_sym.tnl" );
       std::printf( "%s SymmetricMatrix\n", failures ? "FAIL" : "PASS" );
    }
+   // binary matrix (Real = bool, int vectors): BinarySparseMatrixTest.hpp:592-635
+   {
+      using Bin = Matrices::SparseMatrix< bool, Devices::Hip, int, Matrices::GeneralMatrix, Seg::Ellpack >;
+      Bin B( 4, 4 );
+      B.setRowCapacities( std::vector< int >{ 1, 2, 1, 1 } );
+      B.setElement( 0, 0, 1 );
+      B.setElement( 1, 1, 1 );
+      B.setElement( 1, 3, 1 );
+      B.setElement( 2, 1, 1 );
+      B.setElement( 3, 2, 1 );
+      Containers::Vector< int, Devices::Hip, int > xb( 4, 2 ), yb( 4, 0 );
+      B.vectorProduct( xb, yb );
+      const auto yh = yb.toHost();
+      EXPECT_EQ( yh[ 0 ], 2 );
+      EXPECT_EQ( yh[ 1 ], 4 );
+      EXPECT_EQ( yh[ 2 ], 2 );
+      EXPECT_EQ( yh[ 3 ], 2 );
+      std::printf( "%s binary SparseMatrix\n", failures ? "FAIL" : "PASS" );
+   }
    std::printf( failures ? "FAILURES %d\n" : "ALL PASSED\n", failures );
    return failures ? 1 : 0;
 }

@@ -46,7 +46,9 @@ template< typename Real,
           typename Index = int,
           typename MatrixType = GeneralMatrix,
           template< typename, typename, typename > class Segments = Algorithms::Segments::CSRDefault,
-          typename ComputeReal = typename std::conditional< std::is_same< Real, bfloat16 >::value, float, Real >::type >
+          typename ComputeReal = typename std::conditional<
+             std::is_same< Real, bfloat16 >::value, float,
+             typename std::conditional< std::is_same< Real, bool >::value, Index, Real >::type >::type >
 class SparseMatrix
 {
    static_assert( std::is_same< Device, Devices::Hip >::value, "this SparseMatrix is the Devices::Hip one" );
@@ -57,12 +59,17 @@ public:
    using DeviceType = Device;
    using IndexType = Index;
    using ComputeRealType = ComputeReal;
-   using VectorReal = typename std::conditional< std::is_same< Real, double >::value, double, float >::type;
+   // vectors: double for double values, Index for binary (bool) matrices, float otherwise
+   using VectorReal = typename std::conditional<
+      std::is_same< Real, double >::value, double,
+      typename std::conditional< std::is_same< Real, bool >::value, int, float >::type >::type;
    using SegmentsTag = Segments< Device, Index, void >;
    static constexpr int format = SegmentsTag::format;
    static constexpr bool symmetric = MatrixType::isSymmetric();
-   static constexpr int valueDtype =
-      std::is_same< Real, double >::value ? TNLHIP_F64 : std::is_same< Real, float >::value ? TNLHIP_F32 : TNLHIP_BF16;
+   static constexpr int valueDtype = std::is_same< Real, double >::value  ? TNLHIP_F64
+                                     : std::is_same< Real, float >::value ? TNLHIP_F32
+                                     : std::is_same< Real, bool >::value  ? TNLHIP_BINARY
+                                                                          : TNLHIP_BF16;
 
    SparseMatrix() = default;
    SparseMatrix( Index rows, Index columns ) { setDimensions( rows, columns ); }
