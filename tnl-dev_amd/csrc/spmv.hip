@@ -324,6 +324,7 @@ __global__ __launch_bounds__( 256 ) void csrAdaptiveKernel( int32_t nblocks, con
 // flight; the sum still runs left to right over the non-padding slots only, which is the
 // Host result bit for bit.  Each workgroup owns `rowsPerBlock` consecutive rows.
 constexpr int kChunk = 8;
+constexpr int kRowsInFlight = 2;  // rows per lane per iteration of the padded kernels
 
 // Two vertically adjacent slots (rows r, r+1 at one local index) are one aligned 8-byte
 // index load and one 2-element value load.
@@ -379,6 +380,58 @@ __device__ __forceinline__ void paddedRows( const Slots& slots, const int32_t* _
                                             const V* __restrict__ val, const Y* __restrict__ x, Keeper< A, Y > keep,
                                             int32_t first, RowSeq seq )
 {
+   if constexpr( ! PAIR ) {
+      // kRowsInFlight rows of the sequence per lane (r, r + step, ...), their loads issued
+      // together; each row keeps its own left-to-right sum
+      constexpr int Q = kRowsInFlight;
+      const int64_t step = slots.step();
+      int64_t r = seq.start;
+      for( ; r + ( Q - 1 ) * seq.step < seq.end; r += Q * seq.step ) {
+         int64_t b[ Q ];
+         int32_t w[ Q ];
+         int32_t wmax = 0;
+#pragma unroll
+         for( int q = 0; q < Q; q++ ) {
+            b[ q ] = slots.base( r + q * seq.step );
+            w[ q ] = slots.width( r + q * seq.step );
+            wmax = max( wmax, w[ q ] );
+         }
+         A s[ Q ];
+#pragma unroll
+         for( int q = 0; q < Q; q++ )
+            s[ q ] = 0;
+         for( int32_t l0 = 0; l0 < wmax; l0 += kChunk ) {
+            int32_t c[ Q ][ kChunk ];
+            V v[ Q ][ kChunk ];
+#pragma unroll
+            for( int q = 0; q < Q; q++ )
+#pragma unroll
+               for( int j = 0; j < kChunk; j++ ) {
+                  const bool in = l0 + j < w[ q ];
+                  const int64_t g = b[ q ] + step * ( l0 + j );
+                  c[ q ][ j ] = in ? ldnt( col + g ) : -1;
+                  v[ q ][ j ] = in ? ldnt( val + g ) : V( 0 );
+               }
+            Y xv[ Q ][ kChunk ];
+#pragma unroll
+            for( int q = 0; q < Q; q++ )
+#pragma unroll
+               for( int j = 0; j < kChunk; j++ )
+                  xv[ q ][ j ] = x[ c[ q ][ j ] < 0 ? 0 : c[ q ][ j ] ];
+#pragma unroll
+            for( int q = 0; q < Q; q++ )
+#pragma unroll
+               for( int j = 0; j < kChunk; j++ )
+                  if( c[ q ][ j ] >= 0 )
+                     s[ q ] = mulAddExact( s[ q ], toCompute( v[ q ][ j ], A() ), (A) xv[ q ][ j ] );
+         }
+#pragma unroll
+         for( int q = 0; q < Q; q++ )
+            if( r + q * seq.step >= first )
+               keep( r + q * seq.step, s[ q ] );
+      }
+      seq.start = r;  // the rows left over run one per iteration below
+   }
    for( int64_t r = seq.start; r < seq.end; r += seq.step ) {
       const int64_t b = slots.base( r );
       const int32_t width = slots.width( r );
