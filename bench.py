@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 MALL_BYTES = 256 << 20
 # HBM bytes per launch at 160^3 FP64 from rocprofv3 --pmc FETCH_SIZE (x2, the factor a plain
 # 512 MiB stream reads) and WRITE_SIZE runs: profiles/spmv_laplacian160_traffic_pmc.md
-MEASURED_TRAFFIC = {"SlicedEllpack": 487.3e6, "Ellpack": 484.8e6, "ChunkedEllpack": 490.5e6, "CSRAdaptive": 493.2e6,
+MEASURED_TRAFFIC = {"SlicedEllpack": 487.1e6, "Ellpack": 484.2e6, "ChunkedEllpack": 490.5e6, "CSRAdaptive": 493.1e6,
                     "BiEllpack": 508.5e6}
 FORMATS = ["CSRScalar", "CSRVector", "CSRLight", "CSRHybrid", "CSRAdaptive", "Ellpack", "SlicedEllpack",
            "ChunkedEllpack", "BiEllpack"]
