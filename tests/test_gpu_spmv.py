@@ -80,6 +80,35 @@ def test_tnl_unit_test_golden_values(tnl, dev, fmt):
             np.testing.assert_array_equal(y, np.asarray(case["y"], np.float64), err_msg=f"{fmt} {dt} {name}")
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_csr_scalar_staged_and_walked_chunks_bit_exact(tnl, dev, dtype):
+    """CSRScalar stages the elements of each 64-row chunk through LDS when they fit (512 FP64 or
+    1024 FP32 elements) and walks the rows in global memory otherwise; both give the Host
+    traversal's bits, also for chunks cut by a row range and for empty chunks."""
+    rng = np.random.default_rng(21)
+    n = 6000
+    lens = rng.integers(0, 15, n)          # about 450 elements per chunk: most are staged
+    lens[640:900] = 0                      # empty chunks and a chunk that starts empty
+    lens[2048:2112] = 40                   # one chunk far over the LDS slice: walked
+    lens[3000:3064] = rng.integers(12, 20, 64)  # over 512 elements: walked in FP64, staged in FP32
+    rp = np.zeros(n + 1, np.int32)
+    np.cumsum(lens, out=rp[1:])
+    cols = np.concatenate([np.sort(rng.choice(n, k, replace=False)) for k in lens]).astype(np.int32)
+    vals = rng.uniform(-1, 1, len(cols))
+    x = rng.uniform(-1, 1, n)
+    y0 = rng.uniform(-1, 1, n)
+    npdt = np.float64 if dtype == "f64" else np.float32
+    seg = O.Segments("CSR", lens)
+    sv, sc = seg.convert(rp, cols, vals)
+    A = tnl.SparseMatrix.from_csr(rp, cols, vals, format="CSRScalar", dtype=dtype)
+    for alpha, first, last in [(1.0, 0, 0), (2.5, 0, 0), (1.0, 17, 3001), (1.0, 2050, 2051), (-1.0, 700, n)]:
+        print(f"CSRScalar {dtype} alpha={alpha} rows [{first}, {last})", flush=True)
+        want = seg.spmv(sv.astype(npdt), sc, x.astype(npdt), y0.astype(npdt), alpha, 0.0, first, last, dtype=npdt)
+        got = run_spmv(tnl, A, x.astype(npdt).astype(np.float64), dev, y0.astype(npdt).astype(np.float64),
+                       alpha, 0.0, first, last)
+        np.testing.assert_array_equal(got, want.astype(np.float64), err_msg=f"{dtype} {alpha} {first} {last}")
+
+
 @pytest.mark.parametrize("fmt", FORMATS)
 def test_keepers_row_ranges_and_value_types(tnl, dev, fmt):
     rng = np.random.default_rng(3)
@@ -211,7 +240,8 @@ def test_invalid_arguments_raise(tnl, dev):
 @pytest.mark.parametrize("fmt,params", [("Ellpack", {"xcd_remap": 3}), ("Ellpack", {"xcd_remap": 3, "pair_rows": 1}),
                                         ("SlicedEllpack", {"xcd_remap": 3}),
                                         ("SlicedEllpack", {"xcd_remap": 3, "pair_rows": 1, "blocks_per_cu": 4}),
-                                        ("CSRAdaptive", {"xcd_remap": 1}), ("CSRAdaptive", {"xcd_remap": 3})])
+                                        ("CSRAdaptive", {"xcd_remap": 1}), ("CSRAdaptive", {"xcd_remap": 3}),
+                                        ("CSRScalar", {"xcd_remap": 1})])
 def test_launch_distributions_give_the_same_bits(tnl, dev, fmt, params):
     """Row distributions over the XCDs change only which workgroup computes a row: every keeper
     mode and row range gives the default launch's bits (a row computed twice or skipped would
