@@ -866,7 +866,9 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
       }
       case TNLHIP_CHUNKED_ELLPACK:
          if( M.meta.numberOfSlices > 0 )
-            chunkedKernel< V, A, Y ><<< gridFor( M.meta.numberOfSlices, 1 ), 256, 0, st >>>(
+            // one workgroup per slice: at 160^3 70.6 us, against 78.1 / 72.7 us for grids of 8 / 32
+            // workgroups per CU striding over the slices
+            chunkedKernel< V, A, Y ><<< M.meta.numberOfSlices, 256, 0, st >>>(
                M.meta.numberOfSlices, reinterpret_cast< const int4* >( M.slices.ptr ), M.rowToChunk.ptr, col, val, x,
                keep, first, last );
          break;
@@ -874,7 +876,9 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
          const int32_t strips = M.meta.virtualRows / kStrip;
          const int32_t pairs = ( strips + 1 ) / 2;
          if( strips > 0 && M.meta.allocated > 0 )
-            biellpackKernel< V, A, Y ><<< gridFor( pairs, 4 ), 256, 0, st >>>(
+            // 32 workgroups per CU (two strip pairs per wave at 160^3): 79.8 us, against 83.0 us
+            // at 8 and 84.3 us with one workgroup per 4 pairs
+            biellpackKernel< V, A, Y ><<< gridFor( pairs, 4, 32 ), 256, 0, st >>>(
                M.rows, strips, M.rowPerm.ptr, M.groupPointers.ptr, col, val, x, keep, first, last );
          else
             // no stored element at all (TNL's Host path returns without touching y here,
