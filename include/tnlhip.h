@@ -343,6 +343,8 @@ typedef struct
    double history[ TNLHIP_REFINEMENT_HISTORY ]; /* FP64 residue before each outer step */
 } tnlhip_refinement_result;
 
+/* defaults: tolerance 1e-10, 50 outer steps, FP32 Jacobi GMRES(30) CWY inner solve to 1e-2,
+ * stagnation limit 2 */
 void tnlhip_refinement_params_default( tnlhip_refinement_params* params );
 
 /* A: FP64 matrix; A_low: the same matrix in FP32 / BF16 (NULL: converted internally from A,

@@ -51,3 +51,20 @@ def test_refinement_with_given_low_matrix(tnl, dev):
                          max_iterations=5000)
     assert r["converged"]
     assert np.linalg.norm(b - spmv_host(rp, c, v, xd.cpu().numpy())) / np.linalg.norm(b) < 1.01e-10
+
+
+@pytest.mark.parametrize("peclet", [1.0, 100.0])
+def test_refinement_with_default_inner_target(tnl, dev, peclet):
+    """The default parameters (FP32 Jacobi GMRES(30) CWY to 1e-2 per outer step) reach 1e-10 in
+    FP64, each outer step cutting the residual at least tenfold."""
+    rp, c, v = W.convection_diffusion_3d(16, peclet)
+    n = len(rp) - 1
+    b = W.uniform_vector(n, 8)
+    A = tnl.SparseMatrix.from_csr(rp, c, v, format="SlicedEllpack", dtype="f64")
+    xd = torch.zeros(n, dtype=torch.float64, device=dev)
+    r = tnl.refine_solve(A, torch.tensor(b, device=dev), xd)
+    print(peclet, r["outer_iterations"], r["inner_iterations"], r["history"])
+    assert r["converged"] and r["residue"] < 1e-10
+    assert np.linalg.norm(b - spmv_host(rp, c, v, xd.cpu().numpy())) / np.linalg.norm(b) < 1.01e-10
+    h = r["history"]
+    assert all(h[i + 1] < 0.1 * h[i] for i in range(len(h) - 1))

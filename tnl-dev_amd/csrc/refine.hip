@@ -71,7 +71,10 @@ void tnlhip_refinement_params_default( tnlhip_refinement_params* p )
    p->inner_value_dtype = TNLHIP_F32;
    p->stagnation_limit = 2;
    tnlhip_solver_params_default( &p->inner );
-   p->inner.convergence_residue = 1e-4;
+   // a loose inner target: each outer step restarts the Krylov space, which also lifts the
+   // stagnation of restarted GMRES; at 128^3 convection-diffusion 1e-2 needed the fewest inner
+   // iterations in all (5 outer steps, 1300 inner against 2238 at 1e-4 and 1960 for FP64 GMRES)
+   p->inner.convergence_residue = 1e-2;
    p->inner.max_iterations = 100000;
    p->inner.gmres_variant = TNLHIP_GMRES_CWY;
    p->inner.restarting_min = 30;
