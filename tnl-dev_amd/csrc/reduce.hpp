@@ -59,7 +59,12 @@ __device__ __forceinline__ void storeBlockPartials( const double ( &v )[ W ], Re
 
 // One kReduceBlock workgroup: out[k] = sum over the `grid` partials of column k, in a fixed
 // order; out (W doubles, usually LDS) is valid in every thread on return.  Up to 64 columns
-// get kReduceBlock / W lanes each (independent loads in flight), one column gets the block.
+// get kReduceBlock / W lanes each, one column gets the block.  The partials were written by
+// the previous launch, possibly through another XCD's L2, so every load is a trip to memory:
+// each lane keeps kSumStreams independent loads in flight (a 32-column sum of 768 partials is
+// 6 dependent trips instead of 24) and folds its accumulators in a fixed tree.
+constexpr int kSumStreams = 16;
+
 template< int W >
 __device__ __forceinline__ void sumPartials( ReduceSlot slot, int grid, double* out )
 {
@@ -68,17 +73,25 @@ __device__ __forceinline__ void sumPartials( ReduceSlot slot, int grid, double* 
       constexpr int G = kReduceBlock / W;  // lanes per column, a power of two <= 64
       const int k = threadIdx.x / G, g = threadIdx.x % G;
       const double* p = slot.partials + (int64_t) k * grid;
-      double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+      double a[ kSumStreams ];
+#pragma unroll
+      for( int u = 0; u < kSumStreams; u++ )
+         a[ u ] = 0;
       int j = g;
-      for( ; j + 3 * G < grid; j += 4 * G ) {
-         s0 += p[ j ];
-         s1 += p[ j + G ];
-         s2 += p[ j + 2 * G ];
-         s3 += p[ j + 3 * G ];
+      for( ; j + ( kSumStreams - 1 ) * G < grid; j += kSumStreams * G ) {
+#pragma unroll
+         for( int u = 0; u < kSumStreams; u++ )
+            a[ u ] += p[ j + u * G ];
       }
       for( ; j < grid; j += G )
-         s0 += p[ j ];
-      double s = ( s0 + s1 ) + ( s2 + s3 );
+         a[ 0 ] += p[ j ];
+#pragma unroll
+      for( int w = kSumStreams / 2; w >= 1; w /= 2 ) {
+#pragma unroll
+         for( int u = 0; u < w; u++ )
+            a[ u ] += a[ u + w ];
+      }
+      double s = a[ 0 ];
       s = groupSum< G >( s );
       if( g == 0 )
          out[ k ] = s;
@@ -89,9 +102,18 @@ __device__ __forceinline__ void sumPartials( ReduceSlot slot, int grid, double* 
       const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
 #pragma unroll
       for( int k = 0; k < W; k++ ) {
-         double s = 0;
-         for( int j = threadIdx.x; j < grid; j += kReduceBlock )
-            s += slot.partials[ (int64_t) k * grid + j ];
+         const double* p = slot.partials + (int64_t) k * grid;
+         double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+         int j = threadIdx.x;
+         for( ; j + 3 * kReduceBlock < grid; j += 4 * kReduceBlock ) {
+            a0 += p[ j ];
+            a1 += p[ j + kReduceBlock ];
+            a2 += p[ j + 2 * kReduceBlock ];
+            a3 += p[ j + 3 * kReduceBlock ];
+         }
+         for( ; j < grid; j += kReduceBlock )
+            a0 += p[ j ];
+         double s = ( a0 + a1 ) + ( a2 + a3 );
          s = groupSum< kWave >( s );
          if( lane == 0 )
             lds[ k ][ wave ] = s;
