@@ -203,14 +203,16 @@ __global__ __launch_bounds__( 256 ) void csrGroupedKernel( const int32_t* __rest
 
 // CSRScalar (one lane per row, Kernels/CSRScalarKernel.hpp) with the elements staged through
 // LDS.  Each wavefront owns 64 consecutive rows.  When their elements fit the wave's LDS slice
-// (K x 64 of them), the wave loads them with coalesced, unrolled loads, stores each rounded
-// product in LDS (a padding slot stores -0.0, the identity of IEEE addition, so adding it
-// equals skipping it), and every lane then sums its own row left to right out of LDS: the
-// Host traversal's result bit for bit, at streaming speed.  Waves whose rows hold more
-// elements walk their rows in global memory, one lane per row, with the same arithmetic.
-// One wave per 64 rows and no grid-stride loop: at 160^3 neither a persistent grid that
-// prefetches the next chunk's offsets (78-92 us by grid size) nor two chunks per wave (83 us)
-// beat it (79 us).
+// (K x 64 of them: 512 FP64 or 1024 FP32 products), the wave loads them with coalesced loads,
+// kStageLoads per lane in flight per round, stores each rounded product in LDS (a padding slot
+// stores -0.0, the identity of IEEE addition, so adding it equals skipping it), and every lane
+// then sums its own row left to right out of LDS: the Host traversal's result bit for bit, at
+// streaming speed.  Waves whose rows hold more elements walk their rows in global memory, one
+// lane per row, with the same arithmetic.  One wave per 64 rows and no grid-stride loop: at
+// 160^3 FP64 neither a persistent grid that prefetches the next block's offsets (78-92 us by
+// grid size) nor two 64-row blocks per wave (83 us; FP32 61.1 against 59.4 us) beat it (79 us).
+constexpr int kStageLoads = 8;
+
 template< int K, typename V, typename A, typename Y >
 __global__ __launch_bounds__( 256 ) void csrScalarStagedKernel( const int32_t* __restrict__ off,
                                                                 const int32_t* __restrict__ col,
@@ -226,7 +228,7 @@ __global__ __launch_bounds__( 256 ) void csrScalarStagedKernel( const int32_t* _
    if( r0 >= last )  // uniform over the wave; the kernel has no workgroup barrier
       return;
    const int64_t r = r0 + lane;
-   const int32_t used = (int32_t) min( (int64_t) kWave, last - r0 );  // rows of this wave
+   const int32_t used = (int32_t) min( (int64_t) kWave, last - r0 );  // rows of this wave, >= 1
    const bool mine = lane < used;
    const int32_t rb = mine ? off[ r ] : 0;
    const int32_t re = mine ? off[ r + 1 ] : 0;
@@ -238,23 +240,31 @@ __global__ __launch_bounds__( 256 ) void csrScalarStagedKernel( const int32_t* _
       return;
    }
    if( size <= K * kWave ) {
-      int32_t c[ K ];
-      V v[ K ];
+      // rounds of kStageLoads loads per lane (one for FP64, up to two for FP32) keep the
+      // registers, hence the waves per SIMD, of the one-round case
 #pragma unroll
-      for( int k = 0; k < K; k++ ) {
-         const int32_t e = min( lane + k * kWave, size - 1 );
-         c[ k ] = ldnt( col + b + e );
-         v[ k ] = ldnt( val + b + e );
-      }
-      Y xv[ K ];
+      for( int q = 0; q < K / kStageLoads; q++ ) {
+         const int32_t e0 = q * kStageLoads * kWave;
+         if( e0 >= size )  // uniform over the wave
+            break;
+         int32_t c[ kStageLoads ];
+         V v[ kStageLoads ];
 #pragma unroll
-      for( int k = 0; k < K; k++ )
-         xv[ k ] = c[ k ] >= 0 ? x[ c[ k ] ] : Y( 0 );  // x may be empty when every slot is padding
+         for( int k = 0; k < kStageLoads; k++ ) {
+            const int32_t g = min( e0 + lane + k * kWave, size - 1 );
+            c[ k ] = ldnt( col + b + g );
+            v[ k ] = ldnt( val + b + g );
+         }
+         Y xv[ kStageLoads ];
 #pragma unroll
-      for( int k = 0; k < K; k++ ) {
-         const int32_t e = lane + k * kWave;
-         if( e < size )
-            lds[ wave ][ e ] = c[ k ] >= 0 ? toCompute( v[ k ], A() ) * (A) xv[ k ] : (A) -0.0;
+         for( int k = 0; k < kStageLoads; k++ )
+            xv[ k ] = c[ k ] >= 0 ? x[ c[ k ] ] : Y( 0 );  // x may be empty when every slot is padding
+#pragma unroll
+         for( int k = 0; k < kStageLoads; k++ ) {
+            const int32_t g = e0 + lane + k * kWave;
+            if( g < size )
+               lds[ wave ][ g ] = c[ k ] >= 0 ? toCompute( v[ k ], A() ) * (A) xv[ k ] : (A) -0.0;
+         }
       }
       waveSync();
       if( mine ) {
@@ -280,7 +290,7 @@ __global__ __launch_bounds__( 256 ) void csrScalarStagedKernel( const int32_t* _
 
 // One wavefront per block descriptor (the scheme of Kernels/CSRAdaptiveKernelView.hpp:33-170,
 // with wave64 budgets).  STREAM: the wave loads the elements of several short rows with fully
-// coalesced, unrolled loads (up to STREAM/64 loads in flight per lane), keeps the products in
+// coalesced loads (rounds of kStageLoads per lane in flight), keeps the products in
 // its own LDS slice, then every lane sums whole rows out of LDS.  VECTOR: one row per wave.
 // LONG: one row split over the waves left in the workgroup, partial sums combined in LDS in
 // a fixed order.  One descriptor per wave and no grid-stride loop, so the single workgroup
@@ -322,23 +332,31 @@ __global__ __launch_bounds__( 256 ) void csrAdaptiveKernel( int32_t nblocks, con
          const int32_t r = lane + q * kWave;
          rbPre[ q ] = r <= d.w ? off[ firstRow + r ] : 0;
       }
-      int32_t c[ K ];
-      V v[ K ];
+      // rounds of kStageLoads loads per lane (one for FP64, up to two for FP32 blocks of 1024
+      // elements): the FP32 kernel keeps the FP64 kernel's registers and waves per SIMD
 #pragma unroll
-      for( int k = 0; k < K; k++ ) {
-         const int32_t e = min( lane + k * kWave, size - 1 );
-         c[ k ] = ldnt( col + b + e );
-         v[ k ] = ldnt( val + b + e );
-      }
-      Y xv[ K ];
+      for( int q = 0; q < K / kStageLoads; q++ ) {
+         const int32_t e0 = q * kStageLoads * kWave;
+         if( e0 >= size )  // uniform over the wave
+            break;
+         int32_t c[ kStageLoads ];
+         V v[ kStageLoads ];
 #pragma unroll
-      for( int k = 0; k < K; k++ )
-         xv[ k ] = x[ c[ k ] < 0 ? 0 : c[ k ] ];
+         for( int k = 0; k < kStageLoads; k++ ) {
+            const int32_t e = min( e0 + lane + k * kWave, size - 1 );
+            c[ k ] = ldnt( col + b + e );
+            v[ k ] = ldnt( val + b + e );
+         }
+         Y xv[ kStageLoads ];
 #pragma unroll
-      for( int k = 0; k < K; k++ ) {
-         const int32_t e = lane + k * kWave;
-         if( e < size )
-            lds[ wave ][ e ] = c[ k ] >= 0 ? toCompute( v[ k ], A() ) * (A) xv[ k ] : (A) 0;
+         for( int k = 0; k < kStageLoads; k++ )
+            xv[ k ] = x[ c[ k ] < 0 ? 0 : c[ k ] ];
+#pragma unroll
+         for( int k = 0; k < kStageLoads; k++ ) {
+            const int32_t e = e0 + lane + k * kWave;
+            if( e < size )
+               lds[ wave ][ e ] = c[ k ] >= 0 ? toCompute( v[ k ], A() ) * (A) xv[ k ] : (A) 0;
+         }
       }
       waveSync();
 #pragma unroll
