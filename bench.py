@@ -30,8 +30,8 @@ HBM_PEAK_GBS = 8000.0
 MALL_BYTES = 256 << 20
 # HBM bytes per launch at 160^3 FP64 from rocprofv3 --pmc FETCH_SIZE (x2, the factor a plain
 # 512 MiB stream reads) and WRITE_SIZE runs: profiles/spmv_laplacian160_traffic_pmc.md
-MEASURED_TRAFFIC = {"SlicedEllpack": 487.1e6, "Ellpack": 484.2e6, "ChunkedEllpack": 490.5e6, "CSRAdaptive": 493.1e6,
-                    "BiEllpack": 508.5e6}
+MEASURED_TRAFFIC = {"SlicedEllpack": 487.1e6, "Ellpack": 484.3e6, "ChunkedEllpack": 486.7e6, "CSRAdaptive": 493.0e6,
+                    "BiEllpack": 502.8e6, "CSRScalar": 511.5e6}
 FORMATS = ["CSRScalar", "CSRVector", "CSRLight", "CSRHybrid", "CSRAdaptive", "Ellpack", "SlicedEllpack",
            "ChunkedEllpack", "BiEllpack"]
 HEADLINE = "SlicedEllpack"
@@ -477,6 +477,16 @@ def main():
         log(f"{fmt:15s} {mean * 1e6:9.2f} us  {gbs:8.1f} GB/s  {100 * gbs / HBM_PEAK_GBS:5.1f}% of HBM peak"
             f"   (cold {cold * 1e6:8.2f} us, {B / cold / 1e9:7.1f} GB/s)")
         del A
+        # the same format with FP32 values and vectors
+        A32 = tnl.SparseMatrix.from_csr(rp, cols, vals, format=fmt, dtype="f32")
+        B32 = W.algorithmic_bytes(n, n, nnz, 4)
+        s32, c32 = time_spmv(tnl, torch, A32, x.float(), y.float(), args.steps, args.warmup, flush)
+        m32 = s32 if B32 > MALL_BYTES else c32
+        results[fmt].update({"f32_us": round(m32 * 1e6, 2), "f32_GB/s": round(B32 / m32 / 1e9, 1),
+                             "f32_frac_hbm": round(B32 / m32 / 1e9 / HBM_PEAK_GBS, 4)})
+        log(f"{fmt + ' f32':15s} {m32 * 1e6:9.2f} us  {B32 / m32 / 1e9:8.1f} GB/s  "
+            f"{100 * B32 / m32 / 1e9 / HBM_PEAK_GBS:5.1f}% of HBM peak")
+        del A32
         torch.cuda.empty_cache()
     # FP32 values/vectors for the headline kernel
     A32 = tnl.SparseMatrix.from_csr(rp, cols, vals, format=HEADLINE, dtype="f32")
