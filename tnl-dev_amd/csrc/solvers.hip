@@ -1605,6 +1605,20 @@ struct FinGmresStepEnd
    }
 };
 
+// The end of step i and the top of step i + 1 in one launch: the top runs only if the end left
+// the cycle open (as its own launch, gated on the same flag, would).
+struct FinGmresStepEndThenStart
+{
+   FinGmresStepEnd end;
+   FinGmresStepStart start;
+   __device__ void operator()() const
+   {
+      end();
+      if( ! end.st->flags[ 0 ] )
+         start();
+   }
+};
+
 // CGS: multi-reduction acc_k = (V_k, w) fused with the previous subtraction w -= V h_prev
 template< typename T, int C >
 struct OpCgs
@@ -2139,7 +2153,8 @@ void solveGMRES( SolverImpl& S, const void* b, void* xv )
          launchElementwise( n, OpScale< T >{ V, r, &st->beta_gm, true, 0 }, always, S.stream );
          for( int i = 0; i < m; i++ ) {
             T* vi = V + i * ld;
-            launchScalar( FinGmresStepStart{ st, q, stop, i }, cycle, S.stream );
+            if( i == 0 )  // later tops run in the previous step's closing launch
+               launchScalar( FinGmresStepStart{ st, q, stop, i }, cycle, S.stream );
             matvec( S, vi, w, cycle );
             double* Hcol = q.H + i * ( m + 1 );
             if( variant == TNLHIP_GMRES_MGS || variant == TNLHIP_GMRES_MGSR ) {
@@ -2164,8 +2179,11 @@ void solveGMRES( SolverImpl& S, const void* b, void* xv )
             }
             // v_{i+1} = w / |w|
             launchElementwise( n, OpScale< T >{ V + ( i + 1 ) * ld, w, &st->normw, true, 0 }, cycle, S.stream );
-            launchScalar( FinGmresStepEnd{ st, q, stop, S.hist.ptr, S.histCap, i, i + 1, i, true, i, true }, cycle,
-                          S.stream );
+            const FinGmresStepEnd end{ st, q, stop, S.hist.ptr, S.histCap, i, i + 1, i, true, i, true };
+            if( i + 1 < m )
+               launchScalar( FinGmresStepEndThenStart{ end, FinGmresStepStart{ st, q, stop, i + 1 } }, cycle, S.stream );
+            else
+               launchScalar( end, cycle, S.stream );
          }
       }
       else {
@@ -2173,7 +2191,8 @@ void solveGMRES( SolverImpl& S, const void* b, void* xv )
          launchElementwise( n, OpCopy< T >{ {}, z, r }, always, S.stream );
          for( int i = 0; i <= m; i++ ) {
             T* yi = V + i * ld;
-            launchScalar( FinGmresStepStart{ st, q, stop, -1 }, cycle, S.stream );
+            if( i == 0 )  // later tops run in the previous step's closing launch
+               launchScalar( FinGmresStepStart{ st, q, stop, -1 }, cycle, S.stream );
             // hauseholder_generate (for i > 0 its passes over z and Y ran fused with the previous
             // step's z = (I - Y T^T Y^T) w; only the small finish is left)
             if( i == 0 )
@@ -2205,8 +2224,11 @@ void solveGMRES( SolverImpl& S, const void* b, void* xv )
                else
                   launchCwyZGenerate< T, 32 >( n, z, w, V, ld, q.aux, i + 1, q, S.ws.slot( 3 ), cycle, S.stream );
             }
-            launchScalar( FinGmresStepEnd{ st, q, stop, S.hist.ptr, S.histCap, i, i, i - 1, i > 0, i - 1, i > 0 },
-                          cycle, S.stream );
+            const FinGmresStepEnd end{ st, q, stop, S.hist.ptr, S.histCap, i, i, i - 1, i > 0, i - 1, i > 0 };
+            if( i < m )
+               launchScalar( FinGmresStepEndThenStart{ end, FinGmresStepStart{ st, q, stop, -1 } }, cycle, S.stream );
+            else
+               launchScalar( end, cycle, S.stream );
          }
       }
       h = readState( S );
