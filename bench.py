@@ -93,6 +93,67 @@ def cpu_baseline(rp, cols, vals, x, ncols):
             "seconds_per_spmv": t}
 
 
+SMALL_SIDE = 256  # 2D 5-point Laplacian 256 x 256 (n = 65,536): the case TNL runs on its Host path
+
+
+def small_system(W):
+    rp, c, v = W.laplacian_2d(SMALL_SIDE)
+    n = len(rp) - 1
+    return rp, c, v, W.uniform_vector(n, 11), W.uniform_vector(n, 12)
+
+
+def small_suite(tnl, torch, W, dev, args):
+    """2D 5-point Laplacian 256^2, FP64, CSR (TNL's default CSR kernel, CSRScalar): one SpMV and
+    unpreconditioned CG to 1e-8 relative residual from x0 = 0 on the GPU (median of 5 solves)."""
+    rp, c, v, xv, bv = small_system(W)
+    n = len(rp) - 1
+    A = tnl.SparseMatrix.from_csr(rp, c, v, format="CSRScalar", dtype="f64")
+    x = torch.tensor(xv, device=dev)
+    y = torch.zeros(n, dtype=torch.float64, device=dev)
+    for _ in range(3):
+        A.vectorProduct(x, y)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(50):
+        A.vectorProduct(x, y)
+    e1.record()
+    e1.synchronize()
+    spmv_s = e0.elapsed_time(e1) * 1e-3 / 50
+    b = torch.tensor(bv, device=dev)
+    s = tnl.CG()
+    s.setMatrix(A)
+    s.setup(convergence_residue=1e-8, max_iterations=100000)
+    times = []
+    for _ in range(5):
+        xs = torch.zeros(n, dtype=torch.float64, device=dev)
+        ok, t = timed(torch, lambda: s.solve(b, xs))
+        times.append(t)
+    t = sorted(times)[len(times) // 2]
+    log(f"small 256^2 CSRScalar: SpMV {spmv_s * 1e6:.2f} us, CG {s.getIterations()} iterations {t * 1e3:.2f} ms, "
+        f"converged={ok}")
+    return {"workload": "2D 5-point Laplacian 256^2 (n=65536), FP64, CSRScalar; CG to 1e-8 from x0 = 0, no preconditioner",
+            "spmv_us": round(spmv_s * 1e6, 2),
+            "spmv_GB/s": round(W.algorithmic_bytes(n, n, len(c), 8) / spmv_s / 1e9, 1),
+            "cg": {"converged": ok, "iterations": s.getIterations(), "residue": s.getResidue(), "seconds": round(t, 5),
+                   "solves_per_s": round(1.0 / t, 1)}}
+
+
+def cpu_solve_baseline(W):
+    """TNL's Host CG (oracle/_ref, OpenMP over the box's cores) on the 256^2 system of small_suite."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    if not pyoracle.ref_available():
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    rp, c, v, _, bv = small_system(W)
+    r = pyoracle.solve("cg", rp, c, v, bv, backend="ref", threads=threads, convergence_residue=1e-8,
+                       max_iterations=100000)
+    return {"workload": "TNL Host CG, 2D 5-point Laplacian 256^2, to 1e-8 from x0 = 0", "cores": threads,
+            "kind": "reference", "iterations": r["iterations"], "seconds": round(r["seconds"], 5),
+            "solves_per_s": round(1.0 / r["seconds"], 1)}
+
+
 def timed(torch, fn):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -549,6 +610,7 @@ def main():
     }
     del x, y, flush
     torch.cuda.empty_cache()
+    out["small"] = small_suite(tnl, torch, W, dev, args)
     if not args.no_next:
         out["next"] = next_suite(tnl, torch, W, dev, args, results.get("SlicedEllpack", {}).get("us"))
     if not args.no_solvers:
@@ -562,6 +624,11 @@ def main():
     if not args.no_cpu:
         log("timing the CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(rp, cols, vals, x_host, n)
+        solve = cpu_solve_baseline(W)
+        if solve is not None:
+            out["cpu_baseline"]["cg_256x256"] = solve
+            log(f"CPU baseline: SpMV {out['cpu_baseline']['value']} GB/s; TNL Host CG 256^2 {solve['iterations']} "
+                f"iterations {solve['seconds'] * 1e3:.1f} ms on {solve['cores']} cores")
     print(json.dumps(out))
 
 
