@@ -42,7 +42,7 @@ void deviceDot( int dtype, int64_t n, const void* a, const void* b, double* resu
                 Workspace& ws, hipStream_t st )
 {
    ReduceSlot slot = ws.slot( 0 );
-   const unsigned grid = kReduceGrid;
+   const unsigned grid = reduceGridFor( 1, n );
    const bool sq = a == b;
    if( dtype == TNLHIP_F64 ) {
       if( sq )
@@ -129,7 +129,7 @@ template< typename T, int C >
 void launchMultidot( int64_t n, int32_t c, const T* V, int64_t ld, const T* w, ReduceSlot slot, double* result,
                      hipStream_t st )
 {
-   const int grid = reduceGridFor( C );
+   const int grid = reduceGridFor( C, n );
    multidotKernel< T, C ><<< grid, kReduceBlock, 0, st >>>( n, c, V, ld, w, slot );
    multidotFinalKernel< C ><<< 1, kReduceBlock, 0, st >>>( slot, grid, c, result );
 }

@@ -20,12 +20,20 @@ constexpr int kReduceBlock = 256;
 constexpr int kReduceGrid = 1024;  // partials per column (4 workgroups per CU on 256 CUs)
 constexpr int kMaxMulti = 32;      // columns per multi-reduction pass
 
-// Workgroups of a pass keeping W running sums per lane.  The 32-column passes need about 140
-// VGPRs (3 resident workgroups per CU), so their grid is one resident round instead of 1024
-// workgroups in a full and a one-third round; the grid (hence the sum order) is fixed per W.
-inline int reduceGridFor( int W )
+// Workgroups of a pass keeping W running sums per lane over n entries.  The 32-column passes
+// need about 140 VGPRs (3 resident workgroups per CU), so their grid is one resident round
+// instead of 1024 workgroups in a full and a one-third round.  One- and two-column passes over
+// short vectors (n < 1024 x 256) launch only the ceil(n / 256) workgroups that hold an entry
+// (one per thread) instead of 1024 mostly empty ones: the partials that remain are the same
+// and sumPartials adds them in the same order, so the sums keep their bits.  The grid (hence
+// the sum order) depends only on W and n.
+inline int reduceGridFor( int W, int64_t n )
 {
-   return W >= 32 ? std::min( kReduceGrid, 3 * deviceProps().cus ) : kReduceGrid;
+   const int full = W >= 32 ? std::min( kReduceGrid, 3 * deviceProps().cus ) : kReduceGrid;
+   if( W > 2 )
+      return full;
+   const int64_t need = std::max< int64_t >( 1, ( n + kReduceBlock - 1 ) / kReduceBlock );
+   return (int) std::min< int64_t >( full, need );
 }
 
 // Scratch of one reduction slot: kReduceGrid partials per column, column-major.

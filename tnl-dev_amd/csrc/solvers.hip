@@ -136,7 +136,7 @@ void launchElementwise( int64_t n, const Op& op, Gate gate, hipStream_t st )
 template< int W, typename Op, typename Fin >
 void launchReduce( int64_t n, const Op& op, const Fin& fin, ReduceSlot slot, Gate gate, hipStream_t st )
 {
-   const int grid = reduceGridFor( W );
+   const int grid = reduceGridFor( W, n );
    fusedReduceKernel< W ><<< grid, kReduceBlock, 0, st >>>( n, op, slot, gate );
    reduceFinalKernel< W ><<< 1, kReduceBlock, 0, st >>>( fin, slot, grid, gate );
    TNLHIP_CHECK( hipGetLastError() );
@@ -2047,17 +2047,18 @@ template< typename T, int C >
 void launchCwyZGenerate( int64_t n, T* z, const T* w, const T* Y, int64_t ld, const double* aux, int count, Small q,
                          ReduceSlot slot, Gate gate, hipStream_t st )
 {
-   fusedReduceKernel< C ><<< reduceGridFor( C ), kReduceBlock, 0, st >>>(
+   fusedReduceKernel< C ><<< reduceGridFor( C, n ), kReduceBlock, 0, st >>>(
       n, OpCwyZGenerate< T, C >{ z, w, Y, ld, aux, count, const_cast< T* >( Y ) + (int64_t) count * ld }, slot, gate );
    TNLHIP_CHECK( hipGetLastError() );
 }
 
 template< typename T, int C >
-void launchGenerateFinal( Small q, T* yi, const T* Y, int64_t ld, int i, ReduceSlot slot, Gate gate, hipStream_t st )
+void launchGenerateFinal( int64_t n, Small q, T* yi, const T* Y, int64_t ld, int i, ReduceSlot slot, Gate gate,
+                          hipStream_t st )
 {
+   // the partials of launchCwyZGenerate over the same n
    reduceFinalKernel< C ><<< 1, kReduceBlock, 0, st >>>( FinGenerateFused< T, C >{ q, yi, Y, ld, i }, slot,
-                                                         reduceGridFor( C ),
-                                                         gate );
+                                                         reduceGridFor( C, n ), gate );
    TNLHIP_CHECK( hipGetLastError() );
 }
 
@@ -2201,11 +2202,11 @@ void solveGMRES( SolverImpl& S, const void* b, void* xv )
             else {
                const int C = fusedBucket( i );
                if( C == 8 )
-                  launchGenerateFinal< T, 8 >( q, yi, V, ld, i, S.ws.slot( 3 ), cycle, S.stream );
+                  launchGenerateFinal< T, 8 >( n, q, yi, V, ld, i, S.ws.slot( 3 ), cycle, S.stream );
                else if( C == 16 )
-                  launchGenerateFinal< T, 16 >( q, yi, V, ld, i, S.ws.slot( 3 ), cycle, S.stream );
+                  launchGenerateFinal< T, 16 >( n, q, yi, V, ld, i, S.ws.slot( 3 ), cycle, S.stream );
                else
-                  launchGenerateFinal< T, 32 >( q, yi, V, ld, i, S.ws.slot( 3 ), cycle, S.stream );
+                  launchGenerateFinal< T, 32 >( n, q, yi, V, ld, i, S.ws.slot( 3 ), cycle, S.stream );
             }
             // hauseholder_apply_trunc (s or H column i-1)
             // apply_trunc, then hauseholder_cwy's aux = T_i Y_i^T e_i (for v = e_i - Y_i aux)
