@@ -5,8 +5,9 @@
 // lower triangle, diagonal included.  When beta != 1 the whole output vector is scaled by beta
 // first (TNL multiplies rather than clears, so a NaN in y survives beta = 0).  Then every stored
 // a_rc (c != -1) of a row r in [first, last) adds (alpha a_rc) x_r to y_c when c < r -- the
-// transposed entry -- and the row sum alpha * sum_c a_rc x_c is added to y_r.  Both additions are
-// device-scope floating-point atomics, as on TNL's GPU path; the row sum keeps TNL's Host order
+// transposed entry -- and the row sum alpha * sum_c a_rc x_c is added to y_r.  The additions are
+// floating-point atomics, as on TNL's GPU path (in LDS for rows of the wave's own block, then
+// one device atomic per row; device atomics for the rest); the row sum keeps TNL's Host order
 // (each product rounded, then added), the order in which the atomics land is not fixed.  One
 // lane per row over the format's index map; the solvers' Jacobi keeper divides by the diagonal
 // afterwards.
@@ -16,6 +17,22 @@ namespace tnlhip {
 
 namespace {
 
+// Each wavefront takes kSymRows consecutive rows (kSymRows / 64 per lane).  Additions that land
+// on a row of the wave's own block (the row sums, and transposed entries a_rc with c in the
+// block, e.g. c = r - 1 and c = r - 160 of a 3D stencil) meet in the wave's LDS slice through
+// LDS atomics; after a wave barrier every row of the block takes one device atomic.  Transposed
+// entries that leave the block go to y directly.  At 160^3 that is about 2.6 device atomics per
+// row instead of 4.  Wave barriers only (no workgroup barrier): the loop over blocks is uniform
+// per wave.
+constexpr int kSymRows = 256;
+
+__device__ __forceinline__ void waveBarrier()
+{
+   __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+   __builtin_amdgcn_wave_barrier();
+   __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+}
+
 template< typename View, typename V, typename A, typename Y >
 __global__ __launch_bounds__( 256 ) void symmetricKernel( int32_t first, int32_t last, View view,
                                                           const int32_t* __restrict__ col, const V* __restrict__ val,
@@ -23,22 +40,48 @@ __global__ __launch_bounds__( 256 ) void symmetricKernel( int32_t first, int32_t
 {
    if( ! gate.open() )
       return;
-   const int64_t stride = (int64_t) gridDim.x * blockDim.x;
-   for( int64_t r = first + (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < last; r += stride ) {
-      const A xr = (A) x[ r ];
-      const int32_t size = view.size( (int32_t) r );
-      A s = 0;
-      for( int32_t l = 0; l < size; l++ ) {
-         const int64_t g = view.global( (int32_t) r, l );
-         const int32_t c = col[ g ];
-         if( c < 0 )
+   constexpr int kPerLane = kSymRows / kWave;
+   __shared__ A acc[ 4 ][ kSymRows ];
+   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+   const int64_t step = (int64_t) gridDim.x * 4 * kSymRows;
+   for( int64_t w0 = first + ( (int64_t) blockIdx.x * 4 + wave ) * kSymRows; w0 < last; w0 += step ) {
+#pragma unroll
+      for( int k = 0; k < kPerLane; k++ )
+         acc[ wave ][ lane + k * kWave ] = 0;
+      waveBarrier();
+#pragma unroll
+      for( int k = 0; k < kPerLane; k++ ) {
+         const int64_t r = w0 + lane + k * kWave;
+         if( r >= last )
             continue;
-         const A a = toCompute( val[ g ], A() );
-         if( c < r )
-            unsafeAtomicAdd( y + c, (Y) ( ( alpha * a ) * xr ) );
-         s = mulAddExact( s, a, (A) x[ c ] );
+         const A xr = (A) x[ r ];
+         const int32_t size = view.size( (int32_t) r );
+         A s = 0;
+         for( int32_t l = 0; l < size; l++ ) {
+            const int64_t g = view.global( (int32_t) r, l );
+            const int32_t c = col[ g ];
+            if( c < 0 )
+               continue;
+            const A a = toCompute( val[ g ], A() );
+            if( c < r ) {
+               const A t = ( alpha * a ) * xr;
+               if( c >= w0 )
+                  atomicAdd( &acc[ wave ][ c - w0 ], t );
+               else
+                  unsafeAtomicAdd( y + c, (Y) t );
+            }
+            s = mulAddExact( s, a, (A) x[ c ] );
+         }
+         atomicAdd( &acc[ wave ][ r - w0 ], alpha * s );
       }
-      unsafeAtomicAdd( y + r, (Y) ( alpha * s ) );
+      waveBarrier();
+#pragma unroll
+      for( int k = 0; k < kPerLane; k++ ) {
+         const int64_t r = w0 + lane + k * kWave;
+         if( r < last )
+            unsafeAtomicAdd( y + r, (Y) acc[ wave ][ lane + k * kWave ] );
+      }
+      waveBarrier();  // the block's sums are read before the next block clears them
    }
 }
 
@@ -71,7 +114,7 @@ void launchSymmetric( const Matrix& M, const Y* x, Y* y, double alpha, double be
       scaleKernel< Y ><<< gridFor( M.rows, 256 ), 256, 0, st >>>( M.rows, y, (Y) beta, diag ? 1 : 0, gate );
    if( last > first )
       withView( M, [ & ]( auto view ) {
-         symmetricKernel<<< gridFor( last - first, 256 ), 256, 0, st >>>(
+         symmetricKernel<<< gridFor( ( (int64_t) last - first + kSymRows - 1 ) / kSymRows, 4 ), 256, 0, st >>>(
             first, last, view, M.columns.ptr, reinterpret_cast< const V* >( M.values.ptr ), x, y, (A) alpha, gate );
       } );
    if( diag && last > first )
