@@ -1703,43 +1703,66 @@ struct FinCgsNorm
 
 struct FinBacksolve
 {
+   // update( k ) (GMRES.hpp:641-719): y = H(0..k, 0..k)^-1 s; for CWY aux = T^T (YL^T y).  The
+   // recurrences are serial; they run on lane 0 over LDS copies that all 64 lanes load first (the
+   // same chain of dependent loads and stores in device memory took about 200 us per cycle).
    State* st;
    Small q;
    bool cwy;
-   __device__ void operator()() const
+   __device__ void operator()( int lane ) const
    {
+      __shared__ double Hs[ kMaxMulti * kMaxMulti ], Ts[ kMaxMulti * kMaxMulti ], YLs[ kMaxMulti * kMaxMulti ];
+      __shared__ double ys[ kMaxMulti ], auxs[ kMaxMulti ];
+      __shared__ int bad;
       State& s = *st;
-      const int m = q.m;
+      const int m = q.m, mx = q.mx;  // m <= mx <= kMaxMulti - 1
       const int k = s.o_steps < m ? s.o_steps : m - 1;
-      double* y = q.y;
-      for( int i = 0; i <= m; i++ )
-         y[ i ] = q.s[ i ];
-      for( int i = k; i >= 0; i-- ) {
-         if( q.H[ i + i * ( m + 1 ) ] == 0 ) {
-            s.breakdown = 1;
-            s.flags[ 0 ] = 1;
-            return;
+      for( int i = lane; i < ( m + 1 ) * m; i += kWave )
+         Hs[ i ] = q.H[ i ];
+      for( int i = lane; i <= m; i += kWave )
+         ys[ i ] = q.s[ i ];
+      if( cwy )
+         for( int i = lane; i < ( mx + 1 ) * ( mx + 1 ); i += kWave ) {
+            Ts[ i ] = q.T[ i ];
+            YLs[ i ] = q.YL[ i ];
          }
-         y[ i ] /= q.H[ i + i * ( m + 1 ) ];
-         for( int j = i - 1; j >= 0; j-- )
-            y[ j ] -= q.H[ j + i * ( m + 1 ) ] * y[ i ];
-      }
-      if( cwy ) {
-         const int mx = q.mx;
-         double* aux = q.aux;
-         for( int j = 0; j <= k; j++ )
-            aux[ j ] = 0;
-         for( int j = 0; j <= k; j++ )
-            for( int i = 0; i <= j; i++ )
-               aux[ i ] += y[ j ] * q.YL[ j + i * ( mx + 1 ) ];
-         for( int i = 0; i <= k; i++ ) {
-            double a2 = 0.0;
-            for( int j = i; j <= k; j++ )
-               a2 += q.T[ i + j * ( mx + 1 ) ] * aux[ j ];
-            aux[ i ] = a2;
+      __syncthreads();
+      if( lane == 0 ) {
+         bad = 0;
+         for( int i = k; i >= 0 && ! bad; i-- ) {
+            if( Hs[ i + i * ( m + 1 ) ] == 0 ) {
+               s.breakdown = 1;
+               s.flags[ 0 ] = 1;
+               bad = 1;
+               break;
+            }
+            ys[ i ] /= Hs[ i + i * ( m + 1 ) ];
+            for( int j = i - 1; j >= 0; j-- )
+               ys[ j ] -= Hs[ j + i * ( m + 1 ) ] * ys[ i ];
+         }
+         if( ! bad ) {
+            if( cwy ) {
+               for( int j = 0; j <= k; j++ )
+                  auxs[ j ] = 0;
+               for( int j = 0; j <= k; j++ )
+                  for( int i = 0; i <= j; i++ )
+                     auxs[ i ] += ys[ j ] * YLs[ j + i * ( mx + 1 ) ];
+               for( int i = 0; i <= k; i++ ) {
+                  double a2 = 0.0;
+                  for( int j = i; j <= k; j++ )
+                     a2 += Ts[ i + j * ( mx + 1 ) ] * auxs[ j ];
+                  auxs[ i ] = a2;
+               }
+            }
+            s.tmp = (double) ( k + 1 );  // number of columns
          }
       }
-      s.tmp = (double) ( k + 1 );  // number of columns
+      __syncthreads();
+      for( int i = lane; i <= m; i += kWave )
+         q.y[ i ] = ys[ i ];
+      if( cwy && ! bad )
+         for( int i = lane; i <= k; i += kWave )
+            q.aux[ i ] = auxs[ i ];
    }
 };
 
@@ -2235,7 +2258,7 @@ void solveGMRES( SolverImpl& S, const void* b, void* xv )
       h = readState( S );
       const int o_steps = h.o_steps;
       // update( k ) with k = o_steps (early) or m - 1
-      launchScalar( FinBacksolve{ st, q, cwy }, always, S.stream );
+      launchWave( FinBacksolve{ st, q, cwy }, always, S.stream );
       h = readState( S );
       if( h.breakdown )
          throw Error( TNLHIP_ERROR_BREAKDOWN, "GMRES: zero diagonal entry of the Hessenberg matrix" );
