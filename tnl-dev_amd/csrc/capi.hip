@@ -460,6 +460,28 @@ int tnlhip_matrix_save( tnlhip_matrix A, const char* path, int layout )
    } );
 }
 
+// Whether every chunk of a ChunkedEllpack storage (device order: slot k of chunk c at
+// pointer + c + 256 k) keeps its padding after its elements.  Storage read from a file may not.
+bool chunkedPaddingTrails( const HostSegments& meta, const std::vector< int32_t >& cols )
+{
+   for( int32_t s = 0; s < meta.numberOfSlices; s++ ) {
+      const int32_t chunkSize = meta.slices[ 4 * s + 1 ], pointer = meta.slices[ 4 * s + 3 ];
+      for( int32_t c = 0; c < kChunksInSlice; c++ ) {
+         bool padding = false;
+         for( int32_t k = 0; k < chunkSize; k++ ) {
+            const int64_t g = (int64_t) pointer + c + (int64_t) kChunksInSlice * k;
+            if( g >= (int64_t) cols.size() )
+               return false;
+            if( cols[ g ] < 0 )
+               padding = true;
+            else if( padding )
+               return false;
+         }
+      }
+   }
+   return true;
+}
+
 int tnlhip_matrix_load( const char* path, int format, int value_dtype, int layout, tnlhip_matrix* out )
 {
    return guarded( [ & ] {
@@ -479,6 +501,8 @@ int tnlhip_matrix_load( const char* path, int format, int value_dtype, int layou
       const int64_t capTotal = kindOf( fmt ) == KIND_CSR ? f.meta.allocated : 0;
       std::unique_ptr< tnlhip_matrix_s > M(
          buildFromSegments( fmt, dtype, f.rows, f.columns, std::move( f.meta ), capTotal ) );
+      if( fmt == TNLHIP_CHUNKED_ELLPACK )
+         M->trailingPadding = chunkedPaddingTrails( M->meta, f.columnIndexes );
       M->values.upload( f.values.data(), (int64_t) f.values.size() );
       M->columns.upload( f.columnIndexes );
       M->nnz = countStored( *M, nullptr );

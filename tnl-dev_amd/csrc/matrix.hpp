@@ -89,7 +89,10 @@ struct Matrix
    std::vector< int32_t > adaptiveHost;
    int32_t adaptiveBlocks = 0;
    int32_t lanes = 0;      // lanes per row for CSR light / hybrid / vector
-   int32_t xcdRemap = 0;      // row distribution of the padded kernels: 0 grid-stride, 1 XCD-remapped blocks
+   int32_t xcdRemap = 0;
+   // every chunk of a ChunkedEllpack matrix keeps its padding after its elements (true for
+   // storage built here from CSR; checked when storage is read from a file)
+   bool trailingPadding = true;      // row distribution of the padded kernels: 0 grid-stride, 1 XCD-remapped blocks
    int32_t pairRows = 0;      // padded kernels: two adjacent rows per lane with 8 / 16-byte loads
    int32_t blocksPerCu = 16;  // workgroups per CU the grid of the padded kernels is sized for
    bool symmetric = false;    // SymmetricMatrix: the lower triangle is stored

@@ -630,13 +630,17 @@ __global__ __launch_bounds__( 256 ) void slicedKernel( SlicedSlots slots, const 
 // One 256-lane workgroup per slice, one lane per chunk (the format is built around 256
 // chunks per slice): chunk c's elements sit at pointer + c + 256*k, coalesced across the
 // workgroup.  Chunk sums meet in LDS and each row adds up its chunk range
-// (rowToChunkMapping is an inclusive per-slice prefix of chunk counts).
+// (rowToChunkMapping is an inclusive per-slice prefix of chunk counts).  A chunk's padding
+// follows its elements, so with `stopAtPadding` a wavefront stops reading once every lane's
+// chunk has reached padding: a slice's chunk size is set by its longest row, and short rows
+// beside long ones would otherwise stream their padding in full.
 template< typename V, typename A, typename Y >
 __global__ __launch_bounds__( 256 ) void chunkedKernel( int32_t nslices, const int4* __restrict__ slices,
                                                         const int32_t* __restrict__ rowToChunk,
                                                         const int32_t* __restrict__ col,
                                                         const V* __restrict__ val, const Y* __restrict__ x,
-                                                        Keeper< A, Y > keep, int32_t first, int32_t last )
+                                                        Keeper< A, Y > keep, int32_t first, int32_t last,
+                                                        int stopAtPadding )
 {
    if( keep.closed() )
       return;
@@ -666,6 +670,9 @@ __global__ __launch_bounds__( 256 ) void chunkedKernel( int32_t nslices, const i
          for( int j = 0; j < kChunk; j++ )
             if( k0 + j < s.y && c[ j ] >= 0 )
                p += toCompute( v[ j ], A() ) * (A) xv[ j ];
+         // the last slot read is padding: so is the rest of this lane's chunk
+         if( stopAtPadding && __all( c[ kChunk - 1 ] < 0 ) )
+            break;  // wave-uniform; the loop holds no barrier
       }
       if( onePerRow ) {  // uniform: chunk t is row s.z + t, no exchange through LDS
          const int32_t row = s.z + t;
@@ -888,7 +895,7 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
             // workgroups per CU striding over the slices
             chunkedKernel< V, A, Y ><<< M.meta.numberOfSlices, 256, 0, st >>>(
                M.meta.numberOfSlices, reinterpret_cast< const int4* >( M.slices.ptr ), M.rowToChunk.ptr, col, val, x,
-               keep, first, last );
+               keep, first, last, M.trailingPadding ? 1 : 0 );
          break;
       case TNLHIP_BIELLPACK: {
          const int32_t strips = M.meta.virtualRows / kStrip;
