@@ -132,3 +132,48 @@ def test_round_trip_and_type_checks(tnl, dev, files, tmp_path):
     np.testing.assert_array_equal(product(tnl, T, xs), product(tnl, S, xs))
     with pytest.raises(tnl.TnlHipError, match="double or float values only"):
         tnl.SparseMatrix.from_csr(rp, ci, v, columns=nc, format="SlicedEllpack", dtype="bf16").save(p)
+
+
+def test_chunked_file_with_padding_inside_a_chunk(tnl, dev, tmp_path):
+    """TNL's save writes a chunk's padding after its elements, and the ChunkedEllpack kernel stops
+    a wavefront once every lane's chunk has reached padding.  A file whose chunk holds padding
+    before an element (slot 7 emptied, slots 8-11 kept; never written by TNL's save nor by this
+    library) is loaded with the full walk, and the product skips the padding slot and counts the
+    elements after it, as every kernel here does with a stored -1 (TNL's Host traversal would
+    end the row at it)."""
+    import torch
+    n = 200
+    lens = np.full(n, 5)
+    lens[0] = 1000  # the long row sets the slice's chunk size (18)
+    lens[150] = 12
+    rng = np.random.default_rng(4)
+    ncols = 2000
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    cols = np.concatenate([np.sort(rng.choice(ncols, k, replace=False)) for k in lens]).astype(np.int32)
+    vals = rng.uniform(-1, 1, len(cols))
+    A = tnl.SparseMatrix.from_csr(rp, cols, vals, columns=ncols, format="ChunkedEllpack", dtype="f64")
+    p = tmp_path / "chunked.tnl"
+    A.save(p)
+    data = bytearray(p.read_bytes())
+    f = TF.fields(data)
+    r2c = TF.read(data, f, "rowToChunkMapping")
+    slices = TF.read(data, f, "slices").reshape(-1, 4)
+    assert slices[0, 0] == n, slices[0]  # one slice holds every row
+    chunk_size, pointer = int(slices[0, 1]), int(slices[0, 3])
+    c150 = int(r2c[149])
+    # row 150 owns one chunk, in a wavefront (chunks 192-255) whose other rows end before slot 7
+    assert r2c[150] - r2c[149] == 1 and chunk_size >= 12 and c150 >= 192, (r2c[148:151], chunk_size)
+    stored = TF.read(data, f, "columnIndexes").copy()
+    g = pointer + c150 + 256 * 7  # slot 7 of row 150's chunk
+    assert stored[g] >= 0 and stored[g + 256] >= 0
+    removed_col, removed_val = int(stored[g]), float(TF.read(data, f, "values")[g])
+    stored[g] = -1
+    off = f["columnIndexes"][0]
+    data[off:off + 4 * len(stored)] = stored.astype(np.int32).tobytes()
+    p.write_bytes(bytes(data))
+    B = tnl.SparseMatrix.load(p)
+    x = torch.linspace(-1.0, 1.0, ncols, dtype=torch.float64, device=dev)
+    got = product(tnl, B, x)
+    want = product(tnl, A, x)
+    want[150] -= removed_val * float(x[removed_col])
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
