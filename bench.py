@@ -486,8 +486,8 @@ def main():
     ap.add_argument("--only", default="", help="run one suite: solvers | serving | gmres | irregular | next")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
-                                   "(requests shard across GPUs by request, see DESIGN.md)"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus above 1 is not supported "
+                                   "(several GPUs would each serve their own requests and matrices)"}))
         sys.exit(2)
 
     import torch
