@@ -39,6 +39,18 @@ FORMATS = ["CSRScalar", "CSRVector", "CSRLight", "CSRHybrid", "CSRAdaptive", "El
 HEADLINE = "ChunkedEllpack"
 
 
+def kernel_of(fmt, n, nnz):
+    """The kernel a format's product runs: CSRVector / CSRLight / CSRHybrid on rows averaging at
+    most 8 stored elements take CSRScalar's staged kernel (lanes of a cooperative row group would
+    mostly idle); CSRAdaptive keeps its block descriptors."""
+    if fmt == "CSRScalar" or (fmt in ("CSRVector", "CSRLight", "CSRHybrid") and nnz <= 8 * n):
+        return "csrScalarStagedKernel"
+    return {"CSRVector": "csrGroupedKernel (one wavefront per row)", "CSRLight": "csrGroupedKernel",
+            "CSRHybrid": "csrGroupedKernel", "CSRAdaptive": "csrAdaptiveKernel", "Ellpack": "ellpackKernel",
+            "SlicedEllpack": "slicedKernel", "ChunkedEllpack": "chunkedKernel",
+            "BiEllpack": "biellpackKernel"}[fmt]
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -533,7 +545,8 @@ def main():
         # measurement, the flushed (cold) figure is reported beside it
         mean = steady if B > MALL_BYTES else cold
         gbs = B / mean / 1e9
-        results[fmt] = {"us": round(mean * 1e6, 2), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+        results[fmt] = {"kernel": kernel_of(fmt, n, nnz),
+                        "us": round(mean * 1e6, 2), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
                         "us_cold": round(cold * 1e6, 2), "GB/s_cold": round(B / cold / 1e9, 1),
                         "tnl_GiB/s": round(tnl_bytes / mean / 2 ** 30, 1),
                         "setup_s": round(setup, 3), "storage": A.getStorageSize()}

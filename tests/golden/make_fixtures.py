@@ -14,7 +14,8 @@ Outputs (numpy .npz, data only):
   tnlfile.npz   TNL's binary SparseMatrix::save of small matrices in every matrix type, both
                 organizations and both Real types (file bytes)
   solvers.npz   residual histories, iteration counts, convergence flags and solutions of
-                CG / BiCGStab / GMRES (all variants, with and without Jacobi) at one thread
+                CG / BiCGStab / GMRES (all variants, with and without Jacobi) at one thread, and
+                the iteration counts of the same runs at 2, 3, 4 and 8 threads
 """
 import os
 import sys
@@ -220,6 +221,11 @@ def write_solver_fixtures():
         sol[f"{key}/params"] = np.array(repr(sorted(kw.items())))
         sol[f"{key}/history"] = r["history"]
         sol[f"{key}/iterations"] = np.array([r["iterations"]])
+        # TNL's own count with 2, 3, 4 and 8 OpenMP threads (another reduction order): how far
+        # the count of this run moves with the rounding of its dot products alone
+        sol[f"{key}/iterations_threads"] = np.array(
+            [r["iterations"]] + [O.solve(method, rp, c, v, sol[f"{name}/b"], backend="ref", threads=t, **kw)["iterations"]
+                                 for t in (2, 3, 4, 8)])
         sol[f"{key}/converged"] = np.array([r["converged"]])
         sol[f"{key}/x"] = r["x"]
     np.savez_compressed(os.path.join(HERE, "solvers.npz"), **sol)

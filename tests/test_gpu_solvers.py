@@ -76,8 +76,9 @@ def _run_device_solver(tnl, dev, method, rp, c, v, b, fmt="CSRLight", dtype="f64
 
 def test_solvers_follow_tnl_histories(tnl, golden, dev):
     """Every fixture run (CG, BiCGStab, all GMRES variants, with and without Jacobi): same
-    convergence flag, iteration count within 2, residual history within 1e-6 relative over the
-    first iterations (the device sums in a different order), true residual at the tolerance."""
+    convergence flag, iteration count within 2 of the range TNL's own count spans over OpenMP
+    thread counts, residual history within 1e-6 relative over the first iterations (the device
+    sums in a different order), true residual at the tolerance."""
     d = golden["solvers"]
     runs = sorted({k.split("/")[0] for k in d.files if k.startswith("run")})
     errors = []
@@ -98,7 +99,12 @@ def _check_run(tnl, d, dev, run):
     ok, its, res, hist, x = _run_device_solver(tnl, dev, method, rp, c, v, b, **params)
     ref_its = int(d[f"{run}/iterations"][0])
     assert ok == bool(d[f"{run}/converged"][0]), run
-    assert abs(its - ref_its) <= max(2, params.get("bicgstab_ell", 1)), (run, its, ref_its)
+    # within 2 (ell for BiCGStab(ell)) of TNL's count, or of the range TNL's own count spans over
+    # OpenMP thread counts: BiCGStab on the 2D Laplacian moves by up to 10 iterations with the
+    # reduction order of its dot products alone
+    spread = d[f"{run}/iterations_threads"]
+    slack = max(2, params.get("bicgstab_ell", 1))
+    assert min(spread) - slack <= its <= max(spread) + slack, (run, its, ref_its, list(spread))
     h_ref = d[f"{run}/history"]
     m = min(len(hist), len(h_ref), 20)
     # residues near 1e-12 and below are exact residues b - A x at the rounding floor of A x

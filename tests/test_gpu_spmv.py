@@ -109,6 +109,32 @@ def test_csr_scalar_staged_and_walked_chunks_bit_exact(tnl, dev, dtype):
         np.testing.assert_array_equal(got, want.astype(np.float64), err_msg=f"{dtype} {alpha} {first} {last}")
 
 
+@pytest.mark.parametrize("fmt,lanes", [("CSRVector", 64), ("CSRVector", 4), ("CSRLight", 8), ("CSRLight", 2),
+                                       ("CSRHybrid", 8), ("CSRHybrid", 16)])
+def test_cooperative_csr_kernels_on_short_rows(tnl, dev, fmt, lanes):
+    """Rows averaging at most 8 elements run CSRScalar's staged kernel in every CSR variant (the
+    Host bits); lanes_per_row set explicitly keeps the cooperative kernel, which agrees with the
+    oracle to 1e-13 of sum |a_ij x_j|, for every keeper and row range."""
+    rng = np.random.default_rng(31)
+    for rp, c, v in (W.laplacian_3d(20), W.irregular(5000, mean=5, sigma=0.8, max_len=200, seed=3)):
+        n = len(rp) - 1
+        x = rng.uniform(-1, 1, n)
+        y0 = rng.uniform(-1, 1, n)
+        seg = O.Segments("CSR", np.diff(rp))
+        sv, sc = seg.convert(rp, c, v)
+        scale = scale_of(rp, c, v, x)
+        A = tnl.SparseMatrix.from_csr(rp, c, v, format=fmt, dtype="f64")
+        exact = run_spmv(tnl, A, x, dev)
+        np.testing.assert_array_equal(exact, seg.spmv(sv, sc, x))  # staged: the Host bits
+        A.set_param("lanes_per_row", lanes)
+        for alpha, beta, first, last in [(1.0, 0.0, 0, 0), (2.0, -0.5, 17, n - 3)]:
+            print(f"{fmt} lanes={lanes} n={n} alpha={alpha} beta={beta} rows [{first}, {last})", flush=True)
+            want = seg.spmv(sv, sc, x, y0, alpha, beta, first, last)
+            got = run_spmv(tnl, A, x, dev, y0, alpha, beta, first, last)
+            tol = FP64_TOL * (abs(alpha) * scale + abs(beta) * np.abs(y0)) + 1e-300
+            assert np.all(np.abs(got - want) <= tol), (fmt, lanes, alpha, beta)
+
+
 @pytest.mark.parametrize("fmt", FORMATS)
 def test_keepers_row_ranges_and_value_types(tnl, dev, fmt):
     rng = np.random.default_rng(3)

@@ -689,6 +689,7 @@ int tnlhip_matrix_set_param( tnlhip_matrix A, const char* name_c, int32_t value 
          TNLHIP_REQUIRE( kindOf( A->format ) == KIND_CSR && A->format != TNLHIP_CSR_SCALAR,
                          "lanes_per_row applies to CSR light / hybrid / vector / adaptive fallback" );
          A->lanes = value;
+         A->lanesExplicit = true;
       }
       else if( name == "xcd_remap" ) {
          // 2 (contiguous blocks without the XCD remap) is switched off: a run of it ended in a

@@ -89,6 +89,7 @@ struct Matrix
    std::vector< int32_t > adaptiveHost;
    int32_t adaptiveBlocks = 0;
    int32_t lanes = 0;      // lanes per row for CSR light / hybrid / vector
+   bool lanesExplicit = false;  // lanes_per_row set by the caller: the cooperative kernel always runs
    int32_t xcdRemap = 0;
    // every chunk of a ChunkedEllpack matrix keeps its padding after its elements (true for
    // storage built here from CSR; checked when storage is read from a file)

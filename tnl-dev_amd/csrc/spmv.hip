@@ -212,6 +212,7 @@ __global__ __launch_bounds__( 256 ) void csrGroupedKernel( const int32_t* __rest
 // 160^3 FP64 neither a persistent grid that prefetches the next block's offsets (78-92 us by
 // grid size) nor two 64-row blocks per wave (83 us; FP32 61.1 against 59.4 us) beat it (79 us).
 constexpr int kStageLoads = 8;
+constexpr int kShortRow = 8;  // mean row length up to which every CSR kernel takes the staged path
 
 template< int K, typename V, typename A, typename Y >
 __global__ __launch_bounds__( 256 ) void csrScalarStagedKernel( const int32_t* __restrict__ off,
@@ -807,16 +808,18 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
    const int64_t rows = last - first;
    if( rows <= 0 )
       return;
+   auto launchStaged = [ & ] {
+      constexpr int K = kAdaptiveStreamBytesPerWave / sizeof( A ) / kWave;
+      unsigned grid = (unsigned) ( ( rows + 255 ) / 256 );
+      if( M.xcdRemap )
+         grid = ( grid + kXcds - 1 ) / kXcds * kXcds;  // logicalBlock needs a multiple of 8
+      csrScalarStagedKernel< K, V, A, Y >
+         <<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last, M.xcdRemap ? 1 : 0 );
+   };
    switch( M.format ) {
-      case TNLHIP_CSR_SCALAR: {
-         constexpr int K = kAdaptiveStreamBytesPerWave / sizeof( A ) / kWave;
-         unsigned grid = (unsigned) ( ( rows + 255 ) / 256 );
-         if( M.xcdRemap )
-            grid = ( grid + kXcds - 1 ) / kXcds * kXcds;  // logicalBlock needs a multiple of 8
-         csrScalarStagedKernel< K, V, A, Y >
-            <<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last, M.xcdRemap ? 1 : 0 );
+      case TNLHIP_CSR_SCALAR:
+         launchStaged();
          break;
-      }
       case TNLHIP_CSR_VECTOR:
       case TNLHIP_CSR_LIGHT:
       case TNLHIP_CSR_HYBRID:
@@ -831,7 +834,14 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
                M.xcdRemap ? 1 : 0 );
             break;
          }
-         const int G = M.format == TNLHIP_CSR_VECTOR ? 64 : M.lanes;
+         // rows of at most kShortRow stored elements on average: most lanes of a cooperative row
+         // group would idle (a wavefront per 7-element row uses 7 of 64), so unless lanes_per_row
+         // was set the rows take CSRScalar's staged kernel (the Host result bit for bit)
+         if( ! M.lanesExplicit && M.meta.allocated <= (int64_t) kShortRow * M.rows ) {
+            launchStaged();
+            break;
+         }
+         const int G = M.format == TNLHIP_CSR_VECTOR && ! M.lanesExplicit ? 64 : M.lanes;
          const int gpb = 256 / G;
          const unsigned grid = gridFor( rows, gpb );
          switch( G ) {
