@@ -2075,13 +2075,35 @@ void launchCwyZGenerate( int64_t n, T* z, const T* w, const T* Y, int64_t ld, co
    TNLHIP_CHECK( hipGetLastError() );
 }
 
+// The generate finish, then apply_trunc and aux = T_i Y_i^T e_i, in one workgroup: every lane
+// calls each part (they hold workgroup barriers), parts are separated by barriers so each sees
+// the previous one's writes.
 template< typename T, int C >
-void launchGenerateFinal( int64_t n, Small q, T* yi, const T* Y, int64_t ld, int i, ReduceSlot slot, Gate gate,
-                          hipStream_t st )
+struct FinGenerateApplyAux
+{
+   FinGenerateFused< T, C > generate;
+   FinYL< T > applyTrunc;
+   FinCwyAux aux;
+   static constexpr bool kLaneParallel = true;
+   __device__ void operator()( const double* v, int lane ) const
+   {
+      generate( v, lane );
+      __syncthreads();
+      applyTrunc( lane );
+      __syncthreads();
+      aux( lane );
+   }
+};
+
+template< typename T, int C >
+void launchGenerateFinal( int64_t n, State* state, Small q, T* yi, const T* z, const T* Y, int64_t ld, int i,
+                          ReduceSlot slot, Gate gate, hipStream_t st )
 {
    // the partials of launchCwyZGenerate over the same n
-   reduceFinalKernel< C ><<< 1, kReduceBlock, 0, st >>>( FinGenerateFused< T, C >{ q, yi, Y, ld, i }, slot,
-                                                         reduceGridFor( C, n ), gate );
+   reduceFinalKernel< C ><<< 1, kReduceBlock, 0, st >>>(
+      FinGenerateApplyAux< T, C >{ FinGenerateFused< T, C >{ q, yi, Y, ld, i }, FinYL< T >{ state, q, yi, z, i, n },
+                                   FinCwyAux{ q, i } },
+      slot, reduceGridFor( C, n ), gate );
    TNLHIP_CHECK( hipGetLastError() );
 }
 
@@ -2223,18 +2245,21 @@ void solveGMRES( SolverImpl& S, const void* b, void* xv )
                launchReduce< 1 >( n, OpTruncate< T >{ {}, yi, z, i }, FinHouseholder< T >{ st, q, yi, i },
                                   S.ws.slot( 0 ), cycle, S.stream );
             else {
+               // with apply_trunc (s or H column i-1) and hauseholder_cwy's aux = T_i Y_i^T e_i
+               // (for v = e_i - Y_i aux) in the same launch
                const int C = fusedBucket( i );
                if( C == 8 )
-                  launchGenerateFinal< T, 8 >( n, q, yi, V, ld, i, S.ws.slot( 3 ), cycle, S.stream );
+                  launchGenerateFinal< T, 8 >( n, st, q, yi, z, V, ld, i, S.ws.slot( 3 ), cycle, S.stream );
                else if( C == 16 )
-                  launchGenerateFinal< T, 16 >( n, q, yi, V, ld, i, S.ws.slot( 3 ), cycle, S.stream );
+                  launchGenerateFinal< T, 16 >( n, st, q, yi, z, V, ld, i, S.ws.slot( 3 ), cycle, S.stream );
                else
-                  launchGenerateFinal< T, 32 >( n, q, yi, V, ld, i, S.ws.slot( 3 ), cycle, S.stream );
+                  launchGenerateFinal< T, 32 >( n, st, q, yi, z, V, ld, i, S.ws.slot( 3 ), cycle, S.stream );
             }
             // hauseholder_apply_trunc (s or H column i-1)
             // apply_trunc, then hauseholder_cwy's aux = T_i Y_i^T e_i (for v = e_i - Y_i aux)
-            launchWave( Then< FinYL< T >, FinCwyAux >{ FinYL< T >{ st, q, yi, z, i, n }, FinCwyAux{ q, i } }, cycle,
-                        S.stream );
+            if( i == 0 )
+               launchWave( Then< FinYL< T >, FinCwyAux >{ FinYL< T >{ st, q, yi, z, i, n }, FinCwyAux{ q, i } },
+                           cycle, S.stream );
             launchElementwise( n, OpCwyV< T >{ vtmp, V, ld, q.aux, i + 1, i }, cycle, S.stream );
             if( i < m ) {
                matvec( S, vtmp, w, cycle );
