@@ -202,13 +202,15 @@ int tnlhip_matrix_get_storage( tnlhip_matrix A, double* values, int32_t* columns
  * of the vector type. */
 int tnlhip_matrix_get_diagonal( tnlhip_matrix A, void* diagonal, void* stream );
 
-/* Overrides a kernel parameter chosen at construction: "lanes_per_row" (CSR light / hybrid /
+/* Overrides a kernel parameter chosen at construction (ChunkedEllpack with FP32 / BF16 values
+ * starts with "xcd_remap" 1, every other matrix with 0): "lanes_per_row" (CSR light / hybrid /
  * vector: a power of two in [1, 64]; once set, that many lanes share each row even when rows
  * average at most 8 stored elements, where these kernels otherwise take CSRScalar's staged
- * one-lane-per-row path); for the Ellpack / SlicedEllpack kernels "xcd_remap"
- * (row distribution: 0 grid-stride, the default, or 1 contiguous row blocks with each XCD
- * owning a contiguous eighth), "pair_rows" (0/1: two adjacent rows per lane with
- * 8 / 16-byte loads) and "blocks_per_cu" (workgroups per CU the grid is sized for). */
+ * one-lane-per-row path); "xcd_remap" (row distribution: 0 the plain order, 1 contiguous row
+ * blocks -- slices for ChunkedEllpack -- with each XCD owning a contiguous eighth; 3 per-XCD
+ * grid-stride, Ellpack / SlicedEllpack only); for the Ellpack / SlicedEllpack kernels
+ * "pair_rows" (0/1: two adjacent rows per lane with 8 / 16-byte loads) and "blocks_per_cu"
+ * (workgroups per CU the grid is sized for). */
 int tnlhip_matrix_set_param( tnlhip_matrix A, const char* name, int32_t value );
 
 /* SparseMatrix::vectorProduct( x, y, alpha, beta, firstRow, lastRow ):

@@ -641,13 +641,14 @@ __global__ __launch_bounds__( 256 ) void chunkedKernel( int32_t nslices, const i
                                                         const int32_t* __restrict__ col,
                                                         const V* __restrict__ val, const Y* __restrict__ x,
                                                         Keeper< A, Y > keep, int32_t first, int32_t last,
-                                                        int stopAtPadding )
+                                                        int stopAtPadding, int remap )
 {
    if( keep.closed() )
       return;
    __shared__ A part[ kChunksInSlice ];
    const int t = threadIdx.x;
-   for( int32_t sl = blockIdx.x; sl < nslices; sl += gridDim.x ) {
+   // remap: XCD x (blockIdx % 8) takes the x-th eighth of the slices, in order
+   for( int64_t sl = logicalBlock( remap ); sl < nslices; sl += gridDim.x ) {
       const int4 s = slices[ sl ];  // size, chunkSize, firstSegment, pointer
       if( s.z >= last || s.z + s.x <= first )
          continue;  // uniform over the workgroup
@@ -903,9 +904,11 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
          if( M.meta.numberOfSlices > 0 )
             // one workgroup per slice: at 160^3 70.6 us, against 78.1 / 72.7 us for grids of 8 / 32
             // workgroups per CU striding over the slices
-            chunkedKernel< V, A, Y ><<< M.meta.numberOfSlices, 256, 0, st >>>(
+            chunkedKernel< V, A, Y ><<< M.xcdRemap ? ( M.meta.numberOfSlices + kXcds - 1 ) / kXcds * kXcds
+                                                   : M.meta.numberOfSlices,
+                                        256, 0, st >>>(
                M.meta.numberOfSlices, reinterpret_cast< const int4* >( M.slices.ptr ), M.rowToChunk.ptr, col, val, x,
-               keep, first, last, M.trailingPadding ? 1 : 0 );
+               keep, first, last, M.trailingPadding ? 1 : 0, M.xcdRemap ? 1 : 0 );
          break;
       case TNLHIP_BIELLPACK: {
          const int32_t strips = M.meta.virtualRows / kStrip;
