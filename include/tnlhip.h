@@ -202,8 +202,8 @@ int tnlhip_matrix_get_storage( tnlhip_matrix A, double* values, int32_t* columns
  * of the vector type. */
 int tnlhip_matrix_get_diagonal( tnlhip_matrix A, void* diagonal, void* stream );
 
-/* Overrides a kernel parameter chosen at construction (ChunkedEllpack with FP32 / BF16 values
- * starts with "xcd_remap" 1, every other matrix with 0): "lanes_per_row" (CSR light / hybrid /
+/* Overrides a kernel parameter chosen at construction (ChunkedEllpack and SlicedEllpack with
+ * FP32 / BF16 values start with "xcd_remap" 1, every other matrix with 0): "lanes_per_row" (CSR light / hybrid /
  * vector: a power of two in [1, 64]; once set, that many lanes share each row even when rows
  * average at most 8 stored elements, where these kernels otherwise take CSRScalar's staged
  * one-lane-per-row path); "xcd_remap" (row distribution: 0 the plain order, 1 contiguous row

@@ -267,7 +267,8 @@ def test_invalid_arguments_raise(tnl, dev):
                                         ("SlicedEllpack", {"xcd_remap": 3}),
                                         ("SlicedEllpack", {"xcd_remap": 3, "pair_rows": 1, "blocks_per_cu": 4}),
                                         ("CSRAdaptive", {"xcd_remap": 1}), ("CSRAdaptive", {"xcd_remap": 3}),
-                                        ("CSRScalar", {"xcd_remap": 1}), ("ChunkedEllpack", {"xcd_remap": 1})])
+                                        ("CSRScalar", {"xcd_remap": 1}), ("ChunkedEllpack", {"xcd_remap": 1}),
+                                        ("SlicedEllpack", {"xcd_remap": 1})])
 def test_launch_distributions_give_the_same_bits(tnl, dev, fmt, params):
     """Row distributions over the XCDs change only which workgroup computes a row: every keeper
     mode and row range gives the default launch's bits (a row computed twice or skipped would

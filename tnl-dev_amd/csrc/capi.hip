@@ -140,10 +140,11 @@ tnlhip_matrix_s* buildFromSegments( int format, int dtype, int32_t rows, int32_t
       M->adaptiveHost = buildAdaptiveBlocks( rows, M->meta.offsets.data(), vsize, kWave, 256, kAdaptiveStreamBytes,
                                              kAdaptiveMaxElements );
    }
-   // ChunkedEllpack with 4-byte values: slices in XCD order (each XCD a contiguous eighth) keep the
-   // x a slice gathers in that XCD's L2: 45.8 -> 42.3 us at 160^3 FP32, while FP64 (twice the x)
-   // measured 70 -> 72 us, so FP64 keeps the plain order
-   if( format == TNLHIP_CHUNKED_ELLPACK && dtype != TNLHIP_F64 )
+   // ChunkedEllpack and SlicedEllpack with 4-byte values: slices / row blocks in XCD order (each
+   // XCD a contiguous eighth) keep the x they gather in that XCD's L2: at 160^3 FP32 ChunkedEllpack
+   // 45.8 -> 42.3 us, SlicedEllpack 47.5 -> 45.7 us; FP64 (twice the x) measured ChunkedEllpack
+   // 70 -> 72 us and FP32 Ellpack 49.4 -> 52.1 us, so those keep the plain order
+   if( ( format == TNLHIP_CHUNKED_ELLPACK || format == TNLHIP_SLICED_ELLPACK ) && dtype != TNLHIP_F64 )
       M->xcdRemap = 1;
    if( format == TNLHIP_CSR_LIGHT )
       M->lanes = lightLanes( capTotal, rows );
