@@ -72,9 +72,10 @@ typedef enum
    TNLHIP_F64 = 0,
    TNLHIP_F32 = 1,
    TNLHIP_BF16 = 2, /* values stored as bfloat16, ComputeReal = float (SparseMatrix.h:40-49) */
-   TNLHIP_BINARY = 3 /* Real = bool: no values stored, every stored entry is 1; ComputeReal = Index
+   TNLHIP_BINARY = 3, /* Real = bool: no values stored, every stored entry is 1; ComputeReal = Index
                         (SparseMatrixView.h:20-29), so vectors are int32 and a product sums x over
                         the stored columns in int32 */
+   TNLHIP_INT32 = 4   /* int32 output vectors of the row reductions (not a matrix value type) */
 } tnlhip_dtype;
 
 typedef struct tnlhip_matrix_s* tnlhip_matrix;
@@ -94,14 +95,25 @@ typedef struct
 
 /* SparseMatrix<Real, Devices::Hip, int, GeneralMatrix, Segments> A;  A = csr;
  *   Matrices/SparseMatrix.hpp:1070-1180 (operator= from a CSR matrix): row capacities are
- *   the source row lengths, entries are copied in source order, explicit zeros are
- *   dropped and the remaining slots hold the padding index -1
+ *   the source's getRowCapacities, i.e. its stored (column != -1) entries per row, entries are
+ *   copied in source order, explicit zeros are dropped and the remaining slots hold the
+ *   padding index -1
  *   (SparseMatrix.hpp:1315-1318).  setRowCapacities, SparseMatrix.hpp:284-305.
  * row_pointers[rows+1], column_indexes[nnz], values[nnz] (float64) are host pointers when
  * host_pointers != 0, HIP device pointers otherwise. */
 int tnlhip_matrix_create_csr( int format, int value_dtype, int32_t rows, int32_t columns, const int32_t* row_pointers,
                               const int32_t* column_indexes, const double* values, int host_pointers,
                               tnlhip_matrix* out );
+
+/* A matrix built as TNL user code builds one element by element: setDimensions, setRowCapacities
+ * with the CSR row lengths, then setElement of every stored entry (SparseMatrix.hpp:284-305,
+ * SparseMatrixView.hpp:270-345) -- unlike tnlhip_matrix_create_csr (operator=), the capacities are
+ * kept as given, slots with column -1 stay padding and explicit zeros are stored.  symmetric != 0
+ * builds a SymmetricMatrix from its lower triangle.  Host arrays.  This is what the C++ front's
+ * SparseMatrix< ..., Devices::Hip > hands over when it first moves to the device. */
+int tnlhip_matrix_create_staged( int format, int value_dtype, int symmetric, int32_t rows, int32_t columns,
+                                 const int32_t* row_pointers, const int32_t* column_indexes, const double* values,
+                                 tnlhip_matrix* out );
 
 /* SparseMatrix<Real, Devices::Hip, int, SymmetricMatrix, Segments> A;  A = csr: the lower
  *   triangle, diagonal included, of a square matrix (an entry above the diagonal is an error).
@@ -113,7 +125,8 @@ int tnlhip_matrix_create_symmetric_csr( int format, int value_dtype, int32_t row
                                         const int32_t* column_indexes, const double* values, int host_pointers,
                                         tnlhip_matrix* out );
 
-/* B = A across formats / value types on the device (SparseMatrix.hpp:1093-1107). */
+/* B = A across formats / value types on the device (SparseMatrix.hpp:1070-1107): B's capacities are
+ * A's getRowCapacities (stored entries per row, not A's segment sizes), explicit zeros dropped. */
 int tnlhip_matrix_convert( tnlhip_matrix source, int format, int value_dtype, tnlhip_matrix* out );
 
 /* Matrix Market files (Matrices/MatrixReader.hpp, MatrixWriter.hpp:105-130).
@@ -122,8 +135,10 @@ int tnlhip_matrix_convert( tnlhip_matrix source, int format, int value_dtype, tn
  *   column -1 where entries repeat), setElement semantics (sorted, last repeated value kept,
  *   explicit zeros stored).  Pass NULL arrays to query rows / columns / storage first.
  * tnlhip_mtx_read: the same into a matrix of any format on the device (readMtx for a
- *   Devices::Hip matrix: read on the host, then copied).
- * tnlhip_mtx_write: MatrixWriter::writeMtx (entries whose value is not zero, 6 digits). */
+ *   Devices::Hip matrix, MatrixReader.hpp:19-26: read on the host, then matrix = hostMatrix, which
+ *   takes the stored entries per row as capacities and drops explicit zeros).
+ * tnlhip_mtx_write: MatrixWriter::writeMtx of a device matrix (MatrixWriter.hpp:35-40, 105-130: a
+ *   host copy by operator=, then the entries whose value is not zero, 6 digits). */
 int tnlhip_mtx_read_csr( const char* path, int32_t* rows, int32_t* columns, int64_t* storage, int32_t* row_pointers,
                          int32_t* column_indexes, double* values );
 int tnlhip_mtx_read( const char* path, int format, int value_dtype, tnlhip_matrix* out );
@@ -218,6 +233,85 @@ int tnlhip_matrix_set_param( tnlhip_matrix A, const char* name, int32_t value );
  *   read y.  Matrices/SparseMatrix.h:1078-1085, SparseMatrixView.hpp:390-491. */
 int tnlhip_spmv( tnlhip_matrix A, const void* x, void* y, double alpha, double beta, int32_t first, int32_t last,
                  void* stream );
+
+/* ------------------------------------------------------------------ row traversal */
+
+/* Fetch and reduction of tnlhip_matrix_reduce_rows: the fetch( rowIdx, columnIdx, value ) and
+ * reduce( a, b ) lambdas of SparseMatrixView::reduceRows that C callers can name. */
+typedef enum
+{
+   TNLHIP_FETCH_VALUE = 0,     /* value */
+   TNLHIP_FETCH_ABS_VALUE = 1, /* |value| */
+   TNLHIP_FETCH_ONE = 2,       /* 1 for every stored entry (getRowCapacities' fetch) */
+   TNLHIP_FETCH_NONZERO = 3,   /* value != 0 (getCompressedRowLengths' fetch) */
+   TNLHIP_FETCH_COLUMN = 4     /* the column index */
+} tnlhip_fetch;
+
+typedef enum
+{
+   TNLHIP_REDUCE_SUM = 0, /* std::plus */
+   TNLHIP_REDUCE_MIN = 1, /* TNL::min */
+   TNLHIP_REDUCE_MAX = 2  /* TNL::max */
+} tnlhip_reduce;
+
+/* SparseMatrixView::reduceRows( first, last, fetch, reduce, keep, identity )
+ * (Matrices/SparseMatrixView.hpp:495-595; Segments' reduceSegments, e.g. CSRView.hpp:185-199):
+ * out[row] = identity (+) fetch(a_row,0) (+) fetch(a_row,1) ... over the row's slots in local order,
+ * a padding slot (column -1) fetching the identity; keep stores into out (a device vector of
+ * out_dtype: TNLHIP_F64, TNLHIP_F32 or TNLHIP_INT32).  last == 0 means all rows.  Each row is
+ * folded left to right as TNL's Host traversal does, so every result is TNL's bit for bit. */
+int tnlhip_matrix_reduce_rows( tnlhip_matrix A, int32_t first, int32_t last, int fetch, int reduce, double identity,
+                               void* out, int out_dtype, void* stream );
+
+/* SparseMatrix::getRowCapacities( vector ) (SparseMatrixView.hpp:143-160): reduceAllRows with fetch 1,
+ * i.e. the number of stored (non-padding) entries of every row; int32 device vector of rows. */
+int tnlhip_matrix_get_row_capacities( tnlhip_matrix A, int32_t* capacities, void* stream );
+
+/* SparseMatrix::getCompressedRowLengths( vector ) (SparseMatrixView.hpp:116-135): the entries of
+ * every row whose value is not zero; int32 device vector of rows. */
+int tnlhip_matrix_get_compressed_row_lengths( tnlhip_matrix A, int32_t* lengths, void* stream );
+
+/* SparseMatrix::getRowCapacity( row ) (SparseMatrixView.hpp:163-172): the segment size of the row. */
+int tnlhip_matrix_get_row_capacity( tnlhip_matrix A, int32_t row, int32_t* capacity );
+
+/* SparseMatrix::getRow( row ) read on the host (SparseMatrixView.hpp:222-250, SparseMatrixRowView):
+ * *size = the segment size; columns[l] / values[l] receive getColumnIndex( l ) / getValue( l ) for
+ * l < size in local order (padding slots: column -1, value 0; binary matrices: 1 per stored entry).
+ * columns / values are host arrays of at least `capacity` entries; pass NULL for both to query the size. */
+int tnlhip_matrix_get_row( tnlhip_matrix A, int32_t row, int32_t* columns, double* values, int32_t capacity,
+                           int32_t* size );
+
+/* Device arrays of a matrix, for traversal kernels compiled by the caller (the C++ front's
+ * forElements / forRows / reduceRows with device lambdas, SparseMatrixView.hpp:495-760).  The
+ * index maps are TNL's column-major ones (views of Algorithms/Segments/*View.hpp): CSR
+ * offsets[r] + l; Ellpack r + aligned_size * l; SlicedEllpack slice_offsets[r / 32] + r % 32 + 32 l;
+ * ChunkedEllpack and BiEllpack as detail/ChunkedEllpack.h:110-145 and detail/BiEllpack.h:150-185.
+ * The pointers stay valid until the matrix is destroyed; a caller that writes values or column
+ * indexes through them calls tnlhip_matrix_values_changed afterwards. */
+typedef struct
+{
+   int32_t format, value_dtype, rows, columns;
+   void* values;             /* NULL for binary matrices */
+   int32_t* column_indexes;
+   const int32_t* offsets;   /* CSR */
+   int32_t segment_size, aligned_size; /* Ellpack */
+   const int32_t* slice_offsets;       /* SlicedEllpack */
+   const int32_t* slice_segment_sizes;
+   const int32_t* row_to_chunk;        /* ChunkedEllpack: rowToChunkMapping, rowToSliceMapping, */
+   const int32_t* row_to_slice;        /* slices (size, chunkSize, firstSegment, pointer) */
+   const int32_t* slices;
+   const int32_t* row_perm;            /* BiEllpack: rowPermArray, groupPointers */
+   const int32_t* group_pointers;
+   int32_t symmetric;
+   int32_t reserved;
+} tnlhip_matrix_view;
+
+int tnlhip_matrix_get_view( tnlhip_matrix A, tnlhip_matrix_view* view );
+
+/* After values or column indexes were written through the view (forElements): waits for the
+ * stream, then refreshes what the library derived from them (cached diagonal, column-striped
+ * plan, nonzero count). */
+int tnlhip_matrix_values_changed( tnlhip_matrix A, void* stream );
 
 /* ------------------------------------------------------------------ reductions / BLAS-1 */
 

@@ -665,6 +665,43 @@ orc_spmv_f32( const orc_segments* s, const float* vals, const int32_t* cols, con
    spmv< float >( s, vals, cols, x, y, alpha, beta, first, last );
 }
 
+// ---------------------------------------------------------------- row reductions
+
+// SparseMatrixView::reduceRows (SparseMatrixView.hpp:495-530): every slot of a row in local
+// order through fetch_ -- a padding slot returns the identity, others fetch( row, column, value )
+// -- folded by aux = reduce( aux, f ) from the identity (the Host reduceSegments loops, e.g.
+// CSRScalarKernel.hpp:70-107), then keep( row, aux ).  fetch: 0 value, 1 |value|, 2 one,
+// 3 value != 0 (getCompressedRowLengths, SparseMatrixView.hpp:116-135), 4 column; reduce:
+// 0 std::plus, 1 TNL::min (std::min), 2 TNL::max (std::max).  getRowCapacities
+// (SparseMatrixView.hpp:143-160) is fetch 2 with std::plus.
+extern "C" void
+orc_reduce_rows( const orc_segments* s, const double* vals, const int32_t* cols, int32_t first, int32_t last,
+                 int32_t fetch, int32_t reduce, double identity, double* out )
+{
+   if( last == 0 )
+      last = s->n;
+   for( I r = first; r < last; r++ ) {
+      double aux = identity;
+      const I size = orc_segments_segment_size( s, r );
+      for( I l = 0; l < size; l++ ) {
+         const int64_t g = orc_segments_global_index( s, r, l );
+         double f = identity;
+         if( cols[ g ] != -1 ) {
+            const double v = vals[ g ];
+            f = fetch == 0 ? v : fetch == 1 ? std::abs( v ) : fetch == 2 ? 1.0 : fetch == 3 ? ( v != 0.0 ? 1.0 : 0.0 )
+                                                                                           : (double) cols[ g ];
+         }
+         if( reduce == 0 )
+            aux = aux + f;
+         else if( reduce == 1 )
+            aux = std::min( aux, f );
+         else
+            aux = std::max( aux, f );
+      }
+      out[ r ] = aux;
+   }
+}
+
 // ---------------------------------------------------------------- CSR-Adaptive descriptors
 
 extern "C" int64_t

@@ -50,6 +50,12 @@ void orc_spmv_f64( const orc_segments* s, const double* vals, const int32_t* col
 void orc_spmv_f32( const orc_segments* s, const float* vals, const int32_t* cols, const float* x, float* y,
                    float alpha, float beta, int32_t first, int32_t last );
 
+/* SparseMatrixView::reduceRows over the segments' slots in local order: fetch 0 value, 1 |value|,
+ * 2 one (getRowCapacities), 3 value != 0 (getCompressedRowLengths), 4 column; reduce 0 sum, 1 min,
+ * 2 max; padding slots fetch the identity.  out[r] for r in [first, last) (last 0 = all). */
+void orc_reduce_rows( const orc_segments* s, const double* vals, const int32_t* cols, int32_t first, int32_t last,
+                      int32_t fetch, int32_t reduce, double identity, double* out );
+
 /* CSR-Adaptive block descriptors (CSRAdaptiveKernel::initValueSize) for one value size.
  * Returns the number of descriptors; out (6 int32 per descriptor: type, firstSegment,
  * blockSize, segmentsInBlock, warpIdx, warpsCount) may be NULL to query.

@@ -85,6 +85,9 @@ def lib():
         L.orc_spmv_f32.argtypes = [C.c_void_p, _f32p, _i32p, _f32p, _f32p, C.c_float, C.c_float, C.c_int32, C.c_int32]
         L.orc_csr_adaptive_blocks.restype = C.c_int64
         L.orc_csr_adaptive_blocks.argtypes = [C.c_int32, _i32p] + [C.c_int32] * 5 + [C.c_void_p]
+        L.orc_reduce_rows.restype = None
+        L.orc_reduce_rows.argtypes = [C.c_void_p, _f64p, _i32p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                      C.c_double, _f64p]
         L.orc_dot_f64.restype = C.c_double
         L.orc_dot_f64.argtypes = [C.c_int64, _f64p, _f64p]
         L.orc_dot_f32.restype = C.c_float
@@ -127,6 +130,9 @@ def ref():
         L.ref_convert_spmv.restype = C.c_int
         L.ref_convert_spmv.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p,
                                        _f64p, _i32p, C.c_longlong]
+        L.ref_row_reductions.restype = C.c_int
+        L.ref_row_reductions.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f64p, C.c_int, C.c_int,
+                                         C.c_double, _f64p, _i32p, _i32p, _i32p]
         L.ref_csr_adaptive_blocks.restype = C.c_longlong
         L.ref_csr_adaptive_blocks.argtypes = [C.c_int, _i32p, C.c_int, C.c_void_p]
         L.ref_time_spmv_csr.restype = C.c_double
@@ -138,6 +144,8 @@ def ref():
         L.ref_read_mtx.restype = C.c_int
         L.ref_read_mtx.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong),
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
+        L.ref_write_mtx_device.restype = C.c_int
+        L.ref_write_mtx_device.argtypes = [C.c_char_p, C.c_int, C.c_int, _i32p, _i32p, _f64p]
         L.ref_write_mtx.restype = C.c_int
         L.ref_write_mtx.argtypes = [C.c_char_p, C.c_int, C.c_int, _i32p, _i32p, _f64p]
         L.ref_save_tnl.restype = C.c_int
@@ -173,6 +181,12 @@ def ref_read_mtx(path):
     L.ref_read_mtx(str(path).encode(), C.byref(rows), C.byref(cols), C.byref(st), rp.ctypes.data, ci.ctypes.data,
                    v.ctypes.data, err, 1024)
     return rows.value, cols.value, rp, ci[:st.value], v[:st.value]
+
+
+def ref_write_mtx_device(path, rows, cols, row_ptr, col_idx, vals):
+    """writeMtx of a device matrix holding TNL's host read result (two operator= conversions)."""
+    ref().ref_write_mtx_device(str(path).encode(), int(rows), int(cols), np.ascontiguousarray(row_ptr, np.int32),
+                               np.ascontiguousarray(col_idx, np.int32), np.ascontiguousarray(vals, np.float64))
 
 
 def ref_write_mtx(path, rows, cols, row_ptr, col_idx, vals):
@@ -267,6 +281,10 @@ class Segments:
     def metadata(self):
         return {nm: self.get(nm) for nm in METADATA_NAMES[self.kind]}
 
+    def get_segment_size(self, seg):
+        assert self.backend == "oracle"
+        return int(self._L.orc_segments_segment_size(self._h, seg))
+
     def global_index(self, seg, local):
         assert self.backend == "oracle"
         return int(self._L.orc_segments_global_index(self._h, seg, local))
@@ -294,6 +312,44 @@ class Segments:
         f = self._L.orc_spmv_f64 if dtype == np.float64 else self._L.orc_spmv_f32
         f(self._h, v, c, xx, y, alpha, beta, first, last)
         return y
+
+
+FETCHES = {"value": 0, "abs": 1, "one": 2, "nonzero": 3, "column": 4}
+REDUCES = {"sum": 0, "min": 1, "max": 2}
+
+
+def reduce_rows(seg, vals, cols, fetch, reduce, identity, first=0, last=0):
+    """SparseMatrixView::reduceRows over the oracle segments' storage (vals, cols in storage order)."""
+    out = np.zeros(max(seg.n, 1), np.float64)
+    v = np.ascontiguousarray(vals, np.float64)
+    c = np.ascontiguousarray(cols, np.int32)
+    if len(v) == 0:
+        v = np.zeros(1, np.float64)
+        c = np.full(1, -1, np.int32)
+    seg._L.orc_reduce_rows(seg._h, v, c, int(first), int(last), FETCHES[fetch], REDUCES[reduce], float(identity), out)
+    return out[:seg.n]
+
+
+def ref_row_reductions(kind, row_ptr, cols, vals, ncols, fetch, reduce, identity, column_major=True):
+    """TNL's reduceAllRows / getRowCapacities / getCompressedRowLengths / getRowCapacity on a Host
+    matrix of `kind` assigned from the CSR (the CSR kind keeps the CSR storage as given)."""
+    k = KINDS[kind] if isinstance(kind, str) else kind
+    rp = np.ascontiguousarray(row_ptr, np.int32)
+    n = len(rp) - 1
+    c = np.ascontiguousarray(cols, np.int32)
+    v = np.ascontiguousarray(vals, np.float64)
+    if len(c) == 0:
+        c = np.full(1, -1, np.int32)
+        v = np.zeros(1, np.float64)
+    out = np.zeros(max(n, 1), np.float64)
+    caps = np.zeros(max(n, 1), np.int32)
+    lens = np.zeros(max(n, 1), np.int32)
+    sizes = np.zeros(max(n, 1), np.int32)
+    rc = ref().ref_row_reductions(k, 1 if column_major else 0, n, int(ncols), rp, c, v, FETCHES[fetch],
+                                  REDUCES[reduce], float(identity), out, caps, lens, sizes)
+    if rc:
+        raise RuntimeError(f"ref_row_reductions failed: {rc}")
+    return out[:n], caps[:n], lens[:n], sizes[:n]
 
 
 def ref_convert_spmv(kind, row_ptr, cols, vals, ncols, x, column_major=True, y=None):

@@ -322,6 +322,51 @@ struct LoadFn
 
 }  // namespace
 
+// TNL's row traversals on a matrix of format S filled from CSR (the same CSR type as the source
+// keeps its storage as given, explicit zeros and padding included; other types are assigned
+// from it): reduceAllRows with the fetch / reduce pair of orc_reduce_rows, getRowCapacities,
+// getCompressedRowLengths and getRowCapacity of every row.
+template< template< typename, typename, typename > class S >
+int rowReductions( int n, int ncols, const int* rp, const int* cols, const double* vals, int fk, int rk,
+                   double identity, double* out, int* capacities, int* compressed, int* segmentSizes )
+{
+   CsrMatrix A;
+   fillCSR( A, n, ncols, rp, cols, vals );
+   Matrices::SparseMatrix< double, Host, int, Matrices::GeneralMatrix, S > B;
+   B = A;
+   auto run = [ & ]( const auto& M ) {
+      auto fetch = [ = ]( int, int column, const double& value ) -> double
+      {
+         switch( fk ) {
+            case 0: return value;
+            case 1: return std::abs( value );
+            case 2: return 1.0;
+            case 3: return value != 0.0 ? 1.0 : 0.0;
+            default: return (double) column;
+         }
+      };
+      auto reduce = [ = ]( const double& a, const double& b ) -> double
+      {
+         return rk == 0 ? a + b : rk == 1 ? TNL::min( a, b ) : TNL::max( a, b );
+      };
+      auto keep = [ & ]( int row, const double& v ) mutable { out[ row ] = v; };
+      M.reduceAllRows( fetch, reduce, keep, identity );
+      IVec caps, lens;
+      M.getRowCapacities( caps );
+      M.getCompressedRowLengths( lens );
+      for( int i = 0; i < n; i++ ) {
+         capacities[ i ] = caps[ i ];
+         compressed[ i ] = lens[ i ];
+         segmentSizes[ i ] = M.getRowCapacity( i );
+      }
+   };
+   if( std::is_same< S< Host, int, Alloc >, Seg::CSRScalar< Host, int, Alloc > >::value )
+      run( A );
+   else
+      run( B );
+   return 0;
+}
+
 extern "C" {
 
 void* ref_segments_create( int kind, int n, const int* caps, int colmajor )
@@ -370,6 +415,27 @@ int ref_convert_spmv( int kind, int colmajor, int n, int ncols, const int* rp, c
       case 7: return convertSpmv< ChC >( n, ncols, rp, cols, vals, x, y, out_vals, out_cols, storage_cap );
       case 8: return convertSpmv< BiR >( n, ncols, rp, cols, vals, x, y, out_vals, out_cols, storage_cap );
       case 9: return convertSpmv< BiC >( n, ncols, rp, cols, vals, x, y, out_vals, out_cols, storage_cap );
+   }
+   return -2;
+}
+
+int ref_row_reductions( int kind, int colmajor, int n, int ncols, const int* rp, const int* cols, const double* vals,
+                        int fetch, int reduce, double identity, double* out, int* capacities, int* compressed,
+                        int* segmentSizes )
+{
+   switch( kind * 2 + ( colmajor ? 1 : 0 ) ) {
+      case 0:
+      case 1:
+         return rowReductions< Seg::CSRScalar >( n, ncols, rp, cols, vals, fetch, reduce, identity, out, capacities,
+                                                 compressed, segmentSizes );
+      case 2: return rowReductions< EllRow >( n, ncols, rp, cols, vals, fetch, reduce, identity, out, capacities, compressed, segmentSizes );
+      case 3: return rowReductions< EllC >( n, ncols, rp, cols, vals, fetch, reduce, identity, out, capacities, compressed, segmentSizes );
+      case 4: return rowReductions< SlR >( n, ncols, rp, cols, vals, fetch, reduce, identity, out, capacities, compressed, segmentSizes );
+      case 5: return rowReductions< SlC >( n, ncols, rp, cols, vals, fetch, reduce, identity, out, capacities, compressed, segmentSizes );
+      case 6: return rowReductions< ChR >( n, ncols, rp, cols, vals, fetch, reduce, identity, out, capacities, compressed, segmentSizes );
+      case 7: return rowReductions< ChC >( n, ncols, rp, cols, vals, fetch, reduce, identity, out, capacities, compressed, segmentSizes );
+      case 8: return rowReductions< BiR >( n, ncols, rp, cols, vals, fetch, reduce, identity, out, capacities, compressed, segmentSizes );
+      case 9: return rowReductions< BiC >( n, ncols, rp, cols, vals, fetch, reduce, identity, out, capacities, compressed, segmentSizes );
    }
    return -2;
 }
@@ -590,6 +656,29 @@ int ref_write_mtx( const char* path, int rows, int cols, const int* rp, const in
       A.getValues()[ k ] = v[ k ];
    }
    TNL::Matrices::MatrixWriter< M >::writeMtx( std::string( path ), A );
+   return 0;
+}
+
+// writeMtx of a device matrix read by readMtx (MatrixReader.hpp:19-26, MatrixWriter.hpp:35-40):
+// the host-read matrix is assigned to the device matrix and that back to a host copy, each an
+// operator= (stored entries per row as capacities, zeros dropped); TNL's Cuda build is absent, so
+// the two assignments run between Host matrices of different CSR types, the same conversion code.
+int ref_write_mtx_device( const char* path, int rows, int cols, const int* rp, const int* ci, const double* v )
+{
+   CsrMatrix A( rows, cols );
+   IVec caps( rows );
+   for( int i = 0; i < rows; i++ )
+      caps[ i ] = rp[ i + 1 ] - rp[ i ];
+   A.setRowCapacities( caps );
+   for( int k = 0; k < rp[ rows ]; k++ ) {
+      A.getColumnIndexes()[ k ] = ci[ k ];
+      A.getValues()[ k ] = v[ k ];
+   }
+   Matrices::SparseMatrix< double, Host, int, Matrices::GeneralMatrix, Seg::CSRVector > D;
+   D = A;
+   CsrMatrix H;
+   H = D;
+   TNL::Matrices::MatrixWriter< CsrMatrix >::writeMtx( std::string( path ), H );
    return 0;
 }
 

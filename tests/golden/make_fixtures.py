@@ -122,6 +122,10 @@ def write_mtx_fixtures():
             O.ref_write_mtx(wpath, rows, cols, rp, ci, v)
             with open(wpath) as f:
                 out[f"{name}/written"] = np.array(f.read())
+            # writeMtx of the device matrix readMtx builds from the file (zeros dropped twice over)
+            O.ref_write_mtx_device(wpath, rows, cols, rp, ci, v)
+            with open(wpath) as f:
+                out[f"{name}/written_device"] = np.array(f.read())
     np.savez_compressed(os.path.join(HERE, "mtx.npz"), **out)
     print("mtx.npz", os.path.getsize(os.path.join(HERE, "mtx.npz")))
 

@@ -201,23 +201,28 @@ def test_full_size_laplacian_160(tnl, dev, fmt):
 
 
 def test_device_conversion_between_formats(tnl, dev):
-    """B = A across formats on the device gives TNL's storage for B (capacities of A's
-    segments), checked against the oracle's conversion chain."""
+    """B = A across formats on the device gives TNL's storage for B, checked against the oracle's
+    conversion chain: B's capacities are A's getRowCapacities -- the stored (non-padding) entries
+    of each row, not A's segment sizes (SparseMatrix.hpp:1070-1107) -- and the entries follow A's
+    local order."""
     rp, c, v = W.irregular(3000, mean=8, sigma=1.0, max_len=300, seed=12)
     n = len(rp) - 1
     x = W.uniform_vector(n, 5)
     A = tnl.SparseMatrix.from_csr(rp, c, v, format="ChunkedEllpack", dtype="f64")
-    # TNL: capacities of B are A's segment sizes (getRowCapacities); entries in A's local order
     src = O.Segments("ChunkedEllpack", np.diff(rp))
-    caps = np.array([O.lib().orc_segments_segment_size(src._h, r) for r in range(n)], np.int32)
+    seg_sizes = np.array([src.get_segment_size(r) for r in range(n)], np.int32)
+    assert np.any(seg_sizes > np.diff(rp)), "the source must hold padding for the case to mean anything"
     sv, sc = src.convert(rp, c, v)
+    caps = np.array([np.sum(sc[[src.global_index(r, l) for l in range(seg_sizes[r])]] != -1) for r in range(n)],
+                    np.int32)
+    np.testing.assert_array_equal(A.getRowCapacities().cpu().numpy(), caps)
     rp2 = np.zeros(n + 1, np.int32)
     rp2[1:] = np.cumsum(caps)
     c2 = np.full(rp2[-1], -1, np.int32)
     v2 = np.zeros(rp2[-1])
     for r in range(n):
         k = rp2[r]
-        for l in range(caps[r]):
+        for l in range(seg_sizes[r]):
             g = src.global_index(r, l)
             if sc[g] != -1:
                 c2[k] = sc[g]
@@ -226,6 +231,8 @@ def test_device_conversion_between_formats(tnl, dev):
     for fmt in FORMATS:
         B = A.convert(fmt)
         dst = O.Segments(KIND[fmt], caps)
+        for nm in O.METADATA_NAMES[dst.kind]:
+            np.testing.assert_array_equal(B.metadata(nm), dst.get(nm), err_msg=f"{fmt} {nm}")
         dv, dc = dst.convert(rp2, c2, v2)
         bv, bc = B.storage()
         np.testing.assert_array_equal(bc, dc[:len(bc)], err_msg=fmt)
@@ -290,9 +297,10 @@ def test_launch_distributions_give_the_same_bits(tnl, dev, fmt, params):
 
 @pytest.mark.parametrize("fmt", FORMATS)
 def test_matrix_market_read_into_every_format(tnl, dev, fmt, tmp_path):
-    """readMtx into a device matrix of each format: TNL's capacities (metadata of the format for
-    the file's row capacities), explicit zeros stored, y = A x as the dense product, and
-    writeMtx from the device gives TNL's text."""
+    """readMtx into a device matrix of each format (MatrixReader.hpp:19-26: the host read, then
+    matrix = hostMatrix): capacities are the host matrix's stored entries per row, explicit zeros
+    are dropped; y = A x as the dense product; writeMtx from the device (a host copy by operator=,
+    then the host writer) gives TNL's text with the header counting the nonzero entries."""
     d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mtx.npz"))
     for name in (str(n) for n in d["names"]):
         if f"{name}/error" in d.files:
@@ -303,10 +311,15 @@ def test_matrix_market_read_into_every_format(tnl, dev, fmt, tmp_path):
         rows, cols = (int(x) for x in d[f"{name}/dims"])
         rp, ci, v = d[f"{name}/row_ptr"], d[f"{name}/cols"], d[f"{name}/vals"]
         assert (A.info.rows, A.info.columns) == (rows, cols)
-        assert A.info.nonzeros == int(np.sum(ci != -1)), name
-        seg = O.Segments(KIND[fmt], np.diff(rp).astype(np.int32))
+        assert A.info.nonzeros == int(np.sum((ci != -1) & (v != 0))), name
+        stored = np.array([np.sum(ci[rp[r]:rp[r + 1]] != -1) for r in range(rows)], np.int32)
+        seg = O.Segments(KIND[fmt], stored)
         for nm in O.METADATA_NAMES[seg.kind]:
             np.testing.assert_array_equal(A.metadata(nm), seg.get(nm), err_msg=f"{name} {nm}")
+        sv, sc = seg.convert(rp, ci, v)
+        bv, bc = A.storage()
+        np.testing.assert_array_equal(bc, sc[:len(bc)], err_msg=name)
+        np.testing.assert_array_equal(bv, sv[:len(bv)], err_msg=name)
         x = np.random.default_rng(1).uniform(-1, 1, cols)
         dense = np.zeros((rows, cols))
         for r in range(rows):
@@ -318,7 +331,7 @@ def test_matrix_market_read_into_every_format(tnl, dev, fmt, tmp_path):
         assert np.all(np.abs(got - want) <= 1e-13 * (np.abs(dense) @ np.abs(x)) + 1e-300), name
         out = tmp_path / f"{name}.out.mtx"
         A.write_mtx(out)
-        assert out.read_text() == str(d[f"{name}/written"]), name
+        assert out.read_text() == str(d[f"{name}/written_device"]), name
 
 
 def _symmetric_random(n, seed):

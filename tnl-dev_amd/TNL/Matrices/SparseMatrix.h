@@ -3,9 +3,9 @@
 //
 // Elements set with setElement / addElement are staged on the host in TNL's row layout
 // (capacities from setRowCapacities, columns kept sorted, SparseMatrixView.hpp:270-345) and
-// moved to HBM on first use, through the same conversion TNL's cross-device operator= does
-// (explicit zeros dropped, padding index -1).  vectorProduct, format conversion and the
-// solvers then run on the GPU.
+// moved to HBM on first use with the capacities and entries as set (unused slots hold the padding
+// index -1, explicit zeros set by setElement are stored), as a Devices::Hip matrix built element by
+// element holds them.  vectorProduct, format conversion and the solvers then run on the GPU.
 #pragma once
 
 #include <utility>
@@ -255,18 +255,18 @@ public:
          if( rows > 0 && offsets.size() != (size_t) rows + 1 )
             throw std::logic_error( "matrix dimensions were not set" );
          auto* self = const_cast< SparseMatrix* >( this );
-         if( symmetric )
-            detail::check( tnlhip_matrix_create_symmetric_csr( format, valueDtype, rows, offsets.data(),
-                                                               cols.empty() ? nullptr : cols.data(),
-                                                               vals.empty() ? nullptr : vals.data(), 1,
-                                                               &self->handle ) );
-         else
-            detail::check( tnlhip_matrix_create_csr( format, valueDtype, rows, columns, offsets.data(),
-                                                     cols.empty() ? nullptr : cols.data(),
-                                                     vals.empty() ? nullptr : vals.data(), 1, &self->handle ) );
+         // setRowCapacities + setElement as staged: capacities kept, explicit zeros stored
+         detail::check( tnlhip_matrix_create_staged( format, valueDtype, symmetric ? 1 : 0, rows, columns,
+                                                     offsets.data(), cols.empty() ? nullptr : cols.data(),
+                                                     vals.empty() ? nullptr : vals.data(), &self->handle ) );
       }
       return handle;
    }
+
+   // Counts the device copies this matrix has dropped: with getHandle() it names the device
+   // matrix a solver bound, so a solver notices a matrix rebuilt (even at the same address)
+   // by setElement, operator=, load, readMtxFile or setFromCSR after setMatrix.
+   uint64_t getGeneration() const { return generation; }
 
 private:
    // back to host staging before an element update (the device copy is rebuilt on next use)
@@ -278,13 +278,16 @@ private:
       if( handle ) {
          tnlhip_matrix_destroy( handle );
          handle = nullptr;
+         generation++;
       }
       staged = true;
    }
    void release()
    {
-      if( handle )
+      if( handle ) {
          tnlhip_matrix_destroy( handle );
+         generation++;
+      }
       handle = nullptr;
    }
 
@@ -293,6 +296,7 @@ private:
    std::vector< double > vals;
    mutable tnlhip_matrix handle = nullptr;
    bool staged = true;
+   uint64_t generation = 0;
 };
 
 }  // namespace Matrices

@@ -36,10 +36,13 @@ public:
    LinearSolver( const LinearSolver& ) = delete;
    LinearSolver& operator=( const LinearSolver& ) = delete;
 
+   // TNL keeps the shared pointer and reads the matrix at solve time (LinearSolver.h:127-130):
+   // the matrix may change between setMatrix and solve, so solve() binds it again whenever its
+   // device copy or the preconditioner changed
    void setMatrix( const MatrixPointer& m )
    {
       matrix = m;
-      detail::check( tnlhip_solver_set_matrix( handle, m->getHandle() ) );
+      bind();
    }
    void setPreconditioner( const PreconditionerPointer& p )
    {
@@ -72,6 +75,10 @@ public:
    {
       if( ! matrix )
          throw std::logic_error( "No matrix was set. Call setMatrix() before solve()." );
+      const tnlhip_matrix h = matrix->getHandle();
+      if( h != bound || matrix->getGeneration() != boundGeneration ||
+          ( preconditioner ? preconditioner->getUpdateCount() : 0 ) != boundUpdates )
+         bind();
       detail::check( tnlhip_solver_set_params( handle, &params ) );
       int32_t converged = 0;
       detail::check( tnlhip_solver_solve( handle, b.getData(), x.getData(), &converged, nullptr ) );
@@ -102,6 +109,17 @@ public:
    }
 
 protected:
+   // binds the matrix's current device copy; the solver drops its cached diagonal and factors
+   void bind()
+   {
+      bound = matrix->getHandle();
+      boundGeneration = matrix->getGeneration();
+      boundUpdates = preconditioner ? preconditioner->getUpdateCount() : 0;
+      detail::check( tnlhip_solver_set_matrix( handle, bound ) );
+   }
+
+   tnlhip_matrix bound = nullptr;
+   uint64_t boundGeneration = 0, boundUpdates = 0;
    tnlhip_solver handle = nullptr;
    tnlhip_solver_params params;
    MatrixPointer matrix;

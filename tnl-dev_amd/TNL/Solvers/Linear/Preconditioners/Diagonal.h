@@ -18,8 +18,14 @@ class Preconditioner
 public:
    using MatrixPointer = std::shared_ptr< const Matrix >;
    virtual ~Preconditioner() = default;
-   virtual void update( const MatrixPointer& ) {}
+   virtual void update( const MatrixPointer& ) { updates++; }
    virtual int kind() const { return 0; }
+   // how many times update() ran: a solver re-derives its copy of the preconditioner (the
+   // diagonal, the ILU(0) factors) at the next solve after a change
+   uint64_t getUpdateCount() const { return updates; }
+
+protected:
+   uint64_t updates = 0;
 };
 
 template< typename Matrix >
@@ -32,6 +38,7 @@ public:
    // Diagonal.hpp:22-41: diagonal[i] = A(i, i)
    void update( const MatrixPointer& matrix ) override
    {
+      this->updates++;
       diagonal.setSize( matrix->getRows() );
       detail::check( tnlhip_matrix_get_diagonal( matrix->getHandle(), diagonal.getData(), nullptr ) );
       detail::checkHip( hipStreamSynchronize( nullptr ) );

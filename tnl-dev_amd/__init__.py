@@ -38,7 +38,9 @@ EXPORTS = [
     "tnlhip_server_remove_matrix", "tnlhip_server_submit", "tnlhip_server_progress", "tnlhip_server_result",
     "tnlhip_server_stats", "tnlhip_autotune_csr", "tnlhip_mtx_read_csr", "tnlhip_mtx_read", "tnlhip_mtx_write",
     "tnlhip_mtx_write_csr", "tnlhip_matrix_save", "tnlhip_matrix_load", "tnlhip_file_inspect",
-    "tnlhip_matrix_create_symmetric_csr",
+    "tnlhip_matrix_create_symmetric_csr", "tnlhip_matrix_create_staged", "tnlhip_matrix_reduce_rows",
+    "tnlhip_matrix_get_row_capacities", "tnlhip_matrix_get_compressed_row_lengths", "tnlhip_matrix_get_row_capacity",
+    "tnlhip_matrix_get_row", "tnlhip_matrix_get_view", "tnlhip_matrix_values_changed",
 ]
 # element order of TNL binary files: "cuda" = column-major (TNL's GPU matrices), "host" = row-major
 FILE_LAYOUTS = {"cuda": 0, "host": 1}
@@ -67,6 +69,15 @@ class MatrixInfo(C.Structure):
     _fields_ = [("rows", C.c_int32), ("columns", C.c_int32), ("format", C.c_int32), ("value_dtype", C.c_int32),
                 ("storage_size", C.c_int64), ("allocated", C.c_int64), ("nonzeros", C.c_int64),
                 ("lanes_per_row", C.c_int32), ("symmetric", C.c_int32)]
+
+
+class MatrixView(C.Structure):
+    _fields_ = [("format", C.c_int32), ("value_dtype", C.c_int32), ("rows", C.c_int32), ("columns", C.c_int32),
+                ("values", C.c_void_p), ("column_indexes", C.c_void_p), ("offsets", C.c_void_p),
+                ("segment_size", C.c_int32), ("aligned_size", C.c_int32), ("slice_offsets", C.c_void_p),
+                ("slice_segment_sizes", C.c_void_p), ("row_to_chunk", C.c_void_p), ("row_to_slice", C.c_void_p),
+                ("slices", C.c_void_p), ("row_perm", C.c_void_p), ("group_pointers", C.c_void_p),
+                ("symmetric", C.c_int32), ("reserved", C.c_int32)]
 
 
 class SolverParams(C.Structure):
@@ -135,6 +146,7 @@ def _load():
         "tnlhip_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(i64)]),
         "tnlhip_matrix_create_csr": (C.c_int, [C.c_int, C.c_int, i32, i32, vp, vp, vp, C.c_int, C.POINTER(vp)]),
         "tnlhip_matrix_convert": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(vp)]),
+        "tnlhip_matrix_create_staged": (C.c_int, [C.c_int, C.c_int, C.c_int, i32, i32, vp, vp, vp, C.POINTER(vp)]),
         "tnlhip_matrix_create_symmetric_csr": (C.c_int, [C.c_int, C.c_int, i32, vp, vp, vp, C.c_int, C.POINTER(vp)]),
         "tnlhip_matrix_destroy": (C.c_int, [vp]),
         "tnlhip_matrix_get_info": (C.c_int, [vp, C.POINTER(MatrixInfo)]),
@@ -179,6 +191,13 @@ def _load():
         "tnlhip_matrix_save": (C.c_int, [vp, C.c_char_p, C.c_int]),
         "tnlhip_matrix_load": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
         "tnlhip_file_inspect": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(FileInfo), C.c_char_p, C.c_int]),
+        "tnlhip_matrix_reduce_rows": (C.c_int, [vp, i32, i32, C.c_int, C.c_int, f64, vp, C.c_int, vp]),
+        "tnlhip_matrix_get_row_capacities": (C.c_int, [vp, vp, vp]),
+        "tnlhip_matrix_get_compressed_row_lengths": (C.c_int, [vp, vp, vp]),
+        "tnlhip_matrix_get_row_capacity": (C.c_int, [vp, i32, C.POINTER(i32)]),
+        "tnlhip_matrix_get_row": (C.c_int, [vp, i32, vp, vp, i32, C.POINTER(i32)]),
+        "tnlhip_matrix_values_changed": (C.c_int, [vp, vp]),
+        "tnlhip_matrix_get_view": (C.c_int, [vp, C.POINTER(MatrixView)]),
         "tnlhip_autotune_csr": (C.c_int, [C.c_int, i32, i32, vp, vp, vp, i32, C.c_uint32, i32, C.POINTER(Tuning),
                                           C.POINTER(Tuning), i32, C.POINTER(i32), C.POINTER(vp)]),
     }
@@ -228,6 +247,10 @@ def _dtype_id(dtype):
                   .replace("bfloat16", "bf16")]
 
 
+FETCH = {"value": 0, "abs": 1, "one": 2, "nonzero": 3, "column": 4}
+REDUCE = {"sum": 0, "min": 1, "max": 2}
+
+
 class SparseMatrix:
     """SparseMatrix< Real, Devices::Hip, int, GeneralMatrix, Segments > (Matrices/SparseMatrix.h:40-49)."""
 
@@ -274,6 +297,24 @@ class SparseMatrix:
                                                       _dtype_id(dtype), len(rp) - 1, rp.ctypes.data_as(C.c_void_p),
                                                       ci.ctypes.data_as(C.c_void_p), va.ctypes.data_as(C.c_void_p),
                                                       1, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def staged(cls, row_pointers, column_indexes, values, columns=None, format="CSRScalar", dtype="f64",
+               symmetric=False):
+        """setRowCapacities( row lengths ) + setElement of every stored entry (host numpy arrays):
+        capacities kept, slots with column -1 left as padding, explicit zeros stored."""
+        import numpy as np
+        rp = np.ascontiguousarray(row_pointers, np.int32)
+        ci = np.ascontiguousarray(column_indexes, np.int32)
+        va = np.ascontiguousarray(values, np.float64)
+        rows = len(rp) - 1
+        h = C.c_void_p(0)
+        _check(lib.tnlhip_matrix_create_staged(FORMATS[format] if isinstance(format, str) else format,
+                                               _dtype_id(dtype), 1 if symmetric else 0, rows,
+                                               rows if columns is None else columns, rp.ctypes.data_as(C.c_void_p),
+                                               ci.ctypes.data_as(C.c_void_p), va.ctypes.data_as(C.c_void_p),
+                                               C.byref(h)))
         return cls(h)
 
     @property
@@ -374,6 +415,56 @@ class SparseMatrix:
         info = MatrixInfo()
         _check(lib.tnlhip_matrix_get_info(self._h, C.byref(info)))
         self.info = info
+
+    # ---- row traversal (SparseMatrixView.hpp:116-250, 495-595)
+
+    def reduceRows(self, begin, end, fetch, reduce, identity, out, stream=None):
+        """out[row] = identity (+) fetch(row's slots in local order) for rows [begin, end) (end 0 =
+        all), with fetch in FETCH ("value", "abs", "one", "nonzero", "column") and reduce in REDUCE
+        ("sum", "min", "max"); out is a float64 / float32 / int32 device tensor of length rows."""
+        import torch
+        odt = {torch.float64: 0, torch.float32: 1, torch.int32: 4}[out.dtype]
+        if out.numel() != self.info.rows:
+            raise ValueError("out must have one entry per row")
+        _check(lib.tnlhip_matrix_reduce_rows(self._h, int(begin), int(end), FETCH[fetch], REDUCE[reduce],
+                                             float(identity), _ptr(out), odt, _stream(stream)))
+        return out
+
+    def reduceAllRows(self, fetch, reduce, identity, out, stream=None):
+        return self.reduceRows(0, 0, fetch, reduce, identity, out, stream)
+
+    def getRowCapacities(self, out=None, stream=None):
+        """getRowCapacities: stored (non-padding) entries per row, as TNL's reduceAllRows counts them."""
+        import torch
+        out = torch.empty(self.info.rows, dtype=torch.int32, device="cuda") if out is None else out
+        _check(lib.tnlhip_matrix_get_row_capacities(self._h, _ptr(out), _stream(stream)))
+        return out
+
+    def getCompressedRowLengths(self, out=None, stream=None):
+        """getCompressedRowLengths: entries per row whose value is not zero."""
+        import torch
+        out = torch.empty(self.info.rows, dtype=torch.int32, device="cuda") if out is None else out
+        _check(lib.tnlhip_matrix_get_compressed_row_lengths(self._h, _ptr(out), _stream(stream)))
+        return out
+
+    def getRowCapacity(self, row):
+        """getRowCapacity( row ): the segment size of the row."""
+        c = C.c_int32(0)
+        _check(lib.tnlhip_matrix_get_row_capacity(self._h, int(row), C.byref(c)))
+        return c.value
+
+    def getRow(self, row):
+        """getRow( row ) read on the host: (columns, values) of the row's slots in local order
+        (padding slots: column -1, value 0)."""
+        import numpy as np
+        size = C.c_int32(0)
+        _check(lib.tnlhip_matrix_get_row(self._h, int(row), None, None, 0, C.byref(size)))
+        n = size.value
+        cols = np.zeros(max(n, 1), np.int32)
+        vals = np.zeros(max(n, 1), np.float64)
+        _check(lib.tnlhip_matrix_get_row(self._h, int(row), cols.ctypes.data_as(C.c_void_p),
+                                         vals.ctypes.data_as(C.c_void_p), max(n, 1), C.byref(size)))
+        return cols[:n], vals[:n]
 
     def vectorProduct(self, inVector, outVector, matrixMultiplicator=1.0, outVectorMultiplicator=0.0,
                       firstRow=0, lastRow=0, stream=None):

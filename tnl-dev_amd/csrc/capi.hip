@@ -152,6 +152,7 @@ tnlhip_matrix_s* buildFromSegments( int format, int dtype, int32_t rows, int32_t
       M->lanes = hybridLanes( capTotal, rows );
    else if( format == TNLHIP_CSR_VECTOR )
       M->lanes = kWave;
+   M->defaultLanes = M->lanes;
    M->values.resize( M->meta.allocated * (int64_t) dtypeBytes( dtype ) );
    M->columns.resize( M->meta.allocated );
    M->uploadMetadata();
@@ -188,6 +189,38 @@ __global__ __launch_bounds__( kReduceBlock ) void countFinalKernel( ReduceSlot s
       *result = (int64_t) tot[ 0 ];
 }
 
+// Smallest and largest entry of a device int32 array (column indexes handed over on the device).
+__global__ __launch_bounds__( 256 ) void rangeKernel( int64_t n, const int32_t* __restrict__ v, int32_t* range )
+{
+   int32_t lo = INT32_MAX, hi = INT32_MIN;
+   for( int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t) gridDim.x * 256 ) {
+      lo = min( lo, v[ i ] );
+      hi = max( hi, v[ i ] );
+   }
+   for( int o = kWave / 2; o > 0; o /= 2 ) {
+      lo = min( lo, __shfl_xor( lo, o, kWave ) );
+      hi = max( hi, __shfl_xor( hi, o, kWave ) );
+   }
+   if( threadIdx.x % kWave == 0 ) {
+      atomicMin( range, lo );
+      atomicMax( range + 1, hi );
+   }
+}
+
+void columnRange( const int32_t* cols, int64_t n, int32_t* lo, int32_t* hi, hipStream_t st )
+{
+   DeviceArray< int32_t > r( 2 );
+   const int32_t init[ 2 ] = { INT32_MAX, INT32_MIN };
+   TNLHIP_CHECK( hipMemcpyAsync( r.ptr, init, sizeof( init ), hipMemcpyHostToDevice, st ) );
+   rangeKernel<<< gridFor( n, 256 ), 256, 0, st >>>( n, cols, r.ptr );
+   TNLHIP_CHECK( hipGetLastError() );
+   int32_t out[ 2 ];
+   TNLHIP_CHECK( hipMemcpyAsync( out, r.ptr, sizeof( out ), hipMemcpyDeviceToHost, st ) );
+   TNLHIP_CHECK( hipStreamSynchronize( st ) );
+   *lo = out[ 0 ];
+   *hi = out[ 1 ];
+}
+
 int64_t countStored( const Matrix& M, hipStream_t st )
 {
    if( M.meta.allocated == 0 )
@@ -202,6 +235,15 @@ int64_t countStored( const Matrix& M, hipStream_t st )
    TNLHIP_CHECK( hipStreamSynchronize( st ) );
    return h;
 }
+
+}  // namespace
+
+int64_t countNonzeros( const Matrix& M, hipStream_t st )
+{
+   return M.symmetric ? symmetricNonzeros( M, st ) : countStored( M, st );
+}
+
+namespace {
 
 inline float bf16ToFloat( uint16_t b )
 {
@@ -256,58 +298,124 @@ int tnlhip_device_info( char* name, int name_len, int* compute_units, int64_t* g
    } );
 }
 
+}  // extern "C"
+
+namespace tnlhip {
+namespace {
+
+// A matrix holding a CSR given by host or device arrays.
+//  staged == false: SparseMatrix::operator=( csr ) (SparseMatrix.hpp:1070-1107): the row
+//    capacities are the source's getRowCapacities -- its stored (non-padding) entries per row,
+//    explicit zeros included (SparseMatrixView.hpp:143-160) -- and the entries are copied in
+//    source order with explicit zeros dropped, so a row keeps trailing padding for its zeros.
+//  staged == true: setRowCapacities( row lengths ) followed by setElement of every stored entry
+//    (SparseMatrix.hpp:284-305, SparseMatrixView.hpp:270-345): the capacities are the CSR row
+//    lengths, padding slots (-1) stay padding and explicit zeros are stored.
+tnlhip_matrix_s* createFromCsr( int format, int value_dtype, int32_t rows, int32_t columns, const int32_t* row_pointers,
+                                const int32_t* column_indexes, const double* values, bool host_pointers, bool staged )
+{
+   TNLHIP_REQUIRE( rows >= 0 && columns >= 0, "negative dimensions" );
+   TNLHIP_REQUIRE( row_pointers != nullptr || rows == 0, "row_pointers is NULL" );
+   std::vector< int32_t > rp( (size_t) rows + 1, 0 );
+   if( rows > 0 ) {
+      if( host_pointers )
+         std::memcpy( rp.data(), row_pointers, sizeof( int32_t ) * rp.size() );
+      else
+         TNLHIP_CHECK( hipMemcpy( rp.data(), row_pointers, sizeof( int32_t ) * rp.size(), hipMemcpyDeviceToHost ) );
+   }
+   TNLHIP_REQUIRE( rp[ 0 ] == 0, "row_pointers[0] must be 0" );
+   for( int32_t i = 0; i < rows; i++ )
+      TNLHIP_REQUIRE( rp[ i + 1 ] >= rp[ i ], "row_pointers must be non-decreasing" );
+   const int64_t nnzIn = rp[ rows ];
+   TNLHIP_REQUIRE( nnzIn == 0 || column_indexes != nullptr, "column_indexes is NULL" );
+   if( host_pointers && nnzIn > 0 ) {
+      for( int64_t k = 0; k < nnzIn; k++ )
+         TNLHIP_REQUIRE( column_indexes[ k ] >= -1 && column_indexes[ k ] < columns, "column index out of range" );
+   }
+   else if( nnzIn > 0 ) {
+      // device arrays: the range of the column indexes is checked on the device before any
+      // kernel gathers x through them
+      int32_t lo = 0, hi = 0;
+      columnRange( column_indexes, nnzIn, &lo, &hi, nullptr );
+      TNLHIP_REQUIRE( lo >= -1 && hi < columns, "column index out of range" );
+   }
+   DeviceArray< int32_t > drp, dcol;
+   DeviceArray< double > dval;
+   const int32_t* prp = row_pointers;
+   const int32_t* pcol = column_indexes;
+   // a binary matrix may come without values: every entry is then a one
+   std::vector< double > ones;
+   if( value_dtype == TNLHIP_BINARY && values == nullptr ) {
+      ones.assign( (size_t) std::max< int64_t >( nnzIn, 0 ), 1.0 );
+      values = ones.data();
+      if( ! host_pointers ) {
+         dval.upload( ones );
+         values = dval.ptr;
+      }
+   }
+   TNLHIP_REQUIRE( nnzIn == 0 || values != nullptr, "values is NULL" );
+   const double* pval = values;
+   if( host_pointers ) {
+      drp.upload( rp );
+      dcol.upload( column_indexes, nnzIn );
+      dval.upload( values, nnzIn );
+      prp = drp.ptr;
+      pcol = dcol.ptr;
+      pval = dval.ptr;
+   }
+   std::vector< int32_t > caps( rows );
+   if( staged ) {
+      for( int32_t i = 0; i < rows; i++ )
+         caps[ i ] = rp[ i + 1 ] - rp[ i ];
+   }
+   else if( rows > 0 ) {
+      DeviceArray< int32_t > dcaps( rows );
+      csrStoredCounts( rows, prp, pcol, dcaps.ptr, nullptr );
+      caps = dcaps.download();
+   }
+   std::unique_ptr< tnlhip_matrix_s > M( buildEmpty( format, value_dtype, rows, columns, caps ) );
+   convertFromCsrDevice( *M, prp, pcol, pval, nullptr, staged );
+   M->nnz = countStored( *M, nullptr );
+   return M.release();
+}
+
+}  // namespace
+}  // namespace tnlhip
+
+extern "C" {
+
 int tnlhip_matrix_create_csr( int format, int value_dtype, int32_t rows, int32_t columns, const int32_t* row_pointers,
                               const int32_t* column_indexes, const double* values, int host_pointers,
                               tnlhip_matrix* out )
 {
    return guarded( [ & ] {
       TNLHIP_REQUIRE( out != nullptr, "out is NULL" );
-      TNLHIP_REQUIRE( rows >= 0 && columns >= 0, "negative dimensions" );
-      TNLHIP_REQUIRE( row_pointers != nullptr || rows == 0, "row_pointers is NULL" );
-      std::vector< int32_t > rp( (size_t) rows + 1, 0 );
-      if( rows > 0 ) {
-         if( host_pointers )
-            std::memcpy( rp.data(), row_pointers, sizeof( int32_t ) * rp.size() );
-         else
-            TNLHIP_CHECK( hipMemcpy( rp.data(), row_pointers, sizeof( int32_t ) * rp.size(), hipMemcpyDeviceToHost ) );
+      *out = createFromCsr( format, value_dtype, rows, columns, row_pointers, column_indexes, values, host_pointers != 0,
+                            false );
+   } );
+}
+
+int tnlhip_matrix_create_staged( int format, int value_dtype, int symmetric, int32_t rows, int32_t columns,
+                                 const int32_t* row_pointers, const int32_t* column_indexes, const double* values,
+                                 tnlhip_matrix* out )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( out != nullptr, "out is NULL" );
+      if( symmetric ) {
+         TNLHIP_REQUIRE( rows == columns, "a symmetric matrix is square" );
+         if( value_dtype == TNLHIP_BINARY )
+            throw Error( TNLHIP_ERROR_UNSUPPORTED, "symmetric binary matrices are not supported" );
+         for( int32_t i = 0; i < rows; i++ )
+            for( int32_t k = row_pointers[ i ]; k < row_pointers[ i + 1 ]; k++ )
+               if( column_indexes[ k ] > i )
+                  throw Error( TNLHIP_ERROR_INVALID_ARGUMENT,
+                               "a symmetric matrix stores its lower triangle: entry (" + std::to_string( i ) + ", " +
+                                  std::to_string( column_indexes[ k ] ) + ") lies above the diagonal" );
       }
-      TNLHIP_REQUIRE( rp[ 0 ] == 0, "row_pointers[0] must be 0" );
-      std::vector< int32_t > caps( rows );
-      for( int32_t i = 0; i < rows; i++ ) {
-         caps[ i ] = rp[ i + 1 ] - rp[ i ];
-         TNLHIP_REQUIRE( caps[ i ] >= 0, "row_pointers must be non-decreasing" );
-      }
-      const int64_t nnzIn = rp[ rows ];
-      if( host_pointers && nnzIn > 0 ) {
-         for( int64_t k = 0; k < nnzIn; k++ )
-            TNLHIP_REQUIRE( column_indexes[ k ] >= -1 && column_indexes[ k ] < columns, "column index out of range" );
-      }
-      std::unique_ptr< tnlhip_matrix_s > M( buildEmpty( format, value_dtype, rows, columns, caps ) );
-      DeviceArray< int32_t > drp, dcol;
-      DeviceArray< double > dval;
-      const int32_t* prp = row_pointers;
-      const int32_t* pcol = column_indexes;
-      // a binary matrix may come without values: every entry is then a one
-      std::vector< double > ones;
-      if( value_dtype == TNLHIP_BINARY && values == nullptr ) {
-         ones.assign( (size_t) std::max< int64_t >( nnzIn, 0 ), 1.0 );
-         values = ones.data();
-         if( ! host_pointers ) {
-            dval.upload( ones );
-            values = dval.ptr;
-         }
-      }
-      const double* pval = values;
-      if( host_pointers ) {
-         drp.upload( rp );
-         dcol.upload( column_indexes, nnzIn );
-         dval.upload( values, nnzIn );
-         prp = drp.ptr;
-         pcol = dcol.ptr;
-         pval = dval.ptr;
-      }
-      convertFromCsrDevice( *M, prp, pcol, pval, nullptr );
-      M->nnz = countStored( *M, nullptr );
+      std::unique_ptr< tnlhip_matrix_s > M(
+         createFromCsr( format, value_dtype, rows, columns, row_pointers, column_indexes, values, true, true ) );
+      M->symmetric = symmetric != 0;
+      M->nnz = countNonzeros( *M, nullptr );
       *out = M.release();
    } );
 }
@@ -346,6 +454,7 @@ int tnlhip_matrix_create_symmetric_csr( int format, int value_dtype, int32_t row
       TNLHIP_CHECK_STATUS( tnlhip_matrix_create_csr( format, value_dtype, rows, rows, row_pointers, column_indexes,
                                                      values, host_pointers, &M ) );
       M->symmetric = true;
+      M->nnz = countNonzeros( *M, nullptr );
       *out = M;
    } );
 }
@@ -355,8 +464,9 @@ int tnlhip_matrix_convert( tnlhip_matrix source, int format, int value_dtype, tn
    return guarded( [ & ] {
       TNLHIP_REQUIRE( source && out, "NULL argument" );
       const Matrix& S = *source;
+      // operator= takes the source's getRowCapacities: its stored entries per row
       DeviceArray< int32_t > dcaps( S.rows );
-      capacitiesOf( S, dcaps.ptr, nullptr );
+      storedCounts( S, dcaps.ptr, nullptr );
       std::vector< int32_t > caps = dcaps.download();
       std::unique_ptr< tnlhip_matrix_s > M( buildEmpty( format, value_dtype, S.rows, S.cols, caps ) );
       std::vector< int32_t > rp( (size_t) S.rows + 1, 0 );
@@ -367,8 +477,8 @@ int tnlhip_matrix_convert( tnlhip_matrix source, int format, int value_dtype, tn
       drp.upload( rp );
       gatherToCsr( S, drp.ptr, dcol.ptr, dval.ptr, nullptr );
       convertFromCsrDevice( *M, drp.ptr, dcol.ptr, dval.ptr, nullptr );
-      M->nnz = countStored( *M, nullptr );
       M->symmetric = S.symmetric;
+      M->nnz = countNonzeros( *M, nullptr );
       *out = M.release();
    } );
 }
@@ -398,20 +508,13 @@ int tnlhip_mtx_read( const char* path, int format, int value_dtype, tnlhip_matri
 {
    return guarded( [ & ] {
       TNLHIP_REQUIRE( path && out, "NULL argument" );
+      // MatrixReader< Matrix, Device >::readMtx (MatrixReader.hpp:19-26): read into a host matrix
+      // (padding where entries repeat, explicit zeros stored), then matrix = hostMatrix, which
+      // takes its getRowCapacities and drops the explicit zeros
       const MtxCsr m = readMtxCsr( path );
-      std::vector< int32_t > caps( m.rows );
-      for( int32_t i = 0; i < m.rows; i++ )
-         caps[ i ] = m.rowPointers[ i + 1 ] - m.rowPointers[ i ];
-      std::unique_ptr< tnlhip_matrix_s > M( buildEmpty( format, value_dtype, m.rows, m.columns, caps ) );
-      DeviceArray< int32_t > drp, dcol;
-      DeviceArray< double > dval;
-      drp.upload( m.rowPointers );
-      dcol.upload( m.columnIndexes );
-      dval.upload( m.values );
-      // setElement keeps explicit zeros (SparseMatrixView.hpp:270-345)
-      convertFromCsrDevice( *M, drp.ptr, dcol.ptr, dval.ptr, nullptr, true );
-      M->nnz = countStored( *M, nullptr );
-      *out = M.release();
+      *out = createFromCsr( format, value_dtype, m.rows, m.columns, m.rowPointers.data(),
+                            m.columnIndexes.empty() ? nullptr : m.columnIndexes.data(),
+                            m.values.empty() ? nullptr : m.values.data(), true, false );
    } );
 }
 
@@ -433,8 +536,10 @@ int tnlhip_mtx_write( tnlhip_matrix A, const char* path )
    return guarded( [ & ] {
       TNLHIP_REQUIRE( A && path, "NULL argument" );
       const Matrix& S = *A;
+      // writeMtx of a device matrix writes a host copy, hostMatrix = matrix (MatrixWriter.hpp:35-40):
+      // capacities from getRowCapacities, explicit zeros dropped
       DeviceArray< int32_t > dcaps( S.rows );
-      capacitiesOf( S, dcaps.ptr, nullptr );
+      storedCounts( S, dcaps.ptr, nullptr );
       std::vector< int32_t > caps = dcaps.download();
       std::vector< int32_t > rp( (size_t) S.rows + 1, 0 );
       for( int32_t i = 0; i < S.rows; i++ )
@@ -442,7 +547,7 @@ int tnlhip_mtx_write( tnlhip_matrix A, const char* path )
       DeviceArray< int32_t > drp, dcol( rp[ S.rows ] );
       DeviceArray< double > dval( rp[ S.rows ] );
       drp.upload( rp );
-      gatherToCsr( S, drp.ptr, dcol.ptr, dval.ptr, nullptr, true );  // stored zeros count in the header
+      gatherToCsr( S, drp.ptr, dcol.ptr, dval.ptr, nullptr );
       writeMtx( path, S.rows, S.cols, rp, dcol.download(), dval.download() );
    } );
 }
@@ -511,8 +616,8 @@ int tnlhip_matrix_load( const char* path, int format, int value_dtype, int layou
          M->trailingPadding = chunkedPaddingTrails( M->meta, f.columnIndexes );
       M->values.upload( f.values.data(), (int64_t) f.values.size() );
       M->columns.upload( f.columnIndexes );
-      M->nnz = countStored( *M, nullptr );
       M->symmetric = f.symmetric;
+      M->nnz = countNonzeros( *M, nullptr );
       *out = M.release();
    } );
 }
@@ -690,12 +795,12 @@ int tnlhip_matrix_set_param( tnlhip_matrix A, const char* name_c, int32_t value 
       TNLHIP_REQUIRE( A && name_c, "NULL argument" );
       const std::string name( name_c );
       if( name == "lanes_per_row" ) {
-         TNLHIP_REQUIRE( value >= 1 && value <= kWave && ( value & ( value - 1 ) ) == 0,
-                         "lanes_per_row must be a power of two in [1, 64]" );
+         TNLHIP_REQUIRE( value >= 0 && value <= kWave && ( value & ( value - 1 ) ) == 0,
+                         "lanes_per_row must be 0 (the format's default) or a power of two in [1, 64]" );
          TNLHIP_REQUIRE( kindOf( A->format ) == KIND_CSR && A->format != TNLHIP_CSR_SCALAR,
                          "lanes_per_row applies to CSR light / hybrid / vector / adaptive fallback" );
-         A->lanes = value;
-         A->lanesExplicit = true;
+         A->lanes = value == 0 ? A->defaultLanes : value;
+         A->lanesExplicit = value != 0;
       }
       else if( name == "xcd_remap" ) {
          // 2 (contiguous blocks without the XCD remap) is switched off: a run of it ended in a
@@ -704,6 +809,8 @@ int tnlhip_matrix_set_param( tnlhip_matrix A, const char* name_c, int32_t value 
                          "xcd_remap is 0 (grid-stride), 1 (XCD-remapped blocks) or 3 (per-XCD grid-stride)" );
          A->xcdRemap = value;
       }
+      else if( name == "column_stripes" )
+         buildStripePlan( *A, value, nullptr );
       else if( name == "pair_rows" )
          A->pairRows = value != 0;
       else if( name == "blocks_per_cu" ) {

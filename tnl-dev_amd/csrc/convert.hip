@@ -142,6 +142,27 @@ void convertFromCsrDevice( Matrix& A, const int32_t* rowPtr, const int32_t* cols
       throw Error( TNLHIP_ERROR_CAPACITY, "row capacity exceeded during conversion" );
 }
 
+__global__ void csrStoredCountsKernel( int32_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ cols,
+                                       int32_t* __restrict__ caps )
+{
+   const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+   for( int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride ) {
+      int32_t k = 0;
+      for( int32_t g = rp[ r ]; g < rp[ r + 1 ]; g++ )
+         k += cols[ g ] != -1;
+      caps[ r ] = k;
+   }
+}
+
+void csrStoredCounts( int32_t n, const int32_t* rpDevice, const int32_t* colsDevice, int32_t* capsDevice,
+                      hipStream_t stream )
+{
+   if( n == 0 )
+      return;
+   csrStoredCountsKernel<<< gridFor( n, 256 ), 256, 0, stream >>>( n, rpDevice, colsDevice, capsDevice );
+   TNLHIP_CHECK( hipGetLastError() );
+}
+
 void capacitiesOf( const Matrix& A, int32_t* capsDevice, hipStream_t stream )
 {
    if( A.rows == 0 )

@@ -5,6 +5,7 @@
 #include "common.hpp"
 
 #include <map>
+#include <memory>
 
 namespace tnlhip {
 
@@ -74,6 +75,19 @@ constexpr int kAdaptiveStreamBytesPerWave = 4096;
 // Adaptive block types (detail/CSRAdaptiveKernelBlockDescriptor.h)
 enum { BLOCK_LONG = 0, BLOCK_STREAM = 1, BLOCK_VECTOR = 2 };
 
+// Column-striped copy of a CSR matrix (stripes.hip): entries stripe-major, one partial sum per
+// (stripe, row).  Built on request (tnlhip_matrix_set_param "column_stripes").
+struct StripePlan
+{
+   int32_t stripes = 0;               // S
+   int32_t width = 0;                 // columns per stripe
+   int64_t entries = 0;               // stored entries copied
+   DeviceArray< int32_t > offsets;    // S * rows + 1: start of (stripe s, row r) at s * rows + r
+   DeviceArray< int32_t > columns;    // entries
+   DeviceArray< uint8_t > values;     // entries of the matrix value type
+   DeviceArray< uint8_t > partial;    // S * rows partial sums in the compute type
+};
+
 struct Matrix
 {
    int format = TNLHIP_CSR_SCALAR;
@@ -89,6 +103,7 @@ struct Matrix
    std::vector< int32_t > adaptiveHost;
    int32_t adaptiveBlocks = 0;
    int32_t lanes = 0;      // lanes per row for CSR light / hybrid / vector
+   int32_t defaultLanes = 0;  // the format's own choice, restored by lanes_per_row 0
    bool lanesExplicit = false;  // lanes_per_row set by the caller: the cooperative kernel always runs
    int32_t xcdRemap = 0;
    // every chunk of a ChunkedEllpack matrix keeps its padding after its elements (true for
@@ -99,6 +114,7 @@ struct Matrix
    bool symmetric = false;    // SymmetricMatrix: the lower triangle is stored
    DeviceArray< uint8_t > diagonal;  // cached Jacobi diagonal (vector type)
    bool diagonalValid = false;
+   std::unique_ptr< StripePlan > stripes;  // column-striped plan of a CSR matrix, when built
 
    void uploadMetadata();
 };
@@ -116,6 +132,12 @@ void spmvSymmetric( const Matrix& A, const void* x, void* y, double alpha, doubl
 // Binary matrix product (symmetric.hip): int32 vectors, alpha / beta truncated to int32
 void spmvBinary( const Matrix& A, const void* x, void* y, double alpha, double beta, int32_t first, int32_t last,
                  Gate gate, hipStream_t stream );
+// getNonzeroElementsCount (SparseMatrixView.hpp:180-215): stored entries, or for a symmetric
+// matrix the entries of the whole matrix they stand for (capi.hip, traverse.hip)
+int64_t countNonzeros( const Matrix& M, hipStream_t stream );
+int64_t symmetricNonzeros( const Matrix& M, hipStream_t stream );
+// Builds (S > 0) or drops (S == 0) the column-striped plan of a CSR matrix (stripes.hip).
+void buildStripePlan( Matrix& M, int S, hipStream_t stream );
 void convertFromCsrDevice( Matrix& A, const int32_t* rowPtr, const int32_t* cols, const double* vals,
                            hipStream_t stream,
                            bool keepZeros = false );
@@ -123,7 +145,13 @@ void gatherToCsr( const Matrix& A, const int32_t* rpDevice, int32_t* colsDevice,
                   hipStream_t stream,
                   bool keepZeros = false );
 void extractDiagonal( const Matrix& A, void* diag, hipStream_t stream );
+// segment size of every row (the slots a row owns)
 void capacitiesOf( const Matrix& A, int32_t* capsDevice, hipStream_t stream );
+// getRowCapacities: stored (non-padding) entries of every row of a matrix (traverse.hip) or of a
+// device CSR (convert.hip)
+void storedCounts( const Matrix& A, int32_t* countsDevice, hipStream_t stream );
+void csrStoredCounts( int32_t n, const int32_t* rpDevice, const int32_t* colsDevice, int32_t* capsDevice,
+                      hipStream_t stream );
 
 // Matrix Market files with TNL's MatrixReader / MatrixWriter semantics (mtx.cpp).  The CSR
 // keeps TNL's reading result: row capacities from the file, padding -1, explicit zeros kept.

@@ -2420,6 +2420,11 @@ int tnlhip_solver_set_params( tnlhip_solver s, const tnlhip_solver_params* p )
       TNLHIP_REQUIRE( p->preconditioner >= 0 && p->preconditioner <= 2, "unknown preconditioner" );
       TNLHIP_REQUIRE( p->gmres_variant >= 0 && p->gmres_variant <= 4, "unknown GMRES variant" );
       TNLHIP_REQUIRE( p->bicgstab_ell >= 1 && p->bicgstab_ell <= 16, "bicgstab_ell must be in [1, 16]" );
+      // restarting_max sizes the GMRES basis and its small arrays
+      TNLHIP_REQUIRE( p->restarting_min >= 1 && p->restarting_min <= p->restarting_max && p->restarting_max <= 31,
+                      "GMRES restarting needs 1 <= restarting_min <= restarting_max <= 31" );
+      TNLHIP_REQUIRE( p->restarting_step_min >= 0 && p->restarting_step_min <= p->restarting_step_max,
+                      "GMRES restarting steps need 0 <= restarting_step_min <= restarting_step_max" );
       const bool pre = p->preconditioner != s->params.preconditioner;
       s->params = *p;
       if( pre ) {
@@ -2458,8 +2463,10 @@ int tnlhip_solver_get_status( tnlhip_solver s, int32_t* iterations, double* resi
 
 int64_t tnlhip_solver_get_history( tnlhip_solver s, double* out, int64_t capacity )
 {
-   if( ! s )
+   if( ! s || ( out && capacity < 0 ) ) {
+      setLastError( s ? "negative history capacity" : "NULL solver" );
       return -1;
+   }
    const int64_t n = (int64_t) s->history.size();
    if( out )
       std::memcpy( out, s->history.data(), sizeof( double ) * std::min( n, capacity ) );

@@ -9,44 +9,9 @@
 // order; the one-lane-per-row kernels here (CSR scalar, Ellpack, SlicedEllpack) keep that
 // order and round each product and sum separately, so they reproduce the Host result bit
 // for bit; the cooperative kernels reassociate the row sum.
-#include "views.hpp"
+#include "spmv.hpp"
 
 namespace tnlhip {
-
-// ------------------------------------------------------------------ result keeper
-
-// SparseMatrixView.hpp:447-490: direct / alpha / beta / general keepers; mode 4 divides by
-// a diagonal, which fuses Preconditioners::Diagonal::solve (x = b / diag) into the product.
-template< typename A, typename Y >
-struct Keeper
-{
-   Y* y;
-   A alpha, beta;
-   const Y* diag;
-   int mode;
-   const int32_t* gate;  // solver flags: the launch is skipped when gate[0] != 0 (and, with
-   int gateMode;         // gateMode 2, unless gate[1] != 0)
-   __device__ __forceinline__ bool closed() const
-   {
-      if( gateMode == 0 )
-         return false;
-      if( __builtin_nontemporal_load( gate ) )
-         return true;
-      return gateMode == 2 && ! __builtin_nontemporal_load( gate + 1 );
-   }
-   __device__ __forceinline__ void operator()( int64_t row, A v ) const
-   {
-      A out;
-      switch( mode ) {
-         case 0: out = v; break;
-         case 1: out = alpha * v; break;
-         case 2: out = beta * (A) y[ row ] + v; break;
-         case 3: out = beta * (A) y[ row ] + alpha * v; break;
-         default: out = v / (A) diag[ row ]; break;
-      }
-      __builtin_nontemporal_store( (Y) out, y + row );  // y is written once: keep L2 for x
-   }
-};
 
 template< bool EXACT, typename A >
 __device__ __forceinline__ A madd( A acc, A a, A b )
@@ -107,14 +72,6 @@ __device__ __forceinline__ RowSeq rowSequence( int64_t base, int64_t last, int64
    }
    const int64_t r0 = base + logicalBlock( 1 ) * per;
    return RowSeq{ r0 + (int64_t) threadIdx.x * P, min( r0 + per, last ), (int64_t) blockDim.x * P };
-}
-
-// Wave-local LDS handoff between lanes of one wavefront (no workgroup barrier).
-__device__ __forceinline__ void waveSync()
-{
-   __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
-   __builtin_amdgcn_wave_barrier();
-   __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
 }
 
 // ------------------------------------------------------------------ CSR: G lanes per row
@@ -817,6 +774,17 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
       csrScalarStagedKernel< K, V, A, Y >
          <<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last, M.xcdRemap ? 1 : 0 );
    };
+   if( M.cols == 0 ) {
+      // every stored slot is padding and x is empty: the kernels below gather x[0] for padding
+      // slots, so a matrix without columns takes the all-zero product without touching x
+      zeroRowsKernel< A, Y ><<< gridFor( rows, 256 ), 256, 0, st >>>( keep, first, last );
+      TNLHIP_CHECK( hipGetLastError() );
+      return;
+   }
+   if( M.stripes && kindOf( M.format ) == KIND_CSR ) {
+      stripeLaunch< V, A, Y >( M, x, keep, first, last, st );
+      return;
+   }
    switch( M.format ) {
       case TNLHIP_CSR_SCALAR:
          launchStaged();

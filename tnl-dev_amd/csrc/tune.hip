@@ -40,6 +40,18 @@ std::vector< Candidate > candidatesFor( int format )
    return c;
 }
 
+// Every knob of the candidate, defaults included, so a matrix configured for one candidate
+// carries nothing over from the one timed before it.
+void configure( tnlhip_matrix A, const Candidate& c )
+{
+   const bool csrGroups = c.format == TNLHIP_CSR_LIGHT || c.format == TNLHIP_CSR_HYBRID ||
+                          c.format == TNLHIP_CSR_VECTOR || c.format == TNLHIP_CSR_ADAPTIVE;
+   if( csrGroups )
+      TNLHIP_CHECK_STATUS( tnlhip_matrix_set_param( A, "lanes_per_row", c.lanes ) );
+   TNLHIP_CHECK_STATUS( tnlhip_matrix_set_param( A, "blocks_per_cu", c.blocksPerCu > 0 ? c.blocksPerCu : 16 ) );
+   TNLHIP_CHECK_STATUS( tnlhip_matrix_set_param( A, "pair_rows", c.pairRows > 0 ? 1 : 0 ) );
+}
+
 }  // namespace
 
 using namespace tnlhip;
@@ -84,12 +96,7 @@ int tnlhip_autotune_csr( int value_dtype, int32_t rows, int32_t columns, const i
             throw Error( s, std::string( "autotune: building a candidate failed: " ) + tnlhip_last_error() );
          bool kept = false;
          for( const Candidate& c : candidatesFor( format ) ) {
-            if( c.lanes > 0 )
-               TNLHIP_CHECK_STATUS( tnlhip_matrix_set_param( A, "lanes_per_row", c.lanes ) );
-            if( c.blocksPerCu > 0 )
-               TNLHIP_CHECK_STATUS( tnlhip_matrix_set_param( A, "blocks_per_cu", c.blocksPerCu ) );
-            if( c.pairRows >= 0 )
-               TNLHIP_CHECK_STATUS( tnlhip_matrix_set_param( A, "pair_rows", c.pairRows ) );
+            configure( A, c );
             for( int w = 0; w < 2; w++ )
                TNLHIP_CHECK_STATUS( tnlhip_spmv( A, x, y, 1.0, 0.0, 0, 0, st ) );
             TNLHIP_CHECK( hipEventRecord( e0, st ) );
@@ -113,15 +120,10 @@ int tnlhip_autotune_csr( int value_dtype, int32_t rows, int32_t columns, const i
                }
             }
          }
-         if( kept ) {
-            // leave the kept matrix configured as its best candidate
-            if( best->lanes > 0 )
-               tnlhip_matrix_set_param( A, "lanes_per_row", best->lanes );
-            if( best->blocks_per_cu > 0 )
-               tnlhip_matrix_set_param( A, "blocks_per_cu", best->blocks_per_cu );
-            if( best->pair_rows >= 0 )
-               tnlhip_matrix_set_param( A, "pair_rows", best->pair_rows );
-         }
+         if( kept )
+            // leave the kept matrix configured exactly as its best candidate (format defaults
+            // restored for the knobs the candidate leaves at their default)
+            configure( A, Candidate{ best->format, best->lanes, best->blocks_per_cu, best->pair_rows } );
          else
             tnlhip_matrix_destroy( A );
       }
