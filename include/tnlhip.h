@@ -283,7 +283,7 @@ int tnlhip_matrix_get_row( tnlhip_matrix A, int32_t row, int32_t* columns, doubl
 
 /* Device arrays of a matrix, for traversal kernels compiled by the caller (the C++ front's
  * forElements / forRows / reduceRows with device lambdas, SparseMatrixView.hpp:495-760).  The
- * index maps are TNL's column-major ones (views of Algorithms/Segments/*View.hpp): CSR
+ * index maps are TNL's column-major ones (the segment views of Algorithms/Segments): CSR
  * offsets[r] + l; Ellpack r + aligned_size * l; SlicedEllpack slice_offsets[r / 32] + r % 32 + 32 l;
  * ChunkedEllpack and BiEllpack as detail/ChunkedEllpack.h:110-145 and detail/BiEllpack.h:150-185.
  * The pointers stay valid until the matrix is destroyed; a caller that writes values or column
@@ -408,8 +408,25 @@ int tnlhip_solver_solve( tnlhip_solver solver, const void* b, void* x, int32_t* 
 int tnlhip_solver_get_status( tnlhip_solver solver, int32_t* iterations, double* residue );
 
 /* Every value passed to setResidue during the last solve, in order (the residual history
- * file of IterativeSolver.hpp:189-200); returns the count, out may be NULL. */
+ * file of IterativeSolver.hpp:189-200); returns the count, out may be NULL, -1 for a NULL solver or
+ * a negative capacity. */
 int64_t tnlhip_solver_get_history( tnlhip_solver solver, double* out, int64_t capacity );
+
+/* getIterations() at each of those setResidue calls (the first field of a history-file line). */
+int64_t tnlhip_solver_get_history_iterations( tnlhip_solver solver, int32_t* out, int64_t capacity );
+
+/* IterativeSolver::setSolverMonitor (Solvers/IterativeSolver.h:177-188, IterativeSolverMonitor):
+ * monitor( user, iterations, residue ) is called on the calling thread each time the solve reads
+ * its device state (after each replayed batch of iterations, each GMRES restart cycle, and at the
+ * end); NULL removes it.  The refresh rate of TNL's monitor thread belongs to the monitor, which
+ * the C++ front runs. */
+typedef void ( *tnlhip_monitor_fn )( void* user, int32_t iterations, double residue );
+int tnlhip_solver_set_monitor( tnlhip_solver solver, tnlhip_monitor_fn monitor, void* user );
+
+/* setup( "residual-history-file" ) (IterativeSolver.hpp:57-61, 189-200): opens the file (NULL or ""
+ * closes it); each solve appends its setResidue calls as TNL writes them, a blank line at
+ * iteration 0, then "<iteration>\t<residue>" in std::scientific. */
+int tnlhip_solver_set_residual_history_file( tnlhip_solver solver, const char* path );
 
 /* ------------------------------------------------------------------ mixed-precision refinement */
 

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../Algorithms/Segments/Segments.h"
+#include "../Algorithms/Segments/SegmentsMap.h"
 #include "../Containers/Vector.h"
 
 namespace TNL {
@@ -29,6 +30,99 @@ struct bfloat16
 };
 
 namespace Matrices {
+
+#ifdef __HIPCC__
+namespace traversal {
+
+// One lane per row, slots in local order, padding fetching the identity: the Host traversal of
+// SparseMatrixView::reduceRows (SparseMatrixView.hpp:495-530) with user device lambdas.
+template< typename Real, typename Fetch, typename Reduce, typename Keep, typename FetchValue >
+__global__ void reduceRowsKernel( int begin, int end, Algorithms::Segments::SegmentsMap map, Fetch fetch,
+                                  Reduce reduce, Keep keep, FetchValue identity )
+{
+   const Real* values = static_cast< const Real* >( map.v.values );
+   for( long long r = begin + (long long) blockIdx.x * blockDim.x + threadIdx.x; r < end;
+        r += (long long) gridDim.x * blockDim.x ) {
+      FetchValue acc = identity;
+      const int size = map.getSegmentSize( (int) r );
+      for( int l = 0; l < size; l++ ) {
+         const long long g = map.getGlobalIndex( (int) r, l );
+         const int column = map.v.column_indexes[ g ];
+         FetchValue f = identity;
+         if( column != -1 )
+            f = fetch( (int) r, column, values ? values[ g ] : Real( 1 ) );
+         acc = reduce( acc, f );
+      }
+      keep( (int) r, acc );
+   }
+}
+
+// forElements (SparseMatrixView.hpp:600-660): function( rowIdx, localIdx, columnIdx&, value& ) for
+// every slot of the rows, padding included.
+template< typename Real, typename Function >
+__global__ void forElementsKernel( int begin, int end, Algorithms::Segments::SegmentsMap map, Function function )
+{
+   Real* values = static_cast< Real* >( map.v.values );
+   for( long long r = begin + (long long) blockIdx.x * blockDim.x + threadIdx.x; r < end;
+        r += (long long) gridDim.x * blockDim.x ) {
+      const int size = map.getSegmentSize( (int) r );
+      for( int l = 0; l < size; l++ ) {
+         const long long g = map.getGlobalIndex( (int) r, l );
+         Real one( map.v.column_indexes[ g ] != -1 );
+         function( (int) r, l, map.v.column_indexes[ g ], values ? values[ g ] : one );
+      }
+   }
+}
+
+// SparseMatrixRowView on the device (Matrices/SparseMatrixRowView.h): getSize, getRowIndex,
+// getColumnIndex, getValue, setValue, setColumnIndex, setElement.
+template< typename Real >
+struct DeviceRowView
+{
+   const Algorithms::Segments::SegmentsMap* map;
+   int row;
+   __device__ int getSize() const { return map->getSegmentSize( row ); }
+   __device__ int getRowIndex() const { return row; }
+   __device__ int getColumnIndex( int l ) const { return map->v.column_indexes[ map->getGlobalIndex( row, l ) ]; }
+   __device__ Real getValue( int l ) const
+   {
+      const long long g = map->getGlobalIndex( row, l );
+      return map->v.values ? static_cast< const Real* >( map->v.values )[ g ] : Real( map->v.column_indexes[ g ] != -1 );
+   }
+   __device__ void setValue( int l, const Real& value ) const
+   {
+      if( map->v.values )
+         static_cast< Real* >( map->v.values )[ map->getGlobalIndex( row, l ) ] = value;
+   }
+   __device__ void setColumnIndex( int l, int column ) const
+   {
+      map->v.column_indexes[ map->getGlobalIndex( row, l ) ] = column;
+   }
+   __device__ void setElement( int l, int column, const Real& value ) const
+   {
+      setColumnIndex( l, column );
+      setValue( l, value );
+   }
+};
+
+template< typename Real, typename Function >
+__global__ void forRowsKernel( int begin, int end, Algorithms::Segments::SegmentsMap map, Function function )
+{
+   for( long long r = begin + (long long) blockIdx.x * blockDim.x + threadIdx.x; r < end;
+        r += (long long) gridDim.x * blockDim.x ) {
+      DeviceRowView< Real > row{ &map, (int) r };
+      function( row );
+   }
+}
+
+inline unsigned traversalGrid( int rows )
+{
+   const long long blocks = ( (long long) rows + 255 ) / 256;
+   return (unsigned) ( blocks < 8192 ? ( blocks > 0 ? blocks : 1 ) : 8192 );
+}
+
+}  // namespace traversal
+#endif  // __HIPCC__
 
 struct GeneralMatrix
 {
@@ -248,6 +342,127 @@ public:
       detail::checkHip( hipStreamSynchronize( nullptr ) );
    }
 
+   // ---- row traversals (SparseMatrixView.hpp:116-250, 495-760)
+
+   // getRowCapacities: the stored (non-padding) entries of every row (SparseMatrixView.hpp:143-160)
+   template< typename Vector >
+   void getRowCapacities( Vector& rowCapacities ) const
+   {
+      rowCapacities.setSize( rows );
+      detail::check( tnlhip_matrix_get_row_capacities( getHandle(), rowCapacities.getData(), nullptr ) );
+      detail::checkHip( hipStreamSynchronize( nullptr ) );
+   }
+   // getCompressedRowLengths: the entries of every row whose value is not zero (:116-135)
+   template< typename Vector >
+   void getCompressedRowLengths( Vector& rowLengths ) const
+   {
+      rowLengths.setSize( rows );
+      detail::check( tnlhip_matrix_get_compressed_row_lengths( getHandle(), rowLengths.getData(), nullptr ) );
+      detail::checkHip( hipStreamSynchronize( nullptr ) );
+   }
+   // getRowCapacity( row ): the segment size of the row
+   Index getRowCapacity( Index row ) const
+   {
+      int32_t c = 0;
+      detail::check( tnlhip_matrix_get_row_capacity( getHandle(), row, &c ) );
+      return c;
+   }
+   // getRow( row ) read on the host: a copy of the row's slots in local order (padding: column -1)
+   struct ConstRowCopy
+   {
+      Index rowIdx;
+      std::vector< Index > columns;
+      std::vector< double > values;
+      Index getSize() const { return (Index) columns.size(); }
+      Index getRowIndex() const { return rowIdx; }
+      Index getColumnIndex( Index l ) const { return columns[ l ]; }
+      double getValue( Index l ) const { return values[ l ]; }
+   };
+   ConstRowCopy getRow( Index row ) const
+   {
+      ConstRowCopy r{ row, {}, {} };
+      int32_t size = 0;
+      detail::check( tnlhip_matrix_get_row( getHandle(), row, nullptr, nullptr, 0, &size ) );
+      r.columns.resize( size );
+      r.values.resize( size );
+      if( size > 0 )
+         detail::check( tnlhip_matrix_get_row( getHandle(), row, r.columns.data(), r.values.data(), size, &size ) );
+      return r;
+   }
+   // reduceRows with the fetch / reduce pairs the C ABI names (tnlhip_fetch, tnlhip_reduce); out
+   // is a Vector< double | float | int, Devices::Hip >
+   template< typename OutVector >
+   void reduceRows( Index begin, Index end, int fetch, int reduce, double identity, OutVector& out ) const
+   {
+      using O = typename OutVector::RealType;
+      const int odt = std::is_same< O, double >::value ? TNLHIP_F64 : std::is_same< O, float >::value ? TNLHIP_F32
+                                                                                                     : TNLHIP_INT32;
+      out.setSize( rows );
+      detail::check( tnlhip_matrix_reduce_rows( getHandle(), begin, end, fetch, reduce, identity, out.getData(), odt,
+                                                nullptr ) );
+      detail::checkHip( hipStreamSynchronize( nullptr ) );
+   }
+
+#ifdef __HIPCC__
+   // reduceRows( begin, end, fetch, reduce, keep, identity ) with device lambdas
+   // fetch( rowIdx, columnIdx, value ), reduce( a, b ), keep( rowIdx, result ); synchronous
+   template< typename Fetch, typename Reduce, typename Keep, typename FetchValue >
+   void reduceRows( Index begin, Index end, Fetch fetch, Reduce reduce, Keep keep, const FetchValue& identity ) const
+   {
+      static_assert( ! std::is_same< Real, bfloat16 >::value, "lambda traversals take double, float or bool values" );
+      if( end <= begin )
+         return;
+      const auto map = deviceMap();
+      traversal::reduceRowsKernel< LambdaReal ><<< traversal::traversalGrid( end - begin ), 256 >>>( begin, end, map, fetch,
+                                                                                                  reduce, keep, identity );
+      detail::checkHip( hipGetLastError() );
+      detail::checkHip( hipStreamSynchronize( nullptr ) );
+   }
+   template< typename Fetch, typename Reduce, typename Keep, typename FetchValue >
+   void reduceAllRows( Fetch fetch, Reduce reduce, Keep keep, const FetchValue& identity ) const
+   {
+      reduceRows( 0, rows, fetch, reduce, keep, identity );
+   }
+   // forElements( begin, end, function( rowIdx, localIdx, columnIdx&, value& ) ): every slot of the
+   // rows, padding included; what the function writes is kept (derived data is refreshed)
+   template< typename Function >
+   void forElements( Index begin, Index end, Function function )
+   {
+      static_assert( ! std::is_same< Real, bfloat16 >::value, "lambda traversals take double, float or bool values" );
+      if( end <= begin )
+         return;
+      const auto map = deviceMap();
+      traversal::forElementsKernel< LambdaReal ><<< traversal::traversalGrid( end - begin ), 256 >>>( begin, end, map,
+                                                                                                   function );
+      detail::checkHip( hipGetLastError() );
+      detail::check( tnlhip_matrix_values_changed( handle, nullptr ) );
+      generation++;
+   }
+   template< typename Function >
+   void forAllElements( Function function )
+   {
+      forElements( 0, rows, function );
+   }
+   // forRows( begin, end, function( RowView& ) ) with TNL's row view on the device
+   template< typename Function >
+   void forRows( Index begin, Index end, Function function )
+   {
+      static_assert( ! std::is_same< Real, bfloat16 >::value, "lambda traversals take double, float or bool values" );
+      if( end <= begin )
+         return;
+      const auto map = deviceMap();
+      traversal::forRowsKernel< LambdaReal ><<< traversal::traversalGrid( end - begin ), 256 >>>( begin, end, map, function );
+      detail::checkHip( hipGetLastError() );
+      detail::check( tnlhip_matrix_values_changed( handle, nullptr ) );
+      generation++;
+   }
+   template< typename Function >
+   void forAllRows( Function function )
+   {
+      forRows( 0, rows, function );
+   }
+#endif
+
    // the C-ABI handle (moves staged elements to the device first)
    tnlhip_matrix getHandle() const
    {
@@ -269,6 +484,16 @@ public:
    uint64_t getGeneration() const { return generation; }
 
 private:
+#ifdef __HIPCC__
+   // the value type the lambdas see: the stored Real (bool for a binary matrix, whose slots read 1)
+   using LambdaReal = typename std::conditional< std::is_same< Real, bool >::value, bool, Real >::type;
+   Algorithms::Segments::SegmentsMap deviceMap() const
+   {
+      Algorithms::Segments::SegmentsMap m;
+      detail::check( tnlhip_matrix_get_view( getHandle(), &m.v ) );
+      return m;
+   }
+#endif
    // back to host staging before an element update (the device copy is rebuilt on next use)
    void stage()
    {

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../Config/ParameterContainer.h"
+#include "../IterativeSolverMonitor.h"
 #include "../../Containers/Vector.h"
 #include "Preconditioners/Diagonal.h"
 #include "Preconditioners/ILU0.h"
@@ -49,6 +50,20 @@ public:
       preconditioner = p;
       params.preconditioner = p ? p->kind() : 0;
    }
+   // IterativeSolver::setSolverMonitor / setRefreshRate (IterativeSolver.h:177-188, .hpp:203-219)
+   using SolverMonitorType = IterativeSolverMonitor< double, int >;
+   void setSolverMonitor( SolverMonitorType& m )
+   {
+      monitor = &m;
+      monitor->setRefreshRate( refreshRate );
+      detail::check( tnlhip_solver_set_monitor( handle, &LinearSolver::report, this ) );
+   }
+   void setRefreshRate( int rate )
+   {
+      refreshRate = rate;
+      if( monitor )
+         monitor->setRefreshRate( rate );
+   }
    void setMaxIterations( int n ) { params.max_iterations = n; }
    void setMinIterations( int n ) { params.min_iterations = n; }
    void setConvergenceResidue( double r ) { params.convergence_residue = r; }
@@ -67,6 +82,12 @@ public:
          params.convergence_residue = p.getDouble( prefix + "convergence-residue" );
       if( p.checkParameter( prefix + "divergence-residue" ) )
          params.divergence_residue = p.getDouble( prefix + "divergence-residue" );
+      if( p.checkParameter( prefix + "refresh-rate" ) )
+         setRefreshRate( p.getInt( prefix + "refresh-rate" ) );
+      // the file opens here and each solve appends its residues, as IterativeSolver.hpp:57-61 does
+      if( p.checkParameter( prefix + "residual-history-file" ) )
+         detail::check( tnlhip_solver_set_residual_history_file(
+            handle, p.getString( prefix + "residual-history-file" ).c_str() ) );
       return true;
    }
 
@@ -109,6 +130,16 @@ public:
    }
 
 protected:
+   static void report( void* self, int32_t iterations, double residue )
+   {
+      auto* s = static_cast< LinearSolver* >( self );
+      if( s->monitor )
+         s->monitor->update( iterations, residue );
+   }
+
+   SolverMonitorType* monitor = nullptr;
+   int refreshRate = 500;  // IterativeSolver.h: milliseconds between monitor refreshes
+
    // binds the matrix's current device copy; the solver drops its cached diagonal and factors
    void bind()
    {

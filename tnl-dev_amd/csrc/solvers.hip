@@ -15,6 +15,8 @@
 #include <memory>
 #include <type_traits>
 #include "ilu.hpp"
+
+#include <cstdio>
 #include "matrix.hpp"
 
 #include <cmath>
@@ -51,12 +53,15 @@ __device__ __forceinline__ bool checkNext( const State& s, const Stop& p )
    return ! ( isnan( r ) || s.it > p.maxIt || ( r > p.div && s.it >= p.minIt ) || ( r < p.conv && s.it >= p.minIt ) );
 }
 
-// IterativeSolver::setResidue with the residual history (IterativeSolver.hpp:189-200)
+// IterativeSolver::setResidue with the residual history (IterativeSolver.hpp:189-200): each call
+// records (residue, the iteration count at that moment), the two fields of a history-file line
 __device__ __forceinline__ void setResidue( State& s, double r, double* hist, int64_t cap )
 {
    s.residue = r;
-   if( s.histLen < cap )
-      hist[ s.histLen ] = r;
+   if( s.histLen < cap ) {
+      hist[ 2 * (int64_t) s.histLen ] = r;
+      hist[ 2 * (int64_t) s.histLen + 1 ] = (double) s.it;
+   }
    s.histLen++;
 }
 
@@ -263,12 +268,20 @@ struct SolverImpl
    int32_t iterations = 0;
    double residue = 0;
    std::vector< double > history;
+   std::vector< int32_t > historyIterations;  // getIterations() at each setResidue
+   // IterativeSolver::setSolverMonitor: called with (iterations, residue) at every host poll
+   tnlhip_monitor_fn monitor = nullptr;
+   void* monitorUser = nullptr;
+   // IterativeSolver's residual-history-file, opened by setup (IterativeSolver.hpp:57-61)
+   std::FILE* historyFile = nullptr;
 
    SolverImpl() { tnlhip_solver_params_default( &params ); }
    ~SolverImpl()
    {
       if( stream )
          (void) hipStreamDestroy( stream );
+      if( historyFile )
+         std::fclose( historyFile );
    }
    Stop stopRule() const
    {
@@ -314,6 +327,8 @@ State readState( SolverImpl& S )
    State h;
    TNLHIP_CHECK( hipMemcpyAsync( &h, S.state.ptr, sizeof( State ), hipMemcpyDeviceToHost, S.stream ) );
    TNLHIP_CHECK( hipStreamSynchronize( S.stream ) );
+   if( S.monitor )
+      S.monitor( S.monitorUser, h.it, h.residue );
    return h;
 }
 
@@ -323,9 +338,25 @@ void finishSolve( SolverImpl& S )
    S.iterations = h.it;
    S.residue = h.residue;
    const int64_t len = std::min< int64_t >( h.histLen, S.histCap );
-   S.history.resize( len );
+   std::vector< double > pairs( 2 * len );
    if( len > 0 )
-      TNLHIP_CHECK( hipMemcpy( S.history.data(), S.hist.ptr, sizeof( double ) * len, hipMemcpyDeviceToHost ) );
+      TNLHIP_CHECK( hipMemcpy( pairs.data(), S.hist.ptr, sizeof( double ) * 2 * len, hipMemcpyDeviceToHost ) );
+   S.history.resize( len );
+   S.historyIterations.resize( len );
+   for( int64_t k = 0; k < len; k++ ) {
+      S.history[ k ] = pairs[ 2 * k ];
+      S.historyIterations[ k ] = (int32_t) pairs[ 2 * k + 1 ];
+   }
+   if( S.historyFile ) {
+      // IterativeSolver::setResidue's lines (IterativeSolver.hpp:189-200): a blank line when the
+      // iteration count is 0, then "<iteration>\t<residue in std::scientific>"
+      for( int64_t k = 0; k < len; k++ ) {
+         if( S.historyIterations[ k ] == 0 )
+            std::fputs( "\n", S.historyFile );
+         std::fprintf( S.historyFile, "%d\t%e\n", S.historyIterations[ k ], S.history[ k ] );
+      }
+      std::fflush( S.historyFile );
+   }
 }
 
 void resetState( SolverImpl& S )
@@ -337,7 +368,7 @@ void resetState( SolverImpl& S )
    TNLHIP_CHECK( hipStreamSynchronize( S.stream ) );  // z lives on this stack frame
    const int64_t cap = std::min< int64_t >( (int64_t) S.params.max_iterations + 64, 1 << 20 );
    if( S.histCap != cap ) {
-      S.hist.resize( cap );
+      S.hist.resize( 2 * cap );  // (residue, iteration) pairs
       S.histCap = cap;
    }
 }
@@ -2459,6 +2490,43 @@ int tnlhip_solver_get_status( tnlhip_solver s, int32_t* iterations, double* resi
       if( residue )
          *residue = s->residue;
    } );
+}
+
+int tnlhip_solver_set_monitor( tnlhip_solver s, tnlhip_monitor_fn monitor, void* user )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( s, "NULL solver" );
+      s->monitor = monitor;
+      s->monitorUser = user;
+   } );
+}
+
+int tnlhip_solver_set_residual_history_file( tnlhip_solver s, const char* path )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( s, "NULL solver" );
+      if( s->historyFile ) {
+         std::fclose( s->historyFile );
+         s->historyFile = nullptr;
+      }
+      if( path && path[ 0 ] ) {
+         s->historyFile = std::fopen( path, "w" );
+         if( ! s->historyFile )
+            throw Error( TNLHIP_ERROR_RUNTIME, std::string( "cannot open the residual history file " ) + path );
+      }
+   } );
+}
+
+int64_t tnlhip_solver_get_history_iterations( tnlhip_solver s, int32_t* out, int64_t capacity )
+{
+   if( ! s || ( out && capacity < 0 ) ) {
+      setLastError( s ? "negative history capacity" : "NULL solver" );
+      return -1;
+   }
+   const int64_t n = (int64_t) s->historyIterations.size();
+   if( out )
+      std::memcpy( out, s->historyIterations.data(), sizeof( int32_t ) * std::min( n, capacity ) );
+   return n;
 }
 
 int64_t tnlhip_solver_get_history( tnlhip_solver s, double* out, int64_t capacity )
