@@ -83,28 +83,58 @@ def time_spmv(tnl, torch, A, x, y, steps, warmup, flush):
     return steady, float(np.mean(times))
 
 
+def host_cpu():
+    """CPU model, sockets, physical cores and hardware threads of this host (lscpu's fields, read
+    from /proc/cpuinfo and sysfs)."""
+    model, sockets, cores = "unknown", set(), set()
+    cur = {}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f.read().split("\n") + [""]:
+                if not line.strip():
+                    if cur:
+                        model = cur.get("model name", model)
+                        sockets.add(cur.get("physical id", "0"))
+                        cores.add((cur.get("physical id", "0"), cur.get("core id", cur.get("processor"))))
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                cur[k.strip()] = v.strip()
+    except OSError:
+        pass
+    return {"model": model, "sockets": len(sockets) or 1, "physical_cores": len(cores) or (os.cpu_count() or 1),
+            "hardware_threads": os.cpu_count() or 1,
+            "threads_available": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
 def cpu_baseline(rp, cols, vals, x, ncols):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    cpu = host_cpu()
+    # OpenMP threads: the box's own share of cores (OMP_NUM_THREADS when set, else the cores this
+    # process may run on), and once with one thread
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or cpu["threads_available"] or cpu["physical_cores"]
     n = len(rp) - 1
     nnz = len(cols)
+    from workloads import algorithmic_bytes
+    B = algorithmic_bytes(n, ncols, nnz, 8)
     if pyoracle.ref_available():
         loops = 40
         t, _ = pyoracle.ref_time_spmv_csr(rp, cols, vals, ncols, x, loops=loops, threads=threads)
-        kind, cores, sample = "reference", threads, f"TNL Host CSR vectorProduct, median of {loops} calls on the full matrix"
+        t1, _ = pyoracle.ref_time_spmv_csr(rp, cols, vals, ncols, x, loops=10, threads=1)
+        kind, cores, sample = "reference", threads, (f"TNL Host CSR vectorProduct (OpenMP), median of {loops} calls on "
+                                                     f"the full matrix; one thread: median of 10")
     else:
         seg = pyoracle.Segments("CSR", np.diff(rp))
         loops = 5
         t0 = time.perf_counter()
         for _ in range(loops):
             seg.spmv(vals, cols, x)
-        t = (time.perf_counter() - t0) / loops
+        t = t1 = (time.perf_counter() - t0) / loops
         kind, cores, sample = "port", 1, f"oracle CSR SpMV, mean of {loops} calls on the full matrix"
-    from workloads import algorithmic_bytes
-    gbs = algorithmic_bytes(n, ncols, nnz, 8) / t / 1e9
-    return {"value": round(gbs, 2), "unit": "GB/s", "cores": cores, "kind": kind, "sample": sample,
-            "seconds_per_spmv": t}
+    return {"value": round(B / t / 1e9, 2), "unit": "GB/s", "cores": cores, "kind": kind, "sample": sample,
+            "seconds_per_spmv": t, "one_thread": {"value": round(B / t1 / 1e9, 2), "seconds_per_spmv": t1},
+            "cpu": cpu}
 
 
 SMALL_SIDE = 256  # 2D 5-point Laplacian 256 x 256 (n = 65,536): the case TNL runs on its Host path
@@ -159,13 +189,17 @@ def cpu_solve_baseline(W):
     import pyoracle
     if not pyoracle.ref_available():
         return None
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    cpu = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or cpu["threads_available"] or cpu["physical_cores"]
     rp, c, v, _, bv = small_system(W)
-    r = pyoracle.solve("cg", rp, c, v, bv, backend="ref", threads=threads, convergence_residue=1e-8,
-                       max_iterations=100000)
+    runs = [pyoracle.solve("cg", rp, c, v, bv, backend="ref", threads=threads, convergence_residue=1e-8,
+                           max_iterations=100000) for _ in range(5)]
+    r = sorted(runs, key=lambda q: q["seconds"])[2]  # median of 5 solves
+    r1 = pyoracle.solve("cg", rp, c, v, bv, backend="ref", threads=1, convergence_residue=1e-8, max_iterations=100000)
     return {"workload": "TNL Host CG, 2D 5-point Laplacian 256^2, to 1e-8 from x0 = 0", "cores": threads,
             "kind": "reference", "iterations": r["iterations"], "seconds": round(r["seconds"], 5),
-            "solves_per_s": round(1.0 / r["seconds"], 1)}
+            "solves_per_s": round(1.0 / r["seconds"], 1), "sample": "median of 5 solves",
+            "one_thread": {"seconds": round(r1["seconds"], 5), "solves_per_s": round(1.0 / r1["seconds"], 1)}}
 
 
 def timed(torch, fn):

@@ -498,9 +498,12 @@ int tnlhip_autotune_csr( int value_dtype, int32_t rows, int32_t columns, const i
  * fixed sequence of persistent kernels (replayed as a hipGraph): continuous batching with
  * admission by HBM footprint.  Each request solves (A + shift I) X = B for a resident
  * matrix A to an FP64 relative residual per right-hand side, by FP64 iterative refinement
- * over an FP32 CG inner solve (Solvers/Linear/CG.hpp:17-107 recurrences).  No counterpart in
- * the reference: TNL solves one system per LinearSolver::solve call (LinearSolver.h:115-162);
- * a TNL caller that loops over solve() for many small systems is what this replaces. */
+ * over an FP32 CG inner solve (Solvers/Linear/CG.hpp:17-107 recurrences).  GMRES requests
+ * (method TNLHIP_GMRES, Solvers/Linear/GMRES.hpp:79-180) run beside them: `gmres_workers` of them at
+ * once, each on its own HIP stream, by FP64 refinement over an FP32 Jacobi GMRES(30) of
+ * (A + shift I), admitted FIFO as workers free up.  No counterpart in the reference: TNL solves one
+ * system per LinearSolver::solve call (LinearSolver.h:115-162); a TNL caller that loops over
+ * solve() for many small systems is what this replaces. */
 typedef struct tnlhip_server_s* tnlhip_server;
 
 typedef struct
@@ -512,7 +515,8 @@ typedef struct
    int32_t max_requests;           /* resident requests at once */
    int32_t iterations_per_launch;  /* iterations per replay (one host synchronization each) */
    int32_t use_graph;              /* 1: capture the replay in a hipGraph */
-   int32_t reserved;
+   int32_t gmres_workers;          /* GMRES requests solved at once, each by its own host thread and
+                                      HIP stream (0: GMRES requests are refused) */
 } tnlhip_server_config;
 
 typedef struct
@@ -521,7 +525,7 @@ typedef struct
    double inner_tolerance;         /* FP32 inner CG relative residual */
    int32_t max_outer_iterations;
    int32_t max_inner_iterations;   /* per outer iteration */
-   int32_t method;                 /* TNLHIP_CG */
+   int32_t method;                 /* TNLHIP_CG (batched track) or TNLHIP_GMRES (worker track) */
    int32_t stagnation_limit;       /* as tnlhip_refinement_params */
 } tnlhip_request_params;
 

@@ -144,6 +144,8 @@ def ref():
         L.ref_read_mtx.restype = C.c_int
         L.ref_read_mtx.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong),
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
+        L.ref_keep_history_file.restype = None
+        L.ref_keep_history_file.argtypes = [C.c_char_p]
         L.ref_write_mtx_device.restype = C.c_int
         L.ref_write_mtx_device.argtypes = [C.c_char_p, C.c_int, C.c_int, _i32p, _i32p, _f64p]
         L.ref_write_mtx.restype = C.c_int
@@ -464,8 +466,10 @@ def ilu0(row_ptr, cols, vals, b=None, backend="oracle"):
     return (Lrp, Lc, Lv), (Urp, Uc, Uv), (None if b is None else x)
 
 
-def solve(method, row_ptr, cols, vals, b, x0=None, backend="oracle", threads=1, history_cap=100000, **params):
-    """Returns dict(x, converged, iterations, residue, history[, seconds])."""
+def solve(method, row_ptr, cols, vals, b, x0=None, backend="oracle", threads=1, history_cap=100000, history_file=None,
+          **params):
+    """Returns dict(x, converged, iterations, residue, history[, seconds]).  With backend "ref" and a
+    history_file, TNL's residual-history-file of the solve is left at that path."""
     rp = np.ascontiguousarray(row_ptr, np.int32)
     n = len(rp) - 1
     c = np.ascontiguousarray(cols, np.int32)
@@ -485,6 +489,8 @@ def solve(method, row_ptr, cols, vals, b, x0=None, backend="oracle", threads=1, 
                          history_cap, C.byref(hl))
     else:
         L = ref()
+        if history_file:
+            L.ref_keep_history_file(str(history_file).encode())
         p = make_params(lib().orc_solver_params_default, **params)
         secs = C.c_double(0)
         its_i = C.c_int(0)

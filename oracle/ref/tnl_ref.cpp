@@ -501,6 +501,13 @@ typedef struct
    int bicgstab_exact_residue, preconditioner, bicgstab_ell;
 } ref_solver_params;
 
+// When set, the next ref_solve writes TNL's residual-history-file to this path and keeps it.
+static std::string g_keepHistory;
+void ref_keep_history_file( const char* path )
+{
+   g_keepHistory = path ? path : "";
+}
+
 // Solve on the Host with TNL's solvers; history via residual-history-file.  Returns
 // 1 converged, 0 not, -1 GMRES breakdown; *seconds = wall time of solve().
 int ref_solve( int method, int n, const int* rp, const int* cols, const double* vals, const double* b, double* x,
@@ -516,10 +523,16 @@ int ref_solve( int method, int n, const int* rp, const int* cols, const double* 
       bv[ i ] = b[ i ];
       xv[ i ] = x[ i ];
    }
-   char path[] = "/tmp/tnlref_histXXXXXX";
-   const int fd = mkstemp( path );
-   if( fd >= 0 )
-      close( fd );
+   char tmp[] = "/tmp/tnlref_histXXXXXX";
+   const bool keep = ! g_keepHistory.empty();
+   if( ! keep ) {
+      const int fd = mkstemp( tmp );
+      if( fd >= 0 )
+         close( fd );
+   }
+   const std::string pathStr = keep ? g_keepHistory : std::string( tmp );
+   const char* path = pathStr.c_str();
+   g_keepHistory.clear();
    using Pre = Solvers::Linear::Preconditioners::Diagonal< CsrMatrix >;
    using Ilu = Solvers::Linear::Preconditioners::ILU0< CsrMatrix >;
    auto pre = std::make_shared< Pre >();
@@ -609,7 +622,8 @@ int ref_solve( int method, int n, const int* rp, const int* cols, const double* 
    }
    if( history_len )
       *history_len = count;
-   std::remove( path );
+   if( ! keep )
+      std::remove( path );
    return rc;
 }
 

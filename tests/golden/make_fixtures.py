@@ -236,6 +236,36 @@ def write_solver_fixtures():
     print("solvers.npz", os.path.getsize(os.path.join(HERE, "solvers.npz")))
 
 
+HISTORY_RUNS = [("lap2d_24", "cg", {}), ("lap2d_24", "bicgstab", {"preconditioner": 1}),
+                ("lap2d_24", "gmres", {"gmres_variant": 4}), ("lap2d_24", "gmres", {"gmres_variant": 1}),
+                ("convdiff_8_pe10", "tfqmr", {"preconditioner": 1, "convergence_residue": 1e-10}),
+                ("lap2d_24", "bicgstabl", {"bicgstab_ell": 2})]
+
+
+def write_history_fixtures():
+    """TNL's residual-history-file text (IterativeSolver.hpp:57-61, 189-200) for solves of the
+    systems held in solvers.npz."""
+    import tempfile
+    sol = np.load(os.path.join(HERE, "solvers.npz"))
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for i, (name, method, kw) in enumerate(HISTORY_RUNS):
+            kw = dict(kw)
+            kw.setdefault("convergence_residue", 1e-8)
+            kw.setdefault("max_iterations", 5000)
+            path = os.path.join(tmp, f"h{i}.txt")
+            O.solve(method, sol[f"{name}/row_ptr"], sol[f"{name}/cols"], sol[f"{name}/vals"], sol[f"{name}/b"],
+                    backend="ref", threads=1, history_file=path, **kw)
+            key = f"hist{i:02d}"
+            out[f"{key}/system"] = np.array(name)
+            out[f"{key}/method"] = np.array(method)
+            out[f"{key}/params"] = np.array(repr(sorted(kw.items())))
+            with open(path) as f:
+                out[f"{key}/text"] = np.array(f.read())
+    np.savez_compressed(os.path.join(HERE, "history.npz"), **out)
+    print("history.npz", os.path.getsize(os.path.join(HERE, "history.npz")))
+
+
 def tnlfile_cases():
     """Matrices for TNL binary file fixtures: no explicit zeros, so that every conversion keeps
     the same entries."""

@@ -124,3 +124,62 @@ def test_edge_cases_and_capacity(tnl):
     with pytest.raises(tnl.TnlHipError):
         S.submit(mid, np.ones((n, 17)))
     S.close()
+
+
+def test_gmres_requests_against_dense_solve(tnl):
+    """GMRES requests (worker track) on a nonsymmetric matrix: FP64-refined X against a dense solve."""
+    rp, c, v = W.convection_diffusion_3d(8, 10.0)
+    n = len(rp) - 1
+    rng = np.random.default_rng(2)
+    B = rng.uniform(-1, 1, (n, 4))
+    S = tnl.Server(vector_capacity=1 << 20, matrix_capacity=1 << 20, max_requests=8, gmres_workers=2)
+    mid = S.add_matrix(rp, c, v)
+    t = S.submit(mid, B, shift=0.05, method="gmres", inner_tolerance=1e-2)
+    done = S.run_all()
+    X, info = done[t]
+    print(info, flush=True)
+    assert info["converged"] and info["residue"] < 1e-10
+    assert np.all(rel_residuals(rp, c, v, 0.05, B, X) < 1.01e-10)
+    A = W.csr_to_dense(rp, c, v, n) + 0.05 * np.eye(n)
+    Xref = np.linalg.solve(A, B)
+    assert np.linalg.norm(X - Xref) / np.linalg.norm(Xref) < 1e-8
+    S.close()
+
+
+def test_gmres_same_bits_alone_and_in_the_mix(tnl):
+    """A GMRES request gives the same bits alone and among CG requests and other GMRES requests
+    running at the same time on other streams; GMRES with no workers is refused."""
+    reqs = requests(7)[:6]
+    cd = W.convection_diffusion_3d(7, 30.0)
+    rng = np.random.default_rng(9)
+    gm = [(0.1 * (i + 1), rng.uniform(-1, 1, (len(cd[0]) - 1, k))) for i, k in enumerate([1, 3, 2, 5])]
+
+    def run(with_cg, with_other_gmres, workers=3):
+        S = tnl.Server(vector_capacity=1 << 20, matrix_capacity=1 << 21, max_requests=8, gmres_workers=workers)
+        mids = {name: S.add_matrix(*sysm) for name, sysm in systems().items()}
+        mcd = S.add_matrix(*cd)
+        tickets = []
+        if with_cg:
+            for name, shift, B in reqs:
+                S.submit(mids[name], B, shift=shift)
+        for i, (shift, B) in enumerate(gm):
+            if with_other_gmres or i == 0:
+                tickets.append(S.submit(mcd, B, shift=shift, method="gmres", inner_tolerance=1e-2))
+        done = S.run_all()
+        S.close()
+        return [done[t] for t in tickets]
+
+    alone = run(False, False, workers=1)[0]
+    mixed = run(True, True)[0]
+    assert alone[1]["converged"]
+    assert np.array_equal(alone[0], mixed[0])
+    assert alone[1]["inner_iterations"] == mixed[1]["inner_iterations"]
+    rp, c, v = cd
+    for (shift, B), (X, info) in zip(gm, run(True, True)):
+        assert info["converged"]
+        assert np.all(rel_residuals(rp, c, v, shift, B, X) < 1.01e-10)
+    S = tnl.Server(vector_capacity=1 << 16, matrix_capacity=1 << 16, max_requests=2, gmres_workers=0)
+    mid = S.add_matrix(*W.laplacian_2d(10))
+    with pytest.raises(tnl.TnlHipError):
+        S.submit(mid, np.ones((100, 1)), method="gmres")
+    S.close()

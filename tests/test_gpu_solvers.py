@@ -1,11 +1,14 @@
 """Reductions, BLAS-1 and the Krylov solvers on the GPU through the C ABI, against the oracle
 (single-thread TNL semantics, pinned to TNL's own residual histories) and the mathematics."""
+import os
 import numpy as np
 import pytest
 import torch
 
 import pyoracle as O
 import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 pytestmark = pytest.mark.gpu
 
@@ -246,3 +249,52 @@ def test_symmetric_matrix_solves(tnl, dev):
     s.setup(preconditioner=2)
     with pytest.raises(tnl.TnlHipError, match="symmetric"):
         s.solve(t(b, dev, torch.float64), torch.zeros(n, dtype=torch.float64, device=dev))
+
+
+def _parse_history(text):
+    """(iteration, residue) lines and the positions of the blank lines of a residual history file."""
+    rows, blanks = [], []
+    for line in text.split("\n"):
+        if line == "":
+            blanks.append(len(rows))
+            continue
+        it, r = line.split("\t")
+        rows.append((int(it), float(r)))
+    return rows, blanks
+
+
+def test_residual_history_file_and_monitor_follow_tnl(tnl, dev, tmp_path):
+    """setup( residual-history-file ) and setSolverMonitor: the file TNL writes
+    (IterativeSolver.hpp:189-200: a blank line at iteration 0, then "<iteration>\\t<residue>"),
+    compared with TNL's own file for the same solve -- iteration numbers equal line by line over the
+    first lines, residues within the device's rounding -- and a monitor that last sees the final
+    (iterations, residue)."""
+    hist = np.load(os.path.join(ROOT, "tests", "golden", "history.npz"))
+    sol = np.load(os.path.join(ROOT, "tests", "golden", "solvers.npz"))
+    runs = sorted({k.split("/")[0] for k in hist.files})
+    for run in runs:
+        sysname, method = str(hist[f"{run}/system"]), str(hist[f"{run}/method"])
+        params = dict(eval(str(hist[f"{run}/params"])))
+        print(f"history {run} {sysname} {method} {params}", flush=True)
+        rp, c, v, b = (sol[f"{sysname}/{k}"] for k in ("row_ptr", "cols", "vals", "b"))
+        A = tnl.SparseMatrix.from_csr(rp, c, v, format="CSRScalar", dtype="f64")
+        s = tnl.getLinearSolver(method)
+        s.setMatrix(A)
+        s.setup(**params)
+        path = tmp_path / f"{run}.txt"
+        s.setResidualHistoryFile(path)
+        seen = []
+        s.setSolverMonitor(lambda it, r: seen.append((it, r)))
+        x = torch.zeros(len(b), dtype=torch.float64, device=dev)
+        s.solve(torch.tensor(b, device=dev), x)
+        s.setResidualHistoryFile(None)  # closes the file
+        ours, ours_blank = _parse_history(path.read_text())
+        ref, ref_blank = _parse_history(str(hist[f"{run}/text"]))
+        assert ours_blank[0] == 0 and ref_blank[0] == 0, run
+        m = min(len(ours), len(ref), 20)
+        assert [i for i, _ in ours[:m]] == [i for i, _ in ref[:m]], run
+        np.testing.assert_allclose([r for _, r in ours[:m]], [r for _, r in ref[:m]], rtol=2e-6, atol=1e-14,
+                                   err_msg=run)
+        assert [i for i, _ in ours] == list(s.getResidualHistoryIterations()), run
+        np.testing.assert_allclose([r for _, r in ours], s.getResidualHistory(), rtol=1e-6, err_msg=run)
+        assert seen and seen[-1] == (s.getIterations(), s.getResidue()), run

@@ -41,6 +41,7 @@ EXPORTS = [
     "tnlhip_matrix_create_symmetric_csr", "tnlhip_matrix_create_staged", "tnlhip_matrix_reduce_rows",
     "tnlhip_matrix_get_row_capacities", "tnlhip_matrix_get_compressed_row_lengths", "tnlhip_matrix_get_row_capacity",
     "tnlhip_matrix_get_row", "tnlhip_matrix_get_view", "tnlhip_matrix_values_changed",
+    "tnlhip_solver_get_history_iterations", "tnlhip_solver_set_monitor", "tnlhip_solver_set_residual_history_file",
 ]
 # element order of TNL binary files: "cuda" = column-major (TNL's GPU matrices), "host" = row-major
 FILE_LAYOUTS = {"cuda": 0, "host": 1}
@@ -69,6 +70,9 @@ class MatrixInfo(C.Structure):
     _fields_ = [("rows", C.c_int32), ("columns", C.c_int32), ("format", C.c_int32), ("value_dtype", C.c_int32),
                 ("storage_size", C.c_int64), ("allocated", C.c_int64), ("nonzeros", C.c_int64),
                 ("lanes_per_row", C.c_int32), ("symmetric", C.c_int32)]
+
+
+MONITOR_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_double)
 
 
 class MatrixView(C.Structure):
@@ -109,7 +113,7 @@ class Tuning(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("matrix_capacity", C.c_int64), ("slice_capacity", C.c_int64), ("vector_capacity", C.c_int64),
                 ("staging_capacity", C.c_int64), ("max_requests", C.c_int32), ("iterations_per_launch", C.c_int32),
-                ("use_graph", C.c_int32), ("reserved", C.c_int32)]
+                ("use_graph", C.c_int32), ("gmres_workers", C.c_int32)]
 
 
 class RequestParams(C.Structure):
@@ -170,6 +174,9 @@ def _load():
         "tnlhip_solver_solve": (C.c_int, [vp, vp, vp, C.POINTER(i32), vp]),
         "tnlhip_solver_get_status": (C.c_int, [vp, C.POINTER(i32), C.POINTER(f64)]),
         "tnlhip_solver_get_history": (i64, [vp, vp, i64]),
+        "tnlhip_solver_get_history_iterations": (i64, [vp, vp, i64]),
+        "tnlhip_solver_set_monitor": (C.c_int, [vp, MONITOR_FN, vp]),
+        "tnlhip_solver_set_residual_history_file": (C.c_int, [vp, C.c_char_p]),
         "tnlhip_refinement_params_default": (None, [C.POINTER(RefinementParams)]),
         "tnlhip_refine_solve": (C.c_int, [vp, vp, C.POINTER(RefinementParams), vp, vp,
                                           C.POINTER(RefinementResult), vp]),
@@ -639,6 +646,28 @@ class LinearSolver:
         lib.tnlhip_solver_get_history(self._h, out.ctypes.data_as(C.c_void_p), n)
         return out[:n]
 
+    def getResidualHistoryIterations(self):
+        """getIterations() at each setResidue call of the last solve."""
+        import numpy as np
+        n = lib.tnlhip_solver_get_history_iterations(self._h, None, 0)
+        out = np.zeros(max(n, 1), np.int32)
+        lib.tnlhip_solver_get_history_iterations(self._h, out.ctypes.data_as(C.c_void_p), n)
+        return out[:n]
+
+    def setSolverMonitor(self, monitor):
+        """IterativeSolver::setSolverMonitor: monitor(iterations, residue) is called each time the
+        solve reads its device state; None removes it."""
+        if monitor is None:
+            self._monitor = None
+            _check(lib.tnlhip_solver_set_monitor(self._h, MONITOR_FN(), None))
+            return
+        self._monitor = MONITOR_FN(lambda _user, it, res: monitor(int(it), float(res)))
+        _check(lib.tnlhip_solver_set_monitor(self._h, self._monitor, None))
+
+    def setResidualHistoryFile(self, path):
+        """setup( residual-history-file ): each solve appends its residues in TNL's format."""
+        _check(lib.tnlhip_solver_set_residual_history_file(self._h, None if not path else str(path).encode()))
+
 
 class CG(LinearSolver):
     method = "cg"
@@ -690,12 +719,13 @@ class Server:
     counterpart).  B and X are host float64 arrays of shape (rows, k) in row-major order."""
 
     def __init__(self, vector_capacity=None, matrix_capacity=None, slice_capacity=None, max_requests=None,
-                 iterations_per_launch=None, use_graph=True, staging_capacity=None):
+                 iterations_per_launch=None, use_graph=True, staging_capacity=None, gmres_workers=None):
         cfg = ServerConfig()
         lib.tnlhip_server_config_default(C.byref(cfg))
         for k, v in dict(vector_capacity=vector_capacity, matrix_capacity=matrix_capacity,
                          slice_capacity=slice_capacity, max_requests=max_requests,
-                         iterations_per_launch=iterations_per_launch, staging_capacity=staging_capacity).items():
+                         iterations_per_launch=iterations_per_launch, staging_capacity=staging_capacity,
+                         gmres_workers=gmres_workers).items():
             if v is not None:
                 setattr(cfg, k, int(v))
         cfg.use_graph = 1 if use_graph else 0
@@ -729,7 +759,8 @@ class Server:
         _check(lib.tnlhip_server_remove_matrix(self._h, int(mid)))
 
     def submit(self, mid, B, shift=0.0, tolerance=1e-10, inner_tolerance=3e-6, max_outer_iterations=20,
-               max_inner_iterations=5000, stagnation_limit=2):
+               max_inner_iterations=5000, stagnation_limit=2, method="cg"):
+        """Queue (A + shift I) X = B; method "cg" (batched track) or "gmres" (worker track)."""
         B = np.ascontiguousarray(B, np.float64)
         if B.ndim == 1:
             B = B[:, None]
@@ -739,6 +770,7 @@ class Server:
         p.tolerance, p.inner_tolerance = float(tolerance), float(inner_tolerance)
         p.max_outer_iterations, p.max_inner_iterations = int(max_outer_iterations), int(max_inner_iterations)
         p.stagnation_limit = int(stagnation_limit)
+        p.method = METHODS[method] if isinstance(method, str) else int(method)
         t = C.c_int64()
         _check(lib.tnlhip_server_submit(self._h, int(mid), float(shift), B.shape[1], B.ctypes.data, X.ctypes.data,
                                         C.byref(p), C.byref(t)))
@@ -747,7 +779,7 @@ class Server:
 
     def progress(self):
         """Admit, run one replay, retire; returns the finished tickets."""
-        cap = max(1, self.config.max_requests)
+        cap = max(1, self.config.max_requests) + 64 * max(1, self.config.gmres_workers)
         out = (C.c_int64 * cap)()
         n = C.c_int32()
         _check(lib.tnlhip_server_progress(self._h, out, cap, C.byref(n)))

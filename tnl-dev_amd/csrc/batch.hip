@@ -26,15 +26,25 @@
 //   correct      CORRECT  x += |r| d                           (FP64)
 //   outerResid   CORRECT  r = b - (A64 + s I) x, (r, r), (b, b) -> outer stop test (FP64)
 //   restart      RESTART  rr = r / |r|, d = 0, p = rr, (rr, rr)
+//
+// GMRES requests (nonsymmetric-capable, Solvers/Linear/GMRES.hpp:79-180) take a second track:
+// `gmres_workers` host threads, each with its own HIP stream, take them from a FIFO as they free
+// up and solve every right-hand side by FP64 refinement over an FP32 Jacobi GMRES(30) inner solve
+// (tnlhip_refine_solve) of (A + shift I), built on the device from the resident matrix.  Their
+// kernels overlap on the device with each other and with the batched CG launches; each result
+// depends only on its own request (deterministic reductions), never on the mix.
 #include "common.hpp"
 #include "matrix.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 namespace tnlhip {
@@ -654,6 +664,11 @@ struct ResidentMatrix
    int32_t sliceBase = -1, sliceCount = 0;
    int users = 0;
    bool removing = false;
+   // GMRES track: the host CSR (kept from add_matrix) and, built on the first GMRES request, the
+   // FP64 SlicedEllpack matrix each request's shifted copy is made from
+   std::shared_ptr< const std::vector< int32_t > > rowPointers, columns;
+   std::shared_ptr< const std::vector< double > > values;
+   std::shared_ptr< tnlhip_matrix_s > base;
 };
 
 struct Request
@@ -668,11 +683,34 @@ struct Request
    double submitted = 0;
 };
 
+// a + shift on every stored diagonal entry of a CSR matrix (the shifted operator of a request)
+__global__ __launch_bounds__( 256 ) void shiftDiagonalKernel( int32_t n, const int32_t* __restrict__ off,
+                                                              const int32_t* __restrict__ col, double* __restrict__ val,
+                                                              double shift )
+{
+   const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+   for( int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride )
+      for( int32_t k = off[ r ]; k < off[ r + 1 ]; k++ )
+         if( col[ k ] == r )
+            val[ k ] += shift;
+}
+
 }  // namespace
 
 }  // namespace tnlhip
 
 using namespace tnlhip;
+
+// Owning handles of the GMRES track (the C ABI's destroy as the deleter).
+struct MatrixHandle
+{
+   tnlhip_matrix m = nullptr;
+   ~MatrixHandle()
+   {
+      if( m )
+         tnlhip_matrix_destroy( m );
+   }
+};
 
 struct tnlhip_server_s
 {
@@ -706,6 +744,19 @@ struct tnlhip_server_s
    hipGraph_t graph = nullptr;
    hipGraphExec_t exec = nullptr;
    int64_t launches = 0;
+   // GMRES track
+   int device = 0;
+   std::vector< std::thread > workers;
+   std::mutex gmresMutex;
+   std::condition_variable gmresWake;
+   std::deque< Request > gmresQueue;
+   std::vector< std::pair< Request, tnlhip_request_result > > gmresDone;
+   int gmresBusy = 0;
+   bool stopping = false;
+
+   void gmresWorker();
+   tnlhip_request_result solveGmres( const Request& q, hipStream_t st, std::vector< double >& hb,
+                                     std::vector< double >& hx );
 
    Arena arena() const
    {
@@ -733,6 +784,13 @@ struct tnlhip_server_s
 
    ~tnlhip_server_s()
    {
+      {
+         std::lock_guard< std::mutex > lock( gmresMutex );
+         stopping = true;
+      }
+      gmresWake.notify_all();
+      for( std::thread& t : workers )
+         t.join();
       if( exec )
          (void) hipGraphExecDestroy( exec );
       if( graph )
@@ -746,6 +804,112 @@ struct tnlhip_server_s
          (void) hipStreamDestroy( stream );
    }
 };
+
+// One GMRES worker: its own stream, requests in FIFO order until the server stops.
+void tnlhip_server_s::gmresWorker()
+{
+   (void) hipSetDevice( device );
+   hipStream_t st = nullptr;
+   if( hipStreamCreate( &st ) != hipSuccess )
+      st = nullptr;
+   std::vector< double > hb, hx;
+   for( ;; ) {
+      Request q;
+      {
+         std::unique_lock< std::mutex > lock( gmresMutex );
+         gmresWake.wait( lock, [ & ] { return stopping || ! gmresQueue.empty(); } );
+         if( gmresQueue.empty() )
+            break;
+         q = gmresQueue.front();
+         gmresQueue.pop_front();
+         gmresBusy++;
+      }
+      tnlhip_request_result r = solveGmres( q, st, hb, hx );
+      {
+         std::lock_guard< std::mutex > lock( gmresMutex );
+         gmresDone.emplace_back( q, r );
+         gmresBusy--;
+      }
+   }
+   if( st )
+      (void) hipStreamDestroy( st );
+}
+
+// (A + shift I) x_j = b_j for every right-hand side j: FP64 refinement over an FP32 Jacobi
+// GMRES(30) inner solve; b and x are the request's interleaved host arrays (entry row * k + j).
+tnlhip_request_result tnlhip_server_s::solveGmres( const Request& q, hipStream_t st, std::vector< double >& hb,
+                                                   std::vector< double >& hx )
+{
+   tnlhip_request_result out{};
+   out.converged = 1;
+   const int status = guarded( [ & ] {
+      std::shared_ptr< tnlhip_matrix_s > base;
+      int32_t n = 0;
+      std::shared_ptr< const std::vector< int32_t > > rp, ci;
+      std::shared_ptr< const std::vector< double > > va;
+      {
+         std::lock_guard< std::mutex > lock( gmresMutex );
+         ResidentMatrix& M = matrices[ q.matrix ];
+         n = M.n;
+         rp = M.rowPointers;
+         ci = M.columns;
+         va = M.values;
+         base = M.base;
+      }
+      if( ! base ) {
+         // built outside the lock (an upload); a second worker may build it too, one copy is kept
+         tnlhip_matrix h = nullptr;
+         TNLHIP_CHECK_STATUS( tnlhip_matrix_create_csr( TNLHIP_CSR_SCALAR, TNLHIP_F64, n, n, rp->data(), ci->data(),
+                                                        va->data(), 1, &h ) );
+         std::shared_ptr< tnlhip_matrix_s > built( h, []( tnlhip_matrix m ) { tnlhip_matrix_destroy( m ); } );
+         std::lock_guard< std::mutex > lock( gmresMutex );
+         ResidentMatrix& M = matrices[ q.matrix ];
+         if( ! M.base )
+            M.base = built;
+         base = M.base;
+      }
+      // the request's operator: a device copy of A with the shift on the diagonal, and its FP32 copy
+      MatrixHandle A64, A32;
+      TNLHIP_CHECK_STATUS( tnlhip_matrix_convert( base.get(), TNLHIP_CSR_SCALAR, TNLHIP_F64, &A64.m ) );
+      Matrix& S64 = *reinterpret_cast< Matrix* >( A64.m );
+      shiftDiagonalKernel<<< gridFor( n, 256 ), 256, 0, st >>>( n, S64.offsets.ptr, S64.columns.ptr,
+                                                               reinterpret_cast< double* >( S64.values.ptr ), q.shift );
+      TNLHIP_CHECK( hipGetLastError() );
+      TNLHIP_CHECK( hipStreamSynchronize( st ) );
+      TNLHIP_CHECK_STATUS( tnlhip_matrix_convert( A64.m, TNLHIP_SLICED_ELLPACK, TNLHIP_F32, &A32.m ) );
+      tnlhip_refinement_params rp_{};
+      tnlhip_refinement_params_default( &rp_ );
+      rp_.tolerance = q.params.tolerance;
+      rp_.max_outer_iterations = q.params.max_outer_iterations;
+      rp_.stagnation_limit = q.params.stagnation_limit;
+      rp_.inner.convergence_residue = q.params.inner_tolerance;
+      rp_.inner.max_iterations = q.params.max_inner_iterations;
+      DeviceArray< double > db( n ), dx( n );
+      hb.resize( n );
+      hx.resize( n );
+      for( int j = 0; j < q.k; j++ ) {
+         for( int32_t i = 0; i < n; i++ )
+            hb[ i ] = q.b[ (int64_t) i * q.k + j ];
+         TNLHIP_CHECK( hipMemcpyAsync( db.ptr, hb.data(), sizeof( double ) * n, hipMemcpyHostToDevice, st ) );
+         TNLHIP_CHECK( hipMemsetAsync( dx.ptr, 0, sizeof( double ) * n, st ) );
+         tnlhip_refinement_result res{};
+         TNLHIP_CHECK_STATUS( tnlhip_refine_solve( A64.m, A32.m, &rp_, db.ptr, dx.ptr, &res, st ) );
+         TNLHIP_CHECK( hipMemcpyAsync( hx.data(), dx.ptr, sizeof( double ) * n, hipMemcpyDeviceToHost, st ) );
+         TNLHIP_CHECK( hipStreamSynchronize( st ) );
+         for( int32_t i = 0; i < n; i++ )
+            q.x[ (int64_t) i * q.k + j ] = hx[ i ];
+         out.converged = out.converged && res.converged;
+         out.outer_iterations = std::max( out.outer_iterations, res.outer_iterations );
+         out.inner_iterations += res.inner_iterations;
+         out.residue = std::max( out.residue, res.residue );
+      }
+   } );
+   if( status != TNLHIP_SUCCESS ) {
+      out.converged = 0;
+      out.residue = std::nan( "" );
+   }
+   return out;
+}
 
 namespace {
 
@@ -859,6 +1023,10 @@ void releaseMatrixIfUnused( tnlhip_server_s& S, int id )
       S.sliceAlloc.release( M.sliceBase, M.sliceCount );
       M.mat = -1;
       M.sliceBase = -1;
+      M.base.reset();
+      M.rowPointers.reset();
+      M.columns.reset();
+      M.values.reset();
    }
 }
 
@@ -878,6 +1046,7 @@ void tnlhip_server_config_default( tnlhip_server_config* c )
    c->iterations_per_launch = 8;
    c->use_graph = 1;
    c->staging_capacity = 32ll << 20;
+   c->gmres_workers = 4;
 }
 
 void tnlhip_request_params_default( tnlhip_request_params* p )
@@ -964,6 +1133,10 @@ int tnlhip_server_create( const tnlhip_server_config* config, tnlhip_server* out
          TNLHIP_CHECK( hipGraphInstantiate( &S->exec, S->graph, nullptr, nullptr, 0 ) );
       }
       TNLHIP_CHECK( hipStreamSynchronize( S->stream ) );
+      TNLHIP_REQUIRE( c.gmres_workers >= 0 && c.gmres_workers <= 16, "gmres_workers must be in [0, 16]" );
+      TNLHIP_CHECK( hipGetDevice( &S->device ) );
+      for( int i = 0; i < c.gmres_workers; i++ )
+         S->workers.emplace_back( [ srv = S.get() ] { srv->gmresWorker(); } );
       *out = S.release();
    } );
 }
@@ -1022,6 +1195,12 @@ int tnlhip_server_add_matrix( tnlhip_server S, int32_t rows, const int32_t* row_
       M.slots = slots;
       M.sliceBase = (int32_t) sb;
       M.sliceCount = nslices;
+      if( ! S->workers.empty() ) {
+         M.rowPointers = std::make_shared< const std::vector< int32_t > >( row_pointers, row_pointers + rows + 1 );
+         M.columns = std::make_shared< const std::vector< int32_t > >( columns, columns + nnz );
+         M.values = std::make_shared< const std::vector< double > >( values, values + nnz );
+      }
+      std::lock_guard< std::mutex > lock( S->gmresMutex );
       int id = -1;
       for( size_t i = 0; i < S->matrices.size(); i++ )
          if( S->matrices[ i ].mat < 0 && S->matrices[ i ].removing ) {
@@ -1044,6 +1223,7 @@ int tnlhip_server_remove_matrix( tnlhip_server S, int32_t id )
       TNLHIP_REQUIRE( S && id >= 0 && id < (int) S->matrices.size() && ! S->matrices[ id ].removing, "unknown matrix" );
       for( const Request& q : S->queue )
          TNLHIP_REQUIRE( q.matrix != id, "the matrix has queued requests" );
+      std::lock_guard< std::mutex > lock( S->gmresMutex );
       S->matrices[ id ].removing = true;
       releaseMatrixIfUnused( *S, id );
    } );
@@ -1061,8 +1241,10 @@ int tnlhip_server_submit( tnlhip_server S, int32_t matrix_id, double shift, int3
       tnlhip_request_params_default( &q.params );
       if( params )
          q.params = *params;
-      if( q.params.method != TNLHIP_CG )
-         throw Error( TNLHIP_ERROR_UNSUPPORTED, "the solve server runs CG requests (symmetric positive definite A + shift I)" );
+      if( q.params.method != TNLHIP_CG && q.params.method != TNLHIP_GMRES )
+         throw Error( TNLHIP_ERROR_UNSUPPORTED, "the solve server runs CG and GMRES requests" );
+      if( q.params.method == TNLHIP_GMRES && S->workers.empty() )
+         throw Error( TNLHIP_ERROR_UNSUPPORTED, "the server was created without GMRES workers (gmres_workers = 0)" );
       TNLHIP_REQUIRE( q.params.tolerance > 0 && q.params.inner_tolerance > 0 && q.params.max_outer_iterations >= 0
                          && q.params.max_inner_iterations > 0,
                       "bad request parameters" );
@@ -1071,6 +1253,17 @@ int tnlhip_server_submit( tnlhip_server S, int32_t matrix_id, double shift, int3
       q.shift = shift;
       q.b = b;
       q.x = x;
+      if( q.params.method == TNLHIP_GMRES ) {
+         q.ticket = S->nextTicket++;
+         {
+            std::lock_guard< std::mutex > lock( S->gmresMutex );
+            S->matrices[ matrix_id ].users++;
+            S->gmresQueue.push_back( q );
+         }
+         S->gmresWake.notify_one();
+         *ticket = q.ticket;
+         return;
+      }
       const int64_t E = requestEntries( *S, q );
       if( E > S->cfg.vector_capacity )
          throw Error( TNLHIP_ERROR_CAPACITY, "request larger than the server's vector arena" );
@@ -1086,6 +1279,20 @@ int tnlhip_server_progress( tnlhip_server S, int64_t* finished, int32_t capacity
       TNLHIP_REQUIRE( S && count && ( capacity == 0 || finished ), "NULL argument" );
       *count = 0;
       hipStream_t st = S->stream;
+      {
+         // finished GMRES requests
+         std::lock_guard< std::mutex > lock( S->gmresMutex );
+         for( auto& d : S->gmresDone ) {
+            d.second.launches = S->launches;
+            S->finished[ d.first.ticket ] = d.second;
+            S->matrices[ d.first.matrix ].users--;
+            releaseMatrixIfUnused( *S, d.first.matrix );
+            if( *count < capacity )
+               finished[ *count ] = d.first.ticket;
+            ( *count )++;
+         }
+         S->gmresDone.clear();
+      }
       admit( *S );
       bool any = false;
       for( int i = 0; i < S->cfg.max_requests; i++ )
@@ -1165,6 +1372,11 @@ int tnlhip_server_stats( tnlhip_server S, tnlhip_server_stats_t* stats )
             stats->resident_entries += (int64_t) S->hStatic[ i ].n * S->hStatic[ i ].k;
          }
       stats->queued_requests = (int32_t) S->queue.size();
+      {
+         std::lock_guard< std::mutex > lock( S->gmresMutex );
+         stats->queued_requests += (int32_t) S->gmresQueue.size();
+         stats->resident_requests += S->gmresBusy;
+      }
       stats->vector_free = S->vecAlloc.freeTotal();
       stats->matrix_free = S->matAlloc.freeTotal();
       stats->launches = S->launches;

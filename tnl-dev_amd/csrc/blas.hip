@@ -146,6 +146,17 @@ void launchMultidotBucket( int64_t n, int32_t c, const T* V, int64_t ld, const T
       launchMultidot< T, 32 >( n, c, V, ld, w, slot, result, st );
 }
 
+// count <= kMaxMulti columns into resultDevice with the partials in `slot` (kMaxMulti * kReduceGrid doubles)
+void multidotSlot( int dtype, int64_t n, int32_t c, const void* V, int64_t ld, const void* w, double* resultDevice,
+                   ReduceSlot slot, hipStream_t st )
+{
+   if( dtype == TNLHIP_F64 )
+      launchMultidotBucket< double >( n, c, (const double*) V, ld, (const double*) w, slot, resultDevice, st );
+   else
+      launchMultidotBucket< float >( n, c, (const float*) V, ld, (const float*) w, slot, resultDevice, st );
+   TNLHIP_CHECK( hipGetLastError() );
+}
+
 void deviceMultidot( int dtype, int64_t n, int32_t count, const void* V, int64_t ld, const void* w,
                      double* resultDevice, Workspace& ws, hipStream_t st )
 {

@@ -46,6 +46,9 @@ void deviceDot( int dtype, int64_t n, const void* a, const void* b, double* resu
 void deviceAxpby( int dtype, int64_t n, double alpha, const void* x, double beta, void* y, hipStream_t st );
 void deviceMultidot( int dtype, int64_t n, int32_t count, const void* V, int64_t ld, const void* w,
                      double* resultDevice, Workspace& ws, hipStream_t st );
+// one pass of up to kMaxMulti columns with the partials in `slot`
+void multidotSlot( int dtype, int64_t n, int32_t count, const void* V, int64_t ld, const void* w, double* resultDevice,
+                   ReduceSlot slot, hipStream_t st );
 void deviceGemv( int dtype, int64_t n, int32_t count, double alpha, const void* V, int64_t ld,
                  const double* coefDevice, double beta, void* y, hipStream_t st );
 void deviceExclusiveScan( int64_t n, const int32_t* in, int32_t* out, int32_t* totalDevice, Workspace& ws,

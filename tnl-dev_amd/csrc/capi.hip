@@ -62,10 +62,16 @@ void Matrix::uploadMetadata()
          rowToSlice.upload( meta.rowToSlice );
          slices.upload( meta.slices );
          break;
-      case KIND_BIELLPACK:
+      case KIND_BIELLPACK: {
          rowPerm.upload( meta.rowPerm );
          groupPointers.upload( meta.groupPointers );
+         // the product needs only each row's rank in its strip: one byte instead of four
+         std::vector< uint8_t > rank( meta.rowPerm.size() );
+         for( size_t r = 0; r < rank.size(); r++ )
+            rank[ r ] = (uint8_t) ( meta.rowPerm[ r ] - (int32_t) ( r / kStrip ) * kStrip );
+         biRank.upload( rank );
          break;
+      }
       default: break;
    }
    if( format == TNLHIP_CSR_ADAPTIVE ) {
@@ -808,6 +814,10 @@ int tnlhip_matrix_set_param( tnlhip_matrix A, const char* name_c, int32_t value 
          TNLHIP_REQUIRE( value == 0 || value == 1 || value == 3,
                          "xcd_remap is 0 (grid-stride), 1 (XCD-remapped blocks) or 3 (per-XCD grid-stride)" );
          A->xcdRemap = value;
+      }
+      else if( name == "stream_y" ) {
+         TNLHIP_REQUIRE( value >= -1 && value <= 1, "stream_y is -1 (kernel default), 0 or 1" );
+         A->streamY = value;
       }
       else if( name == "column_stripes" )
          buildStripePlan( *A, value, nullptr );

@@ -100,6 +100,7 @@ struct Matrix
    // device metadata
    DeviceArray< int32_t > offsets, sliceOffsets, sliceSizes, rowToChunk, rowToSlice, slices, rowPerm, groupPointers;
    DeviceArray< int32_t > adaptive;  // 4 int32 per block: type|warpIdx<<8|warpsCount<<16, first, size, segs
+   DeviceArray< uint8_t > biRank;    // BiEllpack: rowPermArray[row] - 32 * strip, one byte per row
    std::vector< int32_t > adaptiveHost;
    int32_t adaptiveBlocks = 0;
    int32_t lanes = 0;      // lanes per row for CSR light / hybrid / vector
@@ -111,6 +112,7 @@ struct Matrix
    bool trailingPadding = true;      // row distribution of the padded kernels: 0 grid-stride, 1 XCD-remapped blocks
    int32_t pairRows = 0;      // padded kernels: two adjacent rows per lane with 8 / 16-byte loads
    int32_t blocksPerCu = 16;  // workgroups per CU the grid of the padded kernels is sized for
+   int32_t streamY = -1;      // y stores: 1 non-temporal, 0 through L2, -1 the kernel's default
    bool symmetric = false;    // SymmetricMatrix: the lower triangle is stored
    DeviceArray< uint8_t > diagonal;  // cached Jacobi diagonal (vector type)
    bool diagonalValid = false;

@@ -661,12 +661,13 @@ __global__ __launch_bounds__( 256 ) void chunkedKernel( int32_t nslices, const i
 // dense height x width block (height 32 >> g) stored column-major, i.e. contiguous: lane l
 // streams elements l, l+32, ... which all belong to rank l % height, then lanes with equal
 // l % height are folded with xor-shuffles.  Lane l < height keeps rank l's running sum; at
-// the end each lane fetches the sum of its own row's rank (rowPermArray) and stores it, so
-// y is written coalesced.
+// the end each lane fetches the sum of its own row's rank and stores it, so y is written
+// coalesced.  The rank comes from a one-byte copy of rowPermArray[row] - 32 * strip (4 MB
+// instead of 16 MB at 160^3).
 constexpr int kBiUnroll = 8;
 
 template< typename V, typename A, typename Y >
-__global__ __launch_bounds__( 256 ) void biellpackKernel( int32_t n, int32_t strips, const int32_t* __restrict__ rowPerm,
+__global__ __launch_bounds__( 256 ) void biellpackKernel( int32_t n, int32_t strips, const uint8_t* __restrict__ rowRank,
                                                           const int32_t* __restrict__ groupPointers,
                                                           const int32_t* __restrict__ col,
                                                           const V* __restrict__ val, const Y* __restrict__ x,
@@ -690,7 +691,7 @@ __global__ __launch_bounds__( 256 ) void biellpackKernel( int32_t n, int32_t str
       const bool valid = strip < strips;
       const int64_t row = (int64_t) strip * kStrip + l;
       const bool mine = valid && row < n;
-      const int rank = mine ? rowPerm[ row ] - strip * kStrip : 0;
+      const int rank = mine ? rowRank[ row ] : 0;
       int32_t gpNext[ kLogStrip + 1 ];
       fetch( 2 * ( pair + waves ) + half, gpNext );
       A acc = 0;
@@ -774,6 +775,8 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
       csrScalarStagedKernel< K, V, A, Y >
          <<< grid, 256, 0, st >>>( M.offsets.ptr, col, val, x, keep, first, last, M.xcdRemap ? 1 : 0 );
    };
+   if( M.streamY >= 0 )
+      keep.streamY = M.streamY != 0;
    if( M.cols == 0 ) {
       // every stored slot is padding and x is empty: the kernels below gather x[0] for padding
       // slots, so a matrix without columns takes the all-zero product without touching x
@@ -795,6 +798,8 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
       case TNLHIP_CSR_ADAPTIVE: {
          if( M.format == TNLHIP_CSR_ADAPTIVE && first == 0 && last == M.rows && M.adaptiveBlocks > 0 ) {
             constexpr int STREAM = kAdaptiveStreamBytesPerWave / sizeof( A );
+            // STREAM blocks end rows anywhere in a cache line of y
+            keep.streamY = M.streamY == 1;
             unsigned groups = ( M.adaptiveBlocks + 3 ) / 4;
             if( M.xcdRemap )
                groups = ( groups + kXcds - 1 ) / kXcds * kXcds;  // logicalBlock needs a multiple of 8
@@ -885,7 +890,7 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
             // 32 workgroups per CU (two strip pairs per wave at 160^3): 79.8 us, against 83.0 us
             // at 8 and 84.3 us with one workgroup per 4 pairs
             biellpackKernel< V, A, Y ><<< gridFor( pairs, 4, 32 ), 256, 0, st >>>(
-               M.rows, strips, M.rowPerm.ptr, M.groupPointers.ptr, col, val, x, keep, first, last );
+               M.rows, strips, M.biRank.ptr, M.groupPointers.ptr, col, val, x, keep, first, last );
          else
             // no stored element at all (TNL's Host path returns without touching y here,
             // BiEllpackView.hpp:248-249); every row of A x is zero

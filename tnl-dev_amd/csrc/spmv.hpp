@@ -19,6 +19,9 @@ struct Keeper
    int mode;
    const int32_t* gate;  // solver flags: the launch is skipped when gate[0] != 0 (and, with
    int gateMode;         // gateMode 2, unless gate[1] != 0)
+   // y stored with non-temporal stores (the default: y is written once, L2 stays for x); kernels
+   // whose waves end rows inside a cache line store through L2, which merges the partial lines
+   bool streamY = true;
    __device__ __forceinline__ bool closed() const
    {
       if( gateMode == 0 )
@@ -37,7 +40,10 @@ struct Keeper
          case 3: out = beta * (A) y[ row ] + alpha * v; break;
          default: out = v / (A) diag[ row ]; break;
       }
-      __builtin_nontemporal_store( (Y) out, y + row );  // y is written once: keep L2 for x
+      if( streamY )
+         __builtin_nontemporal_store( (Y) out, y + row );  // y is written once: keep L2 for x
+      else
+         y[ row ] = (Y) out;
    }
 };
 
