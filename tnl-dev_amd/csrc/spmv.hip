@@ -10,6 +10,7 @@
 // order and round each product and sum separately, so they reproduce the Host result bit
 // for bit; the cooperative kernels reassociate the row sum.
 #include "spmv.hpp"
+#include "rowdist.hpp"
 
 namespace tnlhip {
 
@@ -32,46 +33,15 @@ __device__ __forceinline__ bf16 ldnt( const bf16* p )
    return __ushort_as_bfloat16( __builtin_nontemporal_load( reinterpret_cast< const unsigned short* >( p ) ) );
 }
 
-// Workgroup -> row block mapping.  Workgroups are dispatched round-robin over the 8 XCDs;
-// with `remap` the logical index puts consecutive logical blocks on the same XCD, so each
-// XCD streams one contiguous eighth of the rows and x's neighbourhoods stay in its own L2.
-constexpr int kXcds = 8;
 __device__ __forceinline__ int64_t logicalBlock( int remap )
 {
-   if( ! remap )
-      return blockIdx.x;
-   const unsigned per = gridDim.x / kXcds;
-   return (int64_t) ( blockIdx.x % kXcds ) * per + blockIdx.x / kXcds;
+   return logicalBlockOf( blockIdx.x, gridDim.x, remap );
 }
 
-// Row sequence of a lane taking P consecutive rows at a time.  dist 0: grid-stride over the
-// whole range; 1: each workgroup owns `per` consecutive rows, logical blocks remapped so an
-// XCD owns a contiguous eighth; 3: each XCD grid-strides over its own contiguous eighth.
-struct RowSeq
-{
-   int64_t start, end, step;
-};
 template< int P >
 __device__ __forceinline__ RowSeq rowSequence( int64_t base, int64_t last, int64_t per, int dist )
 {
-   if( dist == 0 )
-      return RowSeq{ base + ( (int64_t) blockIdx.x * blockDim.x + threadIdx.x ) * P, last,
-                     (int64_t) gridDim.x * blockDim.x * P };
-   if( dist == 3 ) {
-      // workgroups are dispatched to the XCDs round-robin: XCD x (blockIdx % 8) sweeps the
-      // x-th eighth of the rows with a grid-stride loop of its own, so the x entries a row
-      // band gathers stay in that XCD's L2 while the sweep passes
-      const int64_t perXcd = gridDim.x / kXcds;
-      const int64_t pos = blockIdx.x / kXcds;
-      if( pos >= perXcd )
-         return RowSeq{ last, last, 1 };
-      const int64_t chunk = ( ( last - base + kXcds - 1 ) / kXcds + P - 1 ) / P * P;
-      const int64_t r0 = base + (int64_t) ( blockIdx.x % kXcds ) * chunk;
-      return RowSeq{ r0 + ( pos * blockDim.x + threadIdx.x ) * P, min( r0 + chunk, last ),
-                     perXcd * blockDim.x * P };
-   }
-   const int64_t r0 = base + logicalBlock( 1 ) * per;
-   return RowSeq{ r0 + (int64_t) threadIdx.x * P, min( r0 + per, last ), (int64_t) blockDim.x * P };
+   return rowSequenceAt< P >( base, last, per, dist, LaunchIndex{ blockIdx.x, gridDim.x, blockDim.x, threadIdx.x } );
 }
 
 // ------------------------------------------------------------------ CSR: G lanes per row
@@ -847,12 +817,11 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
       case TNLHIP_SLICED_ELLPACK: {
          // grid a multiple of the XCD count; contiguous row blocks when dist != 0
          const bool pair = M.pairRows != 0;
-         const int rowsPerThread = pair ? 2 : 1;
-         unsigned grid = gridFor( rows, 256 * rowsPerThread, M.blocksPerCu );
-         grid = ( grid + kXcds - 1 ) / kXcds * kXcds;
-         const int64_t span = last - ( pair ? ( first & ~1 ) : first );
-         const int64_t unit = 256 * rowsPerThread;
-         const int64_t per = ( ( span + grid - 1 ) / grid + unit - 1 ) / unit * unit;
+         if( ! knownDistribution( M.xcdRemap ) )
+            throw Error( TNLHIP_ERROR_INVALID_ARGUMENT, "unknown row distribution" );
+         const PaddedLaunch pl = paddedLaunch( first, last, pair, M.blocksPerCu, deviceProps().cus );
+         const unsigned grid = (unsigned) pl.grid;
+         const int64_t per = pl.per;
          const EllpackSlots es{ M.meta.segmentSize, M.meta.alignedSize };
          const SlicedSlots ss{ M.sliceOffsets.ptr, M.sliceSizes.ptr };
          if( M.format == TNLHIP_ELLPACK ) {
