@@ -16,6 +16,8 @@ Outputs (numpy .npz, data only):
   solvers.npz   residual histories, iteration counts, convergence flags and solutions of
                 CG / BiCGStab / GMRES (all variants, with and without Jacobi) at one thread, and
                 the iteration counts of the same runs at 2, 3, 4 and 8 threads
+  history.npz   TNL's residual-history-file text for some of those solves
+  SHA256SUMS    checksums of every .npz above (tests/conftest.py checks them at session start)
 """
 import os
 import sys
@@ -266,6 +268,17 @@ def write_history_fixtures():
     print("history.npz", os.path.getsize(os.path.join(HERE, "history.npz")))
 
 
+def write_checksums():
+    """SHA256SUMS in sha256sum's format, one line per fixture file."""
+    import hashlib
+    lines = []
+    for name in sorted(f for f in os.listdir(HERE) if f.endswith(".npz")):
+        with open(os.path.join(HERE, name), "rb") as f:
+            lines.append(f"{hashlib.sha256(f.read()).hexdigest()}  {name}\n")
+    with open(os.path.join(HERE, "SHA256SUMS"), "w") as f:
+        f.writelines(lines)
+
+
 def tnlfile_cases():
     """Matrices for TNL binary file fixtures: no explicit zeros, so that every conversion keeps
     the same entries."""
@@ -331,3 +344,4 @@ if __name__ == "__main__":
         main()
         write_mtx_fixtures()
         write_tnlfile_fixtures()
+    write_checksums()
