@@ -212,9 +212,9 @@ def timed(torch, fn):
 
 def solver_suite(tnl, torch, W, dev, args):
     """Convection-diffusion 7-point upwind (v = (1, 0.5, 0.25)) on a cd_grid^3 box, Jacobi
-    GMRES(30) to 1e-10 from x0 = 0: plain FP64 against FP64 refinement over an FP32 and a BF16
-    inner GMRES(30) (inner tolerance 1e-2), for each Peclet number and orthogonalization
-    variant.  The matrix format of each precision is the autotuner's choice (tuning time is
+    GMRES(30) to 1e-10 from x0 = 0: plain FP64 against FP64 refinement over an FP32 inner
+    GMRES(30) (inner tolerance 1e-2) with Jacobi, and with the Jacobi polynomial over BF16 and
+    over FP32 values, for each Peclet number and orthogonalization variant.  The matrix format of each precision is the autotuner's choice (tuning time is
     reported, not timed)."""
     N = args.cd_grid
     variants = args.gmres_variants.split(",")
@@ -245,20 +245,32 @@ def solver_suite(tnl, torch, W, dev, args):
             log(f"Pe={pe:g} {variant} FP64 GMRES ({best64['format']}): {s.getIterations()} iterations, "
                 f"{t64:.3f} s, converged={ok}")
             x64 = x.clone()
-            for dt in ("f32", "bf16"):
+            # inner solves: FP32 Jacobi GMRES(30); the same with the Jacobi polynomial over the
+            # BF16 copy of A (BF16 values carry the preconditioner's products, the Krylov operator
+            # stays FP32) and over the FP32 copy; optionally the inner operator itself in BF16
+            runs = [("f32", "f32", None, {"preconditioner": 1}),
+                    ("bf16poly", "f32", "bf16", {"preconditioner": 3, "poly_degree": args.poly_degree}),
+                    ("f32poly", "f32", "f32", {"preconditioner": 3, "poly_degree": args.poly_degree})]
+            if args.bf16_operator:
+                runs.append(("bf16", "bf16", None, {"preconditioner": 1}))
+            for name, dt, pdt, pre in runs:
                 A_low = lows[dt][0]
+                P = lows[pdt][0] if pdt else None
                 x = torch.zeros(n, dtype=torch.float64, device=dev)
-                r, t = timed(torch, lambda: tnl.refine_solve(A, b, x, A_low=A_low, tolerance=1e-10,
+                r, t = timed(torch, lambda: tnl.refine_solve(A, b, x, A_low=A_low, P=P, tolerance=1e-10,
                                                              inner_method="gmres", inner_dtype=dt,
                                                              convergence_residue=args.inner_tol, max_iterations=3000,
                                                              restarting_min=30, restarting_max=30,
-                                                             gmres_variant=variant, preconditioner=1))
+                                                             gmres_variant=variant, **pre))
                 err = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))
-                ve[dt + "_refined"] = {"converged": r["converged"], "outer": r["outer_iterations"],
-                                       "inner_iterations": r["inner_iterations"], "seconds": round(t, 4),
-                                       "residue": r["residue"], "speedup_vs_fp64": round(t64 / t, 3),
-                                       "rel_error_vs_fp64_solve": err}
-                log(f"Pe={pe:g} {variant} {dt} refined ({lows[dt][1]['format']}): outer {r['outer_iterations']} "
+                ve[name + "_refined"] = {"converged": r["converged"], "outer": r["outer_iterations"],
+                                         "inner_iterations": r["inner_iterations"], "seconds": round(t, 4),
+                                         "residue": r["residue"], "speedup_vs_fp64": round(t64 / t, 3),
+                                         "rel_error_vs_fp64_solve": err, "inner_operator": dt,
+                                         "preconditioner": "jacobi" if pdt is None else
+                                         f"jacobi polynomial degree {args.poly_degree} over {pdt} values"}
+                fmt = lows[dt][1]["format"] + (f" + {pdt} P {lows[pdt][1]['format']}" if pdt else "")
+                log(f"Pe={pe:g} {variant} {name} refined ({fmt}): outer {r['outer_iterations']} "
                     f"inner {r['inner_iterations']} {t:.3f} s speedup {t64 / t:.2f}x residue {r['residue']:.2e} "
                     f"err vs FP64 {err:.2e} converged={r['converged']}")
             entry[variant] = ve
@@ -522,6 +534,9 @@ def main():
     ap.add_argument("--gmres-variants", default="CWY,CGS")
     ap.add_argument("--inner-tol", type=float, default=1e-2, help="inner relative residual of the refinement")
     ap.add_argument("--gmres-dtype", default="f64")
+    ap.add_argument("--poly-degree", type=int, default=4, help="Jacobi-polynomial degree of the poly-refined runs")
+    ap.add_argument("--bf16-operator", action="store_true",
+                    help="also refine over a BF16 inner operator (converges only when cond(A) * 2^-8 < 1)")
     ap.add_argument("--serve-inner-tol", type=float, default=3e-6)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-irregular", action="store_true")

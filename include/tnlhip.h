@@ -373,11 +373,17 @@ typedef struct
    int32_t restarting_step_min;
    int32_t restarting_step_max;
    int32_t bicgstab_exact_residue;
-   int32_t preconditioner; /* 0 none, 1 Preconditioners::Diagonal (Jacobi), 2 Preconditioners::ILU0 */
+   int32_t preconditioner; /* 0 none, 1 Preconditioners::Diagonal (Jacobi), 2 Preconditioners::ILU0,
+                              3 Jacobi polynomial (no TNL counterpart; see poly_degree) */
    int32_t use_graph;      /* 1: CG / BiCGStab iterations are captured once into a hipGraph and
                               replayed (16 per replay); 0: launched one by one */
    int32_t bicgstab_ell;   /* BiCGStab(ell): Bi-CG steps per minimal-residual update, 1..16
                               ("bicgstab-ell", default 1) */
+   int32_t poly_degree;    /* preconditioner 3: z = p(D^-1 P) D^-1 s, the Neumann series of P's
+                              Jacobi splitting cut after poly_degree products with P (z_0 = D^-1 s,
+                              z_{j+1} = z_j + D^-1 (s - P z_j)), D the diagonal of the solved matrix,
+                              P its preconditioner matrix (tnlhip_solver_set_preconditioner_matrix;
+                              the solved matrix itself by default).  0..32, default 4. */
 } tnlhip_solver_params;
 
 void tnlhip_solver_params_default( tnlhip_solver_params* params );
@@ -399,6 +405,14 @@ int tnlhip_solver_set_matrix( tnlhip_solver solver, tnlhip_matrix A );
  * diagonal before the others and an entry on or above the diagonal (TNLHIP_ERROR_UNSUPPORTED /
  * INVALID_ARGUMENT otherwise). */
 int tnlhip_solver_set_params( tnlhip_solver solver, const tnlhip_solver_params* params );
+
+/* The matrix P of the Jacobi-polynomial preconditioner (params->preconditioner == 3); NULL: the
+ * solved matrix.  No counterpart in the reference.  P has the solved matrix's dimensions and
+ * vector type (FP64 values with FP64 vectors; FP32 or BF16 values with FP32 vectors), is general
+ * (not symmetric or binary) and must outlive the solver's use of it.  A BF16 copy of an FP32
+ * system makes the preconditioner's products, which carry most of the bytes, read 2-byte values
+ * while the Krylov recurrence runs on the FP32 operator. */
+int tnlhip_solver_set_preconditioner_matrix( tnlhip_solver solver, tnlhip_matrix P );
 
 /* bool LinearSolver::solve( b, x ), x holds the initial guess on entry; *converged is
  * checkConvergence() (IterativeSolver.hpp:127-150).  Synchronous. */
@@ -447,6 +461,13 @@ typedef struct
                                     do not halve the residual; 0 = never.  Refinement converges only
                                     when cond(A) times the unit roundoff of A_low is below 1. */
    tnlhip_solver_params inner;   /* inner stopping rule, GMRES variant / restart, preconditioner */
+   int32_t poly_value_dtype;     /* inner.preconditioner == 3: value type of the preconditioner
+                                    matrix P, a copy of A in A_low's format (TNLHIP_BF16, the default,
+                                    or TNLHIP_F32, which reuses A_low when that is FP32).  With FP32
+                                    A_low and BF16 P the inner Krylov space is built on the FP32
+                                    operator, so the BF16 rounding of A limits only how well P
+                                    preconditions, not the accuracy refinement reaches. */
+   tnlhip_matrix poly_matrix;    /* that P, already built (NULL: converted from A on each call) */
 } tnlhip_refinement_params;
 
 typedef struct

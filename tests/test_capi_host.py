@@ -77,3 +77,37 @@ def test_adaptive_blocks_follow_the_tnl_split(tnl):
     want = O.csr_adaptive_blocks(rp, 8, warp_size=64, block_size=256, streamed_shared_bytes=4 * 4096,
                                  max_adaptive_elements=1024)
     np.testing.assert_array_equal(got, want)
+
+
+def test_solver_argument_checks(tnl):
+    """tnlhip_solver_set_params refuses what would mis-size the solver's arrays or name an unknown
+    preconditioner, and the history getters refuse a negative capacity; no GPU is touched."""
+    import ctypes as C
+    lib = tnl.lib
+    h = C.c_void_p()
+    assert lib.tnlhip_solver_create(tnl.METHODS["gmres"], C.byref(h)) == 0
+    try:
+        def status(**kw):
+            p = tnl.SolverParams()
+            lib.tnlhip_solver_params_default(C.byref(p))
+            for k, v in kw.items():
+                setattr(p, k, v)
+            return lib.tnlhip_solver_set_params(h, C.byref(p))
+        assert status() == 0
+        assert status(preconditioner=3, poly_degree=0) == 0
+        assert status(preconditioner=3, poly_degree=32) == 0
+        assert status(preconditioner=4) == 1
+        assert status(preconditioner=3, poly_degree=33) == 1
+        assert status(preconditioner=3, poly_degree=-1) == 1
+        assert status(restarting_min=20, restarting_max=10) == 1
+        assert status(restarting_min=0, restarting_max=10) == 1
+        assert status(restarting_min=10, restarting_max=32) == 1
+        assert status(restarting_step_min=4, restarting_step_max=3) == 1
+        assert status(bicgstab_ell=0) == 1
+        buf = (C.c_double * 4)()
+        assert lib.tnlhip_solver_get_history(h, C.cast(buf, C.c_void_p), -1) == -1
+        ibuf = (C.c_int32 * 4)()
+        assert lib.tnlhip_solver_get_history_iterations(h, C.cast(ibuf, C.c_void_p), -1) == -1
+        assert lib.tnlhip_solver_get_history(h, None, 0) == 0
+    finally:
+        lib.tnlhip_solver_destroy(h)

@@ -42,6 +42,7 @@ EXPORTS = [
     "tnlhip_matrix_get_row_capacities", "tnlhip_matrix_get_compressed_row_lengths", "tnlhip_matrix_get_row_capacity",
     "tnlhip_matrix_get_row", "tnlhip_matrix_get_view", "tnlhip_matrix_values_changed",
     "tnlhip_solver_get_history_iterations", "tnlhip_solver_set_monitor", "tnlhip_solver_set_residual_history_file",
+    "tnlhip_solver_set_preconditioner_matrix",
 ]
 # element order of TNL binary files: "cuda" = column-major (TNL's GPU matrices), "host" = row-major
 FILE_LAYOUTS = {"cuda": 0, "host": 1}
@@ -89,7 +90,8 @@ class SolverParams(C.Structure):
                 ("divergence_residue", C.c_double), ("gmres_variant", C.c_int32), ("restarting_min", C.c_int32),
                 ("restarting_max", C.c_int32), ("restarting_step_min", C.c_int32),
                 ("restarting_step_max", C.c_int32), ("bicgstab_exact_residue", C.c_int32),
-                ("preconditioner", C.c_int32), ("use_graph", C.c_int32), ("bicgstab_ell", C.c_int32)]
+                ("preconditioner", C.c_int32), ("use_graph", C.c_int32), ("bicgstab_ell", C.c_int32),
+                ("poly_degree", C.c_int32)]
 
 
 REFINEMENT_HISTORY = 64
@@ -97,7 +99,8 @@ REFINEMENT_HISTORY = 64
 
 class RefinementParams(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("max_outer_iterations", C.c_int32), ("inner_method", C.c_int32),
-                ("inner_value_dtype", C.c_int32), ("stagnation_limit", C.c_int32), ("inner", SolverParams)]
+                ("inner_value_dtype", C.c_int32), ("stagnation_limit", C.c_int32), ("inner", SolverParams),
+                ("poly_value_dtype", C.c_int32), ("poly_matrix", C.c_void_p)]
 
 
 class RefinementResult(C.Structure):
@@ -170,6 +173,7 @@ def _load():
         "tnlhip_solver_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
         "tnlhip_solver_destroy": (C.c_int, [vp]),
         "tnlhip_solver_set_matrix": (C.c_int, [vp, vp]),
+        "tnlhip_solver_set_preconditioner_matrix": (C.c_int, [vp, vp]),
         "tnlhip_solver_set_params": (C.c_int, [vp, C.POINTER(SolverParams)]),
         "tnlhip_solver_solve": (C.c_int, [vp, vp, vp, C.POINTER(i32), vp]),
         "tnlhip_solver_get_status": (C.c_int, [vp, C.POINTER(i32), C.POINTER(f64)]),
@@ -570,6 +574,13 @@ class ILU0:
     kind = 2
 
 
+class JacobiPolynomial:
+    """Jacobi-polynomial preconditioner (no TNL counterpart): z = sum_{j<=k} (I - D^-1 P)^j D^-1 s,
+    k = setup(poly_degree=...), P the solver's preconditioner matrix (setPreconditionerMatrix; the
+    solved matrix by default)."""
+    kind = 3
+
+
 class LinearSolver:
     """LinearSolver< Matrix > (Solvers/Linear/LinearSolver.h:115-175) with IterativeSolver's setters."""
     method = None
@@ -597,6 +608,11 @@ class LinearSolver:
     def setPreconditioner(self, preconditioner):
         self.params.preconditioner = 0 if preconditioner is None else getattr(preconditioner, "kind", 1)
 
+    def setPreconditionerMatrix(self, matrix):
+        """P of the Jacobi-polynomial preconditioner (None: the solved matrix); kept alive here."""
+        self._poly_matrix = matrix
+        _check(lib.tnlhip_solver_set_preconditioner_matrix(self._h, None if matrix is None else matrix._h))
+
     def setMaxIterations(self, n):
         self.params.max_iterations = int(n)
 
@@ -616,7 +632,7 @@ class LinearSolver:
             if k == "gmres_variant" and isinstance(v, str):
                 v = GMRES_VARIANTS[v]
             if k == "preconditioner" and isinstance(v, str):
-                v = {"none": 0, "diagonal": 1, "ilu0": 2}[v]
+                v = {"none": 0, "diagonal": 1, "ilu0": 2, "jacobi-polynomial": 3}[v]
             setattr(self.params, k, v)
         return True
 
@@ -691,7 +707,7 @@ class BICGStabL(LinearSolver):
 
 
 def refine_solve(A, b, x, A_low=None, tolerance=1e-10, inner_method="gmres", inner_dtype="f32",
-                 max_outer_iterations=50, stagnation_limit=2, stream=None, **inner):
+                 max_outer_iterations=50, stagnation_limit=2, stream=None, poly_dtype=None, P=None, **inner):
     """FP64 iterative refinement around a low-precision inner solve (no TNL counterpart).
     Returns a dict with converged, outer_iterations, inner_iterations, residue, history."""
     p = RefinementParams()
@@ -701,6 +717,10 @@ def refine_solve(A, b, x, A_low=None, tolerance=1e-10, inner_method="gmres", inn
     p.stagnation_limit = int(stagnation_limit)
     p.inner_method = METHODS[inner_method] if isinstance(inner_method, str) else inner_method
     p.inner_value_dtype = _dtype_id(inner_dtype)
+    if poly_dtype is not None:
+        p.poly_value_dtype = _dtype_id(poly_dtype)
+    if P is not None:
+        p.poly_matrix = P._h
     for k, v in inner.items():
         if k == "gmres_variant" and isinstance(v, str):
             v = GMRES_VARIANTS[v]

@@ -1375,7 +1375,9 @@ int tnlhip_server_stats( tnlhip_server S, tnlhip_server_stats_t* stats )
       {
          std::lock_guard< std::mutex > lock( S->gmresMutex );
          stats->queued_requests += (int32_t) S->gmresQueue.size();
-         stats->resident_requests += S->gmresBusy;
+         // on a worker, or finished there and not yet reported by tnlhip_server_progress: a
+         // caller that stops once nothing is resident must not lose the latter
+         stats->resident_requests += S->gmresBusy + (int32_t) S->gmresDone.size();
       }
       stats->vector_free = S->vecAlloc.freeTotal();
       stats->matrix_free = S->matAlloc.freeTotal();

@@ -128,6 +128,10 @@ void spmvLaunch( const Matrix& A, const void* x, void* y, double alpha, double b
 // with gateMode 2, unless gate[1] != 0.  last < 0 means all rows.
 void spmvLaunchGated( const Matrix& A, const void* x, void* y, double alpha, double beta, const void* diag,
                       const int32_t* gate, int gateMode, hipStream_t stream, int32_t first = 0, int32_t last = -1 );
+// One Jacobi step of the splitting of P: znew = z + ( rhs - P z ) / diag over all rows, gated
+// as above (general real-valued P only; znew aliases neither z nor rhs)
+void spmvJacobiStepGated( const Matrix& P, const void* z, void* znew, const void* rhs, const void* diag,
+                          const int32_t* gate, int gateMode, hipStream_t stream );
 // SymmetricMatrix product (symmetric.hip), used by spmvLaunchGated for symmetric matrices
 void spmvSymmetric( const Matrix& A, const void* x, void* y, double alpha, double beta, int32_t first, int32_t last,
                     const void* diag, Gate gate, hipStream_t stream );

@@ -5,6 +5,8 @@
 //      rhat = (float)( r / ||r|| )       (scaled so FP32 / BF16 keep its significant digits)
 //      solve A_low d = rhat              (inner CG / BiCGStab / GMRES, FP32 vectors, values FP32
 //                                         or BF16, to inner_tolerance, from d = 0)
+//                                        (with the Jacobi-polynomial preconditioner, its products
+//                                         run on a further copy P of A, BF16 by default)
 //      x += ||r|| * (double) d           (FP64)
 //      r = b - A64 x                     (FP64)
 //
@@ -80,6 +82,8 @@ void tnlhip_refinement_params_default( tnlhip_refinement_params* p )
    p->inner.restarting_min = 30;
    p->inner.restarting_max = 30;
    p->inner.preconditioner = 1;
+   p->poly_value_dtype = TNLHIP_BF16;
+   p->poly_matrix = nullptr;
 }
 
 int tnlhip_refine_solve( tnlhip_matrix A, tnlhip_matrix A_low, const tnlhip_refinement_params* p, const double* b,
@@ -109,14 +113,17 @@ int tnlhip_refine_solve( tnlhip_matrix A, tnlhip_matrix A_low, const tnlhip_refi
       {
          tnlhip_matrix m;
          tnlhip_solver s;
+         tnlhip_matrix poly;
          ~Guard()
          {
-            if( m )
-               tnlhip_matrix_destroy( m );
             if( s )
                tnlhip_solver_destroy( s );
+            if( poly )
+               tnlhip_matrix_destroy( poly );
+            if( m )
+               tnlhip_matrix_destroy( m );
          }
-      } guard{ owned, nullptr };
+      } guard{ owned, nullptr, nullptr };
       TNLHIP_REQUIRE( reinterpret_cast< Matrix* >( low )->dtype != TNLHIP_F64, "the inner matrix must be FP32 or BF16" );
 
       DeviceArray< double > r( n ), scal( 4 );
@@ -128,6 +135,21 @@ int tnlhip_refine_solve( tnlhip_matrix A, tnlhip_matrix A_low, const tnlhip_refi
          throw Error( s, tnlhip_last_error() );
       TNLHIP_CHECK( hipStreamSynchronize( st ) );
       tnlhip_solver_set_matrix( guard.s, low );
+      if( p->inner.preconditioner == 3 ) {
+         // the polynomial's products on a copy of A with poly_value_dtype values (A_low itself
+         // when it already holds them)
+         const Matrix& L = *reinterpret_cast< Matrix* >( low );
+         TNLHIP_REQUIRE( p->poly_value_dtype == TNLHIP_F32 || p->poly_value_dtype == TNLHIP_BF16,
+                         "poly_value_dtype is F32 or BF16" );
+         tnlhip_matrix P = p->poly_matrix ? p->poly_matrix : low;
+         if( ! p->poly_matrix && L.dtype != p->poly_value_dtype ) {
+            s = tnlhip_matrix_convert( A, L.format, p->poly_value_dtype, &guard.poly );
+            if( s != TNLHIP_SUCCESS )
+               throw Error( s, std::string( "conversion of the preconditioner matrix failed: " ) + tnlhip_last_error() );
+            P = guard.poly;
+         }
+         tnlhip_solver_set_preconditioner_matrix( guard.s, P );
+      }
       tnlhip_solver_params inner = p->inner;
       tnlhip_solver_set_params( guard.s, &inner );
 

@@ -257,6 +257,10 @@ struct SolverImpl
    DeviceArray< uint8_t > diag;
    bool diagValid = false;
    std::unique_ptr< Ilu0 > ilu;  // preconditioner 2, built on first use
+   // preconditioner 3 (Jacobi polynomial): the matrix P of its products (nullptr: A) and two
+   // work vectors (the input s, and the second buffer of the ping-pong between steps)
+   Matrix* polyMatrix = nullptr;
+   DeviceArray< uint8_t > polyVecs;
    DeviceArray< State > state;
    DeviceArray< double > hist;
    int64_t histCap = 0;
@@ -300,7 +304,7 @@ void ensureDiagonal( SolverImpl& S )
       f->build( *S.A, S.vdtype == TNLHIP_F64 ? TNLHIP_F64 : TNLHIP_F32, S.stream );
       S.ilu = std::move( f );
    }
-   if( S.params.preconditioner != 1 || S.diagValid )
+   if( ( S.params.preconditioner != 1 && S.params.preconditioner != 3 ) || S.diagValid )
       return;
    S.diag.resize( S.n * (int64_t) S.vbytes );
    extractDiagonal( *S.A, S.diag.ptr, S.stream );
@@ -373,13 +377,55 @@ void resetState( SolverImpl& S )
    }
 }
 
+// Jacobi-polynomial preconditioner (no TNL counterpart): out = z_k with z_0 = D^-1 s and
+// z_{j+1} = z_j + D^-1 ( s - P z_j ), i.e. the Neumann series sum_{j<=k} (I - D^-1 P)^j D^-1 s of
+// the Jacobi splitting of P, cut after k = poly_degree products.  A fixed linear operator (the
+// same polynomial on every call), so it preconditions GMRES, BiCGStab and TFQMR as Jacobi does
+// and, for symmetric P, CG.  Each step is one product whose keeper does the update; the steps
+// alternate between `out` and a work vector so that the last one lands in `out`.
+template< typename T >
+void polyApplyTyped( SolverImpl& S, const void* in, void* out, Gate gate )
+{
+   const Matrix& P = S.polyMatrix ? *S.polyMatrix : *S.A;
+   T* sv = (T*) S.polyVecs.ptr;
+   T* tv = sv + S.ld;
+   const T* src = (const T*) in;
+   if( in == out ) {
+      launchElementwise( S.n, OpCopy< T >{ {}, sv, src }, gate, S.stream );
+      src = sv;
+   }
+   const int k = S.params.poly_degree;
+   auto buf = [ & ]( int j ) { return ( ( k - j ) % 2 == 0 ) ? (T*) out : tv; };
+   launchElementwise( S.n, OpDivide< T >{ {}, buf( 0 ), src, (const T*) S.diag.ptr }, gate, S.stream );
+   for( int j = 1; j <= k; j++ )
+      spmvJacobiStepGated( P, buf( j - 1 ), buf( j ), src, S.diag.ptr, gate.flags, gate.mode, S.stream );
+}
+
+// out = M^-1 in for the preconditioners applied as a separate solve (ILU(0), Jacobi polynomial);
+// in == out allowed
+void precSolve( SolverImpl& S, const void* in, void* out, Gate gate )
+{
+   if( S.params.preconditioner == 2 )
+      S.ilu->solve( in, out, gate, S.stream );
+   else if( S.vdtype == TNLHIP_F64 )
+      polyApplyTyped< double >( S, in, out, gate );
+   else
+      polyApplyTyped< float >( S, in, out, gate );
+}
+
 // Precondition-aware SpMV: y = M^{-1} A x or A x, gated.
 void matvec( SolverImpl& S, const void* x, void* y, Gate gate )
 {
+   if( S.params.preconditioner == 3 ) {
+      // A x into the polynomial's input vector, then the polynomial into y (no copy)
+      spmvLaunchGated( *S.A, x, S.polyVecs.ptr, 1.0, 0.0, nullptr, gate.flags, gate.mode, S.stream );
+      precSolve( S, S.polyVecs.ptr, y, gate );
+      return;
+   }
    spmvLaunchGated( *S.A, x, y, 1.0, 0.0, S.params.preconditioner == 1 ? S.diag.ptr : nullptr, gate.flags, gate.mode,
                     S.stream );
    if( S.params.preconditioner == 2 )
-      S.ilu->solve( y, y, gate, S.stream );
+      precSolve( S, y, y, gate );
 }
 
 // r = M^{-1} ( b - A x ) (compute_residue of BiCGStab / GMRES) or, with precondition =
@@ -391,8 +437,8 @@ void residual( SolverImpl& S, const void* b, const void* x, T* r, Gate gate, boo
    spmvLaunchGated( *S.A, x, r, -1.0, 1.0, nullptr, gate.flags, gate.mode, S.stream );
    if( precondition && S.params.preconditioner == 1 )
       launchElementwise( S.n, OpDivide< T >{ {}, r, r, (const T*) S.diag.ptr }, gate, S.stream );
-   if( precondition && S.params.preconditioner == 2 )
-      S.ilu->solve( r, r, gate, S.stream );
+   if( precondition && S.params.preconditioner >= 2 )
+      precSolve( S, r, r, gate );
 }
 
 // normb = || M^{-1} b || (1 when zero), written to state.normb
@@ -416,8 +462,8 @@ void computeNormb( SolverImpl& S, const void* b, T* tmp )
                          S.stream );
       src = tmp;
    }
-   if( S.params.preconditioner == 2 ) {
-      S.ilu->solve( b, tmp, Gate{ nullptr, 0 }, S.stream );
+   if( S.params.preconditioner >= 2 ) {
+      precSolve( S, b, tmp, Gate{ nullptr, 0 } );
       src = tmp;
    }
    launchReduce< 1 >( S.n, OpSquares< T >{ {}, src }, FinNormb< T >{ S.state.ptr }, S.ws.slot( 0 ), Gate{ nullptr, 0 },
@@ -641,7 +687,7 @@ struct CgStepOp
    }
 };
 
-// CG with a preconditioner applied as a separate solve (ILU(0)): z = M^-1 r after every
+// CG with a preconditioner applied as a separate solve (ILU(0), Jacobi polynomial): z = M^-1 r after every
 // residual update (CG.hpp:44-46, 88-90)
 template< typename T >
 void solveCGSeparate( SolverImpl& S, const void* b, void* xv )
@@ -655,7 +701,7 @@ void solveCGSeparate( SolverImpl& S, const void* b, void* xv )
    const Stop stop = S.stopRule();
    computeNormb< T >( S, b, z );
    residual< T >( S, b, x, r, always, false );
-   S.ilu->solve( r, z, always, S.stream );
+   precSolve( S, r, z, always );
    launchReduce< 1 >( S.n, CgInitFromZOp< T >{ {}, p, z, r }, FinCgInit{ S.state.ptr, S.hist.ptr, S.histCap, stop },
                       S.ws.slot( 0 ), always, S.stream );
    runChunks( S, [ & ] {
@@ -663,7 +709,7 @@ void solveCGSeparate( SolverImpl& S, const void* b, void* xv )
       launchReduce< 1 >( S.n, OpDot< T >{ {}, Ap, p }, FinCgAlpha{ S.state.ptr, S.hist.ptr, S.histCap },
                          S.ws.slot( 1 ), running, S.stream );
       launchElementwise( S.n, CgStepOp< T >{ x, r, p, Ap, S.state.ptr, 0 }, running, S.stream );
-      S.ilu->solve( r, z, running, S.stream );
+      precSolve( S, r, z, running );
       launchReduce< 1 >( S.n, OpDot< T >{ {}, r, z }, FinCgBeta{ S.state.ptr, S.hist.ptr, S.histCap, stop },
                          S.ws.slot( 2 ), running, S.stream );
       launchElementwise( S.n, CgDirectionOp< T >{ p, z, S.state.ptr, 0 }, running, S.stream );
@@ -673,7 +719,7 @@ void solveCGSeparate( SolverImpl& S, const void* b, void* xv )
 template< typename T >
 void solveCG( SolverImpl& S, const void* b, void* xv )
 {
-   if( S.params.preconditioner == 2 ) {
+   if( S.params.preconditioner >= 2 ) {
       solveCGSeparate< T >( S, b, xv );
       return;
    }
@@ -2386,6 +2432,16 @@ void prepare( SolverImpl& S )
    if( S.method == TNLHIP_GMRES )
       nvec = 4 + ( S.params.restarting_max + 1 ) + 1;
    S.vecs.resize( (int64_t) nvec * S.ld * (int64_t) S.vbytes );
+   if( S.params.preconditioner == 3 ) {
+      if( S.polyMatrix ) {
+         const Matrix& P = *S.polyMatrix;
+         TNLHIP_REQUIRE( P.rows == S.A->rows && P.cols == S.A->cols,
+                         "the preconditioner matrix must have the dimensions of the solved matrix" );
+         TNLHIP_REQUIRE( P.dtype != TNLHIP_BINARY && vectorDtype( P.dtype ) == S.vdtype,
+                         "the preconditioner matrix must use the solved matrix's vector type" );
+      }
+      S.polyVecs.resize( 2 * S.ld * (int64_t) S.vbytes );
+   }
    S.ws.ensure( 4, kMaxMulti, 64 );
    if( S.method == TNLHIP_GMRES ) {
       const int64_t m1 = S.params.restarting_max + 1;
@@ -2416,6 +2472,7 @@ void tnlhip_solver_params_default( tnlhip_solver_params* p )
    p->preconditioner = 0;
    p->use_graph = 1;
    p->bicgstab_ell = 1;  // BICGStabL.hpp:21
+   p->poly_degree = 4;
 }
 
 int tnlhip_solver_create( int method, tnlhip_solver* out )
@@ -2444,11 +2501,23 @@ int tnlhip_solver_set_matrix( tnlhip_solver s, tnlhip_matrix A )
    } );
 }
 
+int tnlhip_solver_set_preconditioner_matrix( tnlhip_solver s, tnlhip_matrix P )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( s, "NULL solver" );
+      Matrix* M = reinterpret_cast< Matrix* >( P );
+      if( M && ( M->dtype == TNLHIP_BINARY || M->symmetric ) )
+         throw Error( TNLHIP_ERROR_UNSUPPORTED, "the Jacobi-polynomial preconditioner needs a general real-valued matrix" );
+      s->polyMatrix = M;
+   } );
+}
+
 int tnlhip_solver_set_params( tnlhip_solver s, const tnlhip_solver_params* p )
 {
    return guarded( [ & ] {
       TNLHIP_REQUIRE( s && p, "NULL argument" );
-      TNLHIP_REQUIRE( p->preconditioner >= 0 && p->preconditioner <= 2, "unknown preconditioner" );
+      TNLHIP_REQUIRE( p->preconditioner >= 0 && p->preconditioner <= 3, "unknown preconditioner" );
+      TNLHIP_REQUIRE( p->poly_degree >= 0 && p->poly_degree <= 32, "poly_degree must be in [0, 32]" );
       TNLHIP_REQUIRE( p->gmres_variant >= 0 && p->gmres_variant <= 4, "unknown GMRES variant" );
       TNLHIP_REQUIRE( p->bicgstab_ell >= 1 && p->bicgstab_ell <= 16, "bicgstab_ell must be in [1, 16]" );
       // restarting_max sizes the GMRES basis and its small arrays

@@ -723,10 +723,16 @@ namespace {
 
 template< typename V, typename A, typename Y >
 void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, int32_t first, int32_t last,
-                  const Y* diag, const int32_t* gate, int gateMode, hipStream_t st )
+                  const Y* diag, const int32_t* gate, int gateMode, hipStream_t st, const Y* rhs = nullptr,
+                  const Y* prev = nullptr )
 {
    Keeper< A, Y > keep{ y, (A) alpha, (A) beta, diag, 0, gate, gate ? gateMode : 0 };
-   if( diag )
+   if( rhs ) {
+      keep.mode = 5;
+      keep.rhs = rhs;
+      keep.prev = prev;
+   }
+   else if( diag )
       keep.mode = 4;
    else if( beta == 0.0 )
       keep.mode = alpha == 1.0 ? 0 : 1;
@@ -902,6 +908,31 @@ void spmvLaunchGated( const Matrix& A, const void* x, void* y, double alpha, dou
                                             (const float*) diag, gate, gateMode, stream );
          break;
       default: throw Error( TNLHIP_ERROR_INVALID_ARGUMENT, "unknown value type" );
+   }
+}
+
+void spmvJacobiStepGated( const Matrix& P, const void* z, void* znew, const void* rhs, const void* diag,
+                          const int32_t* gate, int gateMode, hipStream_t stream )
+{
+   if( P.dtype == TNLHIP_BINARY || P.symmetric )
+      throw Error( TNLHIP_ERROR_UNSUPPORTED, "the Jacobi-polynomial preconditioner needs a general real-valued matrix" );
+   TNLHIP_REQUIRE( z != znew && rhs != znew, "a Jacobi step writes a vector of its own" );
+   switch( P.dtype ) {
+      case TNLHIP_F64:
+         launchTyped< double, double, double >( P, (const double*) z, (double*) znew, 1.0, 0.0, 0, P.rows,
+                                                (const double*) diag, gate, gateMode, stream, (const double*) rhs,
+                                                (const double*) z );
+         break;
+      case TNLHIP_F32:
+         launchTyped< float, float, float >( P, (const float*) z, (float*) znew, 1.0, 0.0, 0, P.rows,
+                                             (const float*) diag, gate, gateMode, stream, (const float*) rhs,
+                                             (const float*) z );
+         break;
+      default:
+         launchTyped< bf16, float, float >( P, (const float*) z, (float*) znew, 1.0, 0.0, 0, P.rows,
+                                            (const float*) diag, gate, gateMode, stream, (const float*) rhs,
+                                            (const float*) z );
+         break;
    }
 }
 

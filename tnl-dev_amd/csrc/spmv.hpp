@@ -9,7 +9,9 @@ namespace tnlhip {
 // ------------------------------------------------------------------ result keeper
 
 // SparseMatrixView.hpp:447-490: direct / alpha / beta / general keepers; mode 4 divides by
-// a diagonal, which fuses Preconditioners::Diagonal::solve (x = b / diag) into the product.
+// a diagonal, which fuses Preconditioners::Diagonal::solve (x = b / diag) into the product;
+// mode 5 is one step of the Jacobi splitting, y = prev + (rhs - v) / diag (the Jacobi-polynomial
+// preconditioner; y must not alias prev, which other rows gather).
 template< typename A, typename Y >
 struct Keeper
 {
@@ -22,6 +24,8 @@ struct Keeper
    // y stored with non-temporal stores (the default: y is written once, L2 stays for x); kernels
    // whose waves end rows inside a cache line store through L2, which merges the partial lines
    bool streamY = true;
+   const Y* rhs = nullptr;   // mode 5
+   const Y* prev = nullptr;  // mode 5
    __device__ __forceinline__ bool closed() const
    {
       if( gateMode == 0 )
@@ -38,7 +42,8 @@ struct Keeper
          case 1: out = alpha * v; break;
          case 2: out = beta * (A) y[ row ] + v; break;
          case 3: out = beta * (A) y[ row ] + alpha * v; break;
-         default: out = v / (A) diag[ row ]; break;
+         case 4: out = v / (A) diag[ row ]; break;
+         default: out = (A) prev[ row ] + ( (A) rhs[ row ] - v ) / (A) diag[ row ]; break;
       }
       if( streamY )
          __builtin_nontemporal_store( (Y) out, y + row );  // y is written once: keep L2 for x
