@@ -138,7 +138,10 @@ def test_bf16_polynomial_gmres_follows_numpy(tnl, dev, variant):
             restarting_min=30, restarting_max=30)
     xd = torch.zeros(n, dtype=torch.float32, device=dev)
     s.solve(torch.tensor(b, dtype=torch.float32, device=dev), xd)
+    # the history opens with the residue 1 of x0 = 0 (set once or, by the variants that recompute
+    # it at the top of the restart cycle, twice), then one residue per step
     got = s.getResidualHistory()
+    got = got[int(np.argmax(got < 0.999)) - 1:]
     print(variant, got[:6], want[:6])
     m = min(len(got), 13)
     assert m >= 10
