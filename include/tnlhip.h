@@ -330,6 +330,17 @@ int tnlhip_axpby( int dtype, int64_t n, double alpha, const void* x, double beta
 int tnlhip_multidot( int dtype, int64_t n, int32_t count, const void* V, int64_t ld, const void* w, double* result,
                      void* stream );
 
+/* tnlhip_multidot without allocating: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same kernels and bits as tnlhip_multidot, but count is limited to 1..32 columns (one
+ * pass; larger counts fail with TNLHIP_ERROR_INVALID_ARGUMENT), `result` (count doubles) is DEVICE
+ * memory, and the call is asynchronous on `stream` (no synchronisation, nothing copied to the
+ * host).  The partial sums live in a fixed 256 KiB device array that each GPU holds until the
+ * process ends and that every call on that GPU shares: calls that overlap on one GPU (different
+ * streams or threads) race and must be serialised by the caller. */
+int tnlhip_multidot_noalloc( int dtype, int64_t n, int32_t count, const void* V, int64_t ld, const void* w,
+                             double* result, void* stream );
+
 /* y = alpha * V x + beta * y with V column-major n x count, x on the host:
  * Matrices::MatrixOperations::gemv, Matrices/MatrixOperations.h:46-150, 232-345. */
 int tnlhip_gemv( int dtype, int64_t n, int32_t count, double alpha, const void* V, int64_t ld, const double* x,

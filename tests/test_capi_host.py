@@ -111,3 +111,17 @@ def test_solver_argument_checks(tnl):
         assert lib.tnlhip_solver_get_history(h, None, 0) == 0
     finally:
         lib.tnlhip_solver_destroy(h)
+
+
+def test_noalloc_multidot_refuses_before_touching_the_gpu(tnl):
+    """tnlhip_multidot_noalloc rejects a column count its fixed device array cannot hold, and bad
+    shapes, with a clear error and without a device call."""
+    import ctypes as C
+    lib = tnl.lib
+    dummy = C.c_void_p(16)  # never dereferenced: every case below fails its argument check
+    for count in (0, 33, 1000):
+        assert lib.tnlhip_multidot_noalloc(0, 64, count, dummy, 64, dummy, dummy, None) == 1
+        assert b"1 to 32 columns" in lib.tnlhip_last_error()
+    assert lib.tnlhip_multidot_noalloc(0, 64, 4, dummy, 32, dummy, dummy, None) == 1  # ld < n
+    assert lib.tnlhip_multidot_noalloc(0, 64, 4, None, 64, dummy, dummy, None) == 1
+    assert lib.tnlhip_multidot_noalloc(2, 64, 4, dummy, 64, dummy, dummy, None) == 1  # BF16 vectors

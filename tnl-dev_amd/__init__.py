@@ -42,7 +42,7 @@ EXPORTS = [
     "tnlhip_matrix_get_row_capacities", "tnlhip_matrix_get_compressed_row_lengths", "tnlhip_matrix_get_row_capacity",
     "tnlhip_matrix_get_row", "tnlhip_matrix_get_view", "tnlhip_matrix_values_changed",
     "tnlhip_solver_get_history_iterations", "tnlhip_solver_set_monitor", "tnlhip_solver_set_residual_history_file",
-    "tnlhip_solver_set_preconditioner_matrix",
+    "tnlhip_solver_set_preconditioner_matrix", "tnlhip_multidot_noalloc",
 ]
 # element order of TNL binary files: "cuda" = column-major (TNL's GPU matrices), "host" = row-major
 FILE_LAYOUTS = {"cuda": 0, "host": 1}
@@ -167,6 +167,7 @@ def _load():
         "tnlhip_norm2": (C.c_int, [C.c_int, i64, vp, C.POINTER(f64), vp]),
         "tnlhip_axpby": (C.c_int, [C.c_int, i64, f64, vp, f64, vp, vp]),
         "tnlhip_multidot": (C.c_int, [C.c_int, i64, i32, vp, i64, vp, vp, vp]),
+        "tnlhip_multidot_noalloc": (C.c_int, [C.c_int, i64, i32, vp, i64, vp, vp, vp]),
         "tnlhip_gemv": (C.c_int, [C.c_int, i64, i32, f64, vp, i64, vp, f64, vp, vp]),
         "tnlhip_exclusive_scan_i32": (C.c_int, [i64, vp, vp, C.c_int, vp]),
         "tnlhip_solver_params_default": (None, [C.POINTER(SolverParams)]),

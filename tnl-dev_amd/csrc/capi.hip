@@ -265,6 +265,35 @@ inline float bf16ToFloat( uint16_t b )
 
 using namespace tnlhip;
 
+// Partial sums of tnlhip_multidot_noalloc: one column pass of up to kMaxMulti columns writes
+// kReduceGrid partials per column.  Each GPU holds its own copy for the life of the process.
+constexpr int kNoAllocColumns = kMaxMulti;
+__device__ __align__( 256 ) double scratch[ (size_t) kNoAllocColumns * kReduceGrid ];
+
+namespace {
+
+// The device address of the current GPU's `scratch`, looked up once per device (with that
+// device current) on its first use; host code passes this address, never the array's name.
+double* noAllocScratch()
+{
+   constexpr int kMaxDevices = 64;
+   static std::mutex lock;
+   static double* address[ kMaxDevices ] = {};
+   int dev = 0;
+   TNLHIP_CHECK( hipGetDevice( &dev ) );
+   TNLHIP_REQUIRE( dev >= 0 && dev < kMaxDevices, "device ordinal out of range" );
+   std::lock_guard< std::mutex > guard( lock );
+   if( ! address[ dev ] ) {
+      void* p = nullptr;
+      TNLHIP_CHECK( hipGetSymbolAddress( &p, HIP_SYMBOL( scratch ) ) );
+      TNLHIP_REQUIRE( p, "no device address for the scratch array" );
+      address[ dev ] = static_cast< double* >( p );
+   }
+   return address[ dev ];
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* tnlhip_status_string( int s )
@@ -897,6 +926,23 @@ int tnlhip_multidot( int dtype, int64_t n, int32_t count, const void* V, int64_t
       TNLHIP_CHECK( hipMemcpyAsync( result, ws.scalars.ptr, sizeof( double ) * count, hipMemcpyDeviceToHost,
                                     (hipStream_t) stream ) );
       TNLHIP_CHECK( hipStreamSynchronize( (hipStream_t) stream ) );
+   } );
+}
+
+int tnlhip_multidot_noalloc( int dtype, int64_t n, int32_t count, const void* V, int64_t ld, const void* w,
+                             double* result, void* stream )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( dtype == TNLHIP_F64 || dtype == TNLHIP_F32, "vectors are F64 or F32" );
+      TNLHIP_REQUIRE( n > 0 && ld >= n && V && w && result, "bad multidot shape or NULL argument" );
+      if( count < 1 || count > kNoAllocColumns )
+         throw Error( TNLHIP_ERROR_INVALID_ARGUMENT,
+                      "tnlhip_multidot_noalloc takes 1 to " + std::to_string( kNoAllocColumns )
+                         + " columns (its partial sums live in a fixed device array); got "
+                         + std::to_string( count ) );
+      // every partial the final pass reads is written by the first pass of the same call, so the
+      // array needs no clearing
+      multidotSlot( dtype, n, count, V, ld, w, result, ReduceSlot{ noAllocScratch() }, (hipStream_t) stream );
    } );
 }
 
