@@ -37,6 +37,7 @@ FORMATS = ["CSRScalar", "CSRVector", "CSRLight", "CSRHybrid", "CSRAdaptive", "El
 # the headline line is the fastest format on this matrix, which the autotuner also picks for it;
 # every format's figure is in "formats"
 HEADLINE = "ChunkedEllpack"
+GMRES_INNER_TOL = 1e-2
 
 
 def kernel_of(fmt, n, nnz):
@@ -318,14 +319,21 @@ def irregular_suite(tnl, torch, W, dev, args):
         B = W.algorithmic_bytes(n, n, nnz, sv)
         x = torch.tensor(x_host, device=dev).to(vt)
         y = torch.zeros(n, dtype=vt, device=dev)
-        for fmt in ("CSRAdaptive", "ChunkedEllpack", "BiEllpack"):
+        # CSRAdaptive also with its column-striped plan (x split into stripes that stay in the
+        # L2 of the XCDs that own them; stripe partials added in stripe order)
+        runs = [("CSRAdaptive", 0), ("ChunkedEllpack", 0), ("BiEllpack", 0)]
+        runs += [("CSRAdaptive", int(S)) for S in args.irr_stripes.split(",") if S and int(S) > 0]
+        for fmt, S in runs:
             A = tnl.SparseMatrix.from_csr(rp, c, v, format=fmt, dtype=dt)
+            if S:
+                A.set_param("column_stripes", S)
             steady, cold = time_spmv(tnl, torch, A, x, y, args.steps, args.warmup, flush)
             mean = steady if B > MALL_BYTES else cold
             gbs = B / mean / 1e9
-            out["spmv"][f"{fmt}_{dt}"] = {"us": round(mean * 1e6, 2), "GB/s": round(gbs, 1),
-                                          "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "us_cold": round(cold * 1e6, 2)}
-            log(f"  {fmt:15s} {dt}: {mean * 1e6:8.2f} us {gbs:7.1f} GB/s {100 * gbs / HBM_PEAK_GBS:5.1f}%")
+            name = fmt if not S else f"{fmt}_stripes{S}"
+            out["spmv"][f"{name}_{dt}"] = {"us": round(mean * 1e6, 2), "GB/s": round(gbs, 1),
+                                           "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "us_cold": round(cold * 1e6, 2)}
+            log(f"  {name:22s} {dt}: {mean * 1e6:8.2f} us {gbs:7.1f} GB/s {100 * gbs / HBM_PEAK_GBS:5.1f}%")
             del A
         del x, y
     del flush
@@ -411,20 +419,22 @@ def next_suite(tnl, torch, W, dev, args, general_us=None):
     return out
 
 
-def serving_ladder(W):
+def serving_ladder(W, per_dim):
     """Resident matrices of the serving workload: 2D 5-point and 3D 7-point Laplacians whose
-    sizes step geometrically from about 1e3 to 1e6 rows (8 sizes per dimension)."""
+    grid sides step geometrically from 32 to 1000 (2D) and 10 to 100 (3D), per_dim sides per
+    dimension, so that rows run from about 1e3 to 1e6."""
     out = []
-    for side in np.unique(np.round(np.geomspace(32, 1000, 8)).astype(int)):
+    for side in np.unique(np.round(np.geomspace(32, 1000, per_dim)).astype(int)):
         out.append(("2d", int(side), W.laplacian_2d(int(side))))
-    for side in np.unique(np.round(np.geomspace(10, 100, 8)).astype(int)):
+    for side in np.unique(np.round(np.geomspace(10, 100, per_dim)).astype(int)):
         out.append(("3d", int(side), W.laplacian_3d(int(side))))
     return out
 
 
-def serving_requests(ladder, count, seed):
-    """count requests: dimension 2 or 3, rows log-uniform in [1e3, 1e6] (the nearest ladder
-    matrix of that dimension), shift ~ U[0, 1], k ~ U{1..16} right-hand sides ~ U[-1, 1]."""
+def serving_requests(ladder, count, seed, gmres_fraction):
+    """count requests: dimension 2 or 3, rows log-uniform in [1e3, 1e6] (the resident matrix of
+    that dimension nearest in log rows), shift ~ U[0, 1], k ~ U{1..16} right-hand sides ~ U[-1, 1],
+    method GMRES with probability gmres_fraction, CG otherwise."""
     rng = np.random.default_rng(seed)
     sizes = {d: [(i, len(m[0]) - 1) for i, (dd, _, m) in enumerate(ladder) if dd == d] for d in ("2d", "3d")}
     reqs = []
@@ -432,7 +442,9 @@ def serving_requests(ladder, count, seed):
         d = "2d" if rng.random() < 0.5 else "3d"
         target = np.exp(rng.uniform(np.log(1e3), np.log(1e6)))
         idx = min(sizes[d], key=lambda t: abs(np.log(t[1]) - np.log(target)))[0]
-        reqs.append((idx, float(rng.uniform(0, 1)), int(rng.integers(1, 17))))
+        shift, k = float(rng.uniform(0, 1)), int(rng.integers(1, 17))
+        method = "gmres" if rng.random() < gmres_fraction else "cg"
+        reqs.append((idx, shift, k, method))
     return reqs
 
 
@@ -452,10 +464,12 @@ def serve(tnl, server, mids, ladder, reqs, pool, arrivals, inner_tol=3e-6):
     while done < len(reqs):
         now = time.perf_counter() - t0
         while nxt < len(reqs) and arrive[nxt] <= now:
-            idx, shift, k = reqs[nxt]
+            idx, shift, k, method = reqs[nxt]
             n = n_of[idx]
             B = pool[offs[nxt]:offs[nxt] + n * k].reshape(n, k)
-            ticket_of[server.submit(mids[idx], B, shift=shift, inner_tolerance=inner_tol)] = nxt
+            # GMRES requests refine over restarted GMRES, which a loose inner target suits best
+            itol = inner_tol if method == "cg" else GMRES_INNER_TOL
+            ticket_of[server.submit(mids[idx], B, shift=shift, inner_tolerance=itol, method=method)] = nxt
             nxt += 1
         st = server.stats()
         if st["resident_requests"] == 0 and st["queued_requests"] == 0:
@@ -478,22 +492,29 @@ def serve(tnl, server, mids, ladder, reqs, pool, arrivals, inner_tol=3e-6):
 
 def serving_suite(tnl, W, args):
     """Continuous-batching solve server: Laplacian + shift I requests with 1-16 right-hand sides,
-    FP64 refinement over FP32 CG to 1e-10 per right-hand side.  First all requests at once
-    (throughput), then Poisson arrivals at 70 % of that throughput (latency)."""
-    ladder = serving_ladder(W)
-    server = tnl.Server(vector_capacity=96 << 20, matrix_capacity=48 << 20, slice_capacity=4 << 20,
-                        max_requests=256, iterations_per_launch=args.serve_iterations)
+    FP64 refinement over FP32 CG (batched track) or FP32 GMRES (worker track) to 1e-10 per
+    right-hand side.  First all requests at once (throughput), then Poisson arrivals at 70 % of
+    that throughput (latency)."""
+    ladder = serving_ladder(W, args.serve_ladder)
+    slots = sum(len(m[1]) + 32 * (len(m[0]) // 32 + 1) * 2 for _, _, m in ladder)  # nnz + slice padding
+    slices = sum(len(m[0]) // 32 + 2 for _, _, m in ladder)
+    server = tnl.Server(vector_capacity=args.serve_entries, matrix_capacity=int(1.2 * slots), slice_capacity=slices,
+                        max_requests=256, iterations_per_launch=args.serve_iterations,
+                        gmres_workers=args.serve_gmres_workers)
     t0 = time.perf_counter()
     mids = [server.add_matrix(*m) for _, _, m in ladder]
     setup = time.perf_counter() - t0
     count = args.serve_requests
-    reqs = serving_requests(ladder, count, 2024)
+    reqs = serving_requests(ladder, count, 2024, args.serve_gmres_fraction)
     pool = np.random.default_rng(7).uniform(-1, 1, 32 * 10 ** 6)
     n_of = [len(m[0]) - 1 for _, _, m in ladder]
-    rhs = sum(k for _, _, k in reqs)
-    log(f"serving: {len(ladder)} resident matrices ({setup:.2f} s to upload), {count} requests, {rhs} right-hand sides")
-    # warm-up: a few requests through the whole path
-    serve(tnl, server, mids, ladder, reqs[:8], pool, None, args.serve_inner_tol)
+    rhs = sum(q[2] for q in reqs)
+    is_gmres = np.array([q[3] == "gmres" for q in reqs])
+    log(f"serving: {len(ladder)} resident matrices ({setup:.2f} s to upload), {count} requests "
+        f"({int(is_gmres.sum())} GMRES), {rhs} right-hand sides")
+    # warm-up: a few requests of each method through the whole path
+    warm = [q for q in reqs if q[3] == "cg"][:6] + [q for q in reqs if q[3] == "gmres"][:2]
+    serve(tnl, server, mids, ladder, warm, pool, None, args.serve_inner_tol)
     wall, lat_sat, info = serve(tnl, server, mids, ladder, reqs, pool, None, args.serve_inner_tol)
     conv = sum(1 for i in info if i["converged"])
     rate = count / wall
@@ -506,18 +527,27 @@ def serving_suite(tnl, W, args):
     p50, p99 = np.percentile(lat, [50, 99])
     log(f"serving (Poisson, {lam:.1f} req/s offered): p50 {p50 * 1e3:.1f} ms, p99 {p99 * 1e3:.1f} ms, "
         f"converged {conv2}/{count}")
+    by_method = {}
+    for name, mask in (("cg", ~is_gmres), ("gmres", is_gmres)):
+        if mask.any():
+            q50, q99 = np.percentile(lat[mask], [50, 99])
+            by_method[name] = {"requests": int(mask.sum()), "p50_ms": round(q50 * 1e3, 2),
+                               "p99_ms": round(q99 * 1e3, 2)}
+            log(f"  {name}: {int(mask.sum())} requests, p50 {q50 * 1e3:.1f} ms, p99 {q99 * 1e3:.1f} ms")
     server.close()
-    entries = sum(n_of[i] * k for i, _, k in reqs)
-    return {"workload": "2D/3D Laplacian + shift*I, shift ~ U[0,1], rows log-uniform 1e3..1e6 (16 resident "
-                        "matrices), 1-16 RHS ~ U[-1,1] per request, FP64 refinement over FP32 CG, tol 1e-10 per RHS",
-            "requests": count, "rhs": rhs, "entries": entries,
+    entries = sum(n_of[q[0]] * q[2] for q in reqs)
+    return {"workload": f"2D/3D Laplacian + shift*I, shift ~ U[0,1], rows log-uniform 1e3..1e6 ({len(ladder)} resident "
+                        "matrices, the nearest in log rows), 1-16 RHS ~ U[-1,1] per request, FP64 refinement over "
+                        "FP32 CG or GMRES, tol 1e-10 per RHS",
+            "requests": count, "gmres_requests": int(is_gmres.sum()), "rhs": rhs, "entries": entries,
             "throughput": {"solves_per_s": round(rate, 2), "rhs_per_s": round(rhs / wall, 2),
                            "seconds": round(wall, 3), "converged": conv},
             "latency": {"offered_req_per_s": round(lam, 2), "p50_ms": round(p50 * 1e3, 2),
                         "p99_ms": round(p99 * 1e3, 2), "mean_ms": round(float(lat.mean()) * 1e3, 2),
-                        "seconds": round(wall2, 3), "converged": conv2},
+                        "seconds": round(wall2, 3), "converged": conv2, "by_method": by_method},
             "inner_iterations_mean": round(float(np.mean([i["inner_iterations"] for i in info])), 1),
-            "iterations_per_launch": args.serve_iterations, "inner_tolerance": args.serve_inner_tol}
+            "iterations_per_launch": args.serve_iterations, "inner_tolerance": args.serve_inner_tol,
+            "gmres_inner_tolerance": GMRES_INNER_TOL}
 
 
 def main():
@@ -542,7 +572,13 @@ def main():
     ap.add_argument("--no-irregular", action="store_true")
     ap.add_argument("--no-next", action="store_true", help="skip the symmetric / binary / ILU(0) suite")
     ap.add_argument("--irr-rows", type=int, default=2000000)
-    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--irr-stripes", default="4,8", help="column-stripe counts of the striped CSRAdaptive runs")
+    ap.add_argument("--serve-requests", type=int, default=10000)
+    ap.add_argument("--serve-ladder", type=int, default=32, help="resident matrix sizes per dimension")
+    ap.add_argument("--serve-gmres-fraction", type=float, default=0.2)
+    ap.add_argument("--serve-gmres-workers", type=int, default=4)
+    ap.add_argument("--serve-entries", type=int, default=256 << 20,
+                    help="vector arena of the server in (row, right-hand side) entries, 40 B each")
     ap.add_argument("--serve-iterations", type=int, default=8)
     ap.add_argument("--only", default="", help="run one suite: solvers | serving | gmres | irregular | next")
     args = ap.parse_args()

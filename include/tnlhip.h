@@ -234,6 +234,16 @@ int tnlhip_matrix_set_param( tnlhip_matrix A, const char* name, int32_t value );
 int tnlhip_spmv( tnlhip_matrix A, const void* x, void* y, double alpha, double beta, int32_t first, int32_t last,
                  void* stream );
 
+/* SparseMatrix::vectorProduct generalized to k right-hand sides read with one pass over A (no TNL
+ * entry point takes several vectors; SparseMatrixView.hpp:390-491 per column):
+ *   Y[first, last) = alpha * A X + beta * Y, X columns x k and Y rows x k row-major device arrays
+ *   (entry row * k + j), 1 <= k <= 64; last == 0 means all rows; beta == 0 does not read Y.
+ *   Column j of Y equals the one-vector product of column j of X in TNL's Host order (each product
+ *   rounded, the row's slots summed left to right), bit for bit.  Any format; general real-valued
+ *   matrices (TNLHIP_ERROR_UNSUPPORTED for symmetric and binary ones). */
+int tnlhip_spmv_multi( tnlhip_matrix A, int32_t k, const void* X, void* Y, double alpha, double beta, int32_t first,
+                       int32_t last, void* stream );
+
 /* ------------------------------------------------------------------ row traversal */
 
 /* Fetch and reduction of tnlhip_matrix_reduce_rows: the fetch( rowIdx, columnIdx, value ) and
@@ -502,14 +512,16 @@ int tnlhip_refine_solve( tnlhip_matrix A, tnlhip_matrix A_low, const tnlhip_refi
 /* ------------------------------------------------------------------ autotuner */
 
 /* One timed configuration: format, lanes per row (CSR light / hybrid; 0 = the kernel's
- * choice), workgroups per CU and rows per lane (padded formats; 0 / -1 = defaults), seconds per
- * y = A x. */
+ * choice), workgroups per CU and rows per lane (padded formats; 0 / -1 = defaults), the slice
+ * size of SlicedEllpack (TNL's SliceSize template parameter: 32 is TNL's default and keeps its
+ * metadata; 16 and 64 are tried too, 0 for other formats), seconds per y = A x. */
 typedef struct
 {
    int32_t format;
    int32_t lanes;
    int32_t blocks_per_cu;
    int32_t pair_rows;
+   int32_t slice_size;
    double seconds;
 } tnlhip_tuning;
 
@@ -532,8 +544,9 @@ int tnlhip_autotune_csr( int value_dtype, int32_t rows, int32_t columns, const i
  * matrix A to an FP64 relative residual per right-hand side, by FP64 iterative refinement
  * over an FP32 CG inner solve (Solvers/Linear/CG.hpp:17-107 recurrences).  GMRES requests
  * (method TNLHIP_GMRES, Solvers/Linear/GMRES.hpp:79-180) run beside them: `gmres_workers` of them at
- * once, each on its own HIP stream, by FP64 refinement over an FP32 Jacobi GMRES(30) of
- * (A + shift I), admitted FIFO as workers free up.  No counterpart in the reference: TNL solves one
+ * once, each on its own HIP stream, by FP64 refinement over an FP32 GMRES(30) of (A + shift I)
+ * preconditioned by the degree-4 Jacobi polynomial over a BF16 copy, admitted FIFO as workers
+ * free up.  No counterpart in the reference: TNL solves one
  * system per LinearSolver::solve call (LinearSolver.h:115-162); a TNL caller that loops over
  * solve() for many small systems is what this replaces. */
 typedef struct tnlhip_server_s* tnlhip_server;

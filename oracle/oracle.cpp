@@ -63,6 +63,7 @@ struct orc_segments
    // SlicedEllpack
    std::vector< I > sliceOffsets, sliceSegmentSizes;
    I slicedSize = 0, slicedAligned = 0;
+   I sliceSize = 32;  // the SliceSize template parameter, SlicedEllpack.h:22 (default 32)
    // ChunkedEllpack
    I chunksInSlice = 256, desiredChunkSize = 16, numberOfSlices = 0, chunkedStorage = 0;
    std::vector< I > rowToChunk, rowToSlice, chunksToSegments, rowPointers;
@@ -75,7 +76,6 @@ struct orc_segments
 
 namespace {
 
-constexpr I kSliceSize = 32;     // SlicedEllpack.h:22
 constexpr I kAlignment = 32;     // Ellpack.h:21
 constexpr I kWarpSize = 32;      // BiEllpack.h:22
 constexpr I kLogWarpSize = 5;
@@ -107,8 +107,9 @@ void buildEllpack( orc_segments& s, const I* caps )
 
 void buildSliced( orc_segments& s, const I* caps )
 {
-   // SlicedEllpack.hpp:64-102 — per slice of 32 segments: max size; offsets = exclusive
-   // scan of 32*max; size = sum(sizes); alignedSize = offsets[last]
+   // SlicedEllpack.hpp:64-102 — per slice of SliceSize segments: max size; offsets = exclusive
+   // scan of SliceSize*max; size = sum(sizes); alignedSize = offsets[last]
+   const I kSliceSize = s.sliceSize;
    const I slices = roundUpDivision( s.n, kSliceSize );
    s.sliceOffsets.assign( slices + 1, 0 );
    s.sliceSegmentSizes.assign( slices, 0 );
@@ -327,6 +328,21 @@ orc_segments_create( int32_t kind, int32_t n, const int32_t* caps, int32_t colum
    return s;
 }
 
+// SlicedEllpack< ..., SliceSize > for another slice size than TNL's default 32
+extern "C" orc_segments*
+orc_segments_create_sliced( int32_t n, const int32_t* caps, int32_t column_major, int32_t slice_size )
+{
+   if( slice_size < 1 )
+      return nullptr;
+   auto* s = new orc_segments;
+   s->kind = ORC_SLICED_ELLPACK;
+   s->n = n;
+   s->colmajor = column_major != 0;
+   s->sliceSize = slice_size;
+   buildSliced( *s, caps );
+   return s;
+}
+
 extern "C" void
 orc_segments_destroy( orc_segments* s )
 {
@@ -352,7 +368,7 @@ orc_segments_segment_size( const orc_segments* s, int32_t seg )
    switch( s->kind ) {
       case ORC_CSR: return s->offsets[ seg + 1 ] - s->offsets[ seg ];
       case ORC_ELLPACK: return s->segmentSize;
-      case ORC_SLICED_ELLPACK: return s->sliceSegmentSizes[ seg / kSliceSize ];
+      case ORC_SLICED_ELLPACK: return s->sliceSegmentSizes[ seg / s->sliceSize ];
       case ORC_CHUNKED_ELLPACK: {
          // detail/ChunkedEllpack.h:74-89
          const I sl = s->rowToSlice[ seg ];
@@ -386,10 +402,10 @@ orc_segments_global_index( const orc_segments* s, int32_t seg, int32_t local )
          return seg + (int64_t) s->alignedSize * local;
       case ORC_SLICED_ELLPACK: {
          // SlicedEllpackView.hpp:108-132
-         const I sl = seg / kSliceSize, in = seg % kSliceSize;
+         const I sl = seg / s->sliceSize, in = seg % s->sliceSize;
          if( ! s->colmajor )
             return s->sliceOffsets[ sl ] + (int64_t) in * s->sliceSegmentSizes[ sl ] + local;
-         return s->sliceOffsets[ sl ] + in + (int64_t) kSliceSize * local;
+         return s->sliceOffsets[ sl ] + in + (int64_t) s->sliceSize * local;
       }
       case ORC_CHUNKED_ELLPACK: {
          // detail/ChunkedEllpack.h:110-145
@@ -577,7 +593,7 @@ void spmv( const orc_segments* s, const Real* vals, const I* cols, const Real* x
       case ORC_SLICED_ELLPACK:
          // SlicedEllpackView.hpp:237-290 (the loop runs over padding; padding adds 0)
          for( I r = first; r < last; r++ ) {
-            const I sl = r / kSliceSize, in = r % kSliceSize;
+            const I sl = r / s->sliceSize, in = r % s->sliceSize;
             Real aux = 0;
             bool compute = true;
             if( ! s->colmajor ) {
@@ -588,7 +604,7 @@ void spmv( const orc_segments* s, const Real* vals, const I* cols, const Real* x
             }
             else {
                const int64_t b = s->sliceOffsets[ sl ] + in, e = s->sliceOffsets[ sl + 1 ];
-               for( int64_t g = b; g < e; g += kSliceSize )
+               for( int64_t g = b; g < e; g += s->sliceSize )
                   aux = aux + fetch( g, compute );
             }
             keep( r, aux );

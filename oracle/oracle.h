@@ -28,6 +28,9 @@ typedef struct orc_segments orc_segments;
 /* Build the segments of `kind` for n segments of the given capacities.
  * column_major selects ElementsOrganization (TNL's GPU default is column-major). */
 orc_segments* orc_segments_create( int32_t kind, int32_t n, const int32_t* capacities, int32_t column_major );
+/* SlicedEllpack with SliceSize = slice_size (SlicedEllpack.h:22 template parameter; 32 by default) */
+orc_segments* orc_segments_create_sliced( int32_t n, const int32_t* capacities, int32_t column_major,
+                                          int32_t slice_size );
 void orc_segments_destroy( orc_segments* s );
 int64_t orc_segments_storage_size( const orc_segments* s );
 int32_t orc_segments_segment_size( const orc_segments* s, int32_t segment );

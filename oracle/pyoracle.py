@@ -71,6 +71,8 @@ def lib():
         L = C.CDLL(ORACLE_SO)
         L.orc_segments_create.restype = C.c_void_p
         L.orc_segments_create.argtypes = [C.c_int32, C.c_int32, _i32p, C.c_int32]
+        L.orc_segments_create_sliced.restype = C.c_void_p
+        L.orc_segments_create_sliced.argtypes = [C.c_int32, _i32p, C.c_int32, C.c_int32]
         L.orc_segments_destroy.argtypes = [C.c_void_p]
         L.orc_segments_storage_size.restype = C.c_int64
         L.orc_segments_storage_size.argtypes = [C.c_void_p]
@@ -122,6 +124,8 @@ def ref():
         L = C.CDLL(REF_SO)
         L.ref_segments_create.restype = C.c_void_p
         L.ref_segments_create.argtypes = [C.c_int, C.c_int, _i32p, C.c_int]
+        L.ref_sliced_create.restype = C.c_void_p
+        L.ref_sliced_create.argtypes = [C.c_int, _i32p, C.c_int, C.c_int]
         L.ref_segments_destroy.argtypes = [C.c_void_p]
         L.ref_segments_storage_size.restype = C.c_longlong
         L.ref_segments_storage_size.argtypes = [C.c_void_p]
@@ -241,18 +245,25 @@ def ref_load_tnl(variant, path, x=None, column_major=True, single=False):
 class Segments:
     """Oracle segments of one kind, built from row capacities."""
 
-    def __init__(self, kind, capacities, column_major=True, backend="oracle"):
+    def __init__(self, kind, capacities, column_major=True, backend="oracle", slice_size=32):
         self.kind = KINDS[kind] if isinstance(kind, str) else kind
         self.caps = np.ascontiguousarray(capacities, dtype=np.int32)
         self.n = len(self.caps)
         self.column_major = column_major
         self.backend = backend
+        sliced = slice_size != 32 and self.kind == KINDS["SlicedEllpack"]
         if backend == "oracle":
             self._L = lib()
-            self._h = self._L.orc_segments_create(self.kind, self.n, self.caps, 1 if column_major else 0)
+            if sliced:
+                self._h = self._L.orc_segments_create_sliced(self.n, self.caps, 1 if column_major else 0, slice_size)
+            else:
+                self._h = self._L.orc_segments_create(self.kind, self.n, self.caps, 1 if column_major else 0)
         else:
             self._L = ref()
-            self._h = self._L.ref_segments_create(self.kind, self.n, self.caps, 1 if column_major else 0)
+            if sliced:
+                self._h = self._L.ref_sliced_create(self.n, self.caps, 1 if column_major else 0, slice_size)
+            else:
+                self._h = self._L.ref_segments_create(self.kind, self.n, self.caps, 1 if column_major else 0)
         if not self._h:
             raise ValueError("bad segments kind")
 

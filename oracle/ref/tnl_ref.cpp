@@ -53,10 +53,10 @@ struct PeekEllpack : Seg::Ellpack< Host, int, Alloc, O, 32 >
    using Base = Seg::Ellpack< Host, int, Alloc, O, 32 >;
    std::vector< int > meta() const { return { this->segmentSize, this->size, this->alignedSize }; }
 };
-template< Seg::ElementsOrganization O >
-struct PeekSliced : Seg::SlicedEllpack< Host, int, Alloc, O, 32 >
+template< Seg::ElementsOrganization O, int SliceSize = 32 >
+struct PeekSliced : Seg::SlicedEllpack< Host, int, Alloc, O, SliceSize >
 {
-   using Base = Seg::SlicedEllpack< Host, int, Alloc, O, 32 >;
+   using Base = Seg::SlicedEllpack< Host, int, Alloc, O, SliceSize >;
    using Base::sliceOffsets;
    using Base::sliceSegmentSizes;
    std::vector< int > meta() const { return { this->size, this->alignedSize }; }
@@ -367,7 +367,45 @@ int rowReductions( int n, int ncols, const int* rp, const int* cols, const doubl
    return 0;
 }
 
+template< Seg::ElementsOrganization O, int S >
+Handle* buildSliced( const IVec& caps )
+{
+   auto* h = new Handle;
+   PeekSliced< O, S > s;
+   s.setSegmentsSizes( caps );
+   h->arrays[ "sliceOffsets" ] = toStd( s.sliceOffsets );
+   h->arrays[ "sliceSegmentSizes" ] = toStd( s.sliceSegmentSizes );
+   h->arrays[ "sliced" ] = s.meta();
+   h->storage = s.getStorageSize();
+   return h;
+}
+
+template< Seg::ElementsOrganization O >
+Handle* buildSlicedOf( int slice, const IVec& caps )
+{
+   switch( slice ) {
+      case 1: return buildSliced< O, 1 >( caps );
+      case 8: return buildSliced< O, 8 >( caps );
+      case 16: return buildSliced< O, 16 >( caps );
+      case 32: return buildSliced< O, 32 >( caps );
+      case 64: return buildSliced< O, 64 >( caps );
+      case 128: return buildSliced< O, 128 >( caps );
+      case 256: return buildSliced< O, 256 >( caps );
+      default: return nullptr;
+   }
+}
+
 extern "C" {
+
+// SlicedEllpack< Host, int, Allocator, Organization, SliceSize > for SliceSize 1, 8, ..., 256
+void* ref_sliced_create( int n, const int* caps, int colmajor, int slice )
+{
+   IVec c( n );
+   for( int i = 0; i < n; i++ )
+      c[ i ] = caps[ i ];
+   return colmajor ? (void*) buildSlicedOf< Seg::ColumnMajorOrder >( slice, c )
+                   : (void*) buildSlicedOf< Seg::RowMajorOrder >( slice, c );
+}
 
 void* ref_segments_create( int kind, int n, const int* caps, int colmajor )
 {

@@ -356,3 +356,28 @@ def test_binary_product_golden_values_and_tnl():
     for alpha, beta, first, last in ((1, 0, 0, 0), (3, 0, 0, 0), (1, 2, 0, 0), (-2, 5, 50, 200)):
         np.testing.assert_array_equal(O.binary_spmv(rp, c, x, y0, alpha, beta, first, last),
                                       O.binary_spmv(rp, c, x, y0, alpha, beta, first, last, backend="ref"))
+
+
+@pytest.mark.parametrize("S", [1, 8, 16, 64, 128, 256])
+def test_sliced_slice_sizes(S):
+    """SlicedEllpack< ..., SliceSize > for other slice sizes than 32: per slice the largest
+    capacity, offsets the exclusive scan of SliceSize times it (SlicedEllpack.hpp:64-102), the
+    oracle equal to TNL's own build in both organizations."""
+    rng = np.random.default_rng(S)
+    for caps in (rng.integers(0, 9, 1000), np.zeros(37, np.int64), rng.integers(1, 40, 2 * S + 3),
+                 np.array([5], np.int64), np.array([], np.int64)):
+        caps = caps.astype(np.int32)
+        n = len(caps)
+        nsl = -(-n // S)
+        mx = np.array([caps[s * S:(s + 1) * S].max() for s in range(nsl)], np.int32)
+        off = np.concatenate([[0], np.cumsum(mx.astype(np.int64) * S)]).astype(np.int32)
+        for org in (True, False):
+            seg = O.Segments("SlicedEllpack", caps, column_major=org, slice_size=S)
+            assert np.array_equal(seg.get("sliceSegmentSizes"), mx)
+            assert np.array_equal(seg.get("sliceOffsets"), off)
+            assert seg.storage_size() == off[-1]
+            if O.ref_available():
+                ref = O.Segments("SlicedEllpack", caps, column_major=org, backend="ref", slice_size=S)
+                for name in ("sliceOffsets", "sliceSegmentSizes", "sliced"):
+                    assert np.array_equal(seg.get(name), ref.get(name)), (S, org, name)
+                assert seg.storage_size() == ref.storage_size()

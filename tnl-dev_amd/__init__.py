@@ -42,7 +42,7 @@ EXPORTS = [
     "tnlhip_matrix_get_row_capacities", "tnlhip_matrix_get_compressed_row_lengths", "tnlhip_matrix_get_row_capacity",
     "tnlhip_matrix_get_row", "tnlhip_matrix_get_view", "tnlhip_matrix_values_changed",
     "tnlhip_solver_get_history_iterations", "tnlhip_solver_set_monitor", "tnlhip_solver_set_residual_history_file",
-    "tnlhip_solver_set_preconditioner_matrix", "tnlhip_multidot_noalloc",
+    "tnlhip_solver_set_preconditioner_matrix", "tnlhip_multidot_noalloc", "tnlhip_spmv_multi",
 ]
 # element order of TNL binary files: "cuda" = column-major (TNL's GPU matrices), "host" = row-major
 FILE_LAYOUTS = {"cuda": 0, "host": 1}
@@ -110,7 +110,7 @@ class RefinementResult(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("format", C.c_int32), ("lanes", C.c_int32), ("blocks_per_cu", C.c_int32), ("pair_rows", C.c_int32),
-                ("seconds", C.c_double)]
+                ("slice_size", C.c_int32), ("seconds", C.c_double)]
 
 
 class ServerConfig(C.Structure):
@@ -163,6 +163,7 @@ def _load():
         "tnlhip_matrix_get_diagonal": (C.c_int, [vp, vp, vp]),
         "tnlhip_matrix_set_param": (C.c_int, [vp, C.c_char_p, i32]),
         "tnlhip_spmv": (C.c_int, [vp, vp, vp, f64, f64, i32, i32, vp]),
+        "tnlhip_spmv_multi": (C.c_int, [vp, i32, vp, vp, f64, f64, i32, i32, vp]),
         "tnlhip_dot": (C.c_int, [C.c_int, i64, vp, vp, C.POINTER(f64), vp]),
         "tnlhip_norm2": (C.c_int, [C.c_int, i64, vp, C.POINTER(f64), vp]),
         "tnlhip_axpby": (C.c_int, [C.c_int, i64, f64, vp, f64, vp, vp]),
@@ -490,6 +491,24 @@ class SparseMatrix:
         _check(lib.tnlhip_spmv(self._h, _ptr(inVector), _ptr(outVector), float(matrixMultiplicator),
                                float(outVectorMultiplicator), int(firstRow), int(lastRow), _stream(stream)))
         return outVector
+
+    def vectorProductMulti(self, inVectors, outVectors, matrixMultiplicator=1.0, outVectorMultiplicator=0.0,
+                           firstRow=0, lastRow=0, stream=None):
+        """vectorProduct for k right-hand sides in one pass over the matrix: inVectors (columns, k)
+        and outVectors (rows, k) are contiguous row-major device tensors, 1 <= k <= 64."""
+        dt = self.vector_dtype
+        if inVectors.dtype != dt or outVectors.dtype != dt:
+            raise TypeError(f"vectors must be {dt}")
+        if inVectors.dim() != 2 or outVectors.dim() != 2 or inVectors.shape[1] != outVectors.shape[1]:
+            raise ValueError("blocks of vectors are (size, k) tensors with the same k")
+        if not (inVectors.is_contiguous() and outVectors.is_contiguous()):
+            raise ValueError("blocks of vectors must be contiguous (row-major)")
+        if inVectors.shape[0] != self.info.columns or outVectors.shape[0] != self.info.rows:
+            raise ValueError("vector sizes do not match the matrix")
+        _check(lib.tnlhip_spmv_multi(self._h, int(inVectors.shape[1]), _ptr(inVectors), _ptr(outVectors),
+                                     float(matrixMultiplicator), float(outVectorMultiplicator), int(firstRow),
+                                     int(lastRow), _stream(stream)))
+        return outVectors
 
 
 def segments_metadata(format, capacities, name):
@@ -852,7 +871,7 @@ def autotune(row_pointers, column_indexes, values, dtype="f64", formats=None, re
 
     def as_dict(t):
         return {"format": FORMAT_NAMES[t.format], "lanes": t.lanes, "blocks_per_cu": t.blocks_per_cu,
-                "pair_rows": t.pair_rows, "seconds": t.seconds}
+                "pair_rows": t.pair_rows, "slice_size": t.slice_size, "seconds": t.seconds}
 
     return SparseMatrix(h), as_dict(best), [as_dict(allt[i]) for i in range(min(cnt.value, cap))]
 

@@ -29,8 +29,9 @@
 //
 // GMRES requests (nonsymmetric-capable, Solvers/Linear/GMRES.hpp:79-180) take a second track:
 // `gmres_workers` host threads, each with its own HIP stream, take them from a FIFO as they free
-// up and solve every right-hand side by FP64 refinement over an FP32 Jacobi GMRES(30) inner solve
-// (tnlhip_refine_solve) of (A + shift I), built on the device from the resident matrix.  Their
+// up and solve every right-hand side by FP64 refinement over an FP32 GMRES(30) inner solve
+// (tnlhip_refine_solve) of (A + shift I), built on the device from the resident matrix, with the
+// degree-4 Jacobi polynomial over a BF16 copy of it as the inner preconditioner.  Their
 // kernels overlap on the device with each other and with the batched CG launches; each result
 // depends only on its own request (deterministic reductions), never on the mix.
 #include "common.hpp"
@@ -835,8 +836,9 @@ void tnlhip_server_s::gmresWorker()
       (void) hipStreamDestroy( st );
 }
 
-// (A + shift I) x_j = b_j for every right-hand side j: FP64 refinement over an FP32 Jacobi
-// GMRES(30) inner solve; b and x are the request's interleaved host arrays (entry row * k + j).
+// (A + shift I) x_j = b_j for every right-hand side j: FP64 refinement over an FP32 GMRES(30)
+// inner solve (Jacobi polynomial over BF16 values); b and x are the request's interleaved host
+// arrays (entry row * k + j).
 tnlhip_request_result tnlhip_server_s::solveGmres( const Request& q, hipStream_t st, std::vector< double >& hb,
                                                    std::vector< double >& hx )
 {
@@ -877,8 +879,14 @@ tnlhip_request_result tnlhip_server_s::solveGmres( const Request& q, hipStream_t
       TNLHIP_CHECK( hipGetLastError() );
       TNLHIP_CHECK( hipStreamSynchronize( st ) );
       TNLHIP_CHECK_STATUS( tnlhip_matrix_convert( A64.m, TNLHIP_SLICED_ELLPACK, TNLHIP_F32, &A32.m ) );
+      // the inner GMRES is preconditioned by the Jacobi polynomial over a BF16 copy
+      MatrixHandle A16;
+      TNLHIP_CHECK_STATUS( tnlhip_matrix_convert( A64.m, TNLHIP_SLICED_ELLPACK, TNLHIP_BF16, &A16.m ) );
       tnlhip_refinement_params rp_{};
       tnlhip_refinement_params_default( &rp_ );
+      rp_.inner.preconditioner = 3;
+      rp_.inner.poly_degree = 4;
+      rp_.poly_matrix = A16.m;
       rp_.tolerance = q.params.tolerance;
       rp_.max_outer_iterations = q.params.max_outer_iterations;
       rp_.stagnation_limit = q.params.stagnation_limit;

@@ -165,12 +165,14 @@ tnlhip_matrix_s* buildFromSegments( int format, int dtype, int32_t rows, int32_t
    return M.release();
 }
 
-tnlhip_matrix_s* buildEmpty( int format, int dtype, int32_t rows, int32_t cols, const std::vector< int32_t >& caps )
+tnlhip_matrix_s* buildEmpty( int format, int dtype, int32_t rows, int32_t cols, const std::vector< int32_t >& caps,
+                             int32_t sliceSize = kSliceSize )
 {
    TNLHIP_REQUIRE( format >= TNLHIP_CSR_SCALAR && format <= TNLHIP_BIELLPACK, "unknown format" );
    TNLHIP_REQUIRE( dtype >= TNLHIP_F64 && dtype <= TNLHIP_BINARY, "unknown value type" );
    TNLHIP_REQUIRE( rows >= 0 && cols >= 0, "negative dimensions" );
    HostSegments meta;
+   meta.sliceSize = sliceSize;
    buildSegments( meta, kindOf( format ), rows, caps.data() );
    int64_t capTotal = 0;
    for( int32_t c : caps )
@@ -494,6 +496,39 @@ int tnlhip_matrix_create_symmetric_csr( int format, int value_dtype, int32_t row
    } );
 }
 
+}  // extern "C"
+
+namespace {
+
+// SlicedEllpack with another slice size (TNL's SliceSize template parameter): the same stored
+// entries, every one of them (explicit zeros included) in its row's order, laid out for slices of
+// S rows; row capacities become the stored entries per row, as operator= makes them.
+void reslice( Matrix& A, int32_t S )
+{
+   DeviceArray< int32_t > dcaps( A.rows );
+   storedCounts( A, dcaps.ptr, nullptr );
+   std::vector< int32_t > caps = dcaps.download();
+   std::unique_ptr< tnlhip_matrix_s > M( buildEmpty( A.format, A.dtype, A.rows, A.cols, caps, S ) );
+   std::vector< int32_t > rp( (size_t) A.rows + 1, 0 );
+   for( int32_t i = 0; i < A.rows; i++ )
+      rp[ i + 1 ] = rp[ i ] + caps[ i ];
+   DeviceArray< int32_t > drp, dcol( rp[ A.rows ] );
+   DeviceArray< double > dval( rp[ A.rows ] );
+   drp.upload( rp );
+   gatherToCsr( A, drp.ptr, dcol.ptr, dval.ptr, nullptr, true );
+   convertFromCsrDevice( *M, drp.ptr, dcol.ptr, dval.ptr, nullptr, true );
+   A.meta = std::move( M->meta );
+   A.values = std::move( M->values );
+   A.columns = std::move( M->columns );
+   A.sliceOffsets = std::move( M->sliceOffsets );
+   A.sliceSizes = std::move( M->sliceSizes );
+   A.diagonalValid = false;
+}
+
+}  // namespace
+
+extern "C" {
+
 int tnlhip_matrix_convert( tnlhip_matrix source, int format, int value_dtype, tnlhip_matrix* out )
 {
    return guarded( [ & ] {
@@ -594,6 +629,9 @@ int tnlhip_matrix_save( tnlhip_matrix A, const char* path, int layout )
       TNLHIP_REQUIRE( layout == TNLHIP_FILE_CUDA_LAYOUT || layout == TNLHIP_FILE_HOST_LAYOUT, "unknown file layout" );
       if( A->dtype == TNLHIP_BF16 || A->dtype == TNLHIP_BINARY )
          throw Error( TNLHIP_ERROR_UNSUPPORTED, "TNL files are written here for double or float values only" );
+      // the type string of a SlicedEllpack file does not name its slice size: TNL reads it with 32
+      if( kindOf( A->format ) == KIND_SLICED && A->meta.sliceSize != kSliceSize )
+         throw Error( TNLHIP_ERROR_UNSUPPORTED, "SlicedEllpack files hold slices of 32 rows (TNL's default SliceSize)" );
       const int64_t n = A->meta.allocated;
       std::vector< int32_t > cols( (size_t) n );
       std::vector< uint8_t > vals( (size_t) n * dtypeBytes( A->dtype ) );
@@ -728,6 +766,8 @@ int64_t tnlhip_matrix_get_metadata( tnlhip_matrix A, const char* name_c, int32_t
             return put( s.sliceSegmentSizes );
          if( name == "sliced" )
             return put( { s.slicedSize, (int32_t) s.storage } );
+         if( name == "sliceSize" )
+            return put( { s.sliceSize } );
          break;
       case KIND_CHUNKED:
          if( name == "rowToChunkMapping" )
@@ -850,6 +890,13 @@ int tnlhip_matrix_set_param( tnlhip_matrix A, const char* name_c, int32_t value 
       }
       else if( name == "column_stripes" )
          buildStripePlan( *A, value, nullptr );
+      else if( name == "slice_size" ) {
+         TNLHIP_REQUIRE( kindOf( A->format ) == KIND_SLICED, "slice_size applies to SlicedEllpack matrices" );
+         TNLHIP_REQUIRE( value >= 1 && value <= 256 && ( value & ( value - 1 ) ) == 0,
+                         "slice_size is a power of two in [1, 256]" );
+         if( value != A->meta.sliceSize )
+            reslice( *A, value );
+      }
       else if( name == "pair_rows" )
          A->pairRows = value != 0;
       else if( name == "blocks_per_cu" ) {
@@ -871,6 +918,19 @@ int tnlhip_spmv( tnlhip_matrix A, const void* x, void* y, double alpha, double b
          last = A->rows;
       TNLHIP_REQUIRE( first >= 0 && first <= last && last <= A->rows, "row range out of bounds" );
       spmvLaunch( *A, x, y, alpha, beta, first, last, nullptr, (hipStream_t) stream );
+   } );
+}
+
+int tnlhip_spmv_multi( tnlhip_matrix A, int32_t k, const void* X, void* Y, double alpha, double beta, int32_t first,
+                       int32_t last, void* stream )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( A, "NULL matrix" );
+      TNLHIP_REQUIRE( ( X && Y ) || A->rows == 0, "NULL block of vectors" );
+      if( last == 0 )
+         last = A->rows;
+      TNLHIP_REQUIRE( first >= 0 && first <= last && last <= A->rows, "row range out of bounds" );
+      spmvMultiLaunch( *A, k, X, Y, alpha, beta, first, last, (hipStream_t) stream );
    } );
 }
 

@@ -50,9 +50,10 @@ struct HostSegments
    std::vector< int32_t > offsets;
    // Ellpack
    int32_t segmentSize = 0, alignedSize = 0;
-   // SlicedEllpack
+   // SlicedEllpack (sliceSize: TNL's SliceSize template parameter, 32 by default; a power of two)
    std::vector< int32_t > sliceOffsets, sliceSegmentSizes;
    int32_t slicedSize = 0;
+   int32_t sliceSize = kSliceSize;
    // ChunkedEllpack
    int32_t numberOfSlices = 0;
    std::vector< int32_t > rowToChunk, rowToSlice, chunksToSegments, rowPointers, slices;  // slices: 4 per slice
@@ -128,6 +129,11 @@ void spmvLaunch( const Matrix& A, const void* x, void* y, double alpha, double b
 // with gateMode 2, unless gate[1] != 0.  last < 0 means all rows.
 void spmvLaunchGated( const Matrix& A, const void* x, void* y, double alpha, double beta, const void* diag,
                       const int32_t* gate, int gateMode, hipStream_t stream, int32_t first = 0, int32_t last = -1 );
+// Y = alpha A X + beta Y for k <= kMaxMultiRhs right-hand sides, X and Y row-major (entry
+// row * k + j) over rows [first, last) (multi.hip)
+constexpr int32_t kMaxMultiRhs = 64;
+void spmvMultiLaunch( const Matrix& A, int32_t k, const void* X, void* Y, double alpha, double beta, int32_t first,
+                      int32_t last, hipStream_t stream );
 // One Jacobi step of the splitting of P: znew = z + ( rhs - P z ) / diag over all rows, gated
 // as above (general real-valued P only; znew aliases neither z nor rhs)
 void spmvJacobiStepGated( const Matrix& P, const void* z, void* znew, const void* rhs, const void* diag,

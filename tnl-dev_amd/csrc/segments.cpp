@@ -44,12 +44,13 @@ void ellpack( HostSegments& s, const int32_t* caps )
 
 void sliced( HostSegments& s, const int32_t* caps )
 {
-   const int32_t nslices = ceilDiv( s.n, kSliceSize );
+   const int32_t S = s.sliceSize;
+   const int32_t nslices = ceilDiv( s.n, S );
    s.sliceOffsets.assign( (size_t) nslices + 1, 0 );
    s.sliceSegmentSizes.assign( nslices, 0 );
    int64_t off = 0, total = 0;
    for( int32_t sl = 0; sl < nslices; sl++ ) {
-      const int32_t b = sl * kSliceSize, e = std::min( b + kSliceSize, s.n );
+      const int32_t b = sl * S, e = std::min( b + S, s.n );
       int32_t mx = 0;
       for( int32_t i = b; i < e; i++ ) {
          mx = std::max( mx, caps[ i ] );
@@ -57,7 +58,7 @@ void sliced( HostSegments& s, const int32_t* caps )
       }
       s.sliceSegmentSizes[ sl ] = mx;
       s.sliceOffsets[ sl ] = (int32_t) off;
-      off += (int64_t) mx * kSliceSize;
+      off += (int64_t) mx * S;
    }
    if( off > std::numeric_limits< int32_t >::max() )
       throw Error( TNLHIP_ERROR_CAPACITY, "SlicedEllpack storage exceeds the int32 index range" );

@@ -524,9 +524,13 @@ struct SlicedSlots
 {
    const int32_t* sliceOffsets;
    const int32_t* sliceSizes;
-   __device__ int64_t base( int64_t r ) const { return sliceOffsets[ r / kSliceSize ] + r % kSliceSize; }
-   __device__ int32_t width( int64_t r ) const { return sliceSizes[ r / kSliceSize ]; }
-   __device__ int64_t step() const { return kSliceSize; }
+   int32_t logSlice;  // log2 of the slice size (TNL's SliceSize, 32 by default)
+   __device__ int64_t base( int64_t r ) const
+   {
+      return sliceOffsets[ r >> logSlice ] + ( r & ( ( 1 << logSlice ) - 1 ) );
+   }
+   __device__ int32_t width( int64_t r ) const { return sliceSizes[ r >> logSlice ]; }
+   __device__ int64_t step() const { return (int64_t) 1 << logSlice; }
 };
 
 template< bool PAIR, typename V, typename A, typename Y >
@@ -829,7 +833,7 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
          const unsigned grid = (unsigned) pl.grid;
          const int64_t per = pl.per;
          const EllpackSlots es{ M.meta.segmentSize, M.meta.alignedSize };
-         const SlicedSlots ss{ M.sliceOffsets.ptr, M.sliceSizes.ptr };
+         const SlicedSlots ss{ M.sliceOffsets.ptr, M.sliceSizes.ptr, log2Exact( M.meta.sliceSize ) };
          if( M.format == TNLHIP_ELLPACK ) {
             if( pair )
                ellpackKernel< true, V, A, Y ><<< grid, 256, 0, st >>>( es, col, val, x, keep, first, last, (int32_t) per,

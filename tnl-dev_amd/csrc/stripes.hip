@@ -12,7 +12,9 @@
 //   - each (stripe, row) pair yields one partial sum, stored stripe-major in `partial`, and a
 //     second kernel adds a row's S partials in stripe order and applies the keeper.
 // The row's elements are summed left to right within each stripe and the stripe sums in stripe
-// order: a fixed order (the same bits on every call), a reassociation of the Host row sum.
+// order: a fixed order (the same bits on every call), a reassociation of the Host row sum (a row's
+// entries within a stripe are added in runs of at most 8 (FP64) / 16 (FP32) consecutive products,
+// the runs left to right).
 // Reference semantics: SparseMatrixView::vectorProduct (Matrices/SparseMatrixView.hpp:390-491);
 // the storage of the matrix is untouched (the plan is a second, reordered copy).
 #include "blas.hpp"
@@ -118,9 +120,18 @@ __device__ __forceinline__ StripeTile stripeTile( int S, int64_t blocksPerStripe
 }
 
 // One wavefront per kStripeWaveRows rows of one stripe.  The wave's entries are taken in
-// chunks of CH: coalesced non-temporal loads (kStageLoads per lane per round), the x gathers
-// (resident in the XCD's L2), the rounded products in the wave's LDS slice; then each lane adds
-// the part of each of its rows that lies in the chunk, left to right.
+// chunks of CH: coalesced non-temporal loads (kLoads per lane per round), the x gathers
+// (resident in the XCD's L2), the rounded products in the wave's LDS slice.  A row's part of a
+// chunk is then summed in two balanced steps instead of by its own lane alone (rows of a
+// lognormal matrix are hundreds of entries long beside rows of one, and one lane walking a long
+// row would hold up its whole wavefront):
+//   1. lane l adds the P = CH / 64 products at positions [P l, P l + P) left to right, cutting
+//      a run wherever a row starts (a flag its owner set); each run's sum replaces the product
+//      at the run's first position;
+//   2. the owner of each row adds, in position order, the run starting at the row's first
+//      position in the chunk and the runs starting at the lane boundaries P m inside the row.
+// Fixed positions and orders: the same bits on every call.  Positions are padded by one slot
+// per P in LDS (index p + p / P), so the P-strided reads of step 1 spread over the banks.
 template< int CH, typename V, typename A, typename Y >
 __global__ __launch_bounds__( 256 ) void stripeProductKernel( int S, int64_t blocksPerStripe, int32_t n,
                                                               int32_t first, int32_t last,
@@ -130,10 +141,13 @@ __global__ __launch_bounds__( 256 ) void stripeProductKernel( int S, int64_t blo
                                                               A* __restrict__ partial, Gate gate )
 {
    constexpr int kLoads = 8;
+   constexpr int P = CH / kWave;  // products per lane in step 1
+   constexpr int CHP = CH + CH / P;
    static_assert( CH % ( kLoads * kWave ) == 0, "chunk must be whole rounds of loads" );
    if( ! gate.open() )
       return;
-   __shared__ A lds[ 4 ][ CH ];
+   __shared__ A lds[ 4 ][ CHP ];
+   __shared__ uint8_t startsRow[ 4 ][ CH ];
    const StripeTile t = stripeTile( S, blocksPerStripe );
    if( t.block >= blocksPerStripe )
       return;  // uniform over the workgroup; the kernel has no workgroup barrier
@@ -156,8 +170,18 @@ __global__ __launch_bounds__( 256 ) void stripeProductKernel( int S, int64_t blo
 #pragma unroll
    for( int q = 0; q < kStripeRowsPerLane; q++ )
       sum[ q ] = 0;
+   A* L = lds[ wave ];
+   uint8_t* F = startsRow[ wave ];
+#pragma unroll
+   for( int i = 0; i < P; i++ )
+      F[ lane * P + i ] = 0;
    for( int32_t c0 = b; c0 < e; c0 += CH ) {  // uniform over the wave
       const int32_t size = min( CH, e - c0 );
+      // row starts inside this chunk (rows with entries only: their starts are distinct)
+#pragma unroll
+      for( int q = 0; q < kStripeRowsPerLane; q++ )
+         if( re[ q ] > rb[ q ] && rb[ q ] >= c0 && rb[ q ] < c0 + size )
+            F[ rb[ q ] - c0 ] = 1;
 #pragma unroll
       for( int u = 0; u < CH / ( kLoads * kWave ); u++ ) {
          const int32_t g0 = u * kLoads * kWave;
@@ -179,16 +203,52 @@ __global__ __launch_bounds__( 256 ) void stripeProductKernel( int S, int64_t blo
          for( int k = 0; k < kLoads; k++ ) {
             const int32_t g = g0 + lane + k * kWave;
             if( g < size )
-               lds[ wave ][ g ] = toCompute( v[ k ], A() ) * (A) xv[ k ];
+               L[ g + g / P ] = toCompute( v[ k ], A() ) * (A) xv[ k ];
          }
       }
       waveSync();
+      // step 1: runs of the lane's P positions, cut at row starts
+      const int32_t p0 = lane * P;
+      if( p0 < size ) {
+         A pr[ P ];
+         uint8_t fl[ P ];
+#pragma unroll
+         for( int i = 0; i < P; i++ ) {
+            pr[ i ] = L[ p0 + i + lane ];
+            fl[ i ] = F[ p0 + i ];
+         }
+         A acc = pr[ 0 ];
+         int at = 0;
+#pragma unroll
+         for( int i = 1; i < P; i++ ) {
+            const bool inChunk = p0 + i < size;
+            if( inChunk && fl[ i ] ) {
+               L[ p0 + at + lane ] = acc;
+               acc = pr[ i ];
+               at = i;
+            }
+            else if( inChunk )
+               acc += pr[ i ];
+         }
+         L[ p0 + at + lane ] = acc;
+      }
+      waveSync();
+      // step 2: each row adds its runs in position order
 #pragma unroll
       for( int q = 0; q < kStripeRowsPerLane; q++ ) {
          const int32_t lo = max( rb[ q ], c0 ) - c0, hi = min( re[ q ], c0 + size ) - c0;
-         for( int32_t k = lo; k < hi; k++ )
-            sum[ q ] += lds[ wave ][ k ];
+         if( lo < hi ) {
+            A s2 = L[ lo + lo / P ];
+            for( int32_t m = lo / P + 1; m * P < hi; m++ )
+               s2 += L[ m * P + m ];
+            sum[ q ] += s2;
+         }
       }
+      // the flags this lane set are cleared before the next chunk sets its own
+#pragma unroll
+      for( int q = 0; q < kStripeRowsPerLane; q++ )
+         if( re[ q ] > rb[ q ] && rb[ q ] >= c0 && rb[ q ] < c0 + size )
+            F[ rb[ q ] - c0 ] = 0;
       waveSync();
    }
 #pragma unroll

@@ -114,7 +114,7 @@ int32_t rowSize( const Matrix& A, int32_t row )
    switch( m.kind ) {
       case KIND_CSR: return m.offsets[ row + 1 ] - m.offsets[ row ];
       case KIND_ELLPACK: return m.segmentSize;
-      case KIND_SLICED: return m.sliceSegmentSizes[ row / kSliceSize ];
+      case KIND_SLICED: return m.sliceSegmentSizes[ row / m.sliceSize ];
       default: {
          // ChunkedEllpack / BiEllpack: the device index map's own count
          DeviceArray< int32_t > d( 1 );

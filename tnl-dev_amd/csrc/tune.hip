@@ -1,5 +1,6 @@
-// Autotuner: picks the segment format and kernel launch parameters (lanes per row for the CSR
-// light / hybrid kernels, rows per lane and workgroups per CU for the padded kernels) for one
+// Autotuner: picks the segment format, the slice size of SlicedEllpack and kernel launch
+// parameters (lanes per row for the CSR light / hybrid kernels, rows per lane and workgroups
+// per CU for the padded kernels) for one
 // matrix by timing y = A x on the device.  No counterpart in the reference: TNL fixes the
 // format at compile time (SparseMatrix's Segments template parameter) and the benchmark
 // (Benchmarks/SpMV/spmv.h:269-345) times each format separately.
@@ -16,6 +17,7 @@ struct Candidate
    int lanes;      // 0: kernel default
    int blocksPerCu;  // 0: default
    int pairRows;   // -1: default
+   int sliceSize;  // SlicedEllpack: 16, 32 (TNL's default), 64; 0 for other formats
 };
 
 std::vector< Candidate > candidatesFor( int format )
@@ -24,18 +26,24 @@ std::vector< Candidate > candidatesFor( int format )
    switch( format ) {
       case TNLHIP_CSR_LIGHT:
       case TNLHIP_CSR_HYBRID:
-         c.push_back( { format, 0, 0, -1 } );
+         c.push_back( { format, 0, 0, -1, 0 } );
          for( int l : { 2, 4, 8, 16, 32 } )
-            c.push_back( { format, l, 0, -1 } );
+            c.push_back( { format, l, 0, -1, 0 } );
          break;
       case TNLHIP_ELLPACK:
-      case TNLHIP_SLICED_ELLPACK:
          for( int p : { 0, 1 } )
             for( int b : { 8, 16 } )
-               c.push_back( { format, 0, b, p } );
+               c.push_back( { format, 0, b, p, 0 } );
+         break;
+      case TNLHIP_SLICED_ELLPACK:
+         // slice sizes outermost: a change of slice size re-lays the storage out
+         for( int s : { 32, 16, 64 } )
+            for( int p : { 0, 1 } )
+               for( int b : { 8, 16 } )
+                  c.push_back( { format, 0, b, p, s } );
          break;
       default:
-         c.push_back( { format, 0, 0, -1 } );
+         c.push_back( { format, 0, 0, -1, 0 } );
    }
    return c;
 }
@@ -50,6 +58,8 @@ void configure( tnlhip_matrix A, const Candidate& c )
       TNLHIP_CHECK_STATUS( tnlhip_matrix_set_param( A, "lanes_per_row", c.lanes ) );
    TNLHIP_CHECK_STATUS( tnlhip_matrix_set_param( A, "blocks_per_cu", c.blocksPerCu > 0 ? c.blocksPerCu : 16 ) );
    TNLHIP_CHECK_STATUS( tnlhip_matrix_set_param( A, "pair_rows", c.pairRows > 0 ? 1 : 0 ) );
+   if( c.format == TNLHIP_SLICED_ELLPACK )
+      TNLHIP_CHECK_STATUS( tnlhip_matrix_set_param( A, "slice_size", c.sliceSize > 0 ? c.sliceSize : 32 ) );
 }
 
 }  // namespace
@@ -106,7 +116,7 @@ int tnlhip_autotune_csr( int value_dtype, int32_t rows, int32_t columns, const i
             TNLHIP_CHECK( hipEventSynchronize( e1 ) );
             float ms = 0;
             TNLHIP_CHECK( hipEventElapsedTime( &ms, e0, e1 ) );
-            tnlhip_tuning t{ format, c.lanes, c.blocksPerCu, c.pairRows, 1e-3 * ms / repetitions };
+            tnlhip_tuning t{ format, c.lanes, c.blocksPerCu, c.pairRows, c.sliceSize, 1e-3 * ms / repetitions };
             if( all && count < all_capacity )
                all[ count ] = t;
             count++;
@@ -123,7 +133,7 @@ int tnlhip_autotune_csr( int value_dtype, int32_t rows, int32_t columns, const i
          if( kept )
             // leave the kept matrix configured exactly as its best candidate (format defaults
             // restored for the knobs the candidate leaves at their default)
-            configure( A, Candidate{ best->format, best->lanes, best->blocks_per_cu, best->pair_rows } );
+            configure( A, Candidate{ best->format, best->lanes, best->blocks_per_cu, best->pair_rows, best->slice_size } );
          else
             tnlhip_matrix_destroy( A );
       }

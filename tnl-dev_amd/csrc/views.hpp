@@ -3,7 +3,8 @@
 // segment has.  Column-major organization throughout (TNL's GPU default).
 //   CSR            offsets[row] + local                            (CSRView::getGlobalIndex)
 //   Ellpack        row + alignedSize * local                       (EllpackView.hpp:246-252)
-//   SlicedEllpack  sliceOffsets[row/32] + row%32 + 32*local         (SlicedEllpackView.hpp:108-132)
+//   SlicedEllpack  sliceOffsets[row/S] + row%S + S*local, S the slice size (32 by default)
+//                                                                  (SlicedEllpackView.hpp:108-132)
 //   ChunkedEllpack pointer + (local % chunkSize)*256 + firstChunk + local / chunkSize
 //                                                                  (detail/ChunkedEllpack.h:110-145)
 //   BiEllpack      groupPointer + rank + local'*height, walking the strip's groups
@@ -32,12 +33,21 @@ struct SlicedView
 {
    const int32_t* sliceOffsets;
    const int32_t* sliceSizes;
-   __device__ int32_t size( int32_t r ) const { return sliceSizes[ r / kSliceSize ]; }
+   int32_t logSlice;  // log2 of the slice size
+   __device__ int32_t size( int32_t r ) const { return sliceSizes[ r >> logSlice ]; }
    __device__ int64_t global( int32_t r, int32_t l ) const
    {
-      return sliceOffsets[ r / kSliceSize ] + r % kSliceSize + (int64_t) kSliceSize * l;
+      return sliceOffsets[ r >> logSlice ] + ( r & ( ( 1 << logSlice ) - 1 ) ) + ( (int64_t) l << logSlice );
    }
 };
+
+inline int32_t log2Exact( int32_t v )
+{
+   int32_t k = 0;
+   while( ( 1 << k ) < v )
+      k++;
+   return k;
+}
 
 struct ChunkedView
 {
@@ -107,7 +117,7 @@ void withView( const Matrix& M, F&& f )
    switch( kindOf( M.format ) ) {
       case KIND_CSR: f( CsrView{ M.offsets.ptr } ); break;
       case KIND_ELLPACK: f( EllpackView{ M.meta.segmentSize, M.meta.alignedSize } ); break;
-      case KIND_SLICED: f( SlicedView{ M.sliceOffsets.ptr, M.sliceSizes.ptr } ); break;
+      case KIND_SLICED: f( SlicedView{ M.sliceOffsets.ptr, M.sliceSizes.ptr, log2Exact( M.meta.sliceSize ) } ); break;
       case KIND_CHUNKED: f( ChunkedView{ M.rowToChunk.ptr, M.rowToSlice.ptr, M.slices.ptr } ); break;
       case KIND_BIELLPACK: f( BiView{ M.rowPerm.ptr, M.groupPointers.ptr } ); break;
    }
