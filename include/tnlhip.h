@@ -514,7 +514,8 @@ int tnlhip_refine_solve( tnlhip_matrix A, tnlhip_matrix A_low, const tnlhip_refi
 /* One timed configuration: format, lanes per row (CSR light / hybrid; 0 = the kernel's
  * choice), workgroups per CU and rows per lane (padded formats; 0 / -1 = defaults), the slice
  * size of SlicedEllpack (TNL's SliceSize template parameter: 32 is TNL's default and keeps its
- * metadata; 16 and 64 are tried too, 0 for other formats), seconds per y = A x. */
+ * metadata; 16 and 64 are tried too, 0 for other formats), the column stripes of CSRAdaptive's
+ * striped plan (0, and 4 and 8 when x exceeds 4 MiB, one XCD's L2), seconds per y = A x. */
 typedef struct
 {
    int32_t format;
@@ -522,6 +523,7 @@ typedef struct
    int32_t blocks_per_cu;
    int32_t pair_rows;
    int32_t slice_size;
+   int32_t column_stripes;
    double seconds;
 } tnlhip_tuning;
 

@@ -144,3 +144,12 @@ def test_convection_diffusion_128_refinement(tnl, dev):
     true = np.linalg.norm(b - O.Segments("CSR", np.diff(rp)).spmv(v, c, x)) / np.linalg.norm(b)
     print(f"refinement: {res}, true residual {true:.3e}")
     assert res["converged"] and true < 1e-10, (res, true)
+    # the same with the degree-4 Jacobi polynomial over BF16 values preconditioning the FP32 GMRES
+    xp = torch.zeros(n, dtype=torch.float64, device=dev)
+    res = tnl.refine_solve(A, torch.tensor(b, device=dev), xp, tolerance=1e-10, inner_method="gmres",
+                           inner_dtype="f32", poly_dtype="bf16", preconditioner=3, poly_degree=4)
+    xpn = xp.cpu().numpy()
+    true = np.linalg.norm(b - O.Segments("CSR", np.diff(rp)).spmv(v, c, xpn)) / np.linalg.norm(b)
+    print(f"BF16-polynomial refinement: {res}, true residual {true:.3e}")
+    assert res["converged"] and true < 1e-10, (res, true)
+    assert np.linalg.norm(xpn - x) / np.linalg.norm(x) < 1e-7

@@ -110,7 +110,7 @@ class RefinementResult(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("format", C.c_int32), ("lanes", C.c_int32), ("blocks_per_cu", C.c_int32), ("pair_rows", C.c_int32),
-                ("slice_size", C.c_int32), ("seconds", C.c_double)]
+                ("slice_size", C.c_int32), ("column_stripes", C.c_int32), ("seconds", C.c_double)]
 
 
 class ServerConfig(C.Structure):
@@ -871,7 +871,8 @@ def autotune(row_pointers, column_indexes, values, dtype="f64", formats=None, re
 
     def as_dict(t):
         return {"format": FORMAT_NAMES[t.format], "lanes": t.lanes, "blocks_per_cu": t.blocks_per_cu,
-                "pair_rows": t.pair_rows, "slice_size": t.slice_size, "seconds": t.seconds}
+                "pair_rows": t.pair_rows, "slice_size": t.slice_size, "column_stripes": t.column_stripes,
+                "seconds": t.seconds}
 
     return SparseMatrix(h), as_dict(best), [as_dict(allt[i]) for i in range(min(cnt.value, cap))]
 

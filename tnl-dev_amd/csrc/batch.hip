@@ -28,14 +28,19 @@
 //   restart      RESTART  rr = r / |r|, d = 0, p = rr, (rr, rr)
 //
 // GMRES requests (nonsymmetric-capable, Solvers/Linear/GMRES.hpp:79-180) take a second track:
-// `gmres_workers` host threads, each with its own HIP stream, take them from a FIFO as they free
-// up and solve every right-hand side by FP64 refinement over an FP32 GMRES(30) inner solve
-// (tnlhip_refine_solve) of (A + shift I), built on the device from the resident matrix, with the
-// degree-4 Jacobi polynomial over a BF16 copy of it as the inner preconditioner.  Their
+// each right-hand side is a job in a FIFO that `gmres_workers` host threads, each with its own
+// non-blocking HIP stream, take as they free up, so the k right-hand sides of a request run side
+// by side.  A job is FP64 refinement over an FP32 GMRES(30) inner solve of (A + shift I) with the
+// degree-4 Jacobi polynomial over BF16 values as the inner preconditioner.  Each worker keeps,
+// per resident matrix, FP64 / FP32 / BF16 CSR copies of A with the layout of the resident FP64
+// CSR copy; a job only rewrites their diagonal entries to a + shift (rounded from the FP64
+// value), and every buffer grows and stays, so after the first job on a size nothing is
+// allocated or freed (a hipFree waits for the whole device, CG launches included).  The jobs'
 // kernels overlap on the device with each other and with the batched CG launches; each result
 // depends only on its own request (deterministic reductions), never on the mix.
 #include "common.hpp"
 #include "matrix.hpp"
+#include "refine.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -670,6 +675,7 @@ struct ResidentMatrix
    std::shared_ptr< const std::vector< int32_t > > rowPointers, columns;
    std::shared_ptr< const std::vector< double > > values;
    std::shared_ptr< tnlhip_matrix_s > base;
+   uint64_t generation = 0;  // distinct for every add_matrix: keys the workers' cached operators
 };
 
 struct Request
@@ -684,16 +690,28 @@ struct Request
    double submitted = 0;
 };
 
-// a + shift on every stored diagonal entry of a CSR matrix (the shifted operator of a request)
-__global__ __launch_bounds__( 256 ) void shiftDiagonalKernel( int32_t n, const int32_t* __restrict__ off,
-                                                              const int32_t* __restrict__ col, double* __restrict__ val,
-                                                              double shift )
+// val[k] = (V)( base[k] + shift ) on every stored diagonal entry of a CSR matrix laid out as the
+// FP64 CSR `base` (the shifted operator of a request; off-diagonal entries are left as they are)
+template< typename V >
+__device__ __forceinline__ V roundTo( double v );
+template<>
+__device__ __forceinline__ double roundTo< double >( double v ) { return v; }
+template<>
+__device__ __forceinline__ float roundTo< float >( double v ) { return (float) v; }
+template<>
+__device__ __forceinline__ bf16 roundTo< bf16 >( double v ) { return __float2bfloat16( (float) v ); }
+
+template< typename V >
+__global__ __launch_bounds__( 256 ) void setShiftedDiagonalKernel( int32_t n, const int32_t* __restrict__ off,
+                                                                   const int32_t* __restrict__ col,
+                                                                   const double* __restrict__ base,
+                                                                   V* __restrict__ val, double shift )
 {
    const int64_t stride = (int64_t) gridDim.x * blockDim.x;
    for( int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride )
       for( int32_t k = off[ r ]; k < off[ r + 1 ]; k++ )
          if( col[ k ] == r )
-            val[ k ] += shift;
+            val[ k ] = roundTo< V >( base[ k ] + shift );
 }
 
 }  // namespace
@@ -706,11 +724,46 @@ using namespace tnlhip;
 struct MatrixHandle
 {
    tnlhip_matrix m = nullptr;
+   MatrixHandle() = default;
+   MatrixHandle( const MatrixHandle& ) = delete;
+   MatrixHandle& operator=( const MatrixHandle& ) = delete;
    ~MatrixHandle()
    {
       if( m )
          tnlhip_matrix_destroy( m );
    }
+};
+
+// A GMRES request on the worker track: its right-hand sides are separate jobs.
+struct GmresTask
+{
+   Request q;
+   int remaining = 0;     // jobs not finished
+   bool started = false;  // a job was taken
+   tnlhip_request_result out{};
+};
+
+struct GmresJob
+{
+   std::shared_ptr< GmresTask > task;
+   int j = 0;  // right-hand side
+};
+
+// One worker's copies of a resident matrix: FP64, FP32 and BF16 CSR with the resident FP64 CSR
+// copy's layout, their diagonal set for `shift`.
+struct GmresOperators
+{
+   MatrixHandle A64, A32, A16;
+   double shift = std::nan( "" );
+};
+
+// Everything a GMRES worker keeps between jobs.
+struct GmresWorkerState
+{
+   RefineContext ctx;
+   DeviceArray< double > db, dx;
+   std::vector< double > hb, hx;
+   std::map< uint64_t, std::unique_ptr< GmresOperators > > ops;  // by resident-matrix generation
 };
 
 struct tnlhip_server_s
@@ -745,19 +798,20 @@ struct tnlhip_server_s
    hipGraph_t graph = nullptr;
    hipGraphExec_t exec = nullptr;
    int64_t launches = 0;
-   // GMRES track
+   // GMRES track: one job per right-hand side (under gmresMutex)
    int device = 0;
    std::vector< std::thread > workers;
    std::mutex gmresMutex;
    std::condition_variable gmresWake;
-   std::deque< Request > gmresQueue;
+   std::deque< GmresJob > gmresQueue;
    std::vector< std::pair< Request, tnlhip_request_result > > gmresDone;
-   int gmresBusy = 0;
+   int gmresBusy = 0;          // requests with a job taken and not all finished
+   int gmresWaiting = 0;       // requests with no job taken yet
+   uint64_t nextGeneration = 1;
    bool stopping = false;
 
    void gmresWorker();
-   tnlhip_request_result solveGmres( const Request& q, hipStream_t st, std::vector< double >& hb,
-                                     std::vector< double >& hx );
+   tnlhip_request_result solveGmresRhs( const Request& q, int j, hipStream_t st, GmresWorkerState& w );
 
    Arena arena() const
    {
@@ -806,47 +860,62 @@ struct tnlhip_server_s
    }
 };
 
-// One GMRES worker: its own stream, requests in FIFO order until the server stops.
+// One GMRES worker: its own non-blocking stream, jobs (right-hand sides) in FIFO order until the
+// server stops.
 void tnlhip_server_s::gmresWorker()
 {
    (void) hipSetDevice( device );
    hipStream_t st = nullptr;
-   if( hipStreamCreate( &st ) != hipSuccess )
+   if( hipStreamCreateWithFlags( &st, hipStreamNonBlocking ) != hipSuccess )
       st = nullptr;
-   std::vector< double > hb, hx;
-   for( ;; ) {
-      Request q;
-      {
-         std::unique_lock< std::mutex > lock( gmresMutex );
-         gmresWake.wait( lock, [ & ] { return stopping || ! gmresQueue.empty(); } );
-         if( gmresQueue.empty() )
-            break;
-         q = gmresQueue.front();
-         gmresQueue.pop_front();
-         gmresBusy++;
-      }
-      tnlhip_request_result r = solveGmres( q, st, hb, hx );
-      {
-         std::lock_guard< std::mutex > lock( gmresMutex );
-         gmresDone.emplace_back( q, r );
-         gmresBusy--;
+   {
+      GmresWorkerState w;
+      for( ;; ) {
+         GmresJob job;
+         {
+            std::unique_lock< std::mutex > lock( gmresMutex );
+            gmresWake.wait( lock, [ & ] { return stopping || ! gmresQueue.empty(); } );
+            if( gmresQueue.empty() )
+               break;
+            job = gmresQueue.front();
+            gmresQueue.pop_front();
+            if( ! job.task->started ) {
+               job.task->started = true;
+               gmresWaiting--;
+               gmresBusy++;
+            }
+         }
+         const tnlhip_request_result r = solveGmresRhs( job.task->q, job.j, st, w );
+         {
+            std::lock_guard< std::mutex > lock( gmresMutex );
+            GmresTask& t = *job.task;
+            t.out.converged = t.out.converged && r.converged;
+            t.out.outer_iterations = std::max( t.out.outer_iterations, r.outer_iterations );
+            t.out.inner_iterations += r.inner_iterations;
+            t.out.residue = std::isnan( r.residue ) || std::isnan( t.out.residue ) ? std::nan( "" )
+                                                                                   : std::max( t.out.residue, r.residue );
+            if( --t.remaining == 0 ) {
+               gmresDone.emplace_back( t.q, t.out );
+               gmresBusy--;
+            }
+         }
       }
    }
    if( st )
       (void) hipStreamDestroy( st );
 }
 
-// (A + shift I) x_j = b_j for every right-hand side j: FP64 refinement over an FP32 GMRES(30)
-// inner solve (Jacobi polynomial over BF16 values); b and x are the request's interleaved host
-// arrays (entry row * k + j).
-tnlhip_request_result tnlhip_server_s::solveGmres( const Request& q, hipStream_t st, std::vector< double >& hb,
-                                                   std::vector< double >& hx )
+// (A + shift I) x_j = b_j for right-hand side j: FP64 refinement over an FP32 GMRES(30) inner solve
+// (Jacobi polynomial over BF16 values); b and x are the request's interleaved host arrays (entry
+// row * k + j).
+tnlhip_request_result tnlhip_server_s::solveGmresRhs( const Request& q, int j, hipStream_t st, GmresWorkerState& w )
 {
    tnlhip_request_result out{};
    out.converged = 1;
    const int status = guarded( [ & ] {
       std::shared_ptr< tnlhip_matrix_s > base;
       int32_t n = 0;
+      uint64_t generation = 0;
       std::shared_ptr< const std::vector< int32_t > > rp, ci;
       std::shared_ptr< const std::vector< double > > va;
       {
@@ -857,6 +926,7 @@ tnlhip_request_result tnlhip_server_s::solveGmres( const Request& q, hipStream_t
          ci = M.columns;
          va = M.values;
          base = M.base;
+         generation = M.generation;
       }
       if( ! base ) {
          // built outside the lock (an upload); a second worker may build it too, one copy is kept
@@ -870,47 +940,73 @@ tnlhip_request_result tnlhip_server_s::solveGmres( const Request& q, hipStream_t
             M.base = built;
          base = M.base;
       }
-      // the request's operator: a device copy of A with the shift on the diagonal, and its FP32 copy
-      MatrixHandle A64, A32;
-      TNLHIP_CHECK_STATUS( tnlhip_matrix_convert( base.get(), TNLHIP_CSR_SCALAR, TNLHIP_F64, &A64.m ) );
-      Matrix& S64 = *reinterpret_cast< Matrix* >( A64.m );
-      shiftDiagonalKernel<<< gridFor( n, 256 ), 256, 0, st >>>( n, S64.offsets.ptr, S64.columns.ptr,
-                                                               reinterpret_cast< double* >( S64.values.ptr ), q.shift );
-      TNLHIP_CHECK( hipGetLastError() );
-      TNLHIP_CHECK( hipStreamSynchronize( st ) );
-      TNLHIP_CHECK_STATUS( tnlhip_matrix_convert( A64.m, TNLHIP_SLICED_ELLPACK, TNLHIP_F32, &A32.m ) );
-      // the inner GMRES is preconditioned by the Jacobi polynomial over a BF16 copy
-      MatrixHandle A16;
-      TNLHIP_CHECK_STATUS( tnlhip_matrix_convert( A64.m, TNLHIP_SLICED_ELLPACK, TNLHIP_BF16, &A16.m ) );
+      const Matrix& B = *reinterpret_cast< const Matrix* >( base.get() );
+      auto it = w.ops.find( generation );
+      if( it == w.ops.end() ) {
+         // this worker's first job on the matrix: drop copies of matrices no longer resident, then
+         // copy A in the resident CSR layout (a conversion keeps the source's entry order and its
+         // stored entries per row)
+         {
+            std::lock_guard< std::mutex > lock( gmresMutex );
+            for( auto o = w.ops.begin(); o != w.ops.end(); ) {
+               bool live = false;
+               for( const ResidentMatrix& M : matrices )
+                  live = live || ( M.generation == o->first && M.mat >= 0 );
+               o = live ? std::next( o ) : w.ops.erase( o );
+            }
+         }
+         auto ops = std::make_unique< GmresOperators >();
+         TNLHIP_CHECK_STATUS( tnlhip_matrix_convert( base.get(), TNLHIP_CSR_SCALAR, TNLHIP_F64, &ops->A64.m ) );
+         TNLHIP_CHECK_STATUS( tnlhip_matrix_convert( base.get(), TNLHIP_CSR_SCALAR, TNLHIP_F32, &ops->A32.m ) );
+         TNLHIP_CHECK_STATUS( tnlhip_matrix_convert( base.get(), TNLHIP_CSR_SCALAR, TNLHIP_BF16, &ops->A16.m ) );
+         it = w.ops.emplace( generation, std::move( ops ) ).first;
+      }
+      GmresOperators& o = *it->second;
+      if( std::memcmp( &o.shift, &q.shift, sizeof( double ) ) != 0 ) {
+         // the request's operator: a + shift on the diagonal, rounded to each copy's value type
+         auto setDiag = [ & ]( tnlhip_matrix h, auto* tag ) {
+            using V = std::remove_pointer_t< decltype( tag ) >;
+            Matrix& C = *reinterpret_cast< Matrix* >( h );
+            C.diagonalValid = false;
+            if( n > 0 )
+               setShiftedDiagonalKernel< V ><<< gridFor( n, 256 ), 256, 0, st >>>(
+                  n, B.offsets.ptr, B.columns.ptr, reinterpret_cast< const double* >( B.values.ptr ),
+                  reinterpret_cast< V* >( C.values.ptr ), q.shift );
+         };
+         setDiag( o.A64.m, (double*) nullptr );
+         setDiag( o.A32.m, (float*) nullptr );
+         setDiag( o.A16.m, (bf16*) nullptr );
+         TNLHIP_CHECK( hipGetLastError() );
+         o.shift = q.shift;
+      }
       tnlhip_refinement_params rp_{};
       tnlhip_refinement_params_default( &rp_ );
       rp_.inner.preconditioner = 3;
       rp_.inner.poly_degree = 4;
-      rp_.poly_matrix = A16.m;
+      rp_.poly_matrix = o.A16.m;
       rp_.tolerance = q.params.tolerance;
       rp_.max_outer_iterations = q.params.max_outer_iterations;
       rp_.stagnation_limit = q.params.stagnation_limit;
       rp_.inner.convergence_residue = q.params.inner_tolerance;
       rp_.inner.max_iterations = q.params.max_inner_iterations;
-      DeviceArray< double > db( n ), dx( n );
-      hb.resize( n );
-      hx.resize( n );
-      for( int j = 0; j < q.k; j++ ) {
-         for( int32_t i = 0; i < n; i++ )
-            hb[ i ] = q.b[ (int64_t) i * q.k + j ];
-         TNLHIP_CHECK( hipMemcpyAsync( db.ptr, hb.data(), sizeof( double ) * n, hipMemcpyHostToDevice, st ) );
-         TNLHIP_CHECK( hipMemsetAsync( dx.ptr, 0, sizeof( double ) * n, st ) );
-         tnlhip_refinement_result res{};
-         TNLHIP_CHECK_STATUS( tnlhip_refine_solve( A64.m, A32.m, &rp_, db.ptr, dx.ptr, &res, st ) );
-         TNLHIP_CHECK( hipMemcpyAsync( hx.data(), dx.ptr, sizeof( double ) * n, hipMemcpyDeviceToHost, st ) );
-         TNLHIP_CHECK( hipStreamSynchronize( st ) );
-         for( int32_t i = 0; i < n; i++ )
-            q.x[ (int64_t) i * q.k + j ] = hx[ i ];
-         out.converged = out.converged && res.converged;
-         out.outer_iterations = std::max( out.outer_iterations, res.outer_iterations );
-         out.inner_iterations += res.inner_iterations;
-         out.residue = std::max( out.residue, res.residue );
-      }
+      growTo( w.db, std::max( n, 1 ) );
+      growTo( w.dx, std::max( n, 1 ) );
+      w.hb.resize( n );
+      w.hx.resize( n );
+      for( int32_t i = 0; i < n; i++ )
+         w.hb[ i ] = q.b[ (int64_t) i * q.k + j ];
+      TNLHIP_CHECK( hipMemcpyAsync( w.db.ptr, w.hb.data(), sizeof( double ) * n, hipMemcpyHostToDevice, st ) );
+      TNLHIP_CHECK( hipMemsetAsync( w.dx.ptr, 0, sizeof( double ) * n, st ) );
+      tnlhip_refinement_result res{};
+      refineWith( w.ctx, o.A64.m, o.A32.m, &rp_, w.db.ptr, w.dx.ptr, &res, st );
+      TNLHIP_CHECK( hipMemcpyAsync( w.hx.data(), w.dx.ptr, sizeof( double ) * n, hipMemcpyDeviceToHost, st ) );
+      TNLHIP_CHECK( hipStreamSynchronize( st ) );
+      for( int32_t i = 0; i < n; i++ )
+         q.x[ (int64_t) i * q.k + j ] = w.hx[ i ];
+      out.converged = res.converged;
+      out.outer_iterations = res.outer_iterations;
+      out.inner_iterations = res.inner_iterations;
+      out.residue = res.residue;
    } );
    if( status != TNLHIP_SUCCESS ) {
       out.converged = 0;
@@ -1097,7 +1193,9 @@ int tnlhip_server_create( const tnlhip_server_config* config, tnlhip_server* out
                          && c.iterations_per_launch > 0 && c.staging_capacity > 0,
                       "server capacities must be positive" );
       TNLHIP_REQUIRE( c.vector_capacity < ( 1ll << 40 ) && c.matrix_capacity < ( 1ll << 40 ), "capacity too large" );
-      TNLHIP_CHECK( hipStreamCreate( &S->stream ) );
+      // non-blocking: the legacy default stream, which the conversions of the GMRES workers use,
+      // does not wait for the batched launches (nor they for it)
+      TNLHIP_CHECK( hipStreamCreateWithFlags( &S->stream, hipStreamNonBlocking ) );
       S->cols.resize( c.matrix_capacity );
       S->v64.resize( c.matrix_capacity );
       S->v32.resize( c.matrix_capacity );
@@ -1209,6 +1307,7 @@ int tnlhip_server_add_matrix( tnlhip_server S, int32_t rows, const int32_t* row_
          M.values = std::make_shared< const std::vector< double > >( values, values + nnz );
       }
       std::lock_guard< std::mutex > lock( S->gmresMutex );
+      M.generation = S->nextGeneration++;
       int id = -1;
       for( size_t i = 0; i < S->matrices.size(); i++ )
          if( S->matrices[ i ].mat < 0 && S->matrices[ i ].removing ) {
@@ -1263,12 +1362,18 @@ int tnlhip_server_submit( tnlhip_server S, int32_t matrix_id, double shift, int3
       q.x = x;
       if( q.params.method == TNLHIP_GMRES ) {
          q.ticket = S->nextTicket++;
+         auto task = std::make_shared< GmresTask >();
+         task->q = q;
+         task->remaining = q.k;
+         task->out.converged = 1;
          {
             std::lock_guard< std::mutex > lock( S->gmresMutex );
             S->matrices[ matrix_id ].users++;
-            S->gmresQueue.push_back( q );
+            for( int j = 0; j < q.k; j++ )
+               S->gmresQueue.push_back( GmresJob{ task, j } );
+            S->gmresWaiting++;
          }
-         S->gmresWake.notify_one();
+         S->gmresWake.notify_all();
          *ticket = q.ticket;
          return;
       }
@@ -1382,7 +1487,7 @@ int tnlhip_server_stats( tnlhip_server S, tnlhip_server_stats_t* stats )
       stats->queued_requests = (int32_t) S->queue.size();
       {
          std::lock_guard< std::mutex > lock( S->gmresMutex );
-         stats->queued_requests += (int32_t) S->gmresQueue.size();
+         stats->queued_requests += S->gmresWaiting;
          // on a worker, or finished there and not yet reported by tnlhip_server_progress: a
          // caller that stops once nothing is resident must not lose the latter
          stats->resident_requests += S->gmresBusy + (int32_t) S->gmresDone.size();

@@ -74,6 +74,15 @@ int guarded( F&& body )
 // Owning device array (RAII over hipMalloc), the counterpart of TNL's Array with
 // Allocators::Cuda.
 template< typename T >
+struct DeviceArray;
+
+// Grows a device array to at least n elements and never shrinks it: a solver or context reused for
+// systems of different sizes keeps its largest buffers instead of freeing them (hipFree waits for
+// the whole device).
+template< typename T >
+void growTo( DeviceArray< T >& a, int64_t n );
+
+template< typename T >
 struct DeviceArray
 {
    T* ptr = nullptr;
@@ -130,6 +139,13 @@ struct DeviceArray
       return v;
    }
 };
+
+template< typename T >
+void growTo( DeviceArray< T >& a, int64_t n )
+{
+   if( a.size < n )
+      a.resize( n );
+}
 
 // ------------------------------------------------------------------ value types
 

@@ -200,7 +200,9 @@ void biellpack( HostSegments& s, const int32_t* caps )
 
 void buildSegments( HostSegments& s, int kind, int32_t n, const int32_t* caps )
 {
+   const int32_t sliceSize = s.sliceSize;  // SlicedEllpack's slice size, set by the caller (32 by default)
    s = HostSegments();
+   s.sliceSize = sliceSize;
    s.kind = kind;
    s.n = n;
    switch( kind ) {

@@ -306,7 +306,7 @@ void ensureDiagonal( SolverImpl& S )
    }
    if( ( S.params.preconditioner != 1 && S.params.preconditioner != 3 ) || S.diagValid )
       return;
-   S.diag.resize( S.n * (int64_t) S.vbytes );
+   growTo( S.diag, S.n * (int64_t) S.vbytes );
    extractDiagonal( *S.A, S.diag.ptr, S.stream );
    S.diagValid = true;
 }
@@ -2427,11 +2427,11 @@ void prepare( SolverImpl& S )
    if( S.method == TNLHIP_BICGSTABL ) {
       nvec = 2 * ( S.params.bicgstab_ell + 1 ) + 3;
       const int64_t l1 = S.params.bicgstab_ell + 1;
-      S.small.resize( 4 * l1 + l1 * l1 + 8 );
+      growTo( S.small, 4 * l1 + l1 * l1 + 8 );
    }
    if( S.method == TNLHIP_GMRES )
       nvec = 4 + ( S.params.restarting_max + 1 ) + 1;
-   S.vecs.resize( (int64_t) nvec * S.ld * (int64_t) S.vbytes );
+   growTo( S.vecs, (int64_t) nvec * S.ld * (int64_t) S.vbytes );
    if( S.params.preconditioner == 3 ) {
       if( S.polyMatrix ) {
          const Matrix& P = *S.polyMatrix;
@@ -2440,12 +2440,12 @@ void prepare( SolverImpl& S )
          TNLHIP_REQUIRE( P.dtype != TNLHIP_BINARY && vectorDtype( P.dtype ) == S.vdtype,
                          "the preconditioner matrix must use the solved matrix's vector type" );
       }
-      S.polyVecs.resize( 2 * S.ld * (int64_t) S.vbytes );
+      growTo( S.polyVecs, 2 * S.ld * (int64_t) S.vbytes );
    }
    S.ws.ensure( 4, kMaxMulti, 64 );
    if( S.method == TNLHIP_GMRES ) {
       const int64_t m1 = S.params.restarting_max + 1;
-      S.small.resize( m1 * S.params.restarting_max + 6 * m1 + 2 * m1 * m1 + 64 );
+      growTo( S.small, m1 * S.params.restarting_max + 6 * m1 + 2 * m1 * m1 + 64 );
    }
    ensureDiagonal( S );
    resetState( S );

@@ -18,32 +18,41 @@ struct Candidate
    int blocksPerCu;  // 0: default
    int pairRows;   // -1: default
    int sliceSize;  // SlicedEllpack: 16, 32 (TNL's default), 64; 0 for other formats
+   int stripes;    // CSRAdaptive: column stripes of the striped plan (0: none)
 };
 
-std::vector< Candidate > candidatesFor( int format )
+std::vector< Candidate > candidatesFor( int format, bool xExceedsL2 )
 {
    std::vector< Candidate > c;
    switch( format ) {
       case TNLHIP_CSR_LIGHT:
       case TNLHIP_CSR_HYBRID:
-         c.push_back( { format, 0, 0, -1, 0 } );
+         c.push_back( { format, 0, 0, -1, 0, 0 } );
          for( int l : { 2, 4, 8, 16, 32 } )
-            c.push_back( { format, l, 0, -1, 0 } );
+            c.push_back( { format, l, 0, -1, 0, 0 } );
          break;
       case TNLHIP_ELLPACK:
          for( int p : { 0, 1 } )
             for( int b : { 8, 16 } )
-               c.push_back( { format, 0, b, p, 0 } );
+               c.push_back( { format, 0, b, p, 0, 0 } );
          break;
       case TNLHIP_SLICED_ELLPACK:
          // slice sizes outermost: a change of slice size re-lays the storage out
          for( int s : { 32, 16, 64 } )
             for( int p : { 0, 1 } )
                for( int b : { 8, 16 } )
-                  c.push_back( { format, 0, b, p, s } );
+                  c.push_back( { format, 0, b, p, s, 0 } );
+         break;
+      case TNLHIP_CSR_ADAPTIVE:
+         c.push_back( { format, 0, 0, -1, 0, 0 } );
+         // the striped plan keeps each stripe of x in the L2 of the XCDs that own it: worth
+         // timing only when x does not fit one XCD's L2 already
+         if( xExceedsL2 )
+            for( int s : { 4, 8 } )
+               c.push_back( { format, 0, 0, -1, 0, s } );
          break;
       default:
-         c.push_back( { format, 0, 0, -1, 0 } );
+         c.push_back( { format, 0, 0, -1, 0, 0 } );
    }
    return c;
 }
@@ -60,6 +69,8 @@ void configure( tnlhip_matrix A, const Candidate& c )
    TNLHIP_CHECK_STATUS( tnlhip_matrix_set_param( A, "pair_rows", c.pairRows > 0 ? 1 : 0 ) );
    if( c.format == TNLHIP_SLICED_ELLPACK )
       TNLHIP_CHECK_STATUS( tnlhip_matrix_set_param( A, "slice_size", c.sliceSize > 0 ? c.sliceSize : 32 ) );
+   if( c.format == TNLHIP_CSR_ADAPTIVE )
+      TNLHIP_CHECK_STATUS( tnlhip_matrix_set_param( A, "column_stripes", c.stripes ) );
 }
 
 }  // namespace
@@ -105,7 +116,8 @@ int tnlhip_autotune_csr( int value_dtype, int32_t rows, int32_t columns, const i
          if( s != TNLHIP_SUCCESS )
             throw Error( s, std::string( "autotune: building a candidate failed: " ) + tnlhip_last_error() );
          bool kept = false;
-         for( const Candidate& c : candidatesFor( format ) ) {
+         const bool xExceedsL2 = (int64_t) columns * (int64_t) vb > ( 4ll << 20 );
+         for( const Candidate& c : candidatesFor( format, xExceedsL2 ) ) {
             configure( A, c );
             for( int w = 0; w < 2; w++ )
                TNLHIP_CHECK_STATUS( tnlhip_spmv( A, x, y, 1.0, 0.0, 0, 0, st ) );
@@ -116,7 +128,7 @@ int tnlhip_autotune_csr( int value_dtype, int32_t rows, int32_t columns, const i
             TNLHIP_CHECK( hipEventSynchronize( e1 ) );
             float ms = 0;
             TNLHIP_CHECK( hipEventElapsedTime( &ms, e0, e1 ) );
-            tnlhip_tuning t{ format, c.lanes, c.blocksPerCu, c.pairRows, c.sliceSize, 1e-3 * ms / repetitions };
+            tnlhip_tuning t{ format, c.lanes, c.blocksPerCu, c.pairRows, c.sliceSize, c.stripes, 1e-3 * ms / repetitions };
             if( all && count < all_capacity )
                all[ count ] = t;
             count++;
@@ -133,7 +145,8 @@ int tnlhip_autotune_csr( int value_dtype, int32_t rows, int32_t columns, const i
          if( kept )
             // leave the kept matrix configured exactly as its best candidate (format defaults
             // restored for the knobs the candidate leaves at their default)
-            configure( A, Candidate{ best->format, best->lanes, best->blocks_per_cu, best->pair_rows, best->slice_size } );
+            configure( A, Candidate{ best->format, best->lanes, best->blocks_per_cu, best->pair_rows, best->slice_size,
+                                     best->column_stripes } );
          else
             tnlhip_matrix_destroy( A );
       }
