@@ -550,6 +550,39 @@ def serving_suite(tnl, W, args):
             "gmres_inner_tolerance": GMRES_INNER_TOL}
 
 
+def serving_kernel_suite(tnl, W, args):
+    """The batched product of the solve server under a uniform load: R identical requests on the
+    3D 7-point Laplacian 100^3 (n = 1e6) + 0.5 I with 16 right-hand sides each, all resident at
+    once, so every launch of the inner product kernel multiplies R blocks of 16 vectors.  Its
+    algorithmic bytes per launch are R * (nnz * (4 + 4) + (n + 1) * 4 + 2 * 16 * n * 4) (FP32 values
+    and vectors, int32 indices: the multi-right-hand-side SpMV bytes); the kernel's duration comes
+    from rocprofv3 on this suite."""
+    R, k, N = args.serve_uniform_requests, 16, 100
+    rp, c, v = W.laplacian_3d(N)
+    n, nnz = len(rp) - 1, len(c)
+    server = tnl.Server(vector_capacity=R * n * k + 1024, matrix_capacity=int(1.3 * nnz) + 4096,
+                        slice_capacity=n // 32 + 64, max_requests=R, iterations_per_launch=args.serve_iterations,
+                        gmres_workers=0)
+    mid = server.add_matrix(rp, c, v)
+    B = np.random.default_rng(3).uniform(-1, 1, (n, k))
+    for _ in range(2):  # the first pass warms up
+        for _ in range(R):
+            server.submit(mid, B, shift=0.5, inner_tolerance=args.serve_inner_tol)
+        t0 = time.perf_counter()
+        done = server.run_all()
+        wall = time.perf_counter() - t0
+    infos = [i for _, i in done.values()]
+    server.close()
+    per_launch = R * (nnz * 8 + (n + 1) * 4 + 2 * k * n * 4)
+    log(f"serving kernel: {R} requests x {k} RHS on the 100^3 Laplacian + 0.5 I: {wall:.3f} s, "
+        f"inner iterations {infos[0]['inner_iterations'] / k:.0f} per RHS, algorithmic bytes per product launch {per_launch}")
+    return {"workload": f"{R} identical requests, 3D Laplacian 100^3 + 0.5 I, 16 RHS each, all resident",
+            "requests": R, "rhs_per_request": k, "n": n, "nnz": nnz, "seconds": round(wall, 4),
+            "converged": all(i["converged"] for i in infos),
+            "inner_iterations_per_rhs": round(infos[0]["inner_iterations"] / k, 1),
+            "product_algorithmic_bytes_per_launch": per_launch}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -577,10 +610,12 @@ def main():
     ap.add_argument("--serve-ladder", type=int, default=32, help="resident matrix sizes per dimension")
     ap.add_argument("--serve-gmres-fraction", type=float, default=0.2)
     ap.add_argument("--serve-gmres-workers", type=int, default=4)
+    ap.add_argument("--serve-uniform-requests", type=int, default=8)
     ap.add_argument("--serve-entries", type=int, default=256 << 20,
                     help="vector arena of the server in (row, right-hand side) entries, 40 B each")
     ap.add_argument("--serve-iterations", type=int, default=8)
-    ap.add_argument("--only", default="", help="run one suite: solvers | serving | gmres | irregular | next")
+    ap.add_argument("--only", default="",
+                    help="run one suite: solvers | serving | serving-kernel | gmres | irregular | next")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus above 1 is not supported "
@@ -595,6 +630,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.only == "serving":
         print(json.dumps(serving_suite(tnl, W, args)))
+        return
+    if args.only == "serving-kernel":
+        print(json.dumps(serving_kernel_suite(tnl, W, args)))
         return
     if args.only == "gmres":
         print(json.dumps(gmres_step_suite(tnl, torch, W, dev, args)))

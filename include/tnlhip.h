@@ -225,7 +225,13 @@ int tnlhip_matrix_get_diagonal( tnlhip_matrix A, void* diagonal, void* stream );
  * blocks -- slices for ChunkedEllpack -- with each XCD owning a contiguous eighth; 3 per-XCD
  * grid-stride, Ellpack / SlicedEllpack only); for the Ellpack / SlicedEllpack kernels
  * "pair_rows" (0/1: two adjacent rows per lane with 8 / 16-byte loads) and "blocks_per_cu"
- * (workgroups per CU the grid is sized for). */
+ * (workgroups per CU the grid is sized for); "stream_y" (-1 the kernel's default, 0 stores of y
+ * through L2, 1 non-temporal stores); "column_stripes" (CSR: 0 drops the column-striped plan,
+ * 2 / 4 / 8 / 16 / 32 builds one, a second, stripe-major copy of the entries whose stripes of x
+ * stay in the L2 of the XCDs that own them; sums reassociated by stripe, to 1e-13 of sum |a x|);
+ * "slice_size" (SlicedEllpack: TNL's SliceSize template parameter, a power of two in [1, 256],
+ * default 32; the storage is re-laid out on the device with capacities = stored entries per row).
+ * Every other name or value fails with TNLHIP_ERROR_INVALID_ARGUMENT. */
 int tnlhip_matrix_set_param( tnlhip_matrix A, const char* name, int32_t value );
 
 /* SparseMatrix::vectorProduct( x, y, alpha, beta, firstRow, lastRow ):

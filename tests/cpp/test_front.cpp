@@ -232,6 +232,62 @@ int main()
                    gmres.getResidue() );
       EXPECT_EQ( ok2, true );
 
+      // FP32 GMRES with the Jacobi polynomial over a BF16 copy (no TNL counterpart): fewer
+      // iterations than Jacobi on the same FP32 system, same tolerance
+      {
+         using MatrixF = Matrices::SparseMatrix< float, Devices::Hip, int, Matrices::GeneralMatrix, Seg::CSRScalar >;
+         using VectorF = Containers::Vector< float, Devices::Hip, int >;
+         auto Af = std::make_shared< MatrixF >( n, n );
+         Af->setRowCapacities( caps );
+         for( int j = 0; j < N; j++ )
+            for( int i = 0; i < N; i++ ) {
+               const int r = i + N * j;
+               if( j > 0 ) Af->setElement( r, r - N, -1 );
+               if( i > 0 ) Af->setElement( r, r - 1, -1 );
+               Af->setElement( r, r, 4 );
+               if( i < N - 1 ) Af->setElement( r, r + 1, -1 );
+               if( j < N - 1 ) Af->setElement( r, r + N, -1 );
+            }
+         std::vector< float > bf( bh.begin(), bh.end() );
+         VectorF b4( bf );
+         int its[ 2 ] = { 0, 0 };
+         for( int which = 0; which < 2; which++ ) {
+            Solvers::Linear::GMRES< MatrixF > g;
+            g.setMatrix( Af );
+            if( which == 0 )
+               g.setPreconditioner( std::make_shared< Solvers::Linear::Preconditioners::Diagonal< MatrixF > >() );
+            else {
+               auto poly = std::make_shared< Solvers::Linear::Preconditioners::JacobiPolynomial< MatrixF > >(
+                  4, TNLHIP_BF16 );
+               poly->update( Af );
+               g.setPreconditioner( poly );
+            }
+            Config::ParameterContainer pf;
+            pf.addParameter< int >( "gmres-restarting-min", 30 );
+            pf.addParameter< int >( "gmres-restarting-max", 30 );
+            pf.addParameter< double >( "convergence-residue", 1e-4 );
+            pf.addParameter< int >( "max-iterations", 20000 );
+            g.setup( pf );
+            VectorF xf( n, 0.0f );
+            const bool okf = g.solve( b4, xf );
+            its[ which ] = g.getIterations();
+            VectorF rf( n );
+            Af->vectorProduct( xf, rf );
+            auto rfh = rf.toHost();
+            double num2 = 0, den2 = 0;
+            for( int i = 0; i < n; i++ ) {
+               num2 += ( bf[ i ] - rfh[ i ] ) * ( bf[ i ] - rfh[ i ] );
+               den2 += (double) bf[ i ] * bf[ i ];
+            }
+            std::printf( "FP32 GMRES %s converged=%d iterations=%d true residual %.3e\n",
+                         which ? "JacobiPolynomial(4, BF16)" : "Jacobi", (int) okf, its[ which ],
+                         std::sqrt( num2 / den2 ) );
+            EXPECT_EQ( okf, true );
+            EXPECT_EQ( std::sqrt( num2 / den2 ) < 1e-3, true );
+         }
+         EXPECT_EQ( its[ 1 ] < its[ 0 ], true );
+      }
+
       // run-time choice by name (LinearSolverTypeResolver.h): BiCGStab with ILU(0)
       auto solver = Solvers::getLinearSolver< Matrix >( "bicgstab" );
       auto pre = Solvers::getPreconditioner< Matrix >( "ilu0" );

@@ -12,6 +12,7 @@
 #include "../../Containers/Vector.h"
 #include "Preconditioners/Diagonal.h"
 #include "Preconditioners/ILU0.h"
+#include "Preconditioners/JacobiPolynomial.h"
 
 namespace TNL {
 namespace Solvers {
@@ -100,6 +101,8 @@ public:
       if( h != bound || matrix->getGeneration() != boundGeneration ||
           ( preconditioner ? preconditioner->getUpdateCount() : 0 ) != boundUpdates )
          bind();
+      if( preconditioner )
+         preconditioner->configure( handle, params );
       detail::check( tnlhip_solver_set_params( handle, &params ) );
       int32_t converged = 0;
       detail::check( tnlhip_solver_solve( handle, b.getData(), x.getData(), &converged, nullptr ) );

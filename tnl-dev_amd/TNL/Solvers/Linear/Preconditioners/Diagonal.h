@@ -20,6 +20,8 @@ public:
    virtual ~Preconditioner() = default;
    virtual void update( const MatrixPointer& ) { updates++; }
    virtual int kind() const { return 0; }
+   // settings of this preconditioner the solver needs beside its kind (applied before each solve)
+   virtual void configure( tnlhip_solver, tnlhip_solver_params& ) const {}
    // how many times update() ran: a solver re-derives its copy of the preconditioner (the
    // diagonal, the ILU(0) factors) at the next solve after a change
    uint64_t getUpdateCount() const { return updates; }

@@ -778,8 +778,10 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
       case TNLHIP_CSR_ADAPTIVE: {
          if( M.format == TNLHIP_CSR_ADAPTIVE && first == 0 && last == M.rows && M.adaptiveBlocks > 0 ) {
             constexpr int STREAM = kAdaptiveStreamBytesPerWave / sizeof( A );
-            // STREAM blocks end rows anywhere in a cache line of y
-            keep.streamY = M.streamY == 1;
+            // y through L2 on request (STREAM blocks end rows anywhere in a cache line of y, so
+            // non-temporal stores write some lines twice); non-temporal by default, which measured
+            // 79-80 us at 160^3 FP64 against 81-83 us through L2
+            keep.streamY = M.streamY != 0;
             unsigned groups = ( M.adaptiveBlocks + 3 ) / 4;
             if( M.xcdRemap )
                groups = ( groups + kXcds - 1 ) / kXcds * kXcds;  // logicalBlock needs a multiple of 8
