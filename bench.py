@@ -59,15 +59,21 @@ def log(*a):
 def time_spmv(tnl, torch, A, x, y, steps, warmup, flush):
     """Returns (back-to-back seconds per SpMV over `steps` launches between two events,
     cold seconds per SpMV with the 512 MiB flush before each launch)."""
+    return time_launches(torch, lambda: A.vectorProduct(x, y), steps, warmup, flush)
+
+
+def time_launches(torch, fn, steps, warmup, flush):
+    """(back-to-back seconds per call of fn over `steps` calls between two events on the current
+    stream, cold seconds per call with the flush buffer rewritten before each call)."""
     st = torch.cuda.current_stream()
     for _ in range(warmup):
-        A.vectorProduct(x, y)
+        fn()
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(st)
     for _ in range(steps):
-        A.vectorProduct(x, y)
+        fn()
     e1.record(st)
     e1.synchronize()
     steady = e0.elapsed_time(e1) * 1e-3 / steps
@@ -77,7 +83,7 @@ def time_spmv(tnl, torch, A, x, y, steps, warmup, flush):
         c0 = torch.cuda.Event(enable_timing=True)
         c1 = torch.cuda.Event(enable_timing=True)
         c0.record(st)
-        A.vectorProduct(x, y)
+        fn()
         c1.record(st)
         c1.synchronize()
         times.append(c0.elapsed_time(c1) * 1e-3)
@@ -361,8 +367,8 @@ def irregular_suite(tnl, torch, W, dev, args):
 
 def next_suite(tnl, torch, W, dev, args, general_us=None):
     """The components next to the path, measured on their own: a SymmetricMatrix SpMV (lower
-    triangle, SlicedEllpack, FP64) and a binary SpMV (SlicedEllpack, int32) on the grid^3
-    Laplacian; on the convection-diffusion cd_grid^3 box (Pe = 10) BiCGStab to 1e-10 with
+    triangle, SlicedEllpack, FP64), a binary SpMV (SlicedEllpack, int32) and products with k = 4
+    and 16 right-hand sides (SlicedEllpack and CSRScalar, FP64) on the grid^3 Laplacian; on the convection-diffusion cd_grid^3 box (Pe = 10) BiCGStab to 1e-10 with
     Jacobi and with ILU(0), and BiCGStab(2) with Jacobi (iterations, seconds, ILU(0) setup)."""
     out = {}
     N = args.grid
@@ -391,7 +397,24 @@ def next_suite(tnl, torch, W, dev, args, general_us=None):
                           "us": round(steady * 1e6, 2), "GB/s": round(bb / steady / 1e9, 1),
                           "frac_hbm": round(bb / steady / 1e9 / HBM_PEAK_GBS, 4)}
     log(f"binary SlicedEllpack {N}^3: {steady * 1e6:.2f} us, {bb / steady / 1e9:.0f} GB/s")
-    del Bm, x, y, xi, yi, flush
+    del Bm, xi, yi
+    # products with k right-hand sides in one pass over A (row-major n x k blocks)
+    out["multi_rhs_spmv"] = {}
+    for fmt in ("SlicedEllpack", "CSRScalar"):
+        A = tnl.SparseMatrix.from_csr(rp, c, v, format=fmt, dtype="f64")
+        for k in (4, 16):
+            X = torch.rand((n, k), dtype=torch.float64, device=dev)
+            Y = torch.zeros((n, k), dtype=torch.float64, device=dev)
+            steady, cold = time_launches(torch, lambda: A.vectorProductMulti(X, Y), args.steps, args.warmup, flush)
+            mb = len(c) * 12 + 4 * (n + 1) + 2 * k * n * 8
+            out["multi_rhs_spmv"][f"{fmt}_k{k}"] = {
+                "us": round(steady * 1e6, 2), "us_cold": round(cold * 1e6, 2), "algorithmic_bytes": mb,
+                "GB/s": round(mb / steady / 1e9, 1), "frac_hbm": round(mb / steady / 1e9 / HBM_PEAK_GBS, 4)}
+            log(f"multi-RHS {fmt} {N}^3 k={k}: {steady * 1e6:.2f} us, {mb / steady / 1e9:.0f} GB/s")
+            del X, Y
+        del A
+    out["multi_rhs_spmv"]["workload"] = f"{N}^3 Laplacian, FP64, k right-hand sides row-major"
+    del x, y, flush
     torch.cuda.empty_cache()
     M = min(args.cd_grid, 64)
     rp, c, v = W.convection_diffusion_3d(M, 10.0)

@@ -88,3 +88,32 @@ def test_block_product_refusals(tnl, dev):
     with pytest.raises(tnl.TnlHipError):
         S.vectorProductMulti(torch.zeros((n, 2), dtype=torch.float64, device=dev),
                              torch.zeros((n, 2), dtype=torch.float64, device=dev))
+
+
+@pytest.mark.parametrize("fmt", FORMATS)
+def test_block_product_without_columns(tnl, dev, fmt):
+    """A matrix with no columns (every row empty) and an empty X: only the keepers act, and X,
+    which holds no entry, is never read."""
+    n, k = 70, 5
+    rp = np.zeros(n + 1, np.int32)
+    A = tnl.SparseMatrix.from_csr(rp, np.zeros(0, np.int32), np.zeros(0), columns=0, format=fmt, dtype="f64")
+    Y0 = np.random.default_rng(4).uniform(-1, 1, (n, k))
+    X = torch.zeros((0, k), dtype=torch.float64, device=dev)
+    for alpha, beta in ((1.0, 0.0), (2.0, 0.5)):
+        Y = torch.tensor(Y0, device=dev)
+        A.vectorProductMulti(X, Y, alpha, beta)
+        assert np.array_equal(Y.cpu().numpy(), beta * Y0), (fmt, alpha, beta)
+        y = torch.tensor(Y0[:, 0].copy(), device=dev)
+        A.vectorProduct(torch.zeros(0, dtype=torch.float64, device=dev), y, alpha, beta)
+        assert np.array_equal(y.cpu().numpy(), beta * Y0[:, 0]), (fmt, alpha, beta)
+
+
+def test_binary_product_without_columns(tnl, dev):
+    n = 45
+    rp = np.zeros(n + 1, np.int32)
+    B = tnl.SparseMatrix.from_csr(rp, np.zeros(0, np.int32), None, columns=0, format="SlicedEllpack", dtype="binary")
+    y0 = torch.arange(n, dtype=torch.int32, device=dev)
+    for alpha, beta in ((1, 0), (3, 2)):
+        y = y0.clone()
+        B.vectorProduct(torch.zeros(0, dtype=torch.int32, device=dev), y, alpha, beta)
+        assert torch.equal(y, beta * y0)

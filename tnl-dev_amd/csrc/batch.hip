@@ -60,6 +60,15 @@ namespace {
 constexpr int kTile = 256;
 constexpr int kMaxRhs = 16;
 constexpr int kServerGrid = 2048;  // 8 workgroups of 256 per CU
+constexpr int kServerXcds = 8;     // workgroups are dispatched round-robin over the 8 XCDs
+// The two product kernels (at most 128 VGPRs: 4 workgroups of 256 per CU) launch exactly what is
+// resident at once, so the workgroups of an XCD walk its tiles together, front to back, and the x
+// rows one tile gathers are still in L2 when the tiles a plane further on gather them again; a
+// larger grid leaves every other block of tiles to a second round of workgroups that starts once
+// the first has finished its whole walk, when those rows are long gone.
+constexpr int kProductGrid = 1024;
+static_assert( kServerGrid % kServerXcds == 0 && kProductGrid % kServerXcds == 0,
+               "server grids must be multiples of the XCD count" );
 constexpr int kRowsPerLane = 8;     // rows per lane in a tile (independent loads in flight)
 
 enum Phase : int32_t { P_IDLE = 0, P_INNER = 1, P_CORRECT = 2, P_RESTART = 3, P_DONE = 4, P_FAILED = 5 };
@@ -90,9 +99,11 @@ struct Sys
    double normb, scale, rho, rho0, alpha, beta, res, prevRes;
 };
 
+// The tile walk: walk positions [xcdStart[x], xcdStart[x + 1]) of tileOrder are XCD x's share.
 struct Ctl
 {
    int32_t numTiles;
+   int32_t xcdStart[ kServerXcds + 1 ];
 };
 
 struct Arena
@@ -100,6 +111,7 @@ struct Arena
    const Ctl* ctl;
    const int32_t* tileReq;
    const int32_t* tileFirst;
+   const int32_t* tileOrder;  // tile ids in walk order
    const ReqStatic* rs;
    ReqDyn* rd;
    Sys* sys;
@@ -157,27 +169,33 @@ __device__ __forceinline__ void rowProducts( const Arena& A, const ReqStatic& R,
             const int32_t s = row >> 5;
             const int32_t b0 = A.slices[ R.sliceBase + s ], b1 = A.slices[ R.sliceBase + s + 1 ];
             width[ u ] = ( b1 - b0 ) >> 5;
-            idx[ u ] = R.mat + b0 + ( row & 31 );
+            if( width[ u ] > 0 )  // else slot 0 of the arena: loaded, never used
+               idx[ u ] = R.mat + b0 + ( row & 31 );
          }
       }
       Acc acc[ 2 ] = { 0, 0 };
       const int32_t maxw = max( width[ 0 ], width[ 1 ] );
+      const int32_t last[ 2 ] = { max( width[ 0 ] - 1, 0 ), max( width[ 1 ] - 1, 0 ) };
       for( int32_t k0 = 0; k0 < maxw; k0 += kSlotChunk ) {
          int32_t col[ 2 ][ kSlotChunk ];
 #pragma unroll
          for( int u = 0; u < 2; u++ )
 #pragma unroll
             for( int k = 0; k < kSlotChunk; k++ )
-               col[ u ][ k ] = k0 + k < width[ u ] ? A.cols[ idx[ u ] + 32 * (int64_t) ( k0 + k ) ] : -1;
+               col[ u ][ k ] = A.cols[ idx[ u ] + 32 * (int64_t) min( k0 + k, last[ u ] ) ];
          Acc term[ 2 ][ kSlotChunk ];
 #pragma unroll
          for( int u = 0; u < 2; u++ )
 #pragma unroll
-            for( int k = 0; k < kSlotChunk; k++ )
-               term[ u ][ k ] = col[ u ][ k ] >= 0
-                                   ? (Acc) vals[ idx[ u ] + 32 * (int64_t) ( k0 + k ) ]
-                                        * (Acc) v[ (int64_t) col[ u ][ k ] * R.k + c.j ]
-                                   : (Acc) 0;
+            for( int k = 0; k < kSlotChunk; k++ ) {
+               // every load is issued (clamped slot; row 0 of x for padding and for slots past
+               // the row) so that all of them are in flight together; the select drops them
+               const bool on = k0 + k < width[ u ] && col[ u ][ k ] >= 0;
+               const Acc a = (Acc) vals[ idx[ u ] + 32 * (int64_t) min( k0 + k, last[ u ] ) ];
+               const Acc b = (Acc) v[ (int64_t) ( on ? col[ u ][ k ] : 0 ) * R.k + c.j ];
+               term[ u ][ k ] = on ? a * b : (Acc) 0;
+               col[ u ][ k ] = on ? 0 : -1;
+            }
 #pragma unroll
          for( int u = 0; u < 2; u++ )
 #pragma unroll
@@ -190,17 +208,139 @@ __device__ __forceinline__ void rowProducts( const Arena& A, const ReqStatic& R,
    }
 }
 
-// Walks the tile table, fetching the next tile's request one step ahead; f( t, req ).
+// Vector form of a tile for requests whose k is a multiple of V (2 or 4): with L = kp / V lanes
+// per row, lane (lr, jq) holds rows base + u * (256 / L) + lr for u < kRowsPerLane / V and the V
+// right-hand sides V jq .. V jq + V - 1, which are adjacent in the interleaved vector arena.  Each
+// slot then costs one V-wide load of x, and one column and one value load shared by V systems,
+// instead of V scalar gathers and V copies of the column and value loads.  The tile covers the
+// same rows as the scalar form.
+template< typename T, int V >
+struct VecOf
+{
+   typedef T type __attribute__( ( ext_vector_type( V ) ) );
+};
+
+struct VecCtx
+{
+   int32_t base, lr, jq, rpp;
+};
+
+template< int V >
+__device__ __forceinline__ VecCtx vecCtx( const Arena& A, int t, int req, const ReqStatic& R )
+{
+   const int L = R.kp / V;
+   VecCtx c;
+   c.lr = threadIdx.x / L;
+   c.jq = threadIdx.x % L;
+   c.rpp = kTile / L;
+   c.base = ( t - A.tileFirst[ req ] ) * R.rpt * kRowsPerLane;
+   return c;
+}
+
+// sum[u] = sum_k A[row_u, k] * v[col * K + V jq + i] for i < V, slots left to right; rows in
+// pairs, kSlotChunk slots of both per round trip, as rowProducts.
+template< typename T, int V >
+__device__ __forceinline__ void rowProductsVec( const Arena& A, const ReqStatic& R, const VecCtx& c, const T* vals,
+                                                const T* v, typename VecOf< T, V >::type ( &sum )[ kRowsPerLane / V ] )
+{
+   using VT = typename VecOf< T, V >::type;
+#pragma unroll
+   for( int u0 = 0; u0 < kRowsPerLane / V; u0 += 2 ) {
+      int32_t width[ 2 ];
+      int64_t idx[ 2 ];
+#pragma unroll
+      for( int u = 0; u < 2; u++ ) {
+         const int32_t row = c.base + ( u0 + u ) * c.rpp + c.lr;
+         width[ u ] = 0;
+         idx[ u ] = 0;
+         if( row < R.n ) {
+            const int32_t s = row >> 5;
+            const int32_t b0 = A.slices[ R.sliceBase + s ], b1 = A.slices[ R.sliceBase + s + 1 ];
+            width[ u ] = ( b1 - b0 ) >> 5;
+            if( width[ u ] > 0 )  // else slot 0 of the arena: loaded, never used
+               idx[ u ] = R.mat + b0 + ( row & 31 );
+         }
+      }
+      VT acc[ 2 ] = { (VT) 0, (VT) 0 };
+      const int32_t maxw = max( width[ 0 ], width[ 1 ] );
+      const int32_t last[ 2 ] = { max( width[ 0 ] - 1, 0 ), max( width[ 1 ] - 1, 0 ) };
+      const T* vj = v + V * c.jq;
+      for( int32_t k0 = 0; k0 < maxw; k0 += kSlotChunk ) {
+         int32_t col[ 2 ][ kSlotChunk ];
+#pragma unroll
+         for( int u = 0; u < 2; u++ )
+#pragma unroll
+            for( int k = 0; k < kSlotChunk; k++ )
+               col[ u ][ k ] = A.cols[ idx[ u ] + 32 * (int64_t) min( k0 + k, last[ u ] ) ];
+         VT term[ 2 ][ kSlotChunk ];
+#pragma unroll
+         for( int u = 0; u < 2; u++ )
+#pragma unroll
+            for( int k = 0; k < kSlotChunk; k++ ) {
+               const bool on = k0 + k < width[ u ] && col[ u ][ k ] >= 0;
+               const T a = vals[ idx[ u ] + 32 * (int64_t) min( k0 + k, last[ u ] ) ];
+               const VT b = *reinterpret_cast< const VT* >( vj + (int64_t) ( on ? col[ u ][ k ] : 0 ) * R.k );
+               term[ u ][ k ] = on ? a * b : (VT) 0;
+               col[ u ][ k ] = on ? 0 : -1;
+            }
+#pragma unroll
+         for( int u = 0; u < 2; u++ )
+#pragma unroll
+            for( int k = 0; k < kSlotChunk; k++ )
+               if( col[ u ][ k ] >= 0 )
+                  acc[ u ] += term[ u ][ k ];
+      }
+      sum[ u0 ] = acc[ 0 ];
+      sum[ u0 + 1 ] = acc[ 1 ];
+   }
+}
+
+// Tile partials of the vector form: lane sums over the lanes of one jq (xor offsets down to L),
+// then the four waves in order; part[t][0][j] for j < kp.
+template< int V >
+__device__ __forceinline__ void tilePartialsVec( double* part, int t, int kp, const double ( &v )[ V ] )
+{
+   __shared__ double lds[ kTile / 64 ][ kMaxRhs ];
+   const int L = kp / V;
+   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+#pragma unroll
+   for( int i = 0; i < V; i++ ) {
+      double x = v[ i ];
+      for( int o = 32; o >= L; o >>= 1 )
+         x += __shfl_xor( x, o );
+      if( lane < L )
+         lds[ wave ][ V * lane + i ] = x;
+   }
+   __syncthreads();
+   if( (int) threadIdx.x < kp ) {
+      const int j = threadIdx.x;
+      double s = lds[ 0 ][ j ];
+      for( int q = 1; q < kTile / 64; q++ )
+         s += lds[ q ][ j ];
+      part[ (int64_t) t * 2 * kMaxRhs + j ] = s;
+   }
+}
+
+// Walks the tile table, fetching the next tile and its request one step ahead; f( t, req ).
+// Workgroup b runs on XCD b % 8 (round-robin dispatch) and walks that XCD's share of the walk
+// order, which holds a contiguous eighth of every request's tiles (rebuildTiles): the rows a
+// tile's stencil reaches (the neighbouring tiles, a plane away) are then taken on the same XCD,
+// whose L2 keeps that x, and every request still spreads over all eight XCDs.
 template< typename F >
 __device__ __forceinline__ void forTiles( const Arena& A, F&& f )
 {
-   const int nt = A.ctl->numTiles;
-   int t = blockIdx.x;
-   int req = t < nt ? A.tileReq[ t ] : 0;
-   while( t < nt ) {
-      const int tn = t + gridDim.x;
-      const int rn = tn < nt ? A.tileReq[ tn ] : 0;
+   const int xcd = blockIdx.x % kServerXcds;
+   const int per = gridDim.x / kServerXcds;
+   const int k1 = A.ctl->xcdStart[ xcd + 1 ];
+   int k = A.ctl->xcdStart[ xcd ] + (int) ( blockIdx.x / kServerXcds );
+   int t = k < k1 ? A.tileOrder[ k ] : 0;
+   int req = k < k1 ? A.tileReq[ t ] : 0;
+   while( k < k1 ) {
+      const int kn = k + per;
+      const int tn = kn < k1 ? A.tileOrder[ kn ] : 0;
+      const int rn = kn < k1 ? A.tileReq[ tn ] : 0;
       f( t, req );
+      k = kn;
       t = tn;
       req = rn;
    }
@@ -244,17 +384,28 @@ __device__ __forceinline__ const double* requestTotals( const double* part, int 
    const int pair = tid % pairs, g = tid / pairs;
    const int w = pair / kp, j = pair % kp;
    const double* p = part + (int64_t) first * 2 * kMaxRhs + w * kMaxRhs + j;
-   double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+   // kStreams independent loads in flight per lane (the partials come from memory: a request of
+   // 8k tiles is 30 dependent trips per lane instead of 120), folded in a fixed tree
+   constexpr int kStreams = 16;
+   double a[ kStreams ];
+#pragma unroll
+   for( int u = 0; u < kStreams; u++ )
+      a[ u ] = 0;
    int ti = g;
-   for( ; ti + 3 * G < tiles; ti += 4 * G ) {
-      s0 += p[ (int64_t) ti * 2 * kMaxRhs ];
-      s1 += p[ (int64_t) ( ti + G ) * 2 * kMaxRhs ];
-      s2 += p[ (int64_t) ( ti + 2 * G ) * 2 * kMaxRhs ];
-      s3 += p[ (int64_t) ( ti + 3 * G ) * 2 * kMaxRhs ];
+   for( ; ti + ( kStreams - 1 ) * G < tiles; ti += kStreams * G ) {
+#pragma unroll
+      for( int u = 0; u < kStreams; u++ )
+         a[ u ] += p[ (int64_t) ( ti + u * G ) * 2 * kMaxRhs ];
    }
    for( ; ti < tiles; ti += G )
-      s0 += p[ (int64_t) ti * 2 * kMaxRhs ];
-   red[ tid ] = ( s0 + s1 ) + ( s2 + s3 );
+      a[ 0 ] += p[ (int64_t) ti * 2 * kMaxRhs ];
+#pragma unroll
+   for( int h = kStreams / 2; h >= 1; h /= 2 ) {
+#pragma unroll
+      for( int u = 0; u < h; u++ )
+         a[ u ] += a[ u + h ];
+   }
+   red[ tid ] = a[ 0 ];
    __syncthreads();
    for( int step = G / 2; step >= 1; step >>= 1 ) {
       if( g < step )
@@ -323,13 +474,62 @@ __device__ __forceinline__ int64_t entryOf( const TileCtx& c, const ReqStatic& R
 
 // ---------------------------------------------------------------- tile kernels
 
-__global__ __launch_bounds__( kTile ) void innerMatvecKernel( Arena A )
+// q = (A32 + s I) p and the (p, q) partials over the vector form of the tile (k % V == 0)
+template< int V >
+__device__ __forceinline__ void innerMatvecVec( const Arena& A, const ReqStatic& R, const ReqDyn& D, int t, int req )
+{
+   using VT = typename VecOf< float, V >::type;
+   constexpr uint32_t all = ( 1u << V ) - 1u;
+   const VecCtx c = vecCtx< V >( A, t, req, R );
+   const uint32_t act = V * c.jq < R.k ? ( D.bits[ P_INNER ] >> ( V * c.jq ) ) & all : 0u;
+   double v[ V ] = {};
+   if( act ) {
+      const float* p = a32( A, R, P32 );
+      float* qv = a32( A, R, Q32 );
+      VT sum[ kRowsPerLane / V ];
+      rowProductsVec< float, V >( A, R, c, A.v32, p, sum );
+#pragma unroll
+      for( int u = 0; u < kRowsPerLane / V; u++ ) {
+         const int32_t row = c.base + u * c.rpp + c.lr;
+         if( row < R.n ) {
+            const int64_t e = (int64_t) row * R.k + V * c.jq;
+            const VT pe = *reinterpret_cast< const VT* >( p + e );
+            const VT r = sum[ u ] + (float) R.shift * pe;
+            if( act == all )
+               *reinterpret_cast< VT* >( qv + e ) = r;
+            else {
+#pragma unroll
+               for( int i = 0; i < V; i++ )
+                  if( ( act >> i ) & 1u )
+                     qv[ e + i ] = r[ i ];
+            }
+#pragma unroll
+            for( int i = 0; i < V; i++ )
+               if( ( act >> i ) & 1u )
+                  v[ i ] += (double) pe[ i ] * (double) r[ i ];
+         }
+      }
+   }
+   tilePartialsVec< V >( A.part, t, R.kp, v );
+   __syncthreads();
+}
+
+// at most 128 VGPRs (4 waves per SIMD, no spills): the unbounded build took 143 (3 waves)
+__global__ __launch_bounds__( kTile, 4 ) void innerMatvecKernel( Arena A )
 {
    forTiles( A, [ & ]( int t, int req ) {
       const ReqDyn D = A.rd[ req ];
       if( ! ( D.mask & ( 1 << P_INNER ) ) )
          return;
       const ReqStatic R = A.rs[ req ];
+      if( R.k % 4 == 0 ) {
+         innerMatvecVec< 4 >( A, R, D, t, req );
+         return;
+      }
+      if( R.k % 2 == 0 ) {
+         innerMatvecVec< 2 >( A, R, D, t, req );
+         return;
+      }
       const TileCtx c = tileCtx( A, t, req, R );
       double v[ 1 ] = { 0 };
       if( laneOn( c, R, D, P_INNER ) ) {
@@ -460,7 +660,8 @@ __global__ __launch_bounds__( kTile ) void correctKernel( Arena A )
    } );
 }
 
-__global__ __launch_bounds__( kTile ) void outerResidKernel( Arena A )
+// at most 128 VGPRs, as innerMatvecKernel
+__global__ __launch_bounds__( kTile, 4 ) void outerResidKernel( Arena A )
 {
    forTiles( A, [ & ]( int t, int req ) {
       const ReqDyn D = A.rd[ req ];
@@ -770,7 +971,7 @@ struct tnlhip_server_s
 {
    tnlhip_server_config cfg{};
    hipStream_t stream = nullptr;
-   DeviceArray< int32_t > cols, slices, tileReq, tileFirst;
+   DeviceArray< int32_t > cols, slices, tileReq, tileFirst, tileOrder;
    DeviceArray< double > v64, a64, part;
    DeviceArray< float > v32, a32;
    DeviceArray< ReqStatic > rs;
@@ -791,6 +992,7 @@ struct tnlhip_server_s
    // pinned mirrors
    int32_t* hTileReq = nullptr;
    int32_t* hTileFirst = nullptr;
+   int32_t* hTileOrder = nullptr;
    ReqDyn* hDyn = nullptr;
    ReqStatic* hStatic = nullptr;
    Sys* hSys = nullptr;
@@ -815,7 +1017,7 @@ struct tnlhip_server_s
 
    Arena arena() const
    {
-      return Arena{ ctl.ptr, tileReq.ptr, tileFirst.ptr, rs.ptr, rd.ptr, sys.ptr, part.ptr, cols.ptr,
+      return Arena{ ctl.ptr, tileReq.ptr, tileFirst.ptr, tileOrder.ptr, rs.ptr, rd.ptr, sys.ptr, part.ptr, cols.ptr,
                     v64.ptr, v32.ptr, slices.ptr, a64.ptr, a32.ptr, cfg.max_requests };
    }
 
@@ -824,13 +1026,13 @@ struct tnlhip_server_s
       const Arena A = arena();
       const int fg = std::min( cfg.max_requests, kServerGrid );
       for( int i = 0; i < iterations; i++ ) {
-         innerMatvecKernel<<< kServerGrid, kTile, 0, stream >>>( A );
+         innerMatvecKernel<<< kProductGrid, kTile, 0, stream >>>( A );
          requestFinalKernel< 1 ><<< fg, kTile, 0, stream >>>( A, P_INNER, StepAlpha{} );
          innerUpdateKernel<<< kServerGrid, kTile, 0, stream >>>( A );
          requestFinalKernel< 1 ><<< fg, kTile, 0, stream >>>( A, P_INNER, StepBeta{} );
          innerDirKernel<<< kServerGrid, kTile, 0, stream >>>( A );
          correctKernel<<< kServerGrid, kTile, 0, stream >>>( A );
-         outerResidKernel<<< kServerGrid, kTile, 0, stream >>>( A );
+         outerResidKernel<<< kProductGrid, kTile, 0, stream >>>( A );
          requestFinalKernel< 2 ><<< fg, kTile, 0, stream >>>( A, P_CORRECT, StepOuter{} );
          restartKernel<<< kServerGrid, kTile, 0, stream >>>( A );
          requestFinalKernel< 1 ><<< fg, kTile, 0, stream >>>( A, P_RESTART, StepRestart{} );
@@ -852,7 +1054,8 @@ struct tnlhip_server_s
          (void) hipGraphDestroy( graph );
       if( stream )
          (void) hipStreamSynchronize( stream );
-      for( void* p : { (void*) hTileReq, (void*) hTileFirst, (void*) hDyn, (void*) hStatic, (void*) hSys, (void*) hCtl } )
+      for( void* p : { (void*) hTileReq, (void*) hTileFirst, (void*) hTileOrder, (void*) hDyn, (void*) hStatic,
+                       (void*) hSys, (void*) hCtl } )
          if( p )
             (void) hipHostFree( p );
       if( stream )
@@ -1110,9 +1313,27 @@ void rebuildTiles( tnlhip_server_s& S )
    }
    TNLHIP_REQUIRE( t <= S.maxTiles, "tile table overflow" );
    S.hCtl->numTiles = t;
+   // walk order: XCD x takes tiles [x T / 8, (x + 1) T / 8) of every request of T tiles, in
+   // request order; tile ids (and so the partials' order) stay request-contiguous
+   int32_t k = 0;
+   for( int x = 0; x < kServerXcds; x++ ) {
+      S.hCtl->xcdStart[ x ] = k;
+      for( int i = 0; i < S.cfg.max_requests; i++ ) {
+         if( ! S.slotUsed[ i ] )
+            continue;
+         const int64_t T = S.hStatic[ i ].tiles;
+         const int32_t a = (int32_t) ( x * T / kServerXcds ), b = (int32_t) ( ( x + 1 ) * T / kServerXcds );
+         for( int32_t q = a; q < b; q++ )
+            S.hTileOrder[ k++ ] = S.hTileFirst[ i ] + q;
+      }
+   }
+   S.hCtl->xcdStart[ kServerXcds ] = k;
+   TNLHIP_REQUIRE( k == t, "tile walk does not cover the tile table" );
    hipStream_t st = S.stream;
-   if( t > 0 )
+   if( t > 0 ) {
       TNLHIP_CHECK( hipMemcpyAsync( S.tileReq.ptr, S.hTileReq, sizeof( int32_t ) * t, hipMemcpyHostToDevice, st ) );
+      TNLHIP_CHECK( hipMemcpyAsync( S.tileOrder.ptr, S.hTileOrder, sizeof( int32_t ) * t, hipMemcpyHostToDevice, st ) );
+   }
    TNLHIP_CHECK( hipMemcpyAsync( S.tileFirst.ptr, S.hTileFirst, sizeof( int32_t ) * S.cfg.max_requests,
                                  hipMemcpyHostToDevice, st ) );
    TNLHIP_CHECK( hipMemcpyAsync( S.ctl.ptr, S.hCtl, sizeof( Ctl ), hipMemcpyHostToDevice, st ) );
@@ -1205,6 +1426,7 @@ int tnlhip_server_create( const tnlhip_server_config* config, tnlhip_server* out
       S->maxTiles = c.vector_capacity / 128 + 2 * (int64_t) c.max_requests;
       TNLHIP_REQUIRE( S->maxTiles < ( 1ll << 31 ), "too many tiles" );
       S->tileReq.resize( S->maxTiles );
+      S->tileOrder.resize( S->maxTiles );
       S->tileFirst.resize( c.max_requests );
       S->part.resize( S->maxTiles * 2 * kMaxRhs );
       S->rs.resize( c.max_requests );
@@ -1220,6 +1442,7 @@ int tnlhip_server_create( const tnlhip_server_config* config, tnlhip_server* out
       S->slotUsed.assign( c.max_requests, false );
       TNLHIP_CHECK( hipHostMalloc( (void**) &S->hTileReq, sizeof( int32_t ) * S->maxTiles ) );
       TNLHIP_CHECK( hipHostMalloc( (void**) &S->hTileFirst, sizeof( int32_t ) * c.max_requests ) );
+      TNLHIP_CHECK( hipHostMalloc( (void**) &S->hTileOrder, sizeof( int32_t ) * S->maxTiles ) );
       TNLHIP_CHECK( hipHostMalloc( (void**) &S->hDyn, sizeof( ReqDyn ) * c.max_requests ) );
       TNLHIP_CHECK( hipHostMalloc( (void**) &S->hStatic, sizeof( ReqStatic ) * c.max_requests ) );
       TNLHIP_CHECK( hipHostMalloc( (void**) &S->hSys, sizeof( Sys ) * c.max_requests * kMaxRhs ) );

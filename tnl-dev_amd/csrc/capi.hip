@@ -913,7 +913,8 @@ int tnlhip_spmv( tnlhip_matrix A, const void* x, void* y, double alpha, double b
 {
    return guarded( [ & ] {
       TNLHIP_REQUIRE( A, "NULL matrix" );
-      TNLHIP_REQUIRE( ( x && y ) || A->rows == 0, "NULL vector" );
+      // x holds A->cols entries and y A->rows: either may be NULL when it holds none
+      TNLHIP_REQUIRE( ( x || A->cols == 0 ) && ( y || A->rows == 0 ), "NULL vector" );
       if( last == 0 )
          last = A->rows;
       TNLHIP_REQUIRE( first >= 0 && first <= last && last <= A->rows, "row range out of bounds" );
@@ -926,7 +927,7 @@ int tnlhip_spmv_multi( tnlhip_matrix A, int32_t k, const void* X, void* Y, doubl
 {
    return guarded( [ & ] {
       TNLHIP_REQUIRE( A, "NULL matrix" );
-      TNLHIP_REQUIRE( ( X && Y ) || A->rows == 0, "NULL block of vectors" );
+      TNLHIP_REQUIRE( ( X || A->cols == 0 ) && ( Y || A->rows == 0 ), "NULL block of vectors" );
       if( last == 0 )
          last = A->rows;
       TNLHIP_REQUIRE( first >= 0 && first <= last && last <= A->rows, "row range out of bounds" );

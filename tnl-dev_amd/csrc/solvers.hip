@@ -192,6 +192,21 @@ struct NoScalars
 };
 
 // y = x (copy)
+// The `count` (at most kMaxMulti) coefficients a pass multiplies its columns by, copied once per
+// workgroup into LDS (every thread of the block calls it, after the launch's uniform gate).  Read
+// from device memory inside the per-row loop they were vector loads, each waited for in turn
+// behind the row's column loads: one memory round trip per coefficient and row.
+// `Slot` keeps the LDS copies of one pass's several arrays apart.
+template< int Slot = 0 >
+__device__ inline const double* stageCoefficients( const double* p, int count )
+{
+   __shared__ double s[ kMaxMulti ];
+   if( (int) threadIdx.x < min( count, kMaxMulti ) )
+      s[ threadIdx.x ] = p[ threadIdx.x ];
+   __syncthreads();
+   return s;
+}
+
 template< typename T >
 struct OpCopy : NoScalars
 {
@@ -1412,15 +1427,23 @@ struct OpBlUpdate
    T* U;
    int64_t ld;
    BlSmall sm;
-   __device__ void load() {}
+   const double* g0;
+   const double* g1;
+   const double* g2;
+   __device__ void load()
+   {
+      g0 = stageCoefficients< 0 >( sm.g0, sm.ell + 1 );
+      g1 = stageCoefficients< 1 >( sm.g1, sm.ell + 1 );
+      g2 = stageCoefficients< 2 >( sm.g2, sm.ell + 1 );
+   }
    __device__ void operator()( int64_t i ) const
    {
       const int ell = sm.ell;
       double ax = 0.0, ar = 0.0, au = 0.0;
       for( int k = 0; k < ell; k++ ) {
-         ax += (double) R[ (int64_t) k * ld + i ] * sm.g2[ k ];
-         ar += (double) R[ (int64_t) ( k + 1 ) * ld + i ] * -sm.g1[ k + 1 ];
-         au += (double) U[ (int64_t) ( k + 1 ) * ld + i ] * -sm.g0[ k + 1 ];
+         ax += (double) R[ (int64_t) k * ld + i ] * g2[ k ];
+         ar += (double) R[ (int64_t) ( k + 1 ) * ld + i ] * -g1[ k + 1 ];
+         au += (double) U[ (int64_t) ( k + 1 ) * ld + i ] * -g0[ k + 1 ];
       }
       x[ i ] = (T) ( ax + (double) x[ i ] );
       R[ i ] = (T) ( ar + (double) R[ i ] );
@@ -1706,7 +1729,11 @@ struct OpCgs
    int count;          // columns of V involved (1 <= count <= C)
    const double* hprev;  // subtraction coefficients (null: none)
    bool norm;          // accumulate |w|^2 into acc[0] instead of the projections
-   __device__ void load() {}
+   __device__ void load()
+   {
+      if( hprev )  // uniform over the launch
+         hprev = stageCoefficients( hprev, count );
+   }
    __device__ void operator()( int64_t i, double ( &acc )[ C ] ) const
    {
       double wi = (double) w[ i ];
@@ -1856,7 +1883,11 @@ struct OpUpdateX
    double sign;
    bool cwy;
    int cols;
-   __device__ void load() { cols = (int) st->tmp; }
+   __device__ void load()
+   {
+      cols = (int) st->tmp;
+      c = stageCoefficients( c, cols );
+   }
    __device__ void operator()( int64_t i ) const
    {
       double t = 0.0;
@@ -1979,7 +2010,7 @@ struct OpCwyV
    const double* auxp;
    int count;
    int i;
-   __device__ void load() {}
+   __device__ void load() { auxp = stageCoefficients( auxp, count ); }
    __device__ void operator()( int64_t j ) const
    {
       double t = 0.0;
@@ -2057,7 +2088,7 @@ struct OpCwyZ
    int64_t ld;
    const double* auxp;
    int count;
-   __device__ void load() {}
+   __device__ void load() { auxp = stageCoefficients( auxp, count ); }
    __device__ void operator()( int64_t j ) const
    {
       double t = 0.0;
@@ -2080,7 +2111,7 @@ struct OpCwyZGenerate
    const double* auxp;
    int count;  // columns 0..i (count = i + 1 <= C - 1)
    T* ynext;
-   __device__ void load() {}
+   __device__ void load() { auxp = stageCoefficients( auxp, count ); }
    __device__ void operator()( int64_t j, double ( &acc )[ C ] ) const
    {
       T y[ C - 1 ];

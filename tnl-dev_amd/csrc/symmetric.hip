@@ -25,6 +25,7 @@ namespace {
 // row instead of 4.  Wave barriers only (no workgroup barrier): the loop over blocks is uniform
 // per wave.
 constexpr int kSymRows = 256;
+constexpr int kSlotChunk = 8;  // slots of a row whose loads are in flight together
 
 __device__ __forceinline__ void waveBarrier()
 {
@@ -57,20 +58,36 @@ __global__ __launch_bounds__( 256 ) void symmetricKernel( int32_t first, int32_t
          const A xr = (A) x[ r ];
          const int32_t size = view.size( (int32_t) r );
          A s = 0;
-         for( int32_t l = 0; l < size; l++ ) {
-            const int64_t g = view.global( (int32_t) r, l );
-            const int32_t c = col[ g ];
-            if( c < 0 )
-               continue;
-            const A a = toCompute( val[ g ], A() );
-            if( c < r ) {
-               const A t = ( alpha * a ) * xr;
-               if( c >= w0 )
-                  atomicAdd( &acc[ wave ][ c - w0 ], t );
-               else
-                  unsafeAtomicAdd( y + c, (Y) t );
+         for( int32_t l0 = 0; l0 < size; l0 += kSlotChunk ) {
+            // the chunk's column, value and x loads go out together (slots past the row re-read
+            // its last slot, padding reads x[0]: a symmetric matrix with rows has columns), then
+            // the slots are taken left to right
+            int32_t c[ kSlotChunk ];
+            A a[ kSlotChunk ], xc[ kSlotChunk ];
+#pragma unroll
+            for( int k = 0; k < kSlotChunk; k++ ) {
+               const int64_t g = view.global( (int32_t) r, min( l0 + k, size - 1 ) );
+               c[ k ] = col[ g ];
+               a[ k ] = toCompute( val[ g ], A() );
             }
-            s = mulAddExact( s, a, (A) x[ c ] );
+#pragma unroll
+            for( int k = 0; k < kSlotChunk; k++ ) {
+               c[ k ] = l0 + k < size ? c[ k ] : -1;
+               xc[ k ] = (A) x[ c[ k ] >= 0 ? c[ k ] : 0 ];
+            }
+#pragma unroll
+            for( int k = 0; k < kSlotChunk; k++ ) {
+               if( c[ k ] < 0 )
+                  continue;
+               if( c[ k ] < r ) {
+                  const A t = ( alpha * a[ k ] ) * xr;
+                  if( c[ k ] >= w0 )
+                     atomicAdd( &acc[ wave ][ c[ k ] - w0 ], t );
+                  else
+                     unsafeAtomicAdd( y + c[ k ], (Y) t );
+               }
+               s = mulAddExact( s, a[ k ], xc[ k ] );
+            }
          }
          atomicAdd( &acc[ wave ][ r - w0 ], alpha * s );
       }
@@ -128,18 +145,29 @@ void launchSymmetric( const Matrix& M, const Y* x, Y* y, double alpha, double be
 template< typename View >
 __global__ __launch_bounds__( 256 ) void binaryKernel( int32_t first, int32_t last, View view,
                                                        const int32_t* __restrict__ col, const int32_t* __restrict__ x,
-                                                       int32_t* y, int32_t alpha, int32_t beta, Gate gate )
+                                                       int32_t* y, int32_t alpha, int32_t beta, int32_t ncols,
+                                                       Gate gate )
 {
    if( ! gate.open() )
       return;
    const int64_t stride = (int64_t) gridDim.x * blockDim.x;
    for( int64_t r = first + (int64_t) blockIdx.x * blockDim.x + threadIdx.x; r < last; r += stride ) {
-      const int32_t size = view.size( (int32_t) r );
-      uint32_t s = 0;  // int32 sum with wrap-around
-      for( int32_t l = 0; l < size; l++ ) {
-         const int32_t c = col[ view.global( (int32_t) r, l ) ];
-         if( c >= 0 )
-            s += (uint32_t) x[ c ];
+      // no columns: every slot is padding and x may be empty
+      const int32_t size = ncols > 0 ? view.size( (int32_t) r ) : 0;
+      uint32_t s = 0;  // int32 sum with wrap-around (any order gives the same bits)
+      for( int32_t l0 = 0; l0 < size; l0 += kSlotChunk ) {
+         // all loads of a chunk issued together: slots past the row re-read its last slot and
+         // padding reads x[0]; the select drops both
+         int32_t c[ kSlotChunk ];
+#pragma unroll
+         for( int k = 0; k < kSlotChunk; k++ )
+            c[ k ] = col[ view.global( (int32_t) r, min( l0 + k, size - 1 ) ) ];
+#pragma unroll
+         for( int k = 0; k < kSlotChunk; k++ ) {
+            const bool on = l0 + k < size && c[ k ] >= 0;
+            const uint32_t xv = (uint32_t) x[ on ? c[ k ] : 0 ];
+            s += on ? xv : 0u;
+         }
       }
       const int32_t v = (int32_t) s;
       if( beta == 0 )
@@ -160,7 +188,7 @@ void spmvBinary( const Matrix& M, const void* x, void* y, double alpha, double b
    withView( M, [ & ]( auto view ) {
       binaryKernel<<< gridFor( last - first, 256 ), 256, 0, st >>>( first, last, view, M.columns.ptr,
                                                                      (const int32_t*) x, (int32_t*) y,
-                                                                     (int32_t) alpha, (int32_t) beta, gate );
+                                                                     (int32_t) alpha, (int32_t) beta, M.cols, gate );
    } );
    TNLHIP_CHECK( hipGetLastError() );
 }
