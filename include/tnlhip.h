@@ -236,7 +236,8 @@ int tnlhip_matrix_set_param( tnlhip_matrix A, const char* name, int32_t value );
 
 /* SparseMatrix::vectorProduct( x, y, alpha, beta, firstRow, lastRow ):
  *   y[first, last) = alpha * A x + beta * y; last == 0 means all rows; beta == 0 does not
- *   read y.  Matrices/SparseMatrix.h:1078-1085, SparseMatrixView.hpp:390-491. */
+ *   read y.  x may be NULL when A has no columns (nothing is read from it), y when A has no
+ *   rows.  Matrices/SparseMatrix.h:1078-1085, SparseMatrixView.hpp:390-491. */
 int tnlhip_spmv( tnlhip_matrix A, const void* x, void* y, double alpha, double beta, int32_t first, int32_t last,
                  void* stream );
 
@@ -246,7 +247,9 @@ int tnlhip_spmv( tnlhip_matrix A, const void* x, void* y, double alpha, double b
  *   (entry row * k + j), 1 <= k <= 64; last == 0 means all rows; beta == 0 does not read Y.
  *   Column j of Y equals the one-vector product of column j of X in TNL's Host order (each product
  *   rounded, the row's slots summed left to right), bit for bit.  Any format; general real-valued
- *   matrices (TNLHIP_ERROR_UNSUPPORTED for symmetric and binary ones). */
+ *   matrices (TNLHIP_ERROR_UNSUPPORTED for symmetric and binary ones).  X may be NULL when A has
+ *   no columns, Y when it has no rows.  X and Y 16-byte aligned with k a multiple of 2 (FP64) or
+ *   4 (FP32 / BF16 values) take the 16-byte-lane kernel; any other layout works, more slowly. */
 int tnlhip_spmv_multi( tnlhip_matrix A, int32_t k, const void* X, void* Y, double alpha, double beta, int32_t first,
                        int32_t last, void* stream );
 

@@ -190,11 +190,13 @@ __device__ __forceinline__ void rowProducts( const Arena& A, const ReqStatic& R,
 #pragma unroll
             for( int k = 0; k < kSlotChunk; k++ ) {
                // every load is issued (clamped slot; row 0 of x for padding and for slots past
-               // the row) so that all of them are in flight together; the select drops them
+               // the row) so that all of them are in flight together, and the product is formed
+               // unconditionally: a select between it and 0 is turned into a branch around the
+               // loads, one round trip per slot; the sum below skips the dropped slots
                const bool on = k0 + k < width[ u ] && col[ u ][ k ] >= 0;
                const Acc a = (Acc) vals[ idx[ u ] + 32 * (int64_t) min( k0 + k, last[ u ] ) ];
                const Acc b = (Acc) v[ (int64_t) ( on ? col[ u ][ k ] : 0 ) * R.k + c.j ];
-               term[ u ][ k ] = on ? a * b : (Acc) 0;
+               term[ u ][ k ] = a * b;
                col[ u ][ k ] = on ? 0 : -1;
             }
 #pragma unroll
@@ -281,7 +283,7 @@ __device__ __forceinline__ void rowProductsVec( const Arena& A, const ReqStatic&
                const bool on = k0 + k < width[ u ] && col[ u ][ k ] >= 0;
                const T a = vals[ idx[ u ] + 32 * (int64_t) min( k0 + k, last[ u ] ) ];
                const VT b = *reinterpret_cast< const VT* >( vj + (int64_t) ( on ? col[ u ][ k ] : 0 ) * R.k );
-               term[ u ][ k ] = on ? a * b : (VT) 0;
+               term[ u ][ k ] = a * b;  // unconditional, as in rowProducts
                col[ u ][ k ] = on ? 0 : -1;
             }
 #pragma unroll
