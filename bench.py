@@ -632,7 +632,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-ladder", type=int, default=32, help="resident matrix sizes per dimension")
     ap.add_argument("--serve-gmres-fraction", type=float, default=0.2)
-    ap.add_argument("--serve-gmres-workers", type=int, default=4)
+    ap.add_argument("--serve-gmres-workers", type=int, default=8,
+                    help="host threads of the server's GMRES track (4: 163 solves/s, 8: 185, 16: 184)")
     ap.add_argument("--serve-uniform-requests", type=int, default=8)
     ap.add_argument("--serve-entries", type=int, default=256 << 20,
                     help="vector arena of the server in (row, right-hand side) entries, 40 B each")

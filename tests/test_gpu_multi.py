@@ -117,3 +117,24 @@ def test_binary_product_without_columns(tnl, dev):
         y = y0.clone()
         B.vectorProduct(torch.zeros(0, dtype=torch.int32, device=dev), y, alpha, beta)
         assert torch.equal(y, beta * y0)
+
+
+def test_block_product_unaligned_blocks_take_the_scalar_lanes(tnl, dev):
+    """X and Y starting 8 bytes past a 16-byte boundary (FP64, k even) take the one-column-per-lane
+    kernel instead of the 16-byte lanes; both give the oracle's bits."""
+    rp, c, v = W.irregular(2501, 8.0, 1.0, 120, seed=21)
+    n, k = len(rp) - 1, 6
+    X = np.random.default_rng(21).uniform(-1, 1, (n, k))
+    want = oracle_columns(rp, c, v, X)
+    A = tnl.SparseMatrix.from_csr(rp, c, v, format="SlicedEllpack", dtype="f64")
+    Xbig = torch.zeros(n * k + 1, dtype=torch.float64, device=dev)
+    Xbig[1:] = torch.tensor(X.ravel(), device=dev)
+    Ybig = torch.full((n * k + 1,), np.nan, dtype=torch.float64, device=dev)
+    Xu, Yu = Xbig[1:].view(n, k), Ybig[1:].view(n, k)
+    assert Xu.data_ptr() % 16 == 8 and Yu.data_ptr() % 16 == 8
+    A.vectorProductMulti(Xu, Yu)
+    Ya = torch.full((n, k), np.nan, dtype=torch.float64, device=dev)
+    A.vectorProductMulti(torch.tensor(X, device=dev), Ya)
+    torch.cuda.synchronize()
+    assert np.array_equal(Yu.cpu().numpy(), want)
+    assert np.array_equal(Ya.cpu().numpy(), want)

@@ -247,23 +247,23 @@ __device__ __forceinline__ void rowProductsVec( const Arena& A, const ReqStatic&
                                                 const T* v, typename VecOf< T, V >::type ( &sum )[ kRowsPerLane / V ] )
 {
    using VT = typename VecOf< T, V >::type;
+   // the slice offsets of all the lane's rows in one round trip (rows past n read the last
+   // row's slice and get width 0; an empty row reads slot 0 of the arena, never used)
+   constexpr int U = kRowsPerLane / V;
+   int32_t wAll[ U ];
+   int64_t iAll[ U ];
 #pragma unroll
-   for( int u0 = 0; u0 < kRowsPerLane / V; u0 += 2 ) {
-      int32_t width[ 2 ];
-      int64_t idx[ 2 ];
+   for( int u = 0; u < U; u++ ) {
+      const int32_t row = c.base + u * c.rpp + c.lr;
+      const int32_t s = min( row, R.n - 1 ) >> 5;
+      const int32_t b0 = A.slices[ R.sliceBase + s ], b1 = A.slices[ R.sliceBase + s + 1 ];
+      wAll[ u ] = row < R.n ? ( b1 - b0 ) >> 5 : 0;
+      iAll[ u ] = wAll[ u ] > 0 ? R.mat + b0 + ( row & 31 ) : 0;
+   }
 #pragma unroll
-      for( int u = 0; u < 2; u++ ) {
-         const int32_t row = c.base + ( u0 + u ) * c.rpp + c.lr;
-         width[ u ] = 0;
-         idx[ u ] = 0;
-         if( row < R.n ) {
-            const int32_t s = row >> 5;
-            const int32_t b0 = A.slices[ R.sliceBase + s ], b1 = A.slices[ R.sliceBase + s + 1 ];
-            width[ u ] = ( b1 - b0 ) >> 5;
-            if( width[ u ] > 0 )  // else slot 0 of the arena: loaded, never used
-               idx[ u ] = R.mat + b0 + ( row & 31 );
-         }
-      }
+   for( int u0 = 0; u0 < U; u0 += 2 ) {
+      int32_t width[ 2 ] = { wAll[ u0 ], wAll[ u0 + 1 ] };
+      int64_t idx[ 2 ] = { iAll[ u0 ], iAll[ u0 + 1 ] };
       VT acc[ 2 ] = { (VT) 0, (VT) 0 };
       const int32_t maxw = max( width[ 0 ], width[ 1 ] );
       const int32_t last[ 2 ] = { max( width[ 0 ] - 1, 0 ), max( width[ 1 ] - 1, 0 ) };
