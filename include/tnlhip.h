@@ -518,6 +518,19 @@ void tnlhip_refinement_params_default( tnlhip_refinement_params* params );
 int tnlhip_refine_solve( tnlhip_matrix A, tnlhip_matrix A_low, const tnlhip_refinement_params* params,
                          const double* b, double* x, tnlhip_refinement_result* result, void* stream );
 
+/* A refinement context keeps the outer loop's vectors and the inner solver (its Krylov basis,
+ * reductions scratch) between solves, so repeated solves allocate and free nothing after the
+ * first one of the largest size (tnlhip_refine_solve allocates and frees them on every call, and
+ * a device free waits for the whole device).  No TNL counterpart.  A context is used by one host
+ * thread at a time, on the device it was first used on; it holds its buffers until destroyed. */
+typedef struct tnlhip_refine_context_s* tnlhip_refine_context;
+int tnlhip_refine_context_create( tnlhip_refine_context* out );
+int tnlhip_refine_context_destroy( tnlhip_refine_context ctx );
+/* tnlhip_refine_solve on the context's buffers (grown as needed) */
+int tnlhip_refine_solve_in( tnlhip_refine_context ctx, tnlhip_matrix A, tnlhip_matrix A_low,
+                            const tnlhip_refinement_params* params, const double* b, double* x,
+                            tnlhip_refinement_result* result, void* stream );
+
 /* ------------------------------------------------------------------ autotuner */
 
 /* One timed configuration: format, lanes per row (CSR light / hybrid; 0 = the kernel's

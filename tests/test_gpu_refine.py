@@ -68,3 +68,25 @@ def test_refinement_with_default_inner_target(tnl, dev, peclet):
     assert np.linalg.norm(b - spmv_host(rp, c, v, xd.cpu().numpy())) / np.linalg.norm(b) < 1.01e-10
     h = r["history"]
     assert all(h[i + 1] < 0.1 * h[i] for i in range(len(h) - 1))
+
+
+def test_refine_context_matches_the_one_shot_solve(tnl, dev):
+    """tnlhip_refine_solve_in on one context, twice in a row (the second system smaller, then the
+    first again), gives the same bits as tnlhip_refine_solve: the context only keeps buffers."""
+    ctx = tnl.RefineContext()
+    kw = dict(tolerance=1e-10, inner_method="gmres", inner_dtype="f32", convergence_residue=1e-2,
+              max_iterations=3000, restarting_min=30, restarting_max=30, preconditioner=3, poly_degree=4)
+    for N, pe in ((20, 10.0), (12, 100.0), (20, 10.0)):
+        rp, c, v = W.convection_diffusion_3d(N, pe)
+        n = len(rp) - 1
+        A = tnl.SparseMatrix.from_csr(rp, c, v, format="CSRAdaptive", dtype="f64")
+        A32 = tnl.SparseMatrix.from_csr(rp, c, v, format="CSRAdaptive", dtype="f32")
+        b = torch.tensor(W.uniform_vector(n, N), device=dev)
+        x1 = torch.zeros(n, dtype=torch.float64, device=dev)
+        x2 = torch.zeros(n, dtype=torch.float64, device=dev)
+        r1 = tnl.refine_solve(A, b, x1, A_low=A32, **kw)
+        r2 = tnl.refine_solve(A, b, x2, A_low=A32, context=ctx, **kw)
+        assert r1["converged"] and r2["converged"]
+        assert (r1["outer_iterations"], r1["inner_iterations"]) == (r2["outer_iterations"], r2["inner_iterations"])
+        assert torch.equal(x1, x2), (N, pe)
+    ctx.close()

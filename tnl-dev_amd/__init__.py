@@ -33,7 +33,8 @@ EXPORTS = [
     "tnlhip_gemv", "tnlhip_exclusive_scan_i32", "tnlhip_solver_params_default", "tnlhip_solver_create",
     "tnlhip_solver_destroy", "tnlhip_solver_set_matrix", "tnlhip_solver_set_params", "tnlhip_solver_solve",
     "tnlhip_solver_get_status", "tnlhip_solver_get_history", "tnlhip_refinement_params_default",
-    "tnlhip_refine_solve", "tnlhip_server_config_default", "tnlhip_request_params_default",
+    "tnlhip_refine_solve", "tnlhip_refine_context_create", "tnlhip_refine_context_destroy",
+    "tnlhip_refine_solve_in", "tnlhip_server_config_default", "tnlhip_request_params_default",
     "tnlhip_server_footprint", "tnlhip_server_create", "tnlhip_server_destroy", "tnlhip_server_add_matrix",
     "tnlhip_server_remove_matrix", "tnlhip_server_submit", "tnlhip_server_progress", "tnlhip_server_result",
     "tnlhip_server_stats", "tnlhip_autotune_csr", "tnlhip_mtx_read_csr", "tnlhip_mtx_read", "tnlhip_mtx_write",
@@ -186,6 +187,10 @@ def _load():
         "tnlhip_refinement_params_default": (None, [C.POINTER(RefinementParams)]),
         "tnlhip_refine_solve": (C.c_int, [vp, vp, C.POINTER(RefinementParams), vp, vp,
                                           C.POINTER(RefinementResult), vp]),
+        "tnlhip_refine_context_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+        "tnlhip_refine_context_destroy": (C.c_int, [vp]),
+        "tnlhip_refine_solve_in": (C.c_int, [vp, vp, vp, C.POINTER(RefinementParams), vp, vp,
+                                             C.POINTER(RefinementResult), vp]),
         "tnlhip_server_config_default": (None, [C.POINTER(ServerConfig)]),
         "tnlhip_request_params_default": (None, [C.POINTER(RequestParams)]),
         "tnlhip_server_footprint": (C.c_int, [i32, i64, i32, i32, C.POINTER(i64), C.POINTER(i64)]),
@@ -726,10 +731,30 @@ class BICGStabL(LinearSolver):
     method = "bicgstabl"
 
 
+class RefineContext:
+    """Buffers and inner solver of refine_solve kept between solves (no TNL counterpart): pass it
+    as refine_solve(..., context=ctx) and only the first solve of the largest size allocates."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib.tnlhip_refine_context_create(C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.tnlhip_refine_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
 def refine_solve(A, b, x, A_low=None, tolerance=1e-10, inner_method="gmres", inner_dtype="f32",
-                 max_outer_iterations=50, stagnation_limit=2, stream=None, poly_dtype=None, P=None, **inner):
+                 max_outer_iterations=50, stagnation_limit=2, stream=None, poly_dtype=None, P=None, context=None,
+                 **inner):
     """FP64 iterative refinement around a low-precision inner solve (no TNL counterpart).
-    Returns a dict with converged, outer_iterations, inner_iterations, residue, history."""
+    Returns a dict with converged, outer_iterations, inner_iterations, residue, history.  With a
+    RefineContext the solve reuses its buffers and inner solver."""
     p = RefinementParams()
     lib.tnlhip_refinement_params_default(C.byref(p))
     p.tolerance = float(tolerance)
@@ -746,8 +771,12 @@ def refine_solve(A, b, x, A_low=None, tolerance=1e-10, inner_method="gmres", inn
             v = GMRES_VARIANTS[v]
         setattr(p.inner, k, v)
     r = RefinementResult()
-    _check(lib.tnlhip_refine_solve(A._h, A_low._h if A_low is not None else None, C.byref(p), _ptr(b), _ptr(x),
-                                   C.byref(r), _stream(stream)))
+    low = A_low._h if A_low is not None else None
+    if context is not None:
+        _check(lib.tnlhip_refine_solve_in(context._h, A._h, low, C.byref(p), _ptr(b), _ptr(x), C.byref(r),
+                                          _stream(stream)))
+    else:
+        _check(lib.tnlhip_refine_solve(A._h, low, C.byref(p), _ptr(b), _ptr(x), C.byref(r), _stream(stream)))
     return {"converged": bool(r.converged), "outer_iterations": r.outer_iterations,
             "inner_iterations": r.inner_iterations, "residue": r.residue,
             "history": list(r.history[:min(r.outer_iterations + 1, REFINEMENT_HISTORY)])}

@@ -229,4 +229,38 @@ int tnlhip_refine_solve( tnlhip_matrix A, tnlhip_matrix A_low, const tnlhip_refi
    } );
 }
 
+struct tnlhip_refine_context_s
+{
+   RefineContext ctx;
+   int device = -1;
+};
+
+int tnlhip_refine_context_create( tnlhip_refine_context* out )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( out, "NULL argument" );
+      *out = new tnlhip_refine_context_s();
+   } );
+}
+
+int tnlhip_refine_context_destroy( tnlhip_refine_context c )
+{
+   return guarded( [ & ] { delete c; } );
+}
+
+int tnlhip_refine_solve_in( tnlhip_refine_context c, tnlhip_matrix A, tnlhip_matrix A_low,
+                            const tnlhip_refinement_params* p, const double* b, double* x,
+                            tnlhip_refinement_result* result, void* stream_ )
+{
+   return guarded( [ & ] {
+      TNLHIP_REQUIRE( c && A && p && b && x && result, "NULL argument" );
+      int dev = 0;
+      TNLHIP_CHECK( hipGetDevice( &dev ) );
+      if( c->device < 0 )
+         c->device = dev;
+      TNLHIP_REQUIRE( c->device == dev, "a refinement context serves the device it was first used on" );
+      refineWith( c->ctx, A, A_low, p, b, x, result, (hipStream_t) stream_ );
+   } );
+}
+
 }  // extern "C"
