@@ -639,7 +639,7 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=1e-2, help="inner relative residual of the refinement")
     ap.add_argument("--solver-reps", type=int, default=3, help="solves per configuration in the solver suite (median)")
     ap.add_argument("--gmres-dtype", default="f64")
-    ap.add_argument("--poly-degree", type=int, default=4, help="Jacobi-polynomial degree of the poly-refined runs")
+    ap.add_argument("--poly-degree", type=int, default=3, help="Jacobi-polynomial degree of the poly-refined runs")
     ap.add_argument("--bf16-operator", action="store_true",
                     help="also refine over a BF16 inner operator (converges only when cond(A) * 2^-8 < 1)")
     ap.add_argument("--serve-inner-tol", type=float, default=3e-6)

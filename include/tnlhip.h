@@ -413,7 +413,11 @@ typedef struct
                               Jacobi splitting cut after poly_degree products with P (z_0 = D^-1 s,
                               z_{j+1} = z_j + D^-1 (s - P z_j)), D the diagonal of the solved matrix,
                               P its preconditioner matrix (tnlhip_solver_set_preconditioner_matrix;
-                              the solved matrix itself by default).  0..32, default 4. */
+                              the solved matrix itself by default).  0..32, default 3: with k
+                              products the preconditioned operator is I - (I - D^-1 P)^(k+1), and
+                              an odd k (even power) keeps its eigenvalues inside (0, 1] for a
+                              splitting whose Jacobi matrix has spectrum in (-1, 1); an even k lets
+                              them reach 2 (twice the condition number). */
 } tnlhip_solver_params;
 
 void tnlhip_solver_params_default( tnlhip_solver_params* params );

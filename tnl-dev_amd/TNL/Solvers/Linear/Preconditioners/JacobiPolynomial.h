@@ -19,7 +19,7 @@ class JacobiPolynomial : public Preconditioner< Matrix >
 public:
    using MatrixPointer = std::shared_ptr< const Matrix >;
 
-   explicit JacobiPolynomial( int degree = 4, int valueType = -1 ) : degree( degree ), valueType( valueType ) {}
+   explicit JacobiPolynomial( int degree = 3, int valueType = -1 ) : degree( degree ), valueType( valueType ) {}
    JacobiPolynomial( const JacobiPolynomial& ) = delete;
    JacobiPolynomial& operator=( const JacobiPolynomial& ) = delete;
    ~JacobiPolynomial() override { release(); }
@@ -54,7 +54,7 @@ private:
          tnlhip_matrix_destroy( copy );
       copy = nullptr;
    }
-   int degree = 4;
+   int degree = 3;
    int valueType = -1;
    tnlhip_matrix copy = nullptr;
 };

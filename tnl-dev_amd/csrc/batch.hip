@@ -31,7 +31,7 @@
 // each right-hand side is a job in a FIFO that `gmres_workers` host threads, each with its own
 // non-blocking HIP stream, take as they free up, so the k right-hand sides of a request run side
 // by side.  A job is FP64 refinement over an FP32 GMRES(30) inner solve of (A + shift I) with the
-// degree-4 Jacobi polynomial over BF16 values as the inner preconditioner.  Each worker keeps,
+// degree-3 Jacobi polynomial over BF16 values as the inner preconditioner.  Each worker keeps,
 // per resident matrix, FP64 / FP32 / BF16 CSR copies of A with the layout of the resident FP64
 // CSR copy; a job only rewrites their diagonal entries to a + shift (rounded from the FP64
 // value), and every buffer grows and stays, so after the first job on a size nothing is
@@ -1191,7 +1191,7 @@ tnlhip_request_result tnlhip_server_s::solveGmresRhs( const Request& q, int j, h
       tnlhip_refinement_params rp_{};
       tnlhip_refinement_params_default( &rp_ );
       rp_.inner.preconditioner = 3;
-      rp_.inner.poly_degree = 4;
+      rp_.inner.poly_degree = 3;
       rp_.poly_matrix = o.A16.m;
       rp_.tolerance = q.params.tolerance;
       rp_.max_outer_iterations = q.params.max_outer_iterations;

@@ -2503,7 +2503,7 @@ void tnlhip_solver_params_default( tnlhip_solver_params* p )
    p->preconditioner = 0;
    p->use_graph = 1;
    p->bicgstab_ell = 1;  // BICGStabL.hpp:21
-   p->poly_degree = 4;
+   p->poly_degree = 3;
 }
 
 int tnlhip_solver_create( int method, tnlhip_solver* out )
