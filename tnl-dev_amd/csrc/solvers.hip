@@ -2111,7 +2111,15 @@ struct OpCwyZGenerate
    const double* auxp;
    int count;  // columns 0..i (count = i + 1 <= C - 1)
    T* ynext;
-   __device__ void load() { auxp = stageCoefficients( auxp, count ); }
+   // staged in LDS except for the 32-column FP32 pass: there the compiler keeps the staged
+   // coefficients in registers across the loop (209 instead of 139 VGPRs, 2 waves per SIMD
+   // instead of 3), which measured 55 against 46 µs; the FP64 pass, whose per-row coefficient
+   // loads had stalled it, gains more (115 -> 88 µs) than it loses
+   __device__ void load()
+   {
+      if constexpr( sizeof( T ) == sizeof( double ) || C < 32 )
+         auxp = stageCoefficients( auxp, count );
+   }
    __device__ void operator()( int64_t j, double ( &acc )[ C ] ) const
    {
       T y[ C - 1 ];
