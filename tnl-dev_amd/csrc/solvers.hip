@@ -669,6 +669,108 @@ struct CgDirectionOp
    __device__ void operator()( int64_t i ) const { p[ i ] = (T) ( (double) zr[ i ] + beta * (double) p[ i ] ); }
 };
 
+// The sum a reduction's final kernel would form, formed instead by every workgroup of the next
+// kernel from the same partials in the same order (sumPartials), so each workgroup holds the
+// same bits; the next kernel's workgroup 0 then does the final kernel's state update.  It saves
+// the final kernel's launch; no workgroup may read a state field that workgroup 0 of the same
+// launch writes.
+__device__ __forceinline__ double foldPartials1( ReduceSlot slot, int grid )
+{
+   __shared__ double tot[ 1 ];
+   sumPartials< 1 >( slot, grid, tot );
+   return tot[ 0 ];
+}
+
+__device__ __forceinline__ bool leadLane()
+{
+   return blockIdx.x == 0 && threadIdx.x == 0;
+}
+
+template< typename T >
+struct CgUpdateFoldOp
+{
+   // FinCgAlpha from the (p, Ap) partials, then CgUpdateOp; workgroup 0 keeps the state (the
+   // previous rho moves to rho_old, read by the direction pass that follows)
+   T* x;
+   T* r;
+   T* z;
+   const T* p;
+   const T* Ap;
+   const T* diag;
+   State* st;
+   ReduceSlot dot;
+   int dotGrid;
+   double* hist;
+   int64_t cap;
+   double alpha = 0;
+   bool skip = false;
+   __device__ void load()
+   {
+      const double pAp = foldPartials1( dot, dotGrid );
+      const double rho = st->s1;
+      skip = pAp == 0.0;
+      alpha = skip ? 0.0 : rho / pAp;
+      if( leadLane() ) {
+         State& s = *st;
+         s.s2 = pAp;
+         s.rho_old = rho;
+         if( skip ) {
+            // p = 0 => r = 0: the solution has been reached (CG.hpp:64-67)
+            setResidue( s, 0.0, hist, cap );
+            s.flags[ 0 ] = 1;
+         }
+         else
+            s.alpha = alpha;
+      }
+   }
+   __device__ void operator()( int64_t i, double ( &acc )[ 1 ] ) const
+   {
+      if( skip )
+         return;
+      x[ i ] = (T) ( (double) x[ i ] + alpha * (double) p[ i ] );
+      const T ri = (T) ( (double) r[ i ] - alpha * (double) Ap[ i ] );
+      r[ i ] = ri;
+      T zi = ri;
+      if( diag ) {
+         zi = ri / diag[ i ];
+         z[ i ] = zi;
+      }
+      acc[ 0 ] += (double) ri * (double) zi;
+   }
+};
+
+template< typename T >
+struct CgDirectionFoldOp
+{
+   // FinCgBeta from the (r, z) partials, then CgDirectionOp; workgroup 0 keeps the state
+   T* p;
+   const T* zr;
+   State* st;
+   ReduceSlot upd;
+   int updGrid;
+   double* hist;
+   int64_t cap;
+   Stop stop;
+   double beta = 0;
+   __device__ void load()
+   {
+      const double rho = foldPartials1( upd, updGrid );
+      const double rhoOld = st->rho_old;
+      beta = rhoOld == 0.0 ? 0.0 : rho / rhoOld;
+      if( leadLane() ) {
+         State& s = *st;
+         s.s2 = rhoOld;
+         s.s1 = rho;
+         s.beta = beta;
+         setResidue( s, sqrt( rho ) / s.normb, hist, cap );
+         if( ! checkNext( s, stop ) )
+            s.flags[ 0 ] = 1;
+         s.it++;
+      }
+   }
+   __device__ void operator()( int64_t i ) const { p[ i ] = (T) ( (double) zr[ i ] + beta * (double) p[ i ] ); }
+};
+
 template< typename T >
 struct CgInitFromZOp : NoScalars
 {
@@ -751,13 +853,20 @@ void solveCG( SolverImpl& S, const void* b, void* xv )
    residual< T >( S, b, x, r, always, false );  // CG.hpp:40-41: r = b - A x
    launchReduce< 1 >( S.n, CgInitOp< T >{ {}, p, z, r, diag }, FinCgInit{ S.state.ptr, S.hist.ptr, S.histCap, stop },
                       S.ws.slot( 0 ), always, S.stream );
+   // four launches per iteration: the two final-sum kernels are folded into the passes that
+   // follow them (CgUpdateFoldOp, CgDirectionFoldOp), same sums, same bits
+   const int grid = reduceGridFor( 1, S.n );
+   const ReduceSlot dot = S.ws.slot( 1 ), upd = S.ws.slot( 2 );
    runChunks( S, [ & ] {
       spmvLaunchGated( *S.A, p, Ap, 1.0, 0.0, nullptr, running.flags, running.mode, S.stream );
-      launchReduce< 1 >( S.n, OpDot< T >{ {}, Ap, p }, FinCgAlpha{ S.state.ptr, S.hist.ptr, S.histCap },
-                         S.ws.slot( 1 ), running, S.stream );
-      launchReduce< 1 >( S.n, CgUpdateOp< T >{ x, r, z, p, Ap, diag, S.state.ptr, 0 },
-                         FinCgBeta{ S.state.ptr, S.hist.ptr, S.histCap, stop }, S.ws.slot( 2 ), running, S.stream );
-      launchElementwise( S.n, CgDirectionOp< T >{ p, diag ? z : r, S.state.ptr, 0 }, running, S.stream );
+      fusedReduceKernel< 1 ><<< grid, kReduceBlock, 0, S.stream >>>( S.n, OpDot< T >{ {}, Ap, p }, dot, running );
+      fusedReduceKernel< 1 ><<< grid, kReduceBlock, 0, S.stream >>>(
+         S.n, CgUpdateFoldOp< T >{ x, r, z, p, Ap, diag, S.state.ptr, dot, grid, S.hist.ptr, S.histCap }, upd,
+         running );
+      elementwiseKernel<<< gridFor( S.n, kReduceBlock ), kReduceBlock, 0, S.stream >>>(
+         S.n, CgDirectionFoldOp< T >{ p, diag ? z : r, S.state.ptr, upd, grid, S.hist.ptr, S.histCap, stop },
+         running );
+      TNLHIP_CHECK( hipGetLastError() );
    } );
 }
 
