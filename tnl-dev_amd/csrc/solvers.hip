@@ -681,6 +681,14 @@ __device__ __forceinline__ double foldPartials1( ReduceSlot slot, int grid )
    return tot[ 0 ];
 }
 
+__device__ __forceinline__ void foldPartials2( ReduceSlot slot, int grid, double ( &v )[ 2 ] )
+{
+   __shared__ double tot[ 2 ];
+   sumPartials< 2 >( slot, grid, tot );
+   v[ 0 ] = tot[ 0 ];
+   v[ 1 ] = tot[ 1 ];
+}
+
 __device__ __forceinline__ bool leadLane()
 {
    return blockIdx.x == 0 && threadIdx.x == 0;
@@ -927,6 +935,27 @@ struct OpBiS
    __device__ void operator()( int64_t i ) const { s[ i ] = (T) ( (double) r[ i ] - alpha * (double) Ap[ i ] ); }
 };
 
+// FinBiAlpha folded into OpBiS: alpha = rho / (Ap, r_ast) from the partials in every workgroup
+// (the final kernel's sum and bits), workgroup 0 stores it for the passes that follow
+template< typename T >
+struct OpBiSFold
+{
+   T* s;
+   const T* r;
+   const T* Ap;
+   State* st;
+   ReduceSlot dot;
+   int dotGrid;
+   double alpha = 0;
+   __device__ void load()
+   {
+      alpha = st->rho / foldPartials1( dot, dotGrid );
+      if( leadLane() )
+         st->alpha = alpha;
+   }
+   __device__ void operator()( int64_t i ) const { s[ i ] = (T) ( (double) r[ i ] - alpha * (double) Ap[ i ] ); }
+};
+
 template< typename T >
 struct OpBiOmega : NoScalars
 {
@@ -972,6 +1001,27 @@ struct OpBiUpdate
       acc[ 0 ] += (double) ri * (double) rast[ i ];
       acc[ 1 ] += (double) ri * (double) ri;
    }
+};
+
+// FinBiOmega folded into OpBiUpdate: omega = (As, s) / (As, As) from the partials in every
+// workgroup, workgroup 0 stores it (alpha comes from the state, stored by OpBiSFold's launch)
+template< typename T >
+struct OpBiUpdateFold
+{
+   OpBiUpdate< T > u;
+   State* st;
+   ReduceSlot om;
+   int omGrid;
+   __device__ void load()
+   {
+      double v[ 2 ];
+      foldPartials2( om, omGrid, v );
+      u.alpha = st->alpha;
+      u.omega = v[ 0 ] / v[ 1 ];
+      if( leadLane() )
+         st->omega = u.omega;
+   }
+   __device__ void operator()( int64_t i, double ( &acc )[ 2 ] ) const { u( i, acc ); }
 };
 
 struct FinBiRho
@@ -1047,10 +1097,15 @@ struct OpBiDirection
    const T* Ap;
    const State* st;
    double beta, bo;
+   State* promote = nullptr;  // set: FinPromoteStop done here by workgroup 0 (one launch fewer)
    __device__ void load()
    {
       beta = st->beta;
       bo = st->beta * st->omega;
+      // stop was set by an earlier launch; workgroups of this one that see the flag already
+      // skip their share of p, which no later step reads once the solve stops
+      if( promote && leadLane() && promote->stop )
+         promote->flags[ 0 ] = 1;
    }
    __device__ void operator()( int64_t i ) const
    {
@@ -1104,14 +1159,20 @@ void solveBiCGStab( SolverImpl& S, const void* b, void* xv )
    residual< T >( S, b, x, r, always );
    launchReduce< 1 >( S.n, OpCopy2< T >{ {}, p, rast, s, r }, FinRhoInit{ S.state.ptr, S.hist.ptr, S.histCap, stop },
                       S.ws.slot( 0 ), always, S.stream );
+   const int g1 = reduceGridFor( 1, S.n ), g2 = reduceGridFor( 2, S.n );
    runChunks( S, [ & ] {
+      // the final sums of (Ap, r_ast) and of omega's two dots are folded into the passes that
+      // follow them (OpBiSFold, OpBiUpdateFold): same partials, same order, same bits
       matvec( S, p, Ap, running );
-      launchReduce< 1 >( S.n, OpDot< T >{ {}, Ap, rast }, FinBiAlpha{ S.state.ptr }, S.ws.slot( 0 ), running, S.stream );
-      launchElementwise( S.n, OpBiS< T >{ s, r, Ap, S.state.ptr, 0 }, running, S.stream );
+      fusedReduceKernel< 1 ><<< g1, kReduceBlock, 0, S.stream >>>( S.n, OpDot< T >{ {}, Ap, rast }, S.ws.slot( 0 ),
+                                                                   running );
+      launchElementwise( S.n, OpBiSFold< T >{ s, r, Ap, S.state.ptr, S.ws.slot( 0 ), g1 }, running, S.stream );
       matvec( S, s, As, running );
-      launchReduce< 2 >( S.n, OpBiOmega< T >{ {}, As, s }, FinBiOmega{ S.state.ptr }, S.ws.slot( 1 ), running,
-                         S.stream );
-      launchReduce< 2 >( S.n, OpBiUpdate< T >{ x, r, p, s, As, rast, S.state.ptr, 0, 0 },
+      fusedReduceKernel< 2 ><<< g2, kReduceBlock, 0, S.stream >>>( S.n, OpBiOmega< T >{ {}, As, s }, S.ws.slot( 1 ),
+                                                                   running );
+      launchReduce< 2 >( S.n,
+                         OpBiUpdateFold< T >{ OpBiUpdate< T >{ x, r, p, s, As, rast, S.state.ptr, 0, 0 }, S.state.ptr,
+                                              S.ws.slot( 1 ), g2 },
                          FinBiRho{ S.state.ptr, S.hist.ptr, S.histCap, stop, eps * eps, exact }, S.ws.slot( 2 ),
                          running, S.stream );
       // restart path (rare): r = M^{-1}(b - A x); r_ast = r; rho = r_ast_sqnorm = (r, r)
@@ -1119,13 +1180,14 @@ void solveBiCGStab( SolverImpl& S, const void* b, void* xv )
       launchReduce< 1 >( S.n, OpBiRestartCopy< T >{ {}, rast, r },
                          FinBiRestart{ S.state.ptr, S.hist.ptr, S.histCap, stop, exact }, S.ws.slot( 3 ), restart,
                          S.stream );
-      launchElementwise( S.n, OpBiDirection< T >{ p, r, Ap, S.state.ptr, 0, 0 }, running, S.stream );
+      launchElementwise( S.n, OpBiDirection< T >{ p, r, Ap, S.state.ptr, 0, 0, exact ? nullptr : S.state.ptr },
+                         running, S.stream );
       if( exact ) {
          residual< T >( S, b, x, s, running );
          launchReduce< 1 >( S.n, OpSquares< T >{ {}, s }, FinExactResidue{ S.state.ptr, S.hist.ptr, S.histCap, stop },
                             S.ws.slot( 0 ), running, S.stream );
+         launchScalar( FinPromoteStop{ S.state.ptr }, running, S.stream );
       }
-      launchScalar( FinPromoteStop{ S.state.ptr }, running, S.stream );
    } );
 }
 
