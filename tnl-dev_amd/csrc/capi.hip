@@ -152,6 +152,12 @@ tnlhip_matrix_s* buildFromSegments( int format, int dtype, int32_t rows, int32_t
    // 70 -> 72 us and FP32 Ellpack 49.4 -> 52.1 us, so those keep the plain order
    if( ( format == TNLHIP_CHUNKED_ELLPACK || format == TNLHIP_SLICED_ELLPACK ) && dtype != TNLHIP_F64 )
       M->xcdRemap = 1;
+   // FP64 Ellpack: each XCD strides over its own eighth of the row blocks (distribution 3):
+   // 160^3, three alternated runs on one box, 78.4 -> 76.1 us; the other FP64 formats measured
+   // slower with an XCD split (SlicedEllpack 75.9 -> 79.1, CSRScalar 79.7 -> 82.9, CSRAdaptive
+   // 82.4 -> 84.9, ChunkedEllpack 72.0 -> 75.1 us) and keep the plain grid-stride
+   if( format == TNLHIP_ELLPACK && dtype == TNLHIP_F64 )
+      M->xcdRemap = 3;
    if( format == TNLHIP_CSR_LIGHT )
       M->lanes = lightLanes( capTotal, rows );
    else if( format == TNLHIP_CSR_HYBRID || format == TNLHIP_CSR_ADAPTIVE )
