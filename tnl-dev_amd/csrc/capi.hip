@@ -158,6 +158,16 @@ tnlhip_matrix_s* buildFromSegments( int format, int dtype, int32_t rows, int32_t
    // 82.4 -> 84.9, ChunkedEllpack 72.0 -> 75.1 us) and keep the plain grid-stride
    if( format == TNLHIP_ELLPACK && dtype == TNLHIP_F64 )
       M->xcdRemap = 3;
+   // FP64 launch shapes from a sweep of workgroups per CU (8-64) and one or two rows per lane at
+   // 160^3 (two alternated runs): Ellpack 32 workgroups per CU, 76.6 -> 73.7 us; SlicedEllpack two
+   // rows per lane and 32 workgroups per CU, 76.1 -> 73.2 us (each row still summed left to right by
+   // one lane, so the bits do not change)
+   if( format == TNLHIP_ELLPACK && dtype == TNLHIP_F64 )
+      M->blocksPerCu = 32;
+   if( format == TNLHIP_SLICED_ELLPACK && dtype == TNLHIP_F64 ) {
+      M->pairRows = 1;
+      M->blocksPerCu = 32;
+   }
    if( format == TNLHIP_CSR_LIGHT )
       M->lanes = lightLanes( capTotal, rows );
    else if( format == TNLHIP_CSR_HYBRID || format == TNLHIP_CSR_ADAPTIVE )

@@ -828,7 +828,10 @@ void launchTyped( const Matrix& M, const Y* x, Y* y, double alpha, double beta, 
       case TNLHIP_ELLPACK:
       case TNLHIP_SLICED_ELLPACK: {
          // grid a multiple of the XCD count; contiguous row blocks when dist != 0
-         const bool pair = M.pairRows != 0;
+         // a row pair is one 8- / 16-byte load of two vertically adjacent slots: SlicedEllpack
+         // with slice size 1 has no such neighbours (row r + 1 lies in the next slice), so it
+         // takes one row per lane whatever pair_rows says
+         const bool pair = M.pairRows != 0 && ( M.format == TNLHIP_ELLPACK || M.meta.sliceSize >= 2 );
          if( ! knownDistribution( M.xcdRemap ) )
             throw Error( TNLHIP_ERROR_INVALID_ARGUMENT, "unknown row distribution" );
          const PaddedLaunch pl = paddedLaunch( first, last, pair, M.blocksPerCu, deviceProps().cus );
