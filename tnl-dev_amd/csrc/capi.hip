@@ -156,13 +156,13 @@ tnlhip_matrix_s* buildFromSegments( int format, int dtype, int32_t rows, int32_t
    // 160^3, three alternated runs on one box, 78.4 -> 76.1 us; the other FP64 formats measured
    // slower with an XCD split (SlicedEllpack 75.9 -> 79.1, CSRScalar 79.7 -> 82.9, CSRAdaptive
    // 82.4 -> 84.9, ChunkedEllpack 72.0 -> 75.1 us) and keep the plain grid-stride
-   if( format == TNLHIP_ELLPACK && dtype == TNLHIP_F64 )
-      M->xcdRemap = 3;
+   if( format == TNLHIP_ELLPACK && ( dtype == TNLHIP_F64 || dtype == TNLHIP_F32 ) )
+      M->xcdRemap = 3;  // FP32 with 32 workgroups per CU, below: 51.5 -> 50.3 us
    // FP64 launch shapes from a sweep of workgroups per CU (8-64) and one or two rows per lane at
    // 160^3 (two alternated runs): Ellpack 32 workgroups per CU, 76.6 -> 73.7 us; SlicedEllpack two
    // rows per lane and 32 workgroups per CU, 76.1 -> 73.2 us (each row still summed left to right by
    // one lane, so the bits do not change)
-   if( format == TNLHIP_ELLPACK && dtype == TNLHIP_F64 )
+   if( format == TNLHIP_ELLPACK && ( dtype == TNLHIP_F64 || dtype == TNLHIP_F32 ) )
       M->blocksPerCu = 32;
    if( format == TNLHIP_SLICED_ELLPACK && dtype == TNLHIP_F64 ) {
       M->pairRows = 1;
