@@ -218,13 +218,16 @@ int tnlhip_matrix_get_storage( tnlhip_matrix A, double* values, int32_t* columns
 int tnlhip_matrix_get_diagonal( tnlhip_matrix A, void* diagonal, void* stream );
 
 /* Overrides a kernel parameter chosen at construction (ChunkedEllpack and SlicedEllpack with
- * FP32 / BF16 values start with "xcd_remap" 1, every other matrix with 0): "lanes_per_row" (CSR light / hybrid /
+ * FP32 / BF16 values start with "xcd_remap" 1, FP64 / FP32 Ellpack with "xcd_remap" 3 and
+ * "blocks_per_cu" 32, FP64 SlicedEllpack with "pair_rows" 1 and "blocks_per_cu" 32, every other
+ * matrix with "xcd_remap" 0, "pair_rows" 0, "blocks_per_cu" 16): "lanes_per_row" (CSR light / hybrid /
  * vector: a power of two in [1, 64]; once set, that many lanes share each row even when rows
  * average at most 8 stored elements, where these kernels otherwise take CSRScalar's staged
  * one-lane-per-row path); "xcd_remap" (row distribution: 0 the plain order, 1 contiguous row
  * blocks -- slices for ChunkedEllpack -- with each XCD owning a contiguous eighth; 3 per-XCD
  * grid-stride, Ellpack / SlicedEllpack only); for the Ellpack / SlicedEllpack kernels
- * "pair_rows" (0/1: two adjacent rows per lane with 8 / 16-byte loads) and "blocks_per_cu"
+ * "pair_rows" (0/1: two adjacent rows per lane with 8 / 16-byte loads; SlicedEllpack with slice
+ * size 1 always takes one row per lane) and "blocks_per_cu"
  * (workgroups per CU the grid is sized for); "stream_y" (-1 the kernel's default, 0 stores of y
  * through L2, 1 non-temporal stores); "column_stripes" (CSR: 0 drops the column-striped plan,
  * 2 / 4 / 8 / 16 / 32 builds one, a second, stripe-major copy of the entries whose stripes of x
